@@ -1,0 +1,117 @@
+/*
+ * nla3d_gpu.h -- C ABI of libnla3d_gpu.so, the MI355X (gfx950) implementation
+ * of nla3d's Newton-Raphson equilibrium iteration.
+ *
+ * Conventions
+ *   - Every function returns int: 0 = OK, negative = error.  The reference
+ *     aborts through LOG(FATAL)/CHECK instead; here the message of the last
+ *     error is kept per handle (nla3d_gpu_last_error) and nothing aborts.
+ *   - Host pointers are owned by the caller and only read/written during the
+ *     call.  Device buffers are owned by the library.
+ *   - Node, element and equation numbers are 1-based, as in nla3d; CSR
+ *     arrays use the reference's 1-based MKL layout (iofeir, columns).
+ *   - One HIP stream per handle.  A handle is not thread-safe; distinct
+ *     handles may be used concurrently.
+ *   - DoF type codes are nla3d's Dof::dofType (Dof.h:15-25):
+ *     UX=0 UY=1 UZ=2 ROTX=3 ROTY=4 ROTZ=5 HYDRO_PRESSURE=6 TEMP=7.
+ *   - Material type codes are MaterialFactory's ids (MaterialFactory.cpp:9-13):
+ *     1 Neo-Hookean {G, K}, 2 Biderman {C10, C20, C30, C01, K},
+ *     3 Mooney-Rivlin {C10, C01, K}.
+ *
+ * Citations are path:line in dmitryikh/nla3d.
+ */
+#ifndef NLA3D_GPU_H
+#define NLA3D_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NLA3D_GPU_OK 0
+#define NLA3D_GPU_EINVAL (-1)     /* bad argument */
+#define NLA3D_GPU_EHIP (-2)       /* HIP runtime error */
+#define NLA3D_GPU_ESTATE (-3)     /* call out of order (e.g. assemble before init) */
+#define NLA3D_GPU_EPATTERN (-4)   /* CSR position absent / maxInRow overflow (SparseMatrix.cpp:84,401,476) */
+#define NLA3D_GPU_ENOTCONV (-5)   /* Newton/Krylov did not converge (FESolver.cpp:490) */
+#define NLA3D_GPU_EDIVERGED (-6)  /* Newton diverged (FESolver.cpp:484) */
+
+/* element kinds (ElementType, sys.h:151-163) */
+#define NLA3D_GPU_SOLID81 1
+#define NLA3D_GPU_QUADTH 2
+#define NLA3D_GPU_SURFACE_LINETH 3
+
+typedef struct nla3d_gpu_ctx* nla3d_gpu_t;
+
+/* ---- lifetime ---------------------------------------------------------- */
+int nla3d_gpu_create(nla3d_gpu_t* h, int device);
+int nla3d_gpu_destroy(nla3d_gpu_t h);
+const char* nla3d_gpu_last_error(nla3d_gpu_t h);
+
+/* ---- model (FEStorage, FEStorage.h:38-270) ------------------------------ */
+/* FEStorage::createNodes + Node::pos (FEStorage.cpp:199-214): xyz[3*n] */
+int nla3d_gpu_set_nodes(nla3d_gpu_t h, uint32_t n_nodes, const double* xyz);
+/* FEStorage::createElements (FEStorage.h:675-696).  conn: nodes per element
+ * (8 SOLID81, 4 QUADTH, 2 SurfaceLINETH), 1-based.  params: 4 doubles per
+ * element or NULL: QUADTH {k, rho, c, volFlux} (QUADTH.h:27-34),
+ * SurfaceLINETH {flux, htc, etemp0, etemp1} (QUADTH.h:49-51). */
+int nla3d_gpu_add_elements(nla3d_gpu_t h, int kind, uint32_t n, const uint32_t* conn, const double* params);
+/* FEStorage::material (FEStorage.h:46) through MaterialFactory + Material::Ci */
+int nla3d_gpu_set_material(nla3d_gpu_t h, int type, const double* MC, int n);
+/* FESolver::addFix / addLoad (FESolver.cpp:250-259) */
+int nla3d_gpu_add_fix(nla3d_gpu_t h, uint32_t node, int dof, double value);
+int nla3d_gpu_add_load(nla3d_gpu_t h, uint32_t node, int dof, double value);
+/* RigidBodyMpc (Mpc.h:49-60): masterNode, slaveNodes, dofs */
+int nla3d_gpu_add_rigid_body_mpc(nla3d_gpu_t h, uint32_t master, uint32_t n_slaves, const uint32_t* slaves,
+                                 int n_dofs, const int* dofs);
+/* Per-row capacity of the sparsity build, the max_in_row argument of
+ * BlockSparseSymMatrix<2> (BlockSparseMatrix.h:42; FEStorage.cpp:465 uses the
+ * default 100).  A row with more entries fails init_solution with
+ * NLA3D_GPU_EPATTERN, as the reference's "maxInRow overflow!" (SparseMatrix.cpp:84).
+ * 0 = no limit (needed when a rigid-body master DoF couples to more than 100
+ * MPC rows).  Must be called before nla3d_gpu_init_solution. */
+int nla3d_gpu_set_max_in_row(nla3d_gpu_t h, uint32_t max_in_row);
+/* initDofs + setConstrainedDofs + assignEquationNumbers + initSolutionData
+ * (FESolver.cpp:418-421; FEStorage.cpp:338-550), then builds the scatter map
+ * and allocates device storage. */
+int nla3d_gpu_init_solution(nla3d_gpu_t h);
+/* sizes[9] = {nNodes, nElements, nDofs, nConstrained, nUnknown, nMpc,
+ *             nnz(block1), nnz(block12), nnz(block2)} */
+int nla3d_gpu_get_sizes(nla3d_gpu_t h, uint64_t* sizes);
+/* equation numbers: node_eq[nNodes*8], elem_eq[nElements*8] indexed by DoF type,
+ * 0 = DoF not registered (FEStorage::getNodeDofEqNumber, FEStorage.h:606-609) */
+int nla3d_gpu_get_equation_numbers(nla3d_gpu_t h, uint32_t* node_eq, uint32_t* elem_eq);
+/* BlockSparseSymMatrix<2>::block(1), block(1,2), block(2): block = 1, 12, 2.
+ * iofeir[nRows+1], columns[nnz], 1-based (SparseMatrix.h:114-116). */
+int nla3d_gpu_get_pattern(nla3d_gpu_t h, int block, uint32_t* iofeir, uint32_t* columns);
+
+/* ---- solution vector and assembly ---------------------------------------- */
+/* vecU (FEStorage.h:250), length nDofs + nMpc, equation order */
+int nla3d_gpu_set_solution(nla3d_gpu_t h, const double* U);
+int nla3d_gpu_get_solution(nla3d_gpu_t h, double* U);
+/* FEStorage::assembleGlobalEqMatrices (FEStorage.cpp:26-78) on the device for
+ * SOLID81 at the current solution: per element update() + buildK()
+ * (SOLID81.cpp:29-115), deterministic scatter into K and F, then the MPC rows. */
+int nla3d_gpu_assemble(nla3d_gpu_t h);
+/* copy K block values (block = 1, 12, 2) or F (length nDofs + nMpc) to the host */
+int nla3d_gpu_get_K(nla3d_gpu_t h, int block, double* values);
+int nla3d_gpu_get_F(nla3d_gpu_t h, double* F);
+/* Per-element SOLID81 buildK output at the current solution, in the reference's
+ * assemble3 order (SOLID81.h:54-94): Kuu packed upper 24x24 (300), Kup (24),
+ * Kpp, Fu (24), Fp -> out[nSolid81 * 350]. */
+int nla3d_gpu_solid81_element_matrices(nla3d_gpu_t h, double* out);
+/* Gauss-point state after update() (SOLID81.h:38-45): per element and Gauss
+ * point O[9], C[6], S[6] -> out[nSolid81 * 8 * 21]. */
+int nla3d_gpu_solid81_gp_state(nla3d_gpu_t h, double* out);
+
+/* ---- timing -------------------------------------------------------------- */
+/* Runs `iters` device assemblies on the resident solution and returns the mean
+ * duration in ms of the element kernel and of the scatter kernel, measured with
+ * HIP events on the handle's stream. */
+int nla3d_gpu_time_assembly(nla3d_gpu_t h, int iters, double* ms_element, double* ms_scatter);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,210 @@
+"""nla3d_amd: MI355X (gfx950) implementation of nla3d's Newton-Raphson equilibrium iteration.
+
+The compute lives in the in-tree shared library ``libnla3d_gpu.so`` (HIP kernels
+plus the host model store), reached through the C ABI declared in
+``include/nla3d_gpu.h``.  This module is a thin ctypes binding of that ABI.
+There is no CPU fallback: if the library is not built, loading fails loudly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnla3d_gpu.so")
+
+# Dof::dofType (Dof.h:15-25)
+UX, UY, UZ, ROTX, ROTY, ROTZ, HYDRO_PRESSURE, TEMP = range(8)
+# element kinds
+SOLID81, QUADTH, SURFACE_LINETH = 1, 2, 3
+# MaterialFactory ids (MaterialFactory.cpp:9-13)
+NEO_HOOKEAN, BIDERMAN, MOONEY_RIVLIN = 1, 2, 3
+
+_lib = None
+
+_P = ctypes.c_void_p
+_U32P = ctypes.POINTER(ctypes.c_uint32)
+_U64P = ctypes.POINTER(ctypes.c_uint64)
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+
+# name -> argtypes; every function returns int (0 = OK)
+ABI = {
+    "nla3d_gpu_create": [ctypes.POINTER(_P), ctypes.c_int],
+    "nla3d_gpu_destroy": [_P],
+    "nla3d_gpu_set_nodes": [_P, ctypes.c_uint32, _DP],
+    "nla3d_gpu_add_elements": [_P, ctypes.c_int, ctypes.c_uint32, _U32P, _DP],
+    "nla3d_gpu_set_material": [_P, ctypes.c_int, _DP, ctypes.c_int],
+    "nla3d_gpu_add_fix": [_P, ctypes.c_uint32, ctypes.c_int, ctypes.c_double],
+    "nla3d_gpu_add_load": [_P, ctypes.c_uint32, ctypes.c_int, ctypes.c_double],
+    "nla3d_gpu_add_rigid_body_mpc": [_P, ctypes.c_uint32, ctypes.c_uint32, _U32P, ctypes.c_int, _IP],
+    "nla3d_gpu_set_max_in_row": [_P, ctypes.c_uint32],
+    "nla3d_gpu_init_solution": [_P],
+    "nla3d_gpu_get_sizes": [_P, _U64P],
+    "nla3d_gpu_get_equation_numbers": [_P, _U32P, _U32P],
+    "nla3d_gpu_get_pattern": [_P, ctypes.c_int, _U32P, _U32P],
+    "nla3d_gpu_set_solution": [_P, _DP],
+    "nla3d_gpu_get_solution": [_P, _DP],
+    "nla3d_gpu_assemble": [_P],
+    "nla3d_gpu_get_K": [_P, ctypes.c_int, _DP],
+    "nla3d_gpu_get_F": [_P, _DP],
+    "nla3d_gpu_solid81_element_matrices": [_P, _DP],
+    "nla3d_gpu_solid81_gp_state": [_P, _DP],
+    "nla3d_gpu_time_assembly": [_P, ctypes.c_int, _DP, _DP],
+}
+
+
+def lib():
+    """Load the in-tree libnla3d_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in ABI.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.nla3d_gpu_last_error.argtypes = [_P]
+        L.nla3d_gpu_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+class Nla3dGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"nla3d_gpu error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+class GpuModel:
+    """One nla3d_gpu handle: an FEStorage-like model plus its device state.
+
+    device=-1 gives a host-only handle (DoF numbering and sparsity, no GPU)."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        self._h = _P()
+        rc = self._L.nla3d_gpu_create(ctypes.byref(self._h), device)
+        if rc != 0:
+            raise Nla3dGpuError(rc, f"cannot create a handle on device {device}")
+        self.device = device
+
+    def _call(self, name, *args):
+        rc = getattr(self._L, name)(self._h, *args)
+        if rc != 0:
+            raise Nla3dGpuError(rc, self._L.nla3d_gpu_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self._L.nla3d_gpu_destroy(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- model set-up (FEStorage / FESolver) ----
+    def set_nodes(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        self._call("nla3d_gpu_set_nodes", len(xyz), _ptr(xyz, ctypes.c_double))
+
+    def add_elements(self, kind, conn, params=None):
+        conn = np.ascontiguousarray(conn, dtype=np.uint32)
+        n = conn.shape[0]
+        pp = None
+        if params is not None:
+            params = np.ascontiguousarray(params, dtype=np.float64).reshape(n, 4)
+            pp = _ptr(params, ctypes.c_double)
+        self._call("nla3d_gpu_add_elements", kind, n, _ptr(conn, ctypes.c_uint32), pp)
+
+    def set_material(self, mtype, consts):
+        mc = np.ascontiguousarray(consts, dtype=np.float64)
+        self._call("nla3d_gpu_set_material", mtype, _ptr(mc, ctypes.c_double), len(mc))
+
+    def add_fix(self, node, dof, value=0.0):
+        self._call("nla3d_gpu_add_fix", int(node), int(dof), float(value))
+
+    def add_load(self, node, dof, value):
+        self._call("nla3d_gpu_add_load", int(node), int(dof), float(value))
+
+    def add_rigid_body_mpc(self, master, slaves, dofs=(UX, UY, UZ)):
+        s = np.ascontiguousarray(slaves, dtype=np.uint32)
+        d = np.ascontiguousarray(dofs, dtype=np.int32)
+        self._call("nla3d_gpu_add_rigid_body_mpc", int(master), len(s), _ptr(s, ctypes.c_uint32), len(d),
+                   _ptr(d, ctypes.c_int))
+
+    def set_max_in_row(self, n):
+        """per-row sparsity capacity (reference default 100; 0 = unlimited)"""
+        self._call("nla3d_gpu_set_max_in_row", int(n))
+
+    def init_solution(self):
+        self._call("nla3d_gpu_init_solution")
+        s = np.zeros(9, dtype=np.uint64)
+        self._call("nla3d_gpu_get_sizes", _ptr(s, ctypes.c_uint64))
+        keys = ["nNodes", "nElements", "nDofs", "nConstrained", "nUnknown", "nMpc", "nnz1", "nnz12", "nnz2"]
+        self.sizes = {k: int(v) for k, v in zip(keys, s)}
+        self.nAll = self.sizes["nDofs"] + self.sizes["nMpc"]
+        return self.sizes
+
+    def equation_numbers(self):
+        ne = np.zeros((self.sizes["nNodes"], 8), dtype=np.uint32)
+        ee = np.zeros((self.sizes["nElements"], 8), dtype=np.uint32)
+        self._call("nla3d_gpu_get_equation_numbers", _ptr(ne, ctypes.c_uint32), _ptr(ee, ctypes.c_uint32))
+        return ne, ee
+
+    def pattern(self, block):
+        nC = self.sizes["nConstrained"]
+        nrows = nC if block in (1, 12) else self.nAll - nC
+        nnz = self.sizes[{1: "nnz1", 12: "nnz12", 2: "nnz2"}[block]]
+        io = np.zeros(nrows + 1, dtype=np.uint32)
+        cols = np.zeros(max(nnz, 1), dtype=np.uint32)
+        self._call("nla3d_gpu_get_pattern", block, _ptr(io, ctypes.c_uint32), _ptr(cols, ctypes.c_uint32))
+        return io, cols[:nnz]
+
+    # ---- solution and assembly ----
+    def set_solution(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.size == self.nAll
+        self._call("nla3d_gpu_set_solution", _ptr(U, ctypes.c_double))
+
+    def get_solution(self):
+        U = np.zeros(self.nAll)
+        self._call("nla3d_gpu_get_solution", _ptr(U, ctypes.c_double))
+        return U
+
+    def assemble(self):
+        self._call("nla3d_gpu_assemble")
+
+    def get_K(self, block):
+        nnz = self.sizes[{1: "nnz1", 12: "nnz12", 2: "nnz2"}[block]]
+        v = np.zeros(max(nnz, 1))
+        self._call("nla3d_gpu_get_K", block, _ptr(v, ctypes.c_double))
+        return v[:nnz]
+
+    def get_F(self):
+        F = np.zeros(self.nAll)
+        self._call("nla3d_gpu_get_F", _ptr(F, ctypes.c_double))
+        return F
+
+    def solid81_element_matrices(self, n_solid):
+        out = np.zeros((n_solid, 350))
+        self._call("nla3d_gpu_solid81_element_matrices", _ptr(out, ctypes.c_double))
+        return out
+
+    def solid81_gp_state(self, n_solid):
+        out = np.zeros((n_solid, 8, 21))
+        self._call("nla3d_gpu_solid81_gp_state", _ptr(out, ctypes.c_double))
+        return out
+
+    def time_assembly(self, iters):
+        a = ctypes.c_double()
+        b = ctypes.c_double()
+        self._call("nla3d_gpu_time_assembly", int(iters), ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value

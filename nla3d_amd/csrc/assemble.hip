@@ -1,0 +1,128 @@
+// Deterministic scatter of element contributions into the global CSR and F.
+//
+// Reference: the element loop of FEStorage::assembleGlobalEqMatrices
+// (FEStorage.cpp:33-38) calls ElementSOLID81::assemble3 (SOLID81.h:54-94), which
+// adds every element value into its CSR entry with
+// BlockSparseSymMatrix::addValue -> SparsityInfo::getIndex (binary search,
+// SparseMatrix.cpp:117-154) and `values[idx] += v`.  Each CSR entry is thus a
+// sum over the elements that touch it, accumulated in element order from zero.
+//
+// Here the (element, slot) -> entry map is resolved once on the host into a
+// list of contributions sorted by (entry, element); the assembly is then a
+// segmented reduction that sums each entry's contributions in element order
+// from +0.0, reproducing the reference summation order without atomics.
+// The K blocks 1, (1,2), 2 and the F vector share one target array.
+
+#include "internal.h"
+
+namespace nla3d_amd {
+namespace {
+
+__global__ __launch_bounds__(256) void segmented_gather_kernel(int nTargets, const int* __restrict__ seg,
+                                                               const int* __restrict__ src,
+                                                               const double* __restrict__ vals,
+                                                               double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nTargets) return;
+  const int s0 = seg[t], s1 = seg[t + 1];
+  double acc = 0.0;
+  for (int s = s0; s < s1; s++) acc += vals[src[s]];
+  out[t] = acc;
+}
+
+}  // namespace
+
+void launch_segmented_gather(hipStream_t stream, int nTargets, const int* seg, const int* src,
+                             const double* src_vals, double* out) {
+  if (nTargets <= 0) return;
+  const int threads = 256;
+  const int blocks = (nTargets + threads - 1) / threads;
+  hipLaunchKernelGGL(segmented_gather_kernel, dim3(blocks), dim3(threads), 0, stream, nTargets, seg, src,
+                     src_vals, out);
+}
+
+}  // namespace nla3d_amd
+
+namespace nla3d_amd {
+namespace {
+
+// RigidBodyMpc::update (Mpc.cpp:70-164) for one MPC row (slave node, dof):
+// re-linearised coefficients of the master rotations and the residual b,
+// written straight into the K entries (term DoF, MPC equation) and F.
+// Each (term, MPC) entry is touched by no element, so it is set, which equals
+// the reference's zero() followed by addValueMPC (FEStorage.cpp:51-61).
+__global__ __launch_bounds__(256) void rigid_body_mpc_kernel(RigidBodyDev rb, const double* __restrict__ xyz,
+                                                             const int* __restrict__ slaveNode,
+                                                             const int* __restrict__ slaveEq,
+                                                             const double* __restrict__ U,
+                                                             const int* __restrict__ tgt,
+                                                             const int* __restrict__ fTgt,
+                                                             double* __restrict__ vals) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= rb.nRows) return;
+  const int s = m / rb.nDofs, dd = m - s * rb.nDofs;
+  const int i = rb.dofs[dd];  // 0,1,2 = UX,UY,UZ
+  double th[3], w0[3];
+  for (int k = 0; k < 3; k++) {
+    th[k] = rb.masterEq[3 + k] >= 0 ? U[rb.masterEq[3 + k]] : 0.0;
+    w0[k] = rb.masterEq[k] >= 0 ? U[rb.masterEq[k]] : 0.0;
+  }
+  const double t = sqrt(th[0] * th[0] + th[1] * th[1] + th[2] * th[2]);
+  double c1, c2, c3, c4;
+  if (t < 1.0e-3) {
+    c1 = 1.0; c2 = 0.5; c3 = -1.0 / 3.0; c4 = -1.0 / 12.0;
+  } else {
+    const double st = sin(t), ct = cos(t);
+    c1 = st / t;
+    c2 = (1.0 - ct) / (t * t);
+    c3 = (t * ct - st) / (t * t * t);
+    c4 = (t * st - 2.0 + 2.0 * ct) / (t * t * t * t);
+  }
+  const double ct = cos(t);
+  // eps(i,k,j) theta_k summed over k = (Theta)_ij with Theta the skew matrix of theta
+  auto eps = [](int a, int b, int c) -> double {
+    if (a == b || b == c || a == c) return 0.0;
+    return ((a == 0 && b == 1) || (a == 1 && b == 2) || (a == 2 && b == 0)) ? 1.0 : -1.0;
+  };
+  const int sn = slaveNode[s];
+  double pq[3];
+  for (int k = 0; k < 3; k++) pq[k] = xyz[(size_t)sn * 3 + k] - rb.masterPos[k];
+  double b = w0[i];
+  for (int j = 0; j < 3; j++) {
+    const double thij = eps(i, 0, j) * th[0] + eps(i, 1, j) * th[1] + eps(i, 2, j) * th[2];
+    const double Cij = ct * (i == j ? 1.0 : 0.0) + c1 * thij + c2 * th[i] * th[j];
+    b += (Cij - (i == j ? 1.0 : 0.0)) * pq[j];
+  }
+  const int se = slaveEq[m];
+  b -= (se >= 0) ? U[se] : 0.0;
+  double coef[3];
+  for (int l = 0; l < 3; l++) {
+    double acc = 0.0;
+    for (int j = 0; j < 3; j++) {
+      const double thij = eps(i, 0, j) * th[0] + eps(i, 1, j) * th[1] + eps(i, 2, j) * th[2];
+      const double cd = (i == j ? 1.0 : 0.0) * (-c1) * th[l] + c3 * th[l] * thij + c1 * eps(i, l, j) +
+                        c4 * th[l] * th[i] * th[j] +
+                        c2 * ((i == l ? 1.0 : 0.0) * th[j] + th[i] * (j == l ? 1.0 : 0.0));
+      acc += cd * pq[j];
+    }
+    coef[l] = -acc;
+  }
+  const int* tg = tgt + (size_t)m * 5;
+  vals[tg[0]] = 1.0;
+  vals[tg[1]] = -1.0;
+  vals[tg[2]] = coef[0];
+  vals[tg[3]] = coef[1];
+  vals[tg[4]] = coef[2];
+  vals[fTgt[m]] = b;
+}
+
+}  // namespace
+
+void launch_rigid_body_mpc(hipStream_t stream, const RigidBodyDev& rb, const double* xyz, const int* slaveNode,
+                           const int* slaveEq, const double* U, const int* tgt, const int* fTgt, double* vals) {
+  if (rb.nRows <= 0) return;
+  hipLaunchKernelGGL(rigid_body_mpc_kernel, dim3((rb.nRows + 255) / 256), dim3(256), 0, stream, rb, xyz,
+                     slaveNode, slaveEq, U, tgt, fTgt, vals);
+}
+
+}  // namespace nla3d_amd

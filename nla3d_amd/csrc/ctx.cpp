@@ -1,0 +1,95 @@
+// Device set-up and the assembly pipeline of a handle.
+#include "ctx.h"
+
+#include <algorithm>
+
+namespace nla3d_amd {
+
+// defined in capi.cpp
+void buildSolid81ScatterMap(struct Ctx& c);
+
+void Ctx::setupDevice() {
+  FEStorage& m = model;
+  const uint32_t nN = m.nNodes();
+  d_xyz.upload(m.nodePos);
+  std::vector<int> eq3((size_t)nN * 3);
+  for (uint32_t n = 0; n < nN; n++)
+    for (int d = 0; d < 3; d++) eq3[(size_t)n * 3 + d] = (int)m.nodeEq[(size_t)n * 8 + d] - 1;
+  d_nodeEq3.upload(eq3);
+  if (m.solidCount) {
+    std::vector<int> conn((size_t)m.solidCount * 8), peq(m.solidCount);
+    for (uint32_t s = 0; s < m.solidCount; s++) {
+      const uint32_t el = m.solidFirst + s;
+      for (int i = 0; i < 8; i++) conn[(size_t)s * 8 + i] = (int)m.elemNodes[m.elemNodePtr[el] + i] - 1;
+      peq[s] = (int)m.elemEq[(size_t)el * 8 + HYDRO_PRESSURE] - 1;
+    }
+    d_conn.upload(conn);
+    d_elemEqP.upload(peq);
+    d_Ke.alloc((size_t)m.solidCount * kSolid81Slots * 8);
+    buildSolid81ScatterMap(*this);
+  }
+  d_U.alloc((size_t)m.nAll() * 8);
+  hipCheck(hipMemset(d_U.p, 0, d_U.bytes), "memset U");
+  d_vals.alloc((size_t)(m.nnzAll() + m.nAll()) * 8);
+  hipCheck(hipMemset(d_vals.p, 0, d_vals.bytes), "memset vals");
+
+  // rigid-body MPC rows: targets of (term DoF, MPC eq) and of F[MPC eq]
+  rbs.clear();
+  rbs.resize(m.rigidBodies.size());
+  for (size_t r = 0; r < m.rigidBodies.size(); r++) {
+    const RigidBodySpec& spec = m.rigidBodies[r];
+    RB& rb = rbs[r];
+    for (int k = 0; k < 6; k++) rb.dev.masterEq[k] = (int)m.nodeDofEq(spec.masterNode, k) - 1;
+    for (int k = 0; k < 3; k++) rb.dev.masterPos[k] = m.nodePos[(size_t)(spec.masterNode - 1) * 3 + k];
+    rb.dev.nDofs = (int)spec.dofs.size();
+    for (int k = 0; k < rb.dev.nDofs; k++) rb.dev.dofs[k] = spec.dofs[k];
+    rb.dev.nRows = (int)(spec.slaveNodes.size() * spec.dofs.size());
+    std::vector<int> sn(spec.slaveNodes.size()), se(rb.dev.nRows), tg((size_t)rb.dev.nRows * 5), ft(rb.dev.nRows);
+    for (size_t s = 0; s < spec.slaveNodes.size(); s++) sn[s] = (int)spec.slaveNodes[s] - 1;
+    for (int row = 0; row < rb.dev.nRows; row++) {
+      const Mpc& mp = m.mpcs[spec.firstMpc + row];
+      se[row] = (int)m.nodeDofEq(mp.eq[0].node, mp.eq[0].dof) - 1;
+      for (int t = 0; t < 5; t++) {
+        int64_t k = m.findEntry(m.nodeDofEq(mp.eq[t].node, mp.eq[t].dof), mp.eqNum);
+        if (k < 0) throw ApiError(-4, "MPC entry absent from the pattern");
+        tg[(size_t)row * 5 + t] = (int)k;
+      }
+      ft[row] = (int)(m.nnzAll() + mp.eqNum - 1);
+    }
+    rb.slaveNode.upload(sn);
+    rb.slaveEq.upload(se);
+    rb.tgt.upload(tg);
+    rb.fTgt.upload(ft);
+  }
+}
+
+void Ctx::runSolid81(double* gp_state) {
+  launch_solid81(stream, (int)model.solidCount, d_xyz.as<double>(), d_conn.as<int>(), d_nodeEq3.as<int>(),
+                 d_elemEqP.as<int>(), d_U.as<double>(), mat, d_Ke.as<double>(), gp_state);
+  hipCheck(hipGetLastError(), "solid81 kernel launch");
+}
+
+void Ctx::scatter() {
+  launch_segmented_gather(stream, nTargets, d_seg.as<int>(), d_src.as<int>(), d_Ke.as<double>(),
+                          d_vals.as<double>());
+  hipCheck(hipGetLastError(), "scatter kernel launch");
+}
+
+void Ctx::applyMpc() {
+  for (auto& rb : rbs) {
+    launch_rigid_body_mpc(stream, rb.dev, d_xyz.as<double>(), rb.slaveNode.as<int>(), rb.slaveEq.as<int>(),
+                          d_U.as<double>(), rb.tgt.as<int>(), rb.fTgt.as<int>(), d_vals.as<double>());
+    hipCheck(hipGetLastError(), "mpc kernel launch");
+  }
+}
+
+// FEStorage::assembleGlobalEqMatrices (FEStorage.cpp:26-78): the gather writes
+// every K and F entry (entries without element contributions become +0.0,
+// which is zeroK/zeroF), then the MPC rows are set.
+void Ctx::assemble() {
+  runSolid81(nullptr);
+  scatter();
+  applyMpc();
+}
+
+}  // namespace nla3d_amd

@@ -1,0 +1,92 @@
+// The object behind an nla3d_gpu_t handle: model store, material, device
+// buffers and the assembly pipeline.
+#pragma once
+
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "model.h"
+
+namespace nla3d_amd {
+
+struct ApiError : std::runtime_error {
+  int code;
+  ApiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hipCheck(hipError_t e, const char* what);
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  ~DevBuf() { free_(); }
+  void alloc(size_t nbytes);
+  void free_();
+  template <class T>
+  T* as() const { return reinterpret_cast<T*>(p); }
+  template <class T>
+  void upload(const std::vector<T>& v) {
+    alloc(v.size() * sizeof(T));
+    if (!v.empty()) hipCheck(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+  }
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool initialized = false;
+
+  FEStorage model;
+  MaterialParams mat{};
+  bool hasMaterial = false;
+  uint32_t maxInRow = 100;  // BlockSparseSymMatrix<2> default (BlockSparseMatrix.h:42)
+
+  // device copies of the mesh and DoF maps (SOLID81 range)
+  DevBuf d_xyz;        // double [nN*3]
+  DevBuf d_conn;       // int [nSolid*8], 0-based node index
+  DevBuf d_nodeEq3;    // int [nN*3], 0-based equation index of UX UY UZ or -1
+  DevBuf d_elemEqP;    // int [nSolid], 0-based equation index of HYDRO_PRESSURE or -1
+  DevBuf d_U;          // double [nAll]
+  DevBuf d_Ke;         // double [nSolid*350], element outputs in slot order
+  DevBuf d_vals;       // double [nnz1 + nnz12 + nnz2 + nAll]: K blocks then F
+  DevBuf d_seg;        // int [nTargets+1]
+  DevBuf d_src;        // int [nContrib]
+  int nTargets = 0;
+  int64_t nContrib = 0;
+
+  // rigid-body MPC collections
+  struct RB {
+    RigidBodyDev dev{};
+    DevBuf slaveNode, slaveEq, tgt, fTgt;
+  };
+  std::vector<RB> rbs;
+
+  ~Ctx() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void requireDevice() const {
+    if (device < 0) throw ApiError(-3, "host-only handle (device -1) has no device storage");
+  }
+  void requireSolid() const {
+    requireDevice();
+    if (!initialized) throw ApiError(-3, "call nla3d_gpu_init_solution first");
+    if (model.solidCount == 0) throw ApiError(-3, "model has no SOLID81 elements");
+    if (!hasMaterial) throw ApiError(-3, "material is not defined");
+  }
+  void setupDevice();
+  void runSolid81(double* gp_state);
+  void scatter();
+  void applyMpc();
+  void assemble();
+};
+
+}  // namespace nla3d_amd

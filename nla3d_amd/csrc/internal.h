@@ -1,0 +1,53 @@
+// Internal declarations shared by the HIP sources of libnla3d_gpu.so.
+// Not part of the C ABI (that is include/nla3d_gpu.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace nla3d_amd {
+
+// Material constants as they cross into device code: a type tag (the
+// MaterialFactory id) and the MC[] array of the reference Material.
+struct MaterialParams {
+  int type;       // 1 neo-Hookean (G, K), 2 Biderman (C10 C20 C30 C01 K), 3 Mooney-Rivlin (C10 C01 K)
+  double mc[5];
+  double k;       // bulk modulus, getK()
+};
+
+// Number of per-element values produced by the SOLID81 kernel, in the
+// library's slot order (see solid81_slot_to_assemble3).
+constexpr int kSolid81Slots = 350;
+constexpr int kSolid81GpState = 21;  // O[9], C[6], S[6] per Gauss point
+
+// Kernels (launchers). All run on `stream`.
+// xyz[nN*3], conn[nE*8] 0-based, node_eq[nN*3] / elem_eq[nE] 0-based
+// equation index (-1 = DoF not present), U[nAll].
+void launch_solid81(hipStream_t stream, int nE, const double* xyz, const int* conn, const int* node_eq,
+                    const int* elem_eq, const double* U, const MaterialParams& mat, double* Ke,
+                    double* gp_state);
+
+// Segmented reduction: out[t] = sum_{s in [seg[t], seg[t+1])} src_vals[src[s]], summed in s order
+// (starting from +0.0).  int32 offsets: the host checks the total count fits.
+void launch_segmented_gather(hipStream_t stream, int nTargets, const int* seg, const int* src,
+                             const double* src_vals, double* out);
+
+// One RigidBodyMpc collection on the device (Mpc.h:49-60).
+struct RigidBodyDev {
+  int masterEq[6];      // 0-based equation index of the master UX..ROTZ (-1 absent)
+  double masterPos[3];
+  int nRows;            // slaves * dofs
+  int nDofs;
+  int dofs[3];          // 0..2 = UX..UZ
+};
+void launch_rigid_body_mpc(hipStream_t stream, const RigidBodyDev& rb, const double* xyz, const int* slaveNode,
+                           const int* slaveEq, const double* U, const int* tgt, const int* fTgt, double* vals);
+
+// Library slot k (0..349) of the SOLID81 kernel output -> index in the
+// reference assemble3 order (SOLID81.h:54-94).
+void solid81_slot_to_assemble3(int* map350);
+
+}  // namespace nla3d_amd

@@ -1,0 +1,92 @@
+/*
+ * nla3d_oracle.h -- CPU oracle for the nla3d Newton-iteration hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product (nla3d_amd/) never links, loads or calls it.
+ *
+ * Every function restates one reference routine in the reference's own
+ * arithmetic order (loop order, accumulation order, temporaries), so that
+ * its output is what nla3d computes for the same inputs.  Citations are
+ * path:line in dmitryikh/nla3d.
+ *
+ * Pinning (what checks each function against something other than itself):
+ *   orc_mat*prod        test/mat_prod/matlab fixtures (100 cases each, 1e-6)
+ *   orc_hex8_jacob      closed forms: affine maps (det, volume, NiXj exact),
+ *                       translation invariance (test/isoparametric_test.cpp)
+ *   orc_quad4_jacob     same, 2D
+ *   orc_getS_UP         closed-form Mooney-Rivlin / neo-Hookean PK2 stress for
+ *                       homogeneous uniaxial/biaxial stretch; S(C=I,p=0)=0;
+ *                       objectivity (S of rotated F)
+ *   orc_getDdDp_UP      central finite differences of orc_getS_UP in C
+ *   orc_solid81_buildK  finite-difference tangent of the residual
+ *                       (K ~= -dF/d(u,p)), rigid-body invariance, sum Fu = 0,
+ *                       Kpp = -V/k, patch test on jittered meshes
+ *   orc_timecontrol     test/TimeControlTest.cpp exact values
+ *   orc_quadth_*        test/QUADTH/temp.txt (steady ANSYS temperatures)
+ */
+#ifndef NLA3D_ORACLE_H
+#define NLA3D_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* material type tags, as MaterialFactory::matId (materials/MaterialFactory.cpp:9-13) */
+enum { ORC_MAT_NEO_HOOKEAN = 1, ORC_MAT_BIDERMAN = 2, ORC_MAT_MOONEY_RIVLIN = 3 };
+
+/* --- dense small-matrix kernels, math/Mat.h:686-822 (row-major, packed upper MatSym) --- */
+void orc_matBTDBprod(int M, int N, const double* B, const double* Dsym, double coef, double* Rsym);
+void orc_matBTVprod(int M, int N, const double* B, const double* V, double coef, double* R);
+void orc_matBVprod(int M, int N, const double* B, const double* V, double coef, double* R);
+void orc_matABprod(int M1, int N1, int N2, const double* A, const double* B, double coef, double* R);
+void orc_matATBprod(int M1, int N1, int N2, const double* A, const double* B, double coef, double* R);
+
+/* --- isoparametric geometry, elements/isoparametric.cpp --- */
+/* X: 8x3 node coordinates.  NiXj: [8 gp][8 node][3], det: [8 gp]; 2x2x2 Gauss. Returns #gp. */
+int orc_hex8_jacob(const double* X, double* NiXj, double* det);
+/* X: 4x3 node coordinates (z ignored).  NiXj: [4 gp][4 node][2], det[4]. */
+int orc_quad4_jacob(const double* X, double* NiXj, double* det);
+
+/* --- hyperelastic materials, materials/materials_hyperelastic.cpp --- */
+double orc_mat_K(int type, const double* MC);
+void orc_getS_UP(int type, const double* MC, const double* C, double press, double* S);
+void orc_getDdDp_UP(int type, const double* MC, const double* C, double press, double* Dd, double* Dp);
+
+/* --- SOLID81, elements/SOLID81.cpp --- */
+/* update(): U[24] nodal displacements (node-major, UX UY UZ), p_e pressure.
+ * out: O[8][9], C[8][6], S[8][6]. */
+void orc_solid81_update(const double* NiXj, const double* U, double p_e, int type, const double* MC,
+                        double* O, double* C, double* S);
+/* buildK(): writes out[350] in assemble3 order: Kuu packed upper 24x24 (300), Kup (24), Kpp, Fu (24), Fp */
+void orc_solid81_buildK(const double* NiXj, const double* det, const double* O, const double* C,
+                        const double* S, double p_e, int type, const double* MC, double* out);
+/* batched convenience: for ne elements with coordinates Xe[ne][8][3], U[ne][24], p[ne]:
+ * makeJacob + update + buildK -> out[ne][350]. */
+void orc_solid81_elements(int ne, const double* Xe, const double* Ue, const double* pe, int type,
+                          const double* MC, double* out);
+
+/* reference assembly loop (element order, binary-search addValue) into the three 1-based CSR
+ * blocks and F.  eq[ne][25]: global 1-based equations of the 24 node DoFs and the pressure.
+ * Returns 0, or -1 if a position is absent from the pattern. */
+int orc_solid81_assemble_ref(int ne, const double* Xe, const double* Ue, const double* pe, int type,
+                             const double* MC, const unsigned* eq, unsigned nC, const unsigned* io1,
+                             const unsigned* c1, double* v1, const unsigned* io12, const unsigned* c12,
+                             double* v12, const unsigned* io2, const unsigned* c2, double* v2, double* F);
+
+/* --- QUADTH / SurfaceLINETH, elements/QUADTH.cpp --- */
+/* Ke packed upper 4x4 (10), Fe (4), Ce packed upper (10) */
+void orc_quadth_element(const double* X, double k, double rho, double c, double volFlux, double* Ke,
+                        double* Fe, double* Ce);
+/* line: X 2x3; Ke packed upper 2x2 (3), Fe (2) */
+void orc_lineth_element(const double* X, double flux, double htc, double etemp0, double etemp1,
+                        double* Ke, double* Fe);
+
+/* --- TimeControl, FESolver.cpp:34-56: nextStep(delta) until it returns false;
+ *     returns the number of steps, times[] the step end times --- */
+int orc_timecontrol(double start, double end, double delta, double* times, int max_steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,473 @@
+"""Python half of the CPU oracle (TEST INFRASTRUCTURE ONLY: the checker).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module.  The product never does.
+
+* ctypes bindings of liboracle.so (nla3d_oracle.c: element, material and dense
+  kernels restated in the reference's arithmetic order);
+* a restatement of the reference's host-side integer work for small models:
+  DoF registration and numbering (FEStorage.cpp:338-432), the SparsityInfo
+  build with maxInRow slots and compress (SparseMatrix.cpp:64-112;
+  FEStorage.cpp:434-550), BlockSparseSymMatrix<2> addValue with the binary
+  search getIndex (BlockSparseMatrix.h:115-168; SparseMatrix.cpp:117-154,
+  467-479), the element-order assembly loop (FEStorage.cpp:26-78;
+  SOLID81.h:54-94) and RigidBodyMpc::update (Mpc.cpp:70-164);
+* TimeControl (FESolver.cpp:34-56) and the Newton loop of
+  NonlinearFESolver::solve (FESolver.cpp:407-508) with a dense FP64 solve in
+  place of PARDISO (PARDISO itself is unavailable: parity unpinned there).
+
+Pure-Python loops: intended for models of at most a few thousand DoFs.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+UX, UY, UZ, ROTX, ROTY, ROTZ, HYDRO_PRESSURE, TEMP = range(8)
+NUM_DOF_TYPES = 8
+SOLID81, QUADTH, SURFACE_LINETH = 1, 2, 3
+
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in {
+            "orc_matBTDBprod": [ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_double, _D],
+            "orc_matBTVprod": [ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_double, _D],
+            "orc_matBVprod": [ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_double, _D],
+            "orc_matABprod": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_double, _D],
+            "orc_matATBprod": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_double, _D],
+            "orc_hex8_jacob": [_D, _D, _D],
+            "orc_quad4_jacob": [_D, _D, _D],
+            "orc_getS_UP": [ctypes.c_int, _D, _D, ctypes.c_double, _D],
+            "orc_getDdDp_UP": [ctypes.c_int, _D, _D, ctypes.c_double, _D, _D],
+            "orc_solid81_update": [_D, _D, ctypes.c_double, ctypes.c_int, _D, _D, _D, _D],
+            "orc_solid81_buildK": [_D, _D, _D, _D, _D, ctypes.c_double, ctypes.c_int, _D, _D],
+            "orc_solid81_elements": [ctypes.c_int, _D, _D, _D, ctypes.c_int, _D, _D],
+            "orc_quadth_element": [_D, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D, _D],
+            "orc_lineth_element": [_D, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D],
+            "orc_timecontrol": [ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, ctypes.c_int],
+        }.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int if name in ("orc_hex8_jacob", "orc_quad4_jacob", "orc_timecontrol") else None
+        _U = ctypes.POINTER(ctypes.c_uint32)
+        L.orc_solid81_assemble_ref.argtypes = [ctypes.c_int, _D, _D, _D, ctypes.c_int, _D, _U, ctypes.c_uint32,
+                                               _U, _U, _D, _U, _U, _D, _U, _U, _D, _D]
+        L.orc_solid81_assemble_ref.restype = ctypes.c_int
+        L.orc_mat_K.argtypes = [ctypes.c_int, _D]
+        L.orc_mat_K.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+# ---------------- element-level wrappers ----------------
+def matBTDBprod(B, Dsym, coef, Rsym):
+    B = _f64(B); Dsym = _f64(Dsym); R = _f64(Rsym).copy()
+    lib().orc_matBTDBprod(B.shape[0], B.shape[1], _p(B), _p(Dsym), coef, _p(R))
+    return R
+
+
+def matBTVprod(B, V, coef, R):
+    B = _f64(B); V = _f64(V); R = _f64(R).copy()
+    lib().orc_matBTVprod(B.shape[0], B.shape[1], _p(B), _p(V), coef, _p(R))
+    return R
+
+
+def matBVprod(B, V, coef, R):
+    B = _f64(B); V = _f64(V); R = _f64(R).copy()
+    lib().orc_matBVprod(B.shape[0], B.shape[1], _p(B), _p(V), coef, _p(R))
+    return R
+
+
+def matABprod(A, B, coef, R):
+    A = _f64(A); B = _f64(B); R = _f64(R).copy()
+    lib().orc_matABprod(A.shape[0], A.shape[1], B.shape[1], _p(A), _p(B), coef, _p(R))
+    return R
+
+
+def matATBprod(A, B, coef, R):
+    A = _f64(A); B = _f64(B); R = _f64(R).copy()
+    lib().orc_matATBprod(A.shape[0], A.shape[1], B.shape[1], _p(A), _p(B), coef, _p(R))
+    return R
+
+
+def hex8_jacob(X):
+    X = _f64(X, (8, 3)); N = np.zeros((8, 8, 3)); det = np.zeros(8)
+    lib().orc_hex8_jacob(_p(X), _p(N), _p(det))
+    return N, det
+
+
+def quad4_jacob(X):
+    X = _f64(X, (4, 3)); N = np.zeros((4, 4, 2)); det = np.zeros(4)
+    lib().orc_quad4_jacob(_p(X), _p(N), _p(det))
+    return N, det
+
+
+def mat_K(mtype, MC):
+    MC = _f64(MC)
+    return lib().orc_mat_K(mtype, _p(MC))
+
+
+def getS_UP(mtype, MC, C, p):
+    MC = _f64(MC); C = _f64(C); S = np.zeros(6)
+    lib().orc_getS_UP(mtype, _p(MC), _p(C), float(p), _p(S))
+    return S
+
+
+def getDdDp_UP(mtype, MC, C, p):
+    MC = _f64(MC); C = _f64(C); Dd = np.zeros(21); Dp = np.zeros(6)
+    lib().orc_getDdDp_UP(mtype, _p(MC), _p(C), float(p), _p(Dd), _p(Dp))
+    return Dd, Dp
+
+
+def solid81_update(NiXj, U, p, mtype, MC):
+    NiXj = _f64(NiXj); U = _f64(U); MC = _f64(MC)
+    O = np.zeros((8, 9)); C = np.zeros((8, 6)); S = np.zeros((8, 6))
+    lib().orc_solid81_update(_p(NiXj), _p(U), float(p), mtype, _p(MC), _p(O), _p(C), _p(S))
+    return O, C, S
+
+
+def solid81_elements(Xe, Ue, pe, mtype, MC):
+    """Per-element makeJacob + update + buildK -> (ne, 350) in assemble3 order."""
+    Xe = _f64(Xe); Ue = _f64(Ue); pe = _f64(pe); MC = _f64(MC)
+    ne = pe.size
+    out = np.zeros((ne, 350))
+    lib().orc_solid81_elements(ne, _p(Xe), _p(Ue), _p(pe), mtype, _p(MC), _p(out))
+    return out
+
+
+def solid81_assemble_ref(Xe, Ue, pe, mtype, MC, eq, nC, pat1, pat12, pat2, nAll):
+    """reference assembly loop in C; pat* = (iofeir, columns) uint32 1-based.
+    Returns ({1: v1, 12: v12, 2: v2}, F)."""
+    U32 = ctypes.POINTER(ctypes.c_uint32)
+    Xe = _f64(Xe); Ue = _f64(Ue); pe = _f64(pe); MC = _f64(MC)
+    eq = np.ascontiguousarray(eq, dtype=np.uint32)
+    pats = [tuple(np.ascontiguousarray(a, dtype=np.uint32) for a in p) for p in (pat1, pat12, pat2)]
+    vals = [np.zeros(max(len(p[1]), 1)) for p in pats]
+    F = np.zeros(nAll)
+    args = []
+    for (io, c), v in zip(pats, vals):
+        args += [io.ctypes.data_as(U32), c.ctypes.data_as(U32), _p(v)]
+    rc = lib().orc_solid81_assemble_ref(pe.size, _p(Xe), _p(Ue), _p(pe), mtype, _p(MC), eq.ctypes.data_as(U32),
+                                        nC, *args, _p(F))
+    if rc != 0:
+        raise RuntimeError("position absent from the pattern")
+    return {1: vals[0][:len(pats[0][1])], 12: vals[1][:len(pats[1][1])], 2: vals[2][:len(pats[2][1])]}, F
+
+
+def quadth_element(X, k, rho, c, volFlux=0.0):
+    X = _f64(X, (4, 3)); Ke = np.zeros(10); Fe = np.zeros(4); Ce = np.zeros(10)
+    lib().orc_quadth_element(_p(X), k, rho, c, volFlux, _p(Ke), _p(Fe), _p(Ce))
+    return Ke, Fe, Ce
+
+
+def lineth_element(X, flux, htc, e0, e1):
+    X = _f64(X, (2, 3)); Ke = np.zeros(3); Fe = np.zeros(2)
+    lib().orc_lineth_element(_p(X), flux, htc, e0, e1, _p(Ke), _p(Fe))
+    return Ke, Fe
+
+
+def timecontrol(start, end, delta):
+    """step end times of TimeControl::nextStep(delta) from currentTime = 0"""
+    t = np.zeros(int((end - start) / delta) * 2 + 8)
+    n = lib().orc_timecontrol(start, end, delta, _p(t), len(t))
+    return t[:n]
+
+
+def load_steps(nsteps, start=0.0, end=1.0):
+    """NonlinearFESolver schedule: delta = (end - start) / N (FESolver.cpp:439)"""
+    return timecontrol(start, end, (end - start) / nsteps)
+
+
+# ---------------- host-side model restatement ----------------
+class SparsityInfo:
+    """SparsityInfo (SparseMatrix.cpp:44-154): per-row slots of maxInRow, compress sorts."""
+
+    def __init__(self, nrows, ncols, max_in_row=100):
+        self.nRows, self.nColumns = nrows, ncols
+        self.maxInRow = min(max_in_row, ncols)
+        self.rows = [[] for _ in range(nrows)]
+        self.compressed = False
+
+    def addEntry(self, i, j):  # 1-based
+        assert 0 < i <= self.nRows and 0 < j <= self.nColumns and not self.compressed
+        r = self.rows[i - 1]
+        if j in r:
+            return
+        if len(r) >= self.maxInRow:
+            raise RuntimeError("maxInRow overflow!")
+        r.append(j)
+
+    def compress(self):
+        self.iofeir = np.zeros(self.nRows + 1, dtype=np.uint32)
+        cols = []
+        nxt = 0
+        for i in range(self.nRows):
+            self.iofeir[i] = nxt + 1
+            rr = sorted(self.rows[i])
+            cols.extend(rr)
+            nxt += len(rr)
+        self.iofeir[self.nRows] = nxt + 1
+        self.columns = np.array(cols, dtype=np.uint32)
+        self.values = np.zeros(len(cols))
+        self.compressed = True
+
+    def getIndex(self, i, j):
+        """binary search exactly as SparsityInfo::getIndex (SparseMatrix.cpp:117-154); -1 = invalid"""
+        st = int(self.iofeir[i - 1]) - 1
+        en = int(self.iofeir[i]) - 1
+        if st == en:
+            return -1
+        en -= 1
+        cols = self.columns
+        while True:
+            if en - st == 1:
+                if cols[st] == j:
+                    return st
+                if cols[en] == j:
+                    return en
+                return -1
+            ind = int((en + st) * 0.5)
+            if cols[ind] == j:
+                return ind
+            if en == st:
+                return -1
+            if cols[ind] > j:
+                en = ind
+            else:
+                st = ind
+
+
+class OracleModel:
+    """FEStorage + NonlinearFESolver restated for small models."""
+
+    def __init__(self, xyz, elements, material=None, fixes=(), loads=(), rigid_bodies=(), elem_params=None):
+        """elements: list of (kind, [1-based nodes]); fixes/loads: (node, dof, value);
+        rigid_bodies: (master, [slaves], [dofs]); material: (type, MC)."""
+        self.xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
+        self.elements = [(k, list(map(int, n))) for k, n in elements]
+        self.material = material
+        self.fixes = list(fixes)
+        self.loads = list(loads)
+        self.rigid_bodies = [(int(m), list(map(int, s)), list(d)) for m, s, d in rigid_bodies]
+        self.elem_params = elem_params
+
+    # FEStorage::initDofs (FEStorage.cpp:338-379)
+    def initDofs(self):
+        nN, nE = len(self.xyz), len(self.elements)
+        self.nodeMask = [set() for _ in range(nN)]
+        self.elemMask = [set() for _ in range(nE)]
+        for e, (kind, nodes) in enumerate(self.elements):
+            if kind == SOLID81:
+                for n in nodes:
+                    self.nodeMask[n - 1] |= {UX, UY, UZ}
+                self.elemMask[e] |= {HYDRO_PRESSURE}
+            else:
+                for n in nodes:
+                    self.nodeMask[n - 1] |= {TEMP}
+        self.mpcs = []  # [terms list of (node, dof, coef), b, eqNum]
+        self.rb_first = []
+        for master, slaves, dofs in self.rigid_bodies:
+            self.rb_first.append(len(self.mpcs))
+            for s in slaves:
+                for d in dofs:
+                    self.mpcs.append([[(s, d, 1.0), (master, d, -1.0), (master, ROTX, 0.0), (master, ROTY, 0.0),
+                                       (master, ROTZ, 0.0)], 0.0, 0])
+            self.nodeMask[master - 1] |= {UX, UY, UZ, ROTX, ROTY, ROTZ}
+        self.nDofs = sum(len(m) for m in self.nodeMask) + sum(len(m) for m in self.elemMask)
+
+    # FESolver::setConstrainedDofs + FEStorage::assignEquationNumbers (FEStorage.cpp:382-432)
+    def assignEquationNumbers(self):
+        constrained = set()
+        for node, dof, _ in self.fixes:
+            constrained.add((node, dof))
+        self.nC = len(constrained)
+        self.nU = self.nDofs - self.nC
+        nxt_s, nxt_c = self.nC + 1, 1
+        self.elemEq = [dict() for _ in self.elements]
+        self.nodeEq = [dict() for _ in range(len(self.xyz))]
+        for e in range(len(self.elements)):
+            for t in range(NUM_DOF_TYPES):
+                if t in self.elemMask[e]:
+                    self.elemEq[e][t] = nxt_s
+                    nxt_s += 1
+        for n in range(len(self.xyz)):
+            for t in range(NUM_DOF_TYPES):
+                if t in self.nodeMask[n]:
+                    if (n + 1, t) in constrained:
+                        self.nodeEq[n][t] = nxt_c
+                        nxt_c += 1
+                    else:
+                        self.nodeEq[n][t] = nxt_s
+                        nxt_s += 1
+        for m in self.mpcs:
+            m[2] = nxt_s
+            nxt_s += 1
+        self.nMpc = len(self.mpcs)
+        self.nAll = self.nDofs + self.nMpc
+
+    def _block(self, i):
+        return (1, i) if i <= self.nC else (2, i - self.nC)
+
+    def _addEntry(self, i, j):  # BlockSparseSymMatrix::addEntry (BlockSparseMatrix.h:139-152)
+        if i > j:
+            i, j = j, i
+        bi, pi = self._block(i)
+        bj, pj = self._block(j)
+        if bi == bj:
+            self.blocks[bi].addEntry(pi, pj)
+        else:
+            self.blocks[12].addEntry(pi, pj)
+
+    # FEStorage::initSolutionData (FEStorage.cpp:434-550)
+    def initSolutionData(self):
+        nC, nS = self.nC, self.nU + self.nMpc
+        self.blocks = {1: SparsityInfo(nC, nC), 12: SparsityInfo(nC, nS), 2: SparsityInfo(nS, nS)}
+        topology = [set() for _ in range(len(self.xyz))]
+        for e, (_, nodes) in enumerate(self.elements):
+            for n in nodes:
+                topology[n - 1].add(e + 1)
+        for nn in range(1, len(self.xyz) + 1):
+            nn_dofs = [self.nodeEq[nn - 1][t] for t in sorted(self.nodeMask[nn - 1])]
+            for en in sorted(topology[nn - 1]):
+                en_dofs = [self.elemEq[en - 1][t] for t in sorted(self.elemMask[en - 1])]
+                for d1 in en_dofs:
+                    for d2 in nn_dofs:
+                        self._addEntry(d1, d2)
+                for nn2 in self.elements[en - 1][1]:
+                    if nn2 < nn:
+                        continue
+                    nn2_dofs = [self.nodeEq[nn2 - 1][t] for t in sorted(self.nodeMask[nn2 - 1])]
+                    for d1 in nn2_dofs:
+                        for d2 in nn_dofs:
+                            self._addEntry(d1, d2)
+        for en in range(1, len(self.elements) + 1):
+            en_dofs = [self.elemEq[en - 1][t] for t in sorted(self.elemMask[en - 1])]
+            for d1 in en_dofs:
+                for d2 in en_dofs:
+                    self._addEntry(d1, d2)
+        for terms, _, eq in self.mpcs:
+            for node, dof, _ in terms:
+                self._addEntry(eq, self.nodeEq[node - 1][dof])
+        for b in self.blocks.values():
+            b.compress()
+
+    def setup(self):
+        self.initDofs()
+        self.assignEquationNumbers()
+        self.initSolutionData()
+        return self
+
+    # BlockSparseSymMatrix::addValue (BlockSparseMatrix.h:155-168)
+    def addValueK(self, i, j, v):
+        if i > j:
+            i, j = j, i
+        bi, pi = self._block(i)
+        bj, pj = self._block(j)
+        blk = self.blocks[bi] if bi == bj else self.blocks[12]
+        idx = blk.getIndex(pi, pj)
+        if idx < 0:
+            raise RuntimeError(f"The position({i}, {j}) is absent in the matrix")
+        blk.values[idx] += v
+
+    def solid81_elements_at(self, U):
+        """per-element 350-vectors (assemble3 order) at solution U, SOLID81 elements in order"""
+        Xe, Ue, pe = [], [], []
+        for e, (kind, nodes) in enumerate(self.elements):
+            if kind != SOLID81:
+                continue
+            Xe.append(self.xyz[np.array(nodes) - 1])
+            Ue.append([U[self.nodeEq[n - 1][d] - 1] for n in nodes for d in range(3)])
+            pe.append(U[self.elemEq[e][HYDRO_PRESSURE] - 1])
+        return solid81_elements(np.array(Xe), np.array(Ue), np.array(pe), *self.material)
+
+    # FEStorage::assembleGlobalEqMatrices (FEStorage.cpp:26-78) for SOLID81 + RigidBodyMpc
+    def assemble(self, U):
+        for b in self.blocks.values():
+            b.values[:] = 0.0
+        F = np.zeros(self.nAll)
+        ke = self.solid81_elements_at(U)
+        s = 0
+        for e, (kind, nodes) in enumerate(self.elements):
+            if kind != SOLID81:
+                continue
+            out = ke[s]
+            s += 1
+            eqn = [self.nodeEq[n - 1][d] for n in nodes for d in range(3)]
+            ep = self.elemEq[e][HYDRO_PRESSURE]
+            k = 0
+            for i in range(8):  # SOLID81.h:62-74
+                for di in range(3):
+                    for j in range(i, 8):
+                        for dj in range(3):
+                            if i == j and dj < di:
+                                continue
+                            self.addValueK(eqn[3 * i + di], eqn[3 * j + dj], out[k])
+                            k += 1
+            for a in range(24):
+                self.addValueK(eqn[a], ep, out[300 + a])
+            self.addValueK(ep, ep, out[324])
+            for a in range(24):
+                F[eqn[a] - 1] += out[325 + a]
+            F[ep - 1] += out[349]
+        self.rigid_body_update(U)
+        for terms, b, eq in self.mpcs:  # FEStorage.cpp:51-61
+            F[eq - 1] = b
+            for node, dof, coef in terms:
+                self.addValueK(eq, self.nodeEq[node - 1][dof], coef)
+        return {b: blk.values.copy() for b, blk in self.blocks.items()}, F
+
+    # RigidBodyMpc::update (Mpc.cpp:70-164)
+    def rigid_body_update(self, U):
+        LC = [[[0, 0, 0], [0, 0, 1], [0, -1, 0]], [[0, 0, -1], [0, 0, 0], [1, 0, 0]],
+              [[0, 1, 0], [-1, 0, 0], [0, 0, 0]]]
+        Id = [[1, 0, 0], [0, 1, 0], [0, 0, 1]]
+        for r, (master, slaves, dofs) in enumerate(self.rigid_bodies):
+            th = [U[self.nodeEq[master - 1][t] - 1] for t in (ROTX, ROTY, ROTZ)]
+            w0 = [U[self.nodeEq[master - 1][t] - 1] for t in (UX, UY, UZ)]
+            tn = math.sqrt(th[0] * th[0] + th[1] * th[1] + th[2] * th[2])
+            if tn < 1.0e-3:
+                c1, c2, c3, c4 = 1.0, 0.5, -1.0 / 3.0, -1.0 / 12.0
+            else:
+                c1 = math.sin(tn) / tn
+                c2 = (1.0 - math.cos(tn)) / (tn * tn)
+                c3 = (tn * math.cos(tn) - math.sin(tn)) / (tn * tn * tn)
+                c4 = (tn * math.sin(tn) - 2.0 + 2.0 * math.cos(tn)) / (tn * tn * tn * tn)
+            C = [[math.cos(tn) * Id[i][j] + c1 * (LC[i][0][j] * th[0] + LC[i][1][j] * th[1] + LC[i][2][j] * th[2])
+                  + c2 * th[i] * th[j] for j in range(3)] for i in range(3)]
+            mp = self.xyz[master - 1]
+            for n, s in enumerate(slaves):
+                pq = self.xyz[s - 1] - mp
+                for dd, d in enumerate(dofs):
+                    i = d
+                    cD = [[Id[i][j] * (-c1) * th[l] + c3 * th[l] * (LC[i][0][j] * th[0] + LC[i][1][j] * th[1]
+                                                                 + LC[i][2][j] * th[2]) + c1 * LC[i][l][j]
+                           + c4 * th[l] * th[i] * th[j] + c2 * (Id[i][l] * th[j] + th[i] * Id[j][l])
+                           for l in range(3)] for j in range(3)]
+                    m = self.mpcs[self.rb_first[r] + n * len(dofs) + dd]
+                    m[1] = (w0[i] + (C[i][0] - Id[i][0]) * pq[0] + (C[i][1] - Id[i][1]) * pq[1]
+                            + (C[i][2] - Id[i][2]) * pq[2] - U[self.nodeEq[s - 1][d] - 1])
+                    terms = m[0]
+                    for l, t in enumerate((ROTX, ROTY, ROTZ)):
+                        coef = -(cD[0][l] * pq[0] + cD[1][l] * pq[1] + cD[2][l] * pq[2])
+                        terms[2 + l] = (terms[2 + l][0], t, coef)
