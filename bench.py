@@ -84,7 +84,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--shape", type=str, default="120,120,60")
-    ap.add_argument("--cpu-sample", type=int, default=40000)
+    ap.add_argument("--cpu-sample", type=int, default=250000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -105,6 +105,49 @@ int nla3d_gpu_solid81_element_matrices(nla3d_gpu_t h, double* out);
  * point O[9], C[6], S[6] -> out[nSolid81 * 8 * 21]. */
 int nla3d_gpu_solid81_gp_state(nla3d_gpu_t h, double* out);
 
+/* ---- solution procedures (FESolver, FESolver.h:60-242) ------------------- */
+/* Options of the Newton-Raphson loop (NonlinearFESolver, FESolver.h:205-218)
+ * and of the Krylov solve that replaces EquationSolver::solveEquations. */
+typedef struct {
+  int n_load_steps;      /* numberOfLoadsteps; TimeControl delta = 1/N (FESolver.cpp:439) */
+  int max_iterations;    /* numberOfIterations (FESolver.h:210) */
+  int criterion;         /* 0: reference, mean|dU_s| / c < c (FESolver.cpp:474-483, 511-521)
+                            1: relative residual, ||rhs_k|| <= c ||rhs_1|| in each load step */
+  double convergence;    /* c: convergenceCriteria (reference default 1e-3) or the residual ratio */
+  double krylov_tol;     /* MINRES: preconditioned residual <= krylov_tol * initial */
+  int krylov_max_iter;
+  int preconditioner;    /* 0: block Jacobi */
+} nla3d_gpu_newton_opts;
+
+typedef struct {
+  int load_steps;            /* load steps completed */
+  int newton_iterations;     /* total equilibrium iterations (TimeControl::getTotalNumberOfEquilibriumSteps) */
+  int last_step_iterations;  /* iterations of the last load step */
+  int linear_solves;
+  long long krylov_iterations;  /* summed over all linear solves */
+  int krylov_breakdown;      /* a MINRES solve hit an indefinite preconditioner */
+  double last_criterion;     /* value of the convergence measure at exit */
+  double last_time;          /* normalised time of the last completed step */
+  double ms_assemble, ms_solve, ms_total;  /* wall time (host clock around device work) */
+} nla3d_gpu_newton_stats;
+
+/* Fills opts with the reference defaults (20 iterations, 10 load steps,
+ * criterion 0 with 1e-3) and krylov_tol 1e-12, krylov_max_iter 20000. */
+int nla3d_gpu_newton_defaults(nla3d_gpu_newton_opts* opts);
+/* One load step of NonlinearFESolver::solve (FESolver.cpp:441-490): Newton
+ * iterations at normalised time t until the criterion holds.  The model state
+ * (vecU, previous constrained values) carries over between calls. */
+int nla3d_gpu_solve_load_step(nla3d_gpu_t h, double t, const nla3d_gpu_newton_opts* opts,
+                              nla3d_gpu_newton_stats* stats);
+/* NonlinearFESolver::solve: TimeControl over [0, 1] with n_load_steps steps,
+ * each solved by nla3d_gpu_solve_load_step. */
+int nla3d_gpu_solve_nonlinear(nla3d_gpu_t h, const nla3d_gpu_newton_opts* opts, nla3d_gpu_newton_stats* stats);
+/* EquationSolver::solveEquations(K.block(2), rhs, x) (EquationSolver.h:22) with the
+ * currently assembled K: MINRES + block Jacobi.  rhs and x are host arrays of
+ * length nUnknown + nMpc.  iters / relres (preconditioned estimate) may be NULL. */
+int nla3d_gpu_solve_block2(nla3d_gpu_t h, const double* rhs, double* x, double tol, int max_iter, int* iters,
+                           double* relres);
+
 /* ---- timing -------------------------------------------------------------- */
 /* Runs `iters` device assemblies on the resident solution and returns the mean
  * duration in ms of the element kernel and of the scatter kernel, measured with

@@ -22,6 +22,26 @@ NEO_HOOKEAN, BIDERMAN, MOONEY_RIVLIN = 1, 2, 3
 
 _lib = None
 
+
+class NewtonOpts(ctypes.Structure):
+    """nla3d_gpu_newton_opts (include/nla3d_gpu.h)"""
+    _fields_ = [("n_load_steps", ctypes.c_int), ("max_iterations", ctypes.c_int), ("criterion", ctypes.c_int),
+                ("convergence", ctypes.c_double), ("krylov_tol", ctypes.c_double),
+                ("krylov_max_iter", ctypes.c_int), ("preconditioner", ctypes.c_int)]
+
+
+class NewtonStats(ctypes.Structure):
+    """nla3d_gpu_newton_stats (include/nla3d_gpu.h)"""
+    _fields_ = [("load_steps", ctypes.c_int), ("newton_iterations", ctypes.c_int),
+                ("last_step_iterations", ctypes.c_int), ("linear_solves", ctypes.c_int),
+                ("krylov_iterations", ctypes.c_longlong), ("krylov_breakdown", ctypes.c_int),
+                ("last_criterion", ctypes.c_double), ("last_time", ctypes.c_double),
+                ("ms_assemble", ctypes.c_double), ("ms_solve", ctypes.c_double), ("ms_total", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _P = ctypes.c_void_p
 _U32P = ctypes.POINTER(ctypes.c_uint32)
 _U64P = ctypes.POINTER(ctypes.c_uint64)
@@ -51,7 +71,12 @@ ABI = {
     "nla3d_gpu_solid81_element_matrices": [_P, _DP],
     "nla3d_gpu_solid81_gp_state": [_P, _DP],
     "nla3d_gpu_time_assembly": [_P, ctypes.c_int, _DP, _DP],
+    "nla3d_gpu_solve_load_step": [_P, ctypes.c_double, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats)],
+    "nla3d_gpu_solve_nonlinear": [_P, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats)],
+    "nla3d_gpu_solve_block2": [_P, _DP, _DP, ctypes.c_double, ctypes.c_int, _IP, _DP],
 }
+# functions that take no handle
+ABI_FREE = {"nla3d_gpu_newton_defaults": [ctypes.POINTER(NewtonOpts)]}
 
 
 def lib():
@@ -62,6 +87,10 @@ def lib():
             raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
         L = ctypes.CDLL(LIB_PATH)
         for name, args in ABI.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        for name, args in ABI_FREE.items():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = ctypes.c_int
@@ -202,6 +231,34 @@ class GpuModel:
         out = np.zeros((n_solid, 8, 21))
         self._call("nla3d_gpu_solid81_gp_state", _ptr(out, ctypes.c_double))
         return out
+
+    # ---- solution procedures ----
+    @staticmethod
+    def newton_defaults(**kw):
+        o = NewtonOpts()
+        lib().nla3d_gpu_newton_defaults(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+    def solve_load_step(self, t, opts, stats=None):
+        stats = stats or NewtonStats()
+        self._call("nla3d_gpu_solve_load_step", float(t), ctypes.byref(opts), ctypes.byref(stats))
+        return stats
+
+    def solve_nonlinear(self, opts, stats=None):
+        stats = stats or NewtonStats()
+        self._call("nla3d_gpu_solve_nonlinear", ctypes.byref(opts), ctypes.byref(stats))
+        return stats
+
+    def solve_block2(self, rhs, tol=1e-12, max_iter=20000):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.zeros_like(rhs)
+        it = ctypes.c_int()
+        rr = ctypes.c_double()
+        self._call("nla3d_gpu_solve_block2", _ptr(rhs, ctypes.c_double), _ptr(x, ctypes.c_double), float(tol),
+                   int(max_iter), ctypes.byref(it), ctypes.byref(rr))
+        return x, it.value, rr.value
 
     def time_assembly(self, iters):
         a = ctypes.c_double()

@@ -18,27 +18,54 @@
 namespace nla3d_amd {
 namespace {
 
+// Targets in [mirBegin, mirEnd) (the block(2) entries) are also written to
+// both mirror positions of the full-symmetric copy used by the Krylov SpMV.
 __global__ __launch_bounds__(256) void segmented_gather_kernel(int nTargets, const int* __restrict__ seg,
                                                                const int* __restrict__ src,
                                                                const double* __restrict__ vals,
-                                                               double* __restrict__ out) {
+                                                               double* __restrict__ out, int mirBegin, int mirEnd,
+                                                               const int* __restrict__ mir,
+                                                               double* __restrict__ full) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nTargets) return;
   const int s0 = seg[t], s1 = seg[t + 1];
   double acc = 0.0;
   for (int s = s0; s < s1; s++) acc += vals[src[s]];
   out[t] = acc;
+  if (full && t >= mirBegin && t < mirEnd) {
+    const int k = t - mirBegin;
+    full[mir[2 * k]] = acc;
+    full[mir[2 * k + 1]] = acc;
+  }
+}
+
+// set full[mir[..]] from the upper values of block(2) (after the MPC rows are written)
+__global__ __launch_bounds__(256) void mirror_kernel(int n, const double* __restrict__ upper,
+                                                     const int* __restrict__ idx, const int* __restrict__ mir,
+                                                     double* __restrict__ full) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = idx[i];
+  const double v = upper[k];
+  full[mir[2 * k]] = v;
+  full[mir[2 * k + 1]] = v;
 }
 
 }  // namespace
 
 void launch_segmented_gather(hipStream_t stream, int nTargets, const int* seg, const int* src,
-                             const double* src_vals, double* out) {
+                             const double* src_vals, double* out, int mirBegin, int mirEnd, const int* mir,
+                             double* full) {
   if (nTargets <= 0) return;
   const int threads = 256;
   const int blocks = (nTargets + threads - 1) / threads;
   hipLaunchKernelGGL(segmented_gather_kernel, dim3(blocks), dim3(threads), 0, stream, nTargets, seg, src,
-                     src_vals, out);
+                     src_vals, out, mirBegin, mirEnd, mir, full);
+}
+
+void launch_mirror(hipStream_t stream, int n, const double* upper2, const int* idx, const int* mir, double* full) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(mirror_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, upper2, idx, mir, full);
 }
 
 }  // namespace nla3d_amd

@@ -70,8 +70,10 @@ void Ctx::runSolid81(double* gp_state) {
 }
 
 void Ctx::scatter() {
-  launch_segmented_gather(stream, nTargets, d_seg.as<int>(), d_src.as<int>(), d_Ke.as<double>(),
-                          d_vals.as<double>());
+  const int off2 = (int)model.offB2();
+  launch_segmented_gather(stream, nTargets, d_seg.as<int>(), d_src.as<int>(), d_Ke.as<double>(), d_vals.as<double>(),
+                          off2, off2 + (int)model.block2.nnz(), d_mir2.as<int>(),
+                          krylovReady ? d_full2.as<double>() : nullptr);
   hipCheck(hipGetLastError(), "scatter kernel launch");
 }
 
@@ -80,6 +82,11 @@ void Ctx::applyMpc() {
     launch_rigid_body_mpc(stream, rb.dev, d_xyz.as<double>(), rb.slaveNode.as<int>(), rb.slaveEq.as<int>(),
                           d_U.as<double>(), rb.tgt.as<int>(), rb.fTgt.as<int>(), d_vals.as<double>());
     hipCheck(hipGetLastError(), "mpc kernel launch");
+  }
+  if (krylovReady && nMpcMir > 0) {
+    launch_mirror(stream, nMpcMir, d_vals.as<double>() + model.offB2(), d_mpcMirIdx.as<int>(), d_mir2.as<int>(),
+                  d_full2.as<double>());
+    hipCheck(hipGetLastError(), "mirror launch");
   }
 }
 

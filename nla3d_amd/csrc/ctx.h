@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "krylov.h"
 #include "model.h"
 
 namespace nla3d_amd {
@@ -62,6 +63,24 @@ struct Ctx {
   int nTargets = 0;
   int64_t nContrib = 0;
 
+  // ---- block(2) Krylov data (solver.cpp) ----
+  int n2 = 0;                 // nUnknown + nMpc
+  DevBuf d_rowptr2, d_col2, d_diagpos2, d_full2;  // full-symmetric CSR of block 2
+  DevBuf d_mir2;              // int [2*nnz2]: full positions of each upper entry of block 2
+  int64_t fullNnz2 = 0;
+  DevBuf d_gStart, d_gInfo, d_gDiagPos, d_binv;  // block-Jacobi groups
+  int nGroups = 0;
+  DevBuf d_k12tPtr, d_k12tPos, d_k12tRow;       // block(1,2) by columns: K12^T products
+  // Newton / Krylov vectors
+  DevBuf d_R;                 // vecR [nAll]
+  DevBuf d_Ucprev, d_fixEq, d_fixVal, d_loadEq, d_loadVal;
+  DevBuf d_rhs, d_dU;         // rhs [n2], increment [nAll]
+  DevBuf d_mw[8], d_partial, d_mstate, d_scal;
+  DevBuf d_mpcMirIdx;         // block(2) upper indices of MPC entries
+  int nMpcMir = 0;
+  MinresState* h_mstate = nullptr;  // pinned host mirror of the MINRES state
+  bool krylovReady = false;
+
   // rigid-body MPC collections
   struct RB {
     RigidBodyDev dev{};
@@ -70,6 +89,7 @@ struct Ctx {
   std::vector<RB> rbs;
 
   ~Ctx() {
+    if (h_mstate) (void)hipHostFree(h_mstate);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -87,6 +107,13 @@ struct Ctx {
   void scatter();
   void applyMpc();
   void assemble();
+
+  // solver.cpp
+  void setupKrylov();
+  KrylovMatrix krylovMatrix() const;
+  Precond precond();
+  // solve block(2) x = b with the current K; b, x device pointers [n2]; returns iterations
+  int minresSolve(const double* b, double* x, double tol, int maxit, double* relres_est, bool* breakdown);
 };
 
 }  // namespace nla3d_amd

@@ -32,8 +32,13 @@ void launch_solid81(hipStream_t stream, int nE, const double* xyz, const int* co
 
 // Segmented reduction: out[t] = sum_{s in [seg[t], seg[t+1])} src_vals[src[s]], summed in s order
 // (starting from +0.0).  int32 offsets: the host checks the total count fits.
+// Targets in [mirBegin, mirEnd) are also stored at full[mir[2k]] and full[mir[2k+1]],
+// k = t - mirBegin (full may be null).
 void launch_segmented_gather(hipStream_t stream, int nTargets, const int* seg, const int* src,
-                             const double* src_vals, double* out);
+                             const double* src_vals, double* out, int mirBegin, int mirEnd, const int* mir,
+                             double* full);
+// full[mir[2k]] = full[mir[2k+1]] = upper2[k] for k = idx[0..n)
+void launch_mirror(hipStream_t stream, int n, const double* upper2, const int* idx, const int* mir, double* full);
 
 // One RigidBodyMpc collection on the device (Mpc.h:49-60).
 struct RigidBodyDev {

@@ -1,0 +1,47 @@
+// Device data of the block(2) Krylov solve (see krylov.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace nla3d_amd {
+
+// full-symmetric CSR of K.block(2), 0-based
+struct KrylovMatrix {
+  int n = 0;
+  const int* rowptr = nullptr;   // [n+1]
+  const int* col = nullptr;      // [nnz]
+  const double* val = nullptr;   // [nnz]
+  const int* diagpos = nullptr;  // [n], position of (i,i) or -1 (multiplier rows have none)
+};
+
+// block-Jacobi groups: each row belongs to exactly one group
+//   gInfo = size | kind << 2 ; kind 0 = node block (size 1..3), 1 = Schur-estimate scalar row
+struct Precond {
+  int nGroups = 0;
+  const int* gStart = nullptr;    // first row of the group
+  const int* gInfo = nullptr;
+  const int* gDiagPos = nullptr;  // [nGroups*9] positions of the block entries in the full CSR (-1 absent)
+  double* binv = nullptr;         // [nGroups*9] inverse blocks (row-major, size x size used)
+};
+
+// scalar state of one MINRES solve, resident on the device
+struct MinresState {
+  double beta1, beta, oldb, alfa, dbar, epsln, phibar, cs, sn, tnorm2, rnorm;
+  double s, c_r1, c_r2;                      // coefficients of the next SpMV / preconditioner pass
+  double oldeps, delta, denom, phi, s_update;  // coefficients of the W / X update
+  int itn, converged, breakdown, skip_update;
+};
+
+struct MinresWork {
+  double *X, *R1, *R2, *T, *Yp, *Z, *W1, *W2;
+  double* partial;  // per-block partial sums
+  MinresState* st;
+};
+
+void krylov_precond_setup(hipStream_t s, const KrylovMatrix& M, const Precond& P);
+void krylov_spmv(hipStream_t s, const KrylovMatrix& M, const double* x, double* y);
+void krylov_dot(hipStream_t s, int n, const double* a, const double* b, double* partial, double* out);
+void minres_start(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, const double* b);
+void minres_iteration(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, double tol);
+
+}  // namespace nla3d_amd

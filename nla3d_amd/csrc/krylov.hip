@@ -1,0 +1,410 @@
+// Krylov solve of K.block(2) x = b on the device: preconditioned MINRES with
+// a block-Jacobi preconditioner, replacing EquationSolver::solveEquations
+// (math/EquationSolver.h:22; the PARDISO call of FESolver.cpp:457).
+//
+// Matrix: block(2) is symmetric and is kept on the device twice: the
+// reference's upper-triangle CSR (for export) and a full-symmetric CSR
+// (0-based, both triangles) that the SpMV reads.  The assembly's segmented
+// reduction writes both (see assemble.hip).
+//
+// MINRES follows Paige & Saunders (SIAM J. Numer. Anal. 12, 1975), in the
+// preconditioned form of their SYMMLQ/MINRES code: per iteration one SpMV,
+// one preconditioner application and two inner products.  All scalar
+// recurrences run in a one-thread kernel on device-resident state, so an
+// iteration needs no host round-trip; inner products are reduced
+// deterministically (fixed-order per-block partials, then a fixed-order
+// single-block finish), so results do not depend on timing.
+//
+// Preconditioner (SPD, as MINRES requires): 3x3 blocks of the node DoFs
+// (inverted once per Newton iteration; a block that is not positive definite
+// falls back to |diag|^-1), and for the pressure and Lagrange-multiplier rows
+// the diagonal Schur estimate  d_i = |K_ii| + sum_{j != i} K_ij^2 / |K_jj|.
+
+#include "internal.h"
+#include "krylov.h"
+
+namespace nla3d_amd {
+namespace {
+
+constexpr int kRedThreads = 256;
+
+__device__ __forceinline__ double warp_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// block-level sum in a fixed order (wave sums, then wave 0 adds the waves in order)
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  v = warp_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < NT / 64; i++) r += sh[i];
+  }
+  return r;  // valid in thread 0
+}
+
+// ---- preconditioner set-up: one thread per group --------------------------
+__global__ __launch_bounds__(256) void bj_setup_kernel(int nGroups, const int* __restrict__ gStart,
+                                                      const int* __restrict__ gInfo,
+                                                      const int* __restrict__ gDiagPos,
+                                                      const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                      const int* __restrict__ diagpos,
+                                                      const double* __restrict__ A, double* __restrict__ binv) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nGroups) return;
+  const int info = gInfo[g];
+  const int size = info & 3, kind = info >> 2;
+  double* out = binv + (size_t)g * 9;
+  for (int k = 0; k < 9; k++) out[k] = 0.0;
+  if (kind == 1) {  // pressure / multiplier row: diagonal Schur estimate
+    const int r = gStart[g];
+    double d = 0.0;
+    for (int k = rowptr[r]; k < rowptr[r + 1]; k++) {
+      const int c = col[k];
+      const double a = A[k];
+      if (c == r) {
+        d += fabs(a);
+      } else {
+        const int dp = diagpos[c];
+        const double ajj = dp >= 0 ? fabs(A[dp]) : 0.0;
+        if (ajj > 0.0) d += a * a / ajj;
+      }
+    }
+    out[0] = d > 0.0 ? 1.0 / d : 1.0;
+    return;
+  }
+  // node block of `size` consecutive equations
+  double B[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      const int p = (i < size && j < size) ? gDiagPos[(size_t)g * 9 + i * 3 + j] : -1;
+      B[i][j] = p >= 0 ? A[p] : 0.0;
+    }
+  // Cholesky of the symmetric part; not positive definite -> |diag|^-1
+  bool spd = true;
+  double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int i = 0; i < size && spd; i++) {
+    for (int j = 0; j <= i; j++) {
+      double s = 0.5 * (B[i][j] + B[j][i]);
+      for (int k = 0; k < j; k++) s -= L[i][k] * L[j][k];
+      if (i == j) {
+        if (!(s > 0.0)) { spd = false; break; }
+        L[i][i] = sqrt(s);
+      } else {
+        L[i][j] = s / L[j][j];
+      }
+    }
+  }
+  if (!spd) {
+    for (int i = 0; i < size; i++) out[i * 3 + i] = fabs(B[i][i]) > 0.0 ? 1.0 / fabs(B[i][i]) : 1.0;
+    return;
+  }
+  // inverse via L^-1 (lower) then B^-1 = L^-T L^-1
+  double Li[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int i = 0; i < size; i++) {
+    Li[i][i] = 1.0 / L[i][i];
+    for (int j = 0; j < i; j++) {
+      double s = 0.0;
+      for (int k = j; k < i; k++) s -= L[i][k] * Li[k][j];
+      Li[i][j] = s / L[i][i];
+    }
+  }
+  for (int i = 0; i < size; i++)
+    for (int j = 0; j < size; j++) {
+      double s = 0.0;
+      for (int k = (i > j ? i : j); k < size; k++) s += Li[k][i] * Li[k][j];
+      out[i * 3 + j] = s;
+    }
+}
+
+// ---- SpMV with the MINRES epilogue ----------------------------------------
+// T = s * A Yp - c1 * R1 ; partial alfa = s * sum Yp . T
+// LPR lanes per row; rows processed in order within a block.
+template <int LPR>
+__global__ __launch_bounds__(256) void spmv_minres_kernel(int n, const int* __restrict__ rowptr,
+                                                         const int* __restrict__ col, const double* __restrict__ A,
+                                                         const double* __restrict__ Yp,
+                                                         const double* __restrict__ R1, double* __restrict__ T,
+                                                         MinresState* __restrict__ st, double* __restrict__ partial) {
+  __shared__ double sh[4];
+  if (st->converged) return;
+  const double s = st->s, c1 = st->c_r1;
+  const int lane = threadIdx.x % LPR;
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  double contrib = 0.0;
+  if (row < n) {
+    double acc = 0.0;
+    for (int k = rowptr[row] + lane; k < rowptr[row + 1]; k += LPR) acc += A[k] * Yp[col[k]];
+    for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+    if (lane == 0) {
+      const double t = s * acc - c1 * R1[row];
+      T[row] = t;
+      contrib = s * Yp[row] * t;
+    }
+  }
+  const double b = block_sum<256>(contrib, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+
+// plain y = A x (used for residual checks and K12-type products on block 2)
+template <int LPR>
+__global__ __launch_bounds__(256) void spmv_kernel(int n, const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                  const double* __restrict__ A, const double* __restrict__ x,
+                                                  double* __restrict__ y) {
+  const int lane = threadIdx.x % LPR;
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  if (row >= n) return;
+  double acc = 0.0;
+  for (int k = rowptr[row] + lane; k < rowptr[row + 1]; k += LPR) acc += A[k] * x[col[k]];
+  for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+  if (lane == 0) y[row] = acc;
+}
+
+// ---- fixed-order finish of a reduction + MINRES scalar steps --------------
+__device__ double finish_sum(const double* partial, int nPart, double* sh) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nPart; i += kRedThreads) v += partial[i];
+  return block_sum<kRedThreads>(v, sh);
+}
+
+// after the SpMV: alfa, and the coefficient of R2 for the preconditioner pass
+__global__ __launch_bounds__(kRedThreads) void minres_scalar1_kernel(MinresState* st, const double* partial,
+                                                                   int nPart) {
+  __shared__ double sh[kRedThreads / 64];
+  if (st->converged) return;
+  const double alfa = finish_sum(partial, nPart, sh);
+  if (threadIdx.x == 0) {
+    st->alfa = alfa;
+    st->c_r2 = alfa / st->beta;
+  }
+}
+
+// after the preconditioner pass: beta, the plane rotation, phi and the
+// coefficients of the W / X update; convergence test on the residual estimate
+__global__ __launch_bounds__(kRedThreads) void minres_scalar2_kernel(MinresState* st, const double* partial,
+                                                                   int nPart, double tol) {
+  __shared__ double sh[kRedThreads / 64];
+  if (st->converged) {
+    if (threadIdx.x == 0) st->skip_update = 1;
+    return;
+  }
+  const double beta2 = finish_sum(partial, nPart, sh);
+  if (threadIdx.x != 0) return;
+  st->itn += 1;
+  const double oldb = st->beta;
+  const double beta = beta2 > 0.0 ? sqrt(beta2) : 0.0;
+  const double alfa = st->alfa;
+  st->tnorm2 += alfa * alfa + oldb * oldb + beta * beta;
+  const double oldeps = st->epsln;
+  const double delta = st->cs * st->dbar + st->sn * alfa;
+  const double gbar = st->sn * st->dbar - st->cs * alfa;
+  st->epsln = st->sn * beta;
+  st->dbar = -st->cs * beta;
+  double gamma = sqrt(gbar * gbar + beta * beta);
+  if (gamma < 1e-300) gamma = 1e-300;
+  st->cs = gbar / gamma;
+  st->sn = beta / gamma;
+  const double phi = st->cs * st->phibar;
+  st->phibar = st->sn * st->phibar;
+  st->oldeps = oldeps;
+  st->delta = delta;
+  st->denom = 1.0 / gamma;
+  st->phi = phi;
+  st->s_update = st->s;  // the v = s * Yp of this iteration
+  st->oldb = oldb;
+  st->beta = beta;
+  st->rnorm = st->phibar;
+  // next iteration's SpMV coefficients
+  st->s = beta > 0.0 ? 1.0 / beta : 0.0;
+  st->c_r1 = oldb > 0.0 ? beta / oldb : 0.0;
+  st->skip_update = 0;
+  if (!(beta2 >= 0.0) || beta2 != beta2) {  // indefinite preconditioner or NaN
+    st->breakdown = 1;
+    st->converged = 1;
+  } else if (st->phibar <= tol * st->beta1 || beta == 0.0) {
+    st->converged = 1;
+  }
+}
+
+// T -= c_r2 * R2 ; Z = M T ; partial beta2 = T . Z   (one thread per group)
+__global__ __launch_bounds__(256) void precond_kernel(int nGroups, const int* __restrict__ gStart,
+                                                     const int* __restrict__ gInfo, const double* __restrict__ binv,
+                                                     double* __restrict__ T, const double* __restrict__ R2,
+                                                     double* __restrict__ Z, MinresState* __restrict__ st,
+                                                     double* __restrict__ partial) {
+  __shared__ double sh[4];
+  if (st->converged) return;
+  const double c2 = st->c_r2;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  double contrib = 0.0;
+  if (g < nGroups) {
+    const int r0 = gStart[g], size = gInfo[g] & 3;
+    double t[3] = {0, 0, 0};
+    for (int i = 0; i < size; i++) {
+      t[i] = T[r0 + i] - c2 * R2[r0 + i];
+      T[r0 + i] = t[i];
+    }
+    const double* B = binv + (size_t)g * 9;
+    for (int i = 0; i < size; i++) {
+      double z = 0.0;
+      for (int j = 0; j < size; j++) z += B[i * 3 + j] * t[j];
+      Z[r0 + i] = z;
+      contrib += t[i] * z;
+    }
+  }
+  const double b = block_sum<256>(contrib, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+
+// Wn = (s Yp - oldeps W1 - delta W2) * denom ; X += phi Wn   (Wn overwrites W1)
+__global__ __launch_bounds__(256) void minres_update_kernel(int n, const double* __restrict__ Yp,
+                                                           double* __restrict__ W1, const double* __restrict__ W2,
+                                                           double* __restrict__ X, const MinresState* __restrict__ st) {
+  if (st->skip_update) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double wn = (st->s_update * Yp[i] - st->oldeps * W1[i] - st->delta * W2[i]) * st->denom;
+  W1[i] = wn;
+  X[i] += st->phi * wn;
+}
+
+// start: R1 = R2 = b (x0 = 0), Z = M b, partial = b . Z
+__global__ __launch_bounds__(256) void precond_start_kernel(int nGroups, const int* __restrict__ gStart,
+                                                           const int* __restrict__ gInfo,
+                                                           const double* __restrict__ binv,
+                                                           const double* __restrict__ b, double* __restrict__ R1,
+                                                           double* __restrict__ R2, double* __restrict__ Z,
+                                                           double* __restrict__ partial) {
+  __shared__ double sh[4];
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  double contrib = 0.0;
+  if (g < nGroups) {
+    const int r0 = gStart[g], size = gInfo[g] & 3;
+    double t[3] = {0, 0, 0};
+    for (int i = 0; i < size; i++) {
+      t[i] = b[r0 + i];
+      R1[r0 + i] = t[i];
+      R2[r0 + i] = t[i];
+    }
+    const double* B = binv + (size_t)g * 9;
+    for (int i = 0; i < size; i++) {
+      double z = 0.0;
+      for (int j = 0; j < size; j++) z += B[i * 3 + j] * t[j];
+      Z[r0 + i] = z;
+      contrib += t[i] * z;
+    }
+  }
+  const double s = block_sum<256>(contrib, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kRedThreads) void minres_init_kernel(MinresState* st, const double* partial, int nPart) {
+  __shared__ double sh[kRedThreads / 64];
+  const double b2 = finish_sum(partial, nPart, sh);
+  if (threadIdx.x != 0) return;
+  const double beta1 = b2 > 0.0 ? sqrt(b2) : 0.0;
+  MinresState z{};
+  z.beta1 = beta1;
+  z.beta = beta1;
+  z.oldb = 0.0;
+  z.phibar = beta1;
+  z.rnorm = beta1;
+  z.cs = -1.0;
+  z.sn = 0.0;
+  z.s = beta1 > 0.0 ? 1.0 / beta1 : 0.0;
+  z.c_r1 = 0.0;
+  z.converged = (beta1 == 0.0) ? 1 : 0;
+  z.breakdown = (b2 < 0.0 || b2 != b2) ? 1 : 0;
+  if (z.breakdown) z.converged = 1;
+  *st = z;
+}
+
+__global__ __launch_bounds__(256) void zero3_kernel(int n, double* __restrict__ a, double* __restrict__ b,
+                                                   double* __restrict__ c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    a[i] = 0.0;
+    b[i] = 0.0;
+    c[i] = 0.0;
+  }
+}
+
+// generic dot partials: sum a_i b_i
+__global__ __launch_bounds__(256) void dot_partial_kernel(int n, const double* __restrict__ a,
+                                                         const double* __restrict__ b, double* __restrict__ partial) {
+  __shared__ double sh[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double v = (i < n) ? a[i] * b[i] : 0.0;
+  const double s = block_sum<256>(v, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kRedThreads) void finish_to_kernel(const double* partial, int nPart, double* out) {
+  __shared__ double sh[kRedThreads / 64];
+  const double v = finish_sum(partial, nPart, sh);
+  if (threadIdx.x == 0) *out = v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+void krylov_precond_setup(hipStream_t s, const KrylovMatrix& M, const Precond& P) {
+  hipLaunchKernelGGL(bj_setup_kernel, dim3(cdiv(P.nGroups, 256)), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo,
+                     P.gDiagPos, M.rowptr, M.col, M.diagpos, M.val, P.binv);
+}
+
+void krylov_spmv(hipStream_t s, const KrylovMatrix& M, const double* x, double* y) {
+  constexpr int LPR = 8;
+  hipLaunchKernelGGL(spmv_kernel<LPR>, dim3(cdiv(M.n * LPR, 256)), dim3(256), 0, s, M.n, M.rowptr, M.col, M.val, x,
+                     y);
+}
+
+void krylov_dot(hipStream_t s, int n, const double* a, const double* b, double* partial, double* out) {
+  const int nb = cdiv(n, 256);
+  hipLaunchKernelGGL(dot_partial_kernel, dim3(nb), dim3(256), 0, s, n, a, b, partial);
+  hipLaunchKernelGGL(finish_to_kernel, dim3(1), dim3(kRedThreads), 0, s, partial, nb, out);
+}
+
+void minres_start(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, const double* b) {
+  const int nbG = cdiv(P.nGroups, 256);
+  hipLaunchKernelGGL(precond_start_kernel, dim3(nbG), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo, P.binv, b,
+                     w.R1, w.R2, w.Yp, w.partial);
+  hipLaunchKernelGGL(minres_init_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, nbG);
+  // x0 = 0, w = w2 = 0 (a fill kernel rather than memset nodes, so the sequence can be graph-captured)
+  hipLaunchKernelGGL(zero3_kernel, dim3(cdiv(M.n, 256)), dim3(256), 0, s, M.n, w.X, w.W1, w.W2);
+}
+
+// one MINRES iteration; buffers rotate on the host (pure pointer swaps)
+void minres_iteration(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, double tol) {
+  constexpr int LPR = 8;
+  const int nbS = cdiv(M.n * LPR, 256);
+  const int nbG = cdiv(P.nGroups, 256);
+  hipLaunchKernelGGL(spmv_minres_kernel<LPR>, dim3(nbS), dim3(256), 0, s, M.n, M.rowptr, M.col, M.val, w.Yp, w.R1,
+                     w.T, w.st, w.partial);
+  hipLaunchKernelGGL(minres_scalar1_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, nbS);
+  hipLaunchKernelGGL(precond_kernel, dim3(nbG), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo, P.binv, w.T, w.R2,
+                     w.Z, w.st, w.partial);
+  hipLaunchKernelGGL(minres_scalar2_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, nbG, tol);
+  hipLaunchKernelGGL(minres_update_kernel, dim3(cdiv(M.n, 256)), dim3(256), 0, s, M.n, w.Yp, w.W1, w.W2, w.X, w.st);
+  // rotations: R1 <- R2, R2 <- T, T <- old R1 ; Yp <- Z ; (W1, W2) <- (W2, Wn)
+  double* t = w.R1;
+  w.R1 = w.R2;
+  w.R2 = w.T;
+  w.T = t;
+  t = w.Yp;
+  w.Yp = w.Z;
+  w.Z = t;
+  t = w.W2;
+  w.W2 = w.W1;  // W1 now holds Wn
+  w.W1 = t;
+}
+
+}  // namespace nla3d_amd

@@ -1,0 +1,18 @@
+// Vector kernels of the Newton loop (newton.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+namespace nla3d_amd {
+
+void newton_apply_bc(hipStream_t s, int nFix, const int* fixEq, const double* fixVal, int nLoad, const int* loadEq,
+                     const double* loadVal, double t, double* U, double* R);
+void newton_rhs(hipStream_t s, int n2, int nC, const double* F, const double* R, const double* U, const double* Ucprev,
+                const int* cptr, const int* cpos, const int* crow, const double* K12, double* rhs);
+void newton_update(hipStream_t s, int nC, int nU, int nMpc, const double* dU, double* U, double* Ucprev);
+void vec_abssum(hipStream_t s, int n, const double* x, double* partial, double* out);
+void vec_sumsq(hipStream_t s, int n, const double* x, double* partial, double* out);
+
+}  // namespace nla3d_amd

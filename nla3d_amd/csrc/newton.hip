@@ -1,0 +1,123 @@
+// Small vector kernels of the Newton-Raphson loop (FESolver.cpp:441-490):
+// boundary conditions, the block(2) right-hand side with the eliminated
+// constrained DoFs, the increment update and the convergence measures.
+#include "newton.h"
+
+namespace nla3d_amd {
+namespace {
+
+__device__ __forceinline__ double wsum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double bsum256(double v, double* sh) {
+  v = wsum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < 4; i++) r += sh[i];
+  return r;
+}
+
+// vecUc[eq] = fix * t (FESolver.cpp:241-246); fixes are distinct DoFs
+__global__ void fix_kernel(int n, const int* __restrict__ eq, const double* __restrict__ val, double t,
+                           double* __restrict__ U) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) U[eq[i]] = val[i] * t;
+}
+
+// vecR[eq] += load * t in list order (FESolver.cpp:235-237)
+__global__ void load_kernel(int n, const int* __restrict__ eq, const double* __restrict__ val, double t,
+                            double* __restrict__ R) {
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int i = 0; i < n; i++) R[eq[i]] += val[i] * t;
+}
+
+// rhs_j = F_sl + R_sl - (K12^T dUc)_j, dUc = Uc - Ucprev (FESolver.cpp:449-454)
+__global__ __launch_bounds__(256) void rhs_kernel(int n2, int nC, const double* __restrict__ F,
+                                                 const double* __restrict__ R, const double* __restrict__ U,
+                                                 const double* __restrict__ Ucprev, const int* __restrict__ cptr,
+                                                 const int* __restrict__ cpos, const int* __restrict__ crow,
+                                                 const double* __restrict__ K12, double* __restrict__ rhs) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n2) return;
+  double v = F[nC + j] + R[nC + j];
+  double acc = 0.0;
+  for (int k = cptr[j]; k < cptr[j + 1]; k++) {
+    const int i = crow[k];
+    acc += K12[cpos[k]] * (U[i] - Ucprev[i]);
+  }
+  rhs[j] = v - acc;
+}
+
+// vecUs += dUs ; vecUl = dUl ; Ucprev = Uc (FESolver.cpp:460-469)
+__global__ __launch_bounds__(256) void update_kernel(int nC, int nU, int nMpc, const double* __restrict__ dU,
+                                                    double* __restrict__ U, double* __restrict__ Ucprev) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nU) U[nC + i] += dU[i];
+  else if (i < nU + nMpc) U[nC + i] = dU[i];
+  if (i < nC) Ucprev[i] = U[i];
+}
+
+__global__ __launch_bounds__(256) void abssum_partial_kernel(int n, const double* __restrict__ x,
+                                                            double* __restrict__ partial) {
+  __shared__ double sh[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double s = bsum256(i < n ? fabs(x[i]) : 0.0, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void sumsq_partial_kernel(int n, const double* __restrict__ x,
+                                                           double* __restrict__ partial) {
+  __shared__ double sh[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double v = i < n ? x[i] : 0.0;
+  const double s = bsum256(v * v, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void finish_kernel(const double* __restrict__ partial, int n,
+                                                    double* __restrict__ out) {
+  __shared__ double sh[4];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) v += partial[i];
+  const double s = bsum256(v, sh);
+  if (threadIdx.x == 0) *out = s;
+}
+
+}  // namespace
+
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+void newton_apply_bc(hipStream_t s, int nFix, const int* fixEq, const double* fixVal, int nLoad, const int* loadEq,
+                     const double* loadVal, double t, double* U, double* R) {
+  if (nFix > 0) hipLaunchKernelGGL(fix_kernel, dim3(cdiv(nFix, 256)), dim3(256), 0, s, nFix, fixEq, fixVal, t, U);
+  if (nLoad > 0) hipLaunchKernelGGL(load_kernel, dim3(1), dim3(64), 0, s, nLoad, loadEq, loadVal, t, R);
+}
+
+void newton_rhs(hipStream_t s, int n2, int nC, const double* F, const double* R, const double* U, const double* Ucprev,
+                const int* cptr, const int* cpos, const int* crow, const double* K12, double* rhs) {
+  hipLaunchKernelGGL(rhs_kernel, dim3(cdiv(n2, 256)), dim3(256), 0, s, n2, nC, F, R, U, Ucprev, cptr, cpos, crow, K12,
+                     rhs);
+}
+
+void newton_update(hipStream_t s, int nC, int nU, int nMpc, const double* dU, double* U, double* Ucprev) {
+  const int n = std::max(nU + nMpc, nC);
+  hipLaunchKernelGGL(update_kernel, dim3(cdiv(std::max(n, 1), 256)), dim3(256), 0, s, nC, nU, nMpc, dU, U, Ucprev);
+}
+
+void vec_abssum(hipStream_t s, int n, const double* x, double* partial, double* out) {
+  const int nb = cdiv(std::max(n, 1), 256);
+  hipLaunchKernelGGL(abssum_partial_kernel, dim3(nb), dim3(256), 0, s, n, x, partial);
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, s, partial, nb, out);
+}
+
+void vec_sumsq(hipStream_t s, int n, const double* x, double* partial, double* out) {
+  const int nb = cdiv(std::max(n, 1), 256);
+  hipLaunchKernelGGL(sumsq_partial_kernel, dim3(nb), dim3(256), 0, s, n, x, partial);
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, s, partial, nb, out);
+}
+
+}  // namespace nla3d_amd

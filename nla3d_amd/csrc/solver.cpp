@@ -1,0 +1,421 @@
+// Block(2) Krylov data set-up, the MINRES driver, and the Newton-Raphson
+// loop of NonlinearFESolver::solve (FESolver.cpp:407-521) on the device.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <string>
+
+#include "ctx.h"
+#include "newton.h"
+
+namespace nla3d_amd {
+
+// ---- set-up --------------------------------------------------------------
+void Ctx::setupKrylov() {
+  FEStorage& m = model;
+  const uint32_t nC = m.nConstrainedDofs;
+  n2 = (int)(m.nAll() - nC);
+  const CsrPattern& U2 = m.block2;
+  const int64_t nnzU = (int64_t)U2.nnz();
+  // full-symmetric CSR: row i = (entries (j,i), j<i, from the upper rows j) ++ (upper row i)
+  std::vector<int> cntLower(n2, 0);
+  for (int i = 0; i < n2; i++)
+    for (uint32_t k = U2.iofeir[i] - 1; k < U2.iofeir[i + 1] - 1; k++) {
+      int c = (int)U2.columns[k] - 1;
+      if (c != i) cntLower[c]++;
+    }
+  std::vector<int64_t> rp(n2 + 1, 0);
+  for (int i = 0; i < n2; i++) rp[i + 1] = rp[i] + cntLower[i] + (U2.iofeir[i + 1] - U2.iofeir[i]);
+  fullNnz2 = rp[n2];
+  if (fullNnz2 >= (1ll << 31)) throw ApiError(-1, "block(2) too large for 32-bit CSR offsets");
+  std::vector<int> col(fullNnz2), mir(2 * nnzU), diagpos(n2, -1);
+  std::vector<int> fillLower(n2, 0);
+  for (int i = 0; i < n2; i++) {
+    const int64_t base = rp[i] + cntLower[i];
+    for (uint32_t k = U2.iofeir[i] - 1; k < U2.iofeir[i + 1] - 1; k++) {
+      const int c = (int)U2.columns[k] - 1;
+      const int64_t pu = base + (k - (U2.iofeir[i] - 1));
+      col[pu] = c;
+      mir[2 * k] = (int)pu;
+      if (c == i) {
+        mir[2 * k + 1] = (int)pu;
+        diagpos[i] = (int)pu;
+      } else {
+        const int64_t pl = rp[c] + fillLower[c]++;
+        col[pl] = i;  // rows are visited in order, so each row's lower part stays sorted
+        mir[2 * k + 1] = (int)pl;
+      }
+    }
+  }
+  std::vector<int> rp32(rp.begin(), rp.end());
+  d_rowptr2.upload(rp32);
+  d_col2.upload(col);
+  d_diagpos2.upload(diagpos);
+  d_mir2.upload(mir);
+  d_full2.alloc((size_t)fullNnz2 * 8);
+  hipCheck(hipMemset(d_full2.p, 0, d_full2.bytes), "memset full2");
+
+  // what each block(2) row is: element DoF (pressure), node DoF, or MPC multiplier
+  std::vector<int8_t> rowKind(n2, -1);  // 0 node, 1 schur (element DoF / multiplier)
+  std::vector<uint8_t> rowType(n2, 0);
+  std::vector<uint32_t> rowNode(n2, 0);
+  for (uint32_t e = 0; e < m.nElements(); e++)
+    for (int t = 0; t < NUM_DOF_TYPES; t++) {
+      uint32_t q = m.elemEq[(size_t)e * 8 + t];
+      if (q > nC) rowKind[q - nC - 1] = 1;
+    }
+  for (uint32_t n = 0; n < m.nNodes(); n++)
+    for (int t = 0; t < NUM_DOF_TYPES; t++) {
+      uint32_t q = m.nodeEq[(size_t)n * 8 + t];
+      if (q > nC) {
+        rowKind[q - nC - 1] = 0;
+        rowType[q - nC - 1] = (uint8_t)t;
+        rowNode[q - nC - 1] = n;
+      }
+    }
+  for (auto& mp : m.mpcs) rowKind[mp.eqNum - nC - 1] = 1;
+  // a node DoF row without a diagonal entry (element-less node) is treated like a multiplier row
+  for (int i = 0; i < n2; i++)
+    if (rowKind[i] == 0 && diagpos[i] < 0) rowKind[i] = 1;
+  std::vector<int> gStart, gInfo, gDiag;
+  auto findPos = [&](int r, int c) -> int {
+    auto b = col.begin() + rp[r], e = col.begin() + rp[r + 1];
+    auto it = std::lower_bound(b, e, c);
+    return (it != e && *it == c) ? (int)(it - col.begin()) : -1;
+  };
+  for (int i = 0; i < n2;) {
+    if (rowKind[i] != 0) {
+      gStart.push_back(i);
+      gInfo.push_back(1 | (1 << 2));
+      for (int k = 0; k < 9; k++) gDiag.push_back(-1);
+      i++;
+      continue;
+    }
+    // consecutive DoFs of the same node within one of the triples {UX,UY,UZ}, {ROTX,ROTY,ROTZ}
+    int sz = 1;
+    while (sz < 3 && i + sz < n2 && rowKind[i + sz] == 0 && rowNode[i + sz] == rowNode[i] &&
+           rowType[i + sz] / 3 == rowType[i] / 3)
+      sz++;
+    gStart.push_back(i);
+    gInfo.push_back(sz);
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) gDiag.push_back((a < sz && b < sz) ? findPos(i + a, i + b) : -1);
+    i += sz;
+  }
+  nGroups = (int)gStart.size();
+  d_gStart.upload(gStart);
+  d_gInfo.upload(gInfo);
+  d_gDiagPos.upload(gDiag);
+  d_binv.alloc((size_t)nGroups * 9 * 8);
+
+  // block(1,2) by columns, for rhs -= K12^T dUc (FESolver.cpp:454)
+  const CsrPattern& B12 = m.block12;
+  std::vector<int> cptr(n2 + 1, 0);
+  for (uint64_t k = 0; k < B12.nnz(); k++) cptr[B12.columns[k]]++;
+  for (int j = 0; j < n2; j++) cptr[j + 1] += cptr[j];
+  std::vector<int> cpos(B12.nnz()), crow(B12.nnz());
+  {
+    std::vector<int> f(cptr.begin(), cptr.end() - 1);
+    for (uint32_t i = 0; i < nC; i++)
+      for (uint32_t k = B12.iofeir[i] - 1; k < B12.iofeir[i + 1] - 1; k++) {
+        int j = (int)B12.columns[k] - 1;
+        int p = f[j]++;
+        cpos[p] = (int)k;
+        crow[p] = (int)i;
+      }
+  }
+  d_k12tPtr.upload(cptr);
+  d_k12tPos.upload(cpos);
+  d_k12tRow.upload(crow);
+
+  // boundary conditions as equation lists (FESolver::applyBoundaryConditions, FESolver.cpp:229-247)
+  std::vector<int> fe, le;
+  std::vector<double> fv, lv;
+  for (auto& f : m.fixs) {
+    fe.push_back((int)m.nodeDofEq(f.node, f.dof) - 1);
+    fv.push_back(f.value);
+  }
+  for (auto& l : m.loads) {
+    if (l.node < 1 || l.node > m.nNodes() || !m.nodeDofEq(l.node, l.dof)) throw ApiError(-1, "load on an unregistered DoF");
+    le.push_back((int)m.nodeDofEq(l.node, l.dof) - 1);
+    lv.push_back(l.value);
+  }
+  d_fixEq.upload(fe);
+  d_fixVal.upload(fv);
+  d_loadEq.upload(le);
+  d_loadVal.upload(lv);
+  d_R.alloc((size_t)m.nAll() * 8);
+  hipCheck(hipMemset(d_R.p, 0, d_R.bytes), "memset R");
+  d_Ucprev.alloc((size_t)std::max<uint32_t>(nC, 1) * 8);
+  hipCheck(hipMemset(d_Ucprev.p, 0, d_Ucprev.bytes), "memset Ucprev");
+  d_rhs.alloc((size_t)n2 * 8);
+  d_dU.alloc((size_t)m.nAll() * 8);
+  for (auto& b : d_mw) b.alloc((size_t)n2 * 8);
+  const int nPart = std::max((n2 * 8 + 255) / 256, (nGroups + 255) / 256) + 16;
+  d_partial.alloc((size_t)nPart * 8);
+  d_mstate.alloc(sizeof(MinresState));
+  d_scal.alloc(16 * 8);
+  if (!h_mstate) hipCheck(hipHostMalloc((void**)&h_mstate, sizeof(MinresState), hipHostMallocDefault), "pinned");
+  // block(2) positions of the MPC entries, mirrored after the MPC rows are set
+  std::vector<int> mpcIdx;
+  for (auto& rb : rbs) {
+    std::vector<int> tg((size_t)rb.dev.nRows * 5);
+    hipCheck(hipMemcpy(tg.data(), rb.tgt.p, tg.size() * 4, hipMemcpyDeviceToHost), "tgt");
+    for (int t : tg)
+      if ((uint64_t)t >= m.offB2() && (uint64_t)t < m.nnzAll()) mpcIdx.push_back(t - (int)m.offB2());
+  }
+  d_mpcMirIdx.upload(mpcIdx);
+  nMpcMir = (int)mpcIdx.size();
+  krylovReady = true;
+}
+
+KrylovMatrix Ctx::krylovMatrix() const {
+  KrylovMatrix M;
+  M.n = n2;
+  M.rowptr = d_rowptr2.as<int>();
+  M.col = d_col2.as<int>();
+  M.val = d_full2.as<double>();
+  M.diagpos = d_diagpos2.as<int>();
+  return M;
+}
+
+Precond Ctx::precond() {
+  Precond P;
+  P.nGroups = nGroups;
+  P.gStart = d_gStart.as<int>();
+  P.gInfo = d_gInfo.as<int>();
+  P.gDiagPos = d_gDiagPos.as<int>();
+  P.binv = d_binv.as<double>();
+  return P;
+}
+
+int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* relres_est, bool* breakdown) {
+  KrylovMatrix M = krylovMatrix();
+  Precond P = precond();
+  MinresWork w;
+  w.X = x;
+  w.R1 = d_mw[0].as<double>();
+  w.R2 = d_mw[1].as<double>();
+  w.T = d_mw[2].as<double>();
+  w.Yp = d_mw[3].as<double>();
+  w.Z = d_mw[4].as<double>();
+  w.W1 = d_mw[5].as<double>();
+  w.W2 = d_mw[6].as<double>();
+  w.partial = d_partial.as<double>();
+  w.st = d_mstate.as<MinresState>();
+  minres_start(stream, M, P, w, b);
+  hipCheck(hipGetLastError(), "minres start");
+  const int check = 16;
+  int it = 0;
+  for (;;) {
+    for (int k = 0; k < check && it < maxit; k++, it++) minres_iteration(stream, M, P, w, tol);
+    hipCheck(hipGetLastError(), "minres iteration");
+    hipCheck(hipMemcpyAsync(h_mstate, w.st, sizeof(MinresState), hipMemcpyDeviceToHost, stream), "state");
+    hipCheck(hipStreamSynchronize(stream), "minres sync");
+    if (h_mstate->converged || it >= maxit) break;
+  }
+  if (relres_est) *relres_est = h_mstate->beta1 > 0 ? h_mstate->phibar / h_mstate->beta1 : 0.0;
+  if (breakdown) *breakdown = h_mstate->breakdown != 0;
+  return h_mstate->itn;
+}
+
+}  // namespace nla3d_amd
+
+// ---------------------------------------------------------------------------
+// Newton-Raphson loop (NonlinearFESolver::solve, FESolver.cpp:407-521)
+// ---------------------------------------------------------------------------
+#include <chrono>
+
+#include "../../include/nla3d_gpu.h"
+
+namespace nla3d_amd {
+
+static double nowMs() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct NewtonRun {
+  Ctx& c;
+  const nla3d_gpu_newton_opts& o;
+  nla3d_gpu_newton_stats& st;
+};
+
+static double readScalar(Ctx& c, int idx) {
+  double v;
+  hipCheck(hipMemcpyAsync(&v, c.d_scal.as<double>() + idx, 8, hipMemcpyDeviceToHost, c.stream), "scalar");
+  hipCheck(hipStreamSynchronize(c.stream), "sync");
+  return v;
+}
+
+// one load step at normalised time t; returns 0 or an ABI error code (message in c.err)
+int newtonLoadStep(Ctx& c, double t, const nla3d_gpu_newton_opts& o, nla3d_gpu_newton_stats& st) {
+  FEStorage& m = c.model;
+  const int nC = (int)m.nConstrainedDofs, nU = (int)m.nUnknownDofs, nM = (int)m.nMpc();
+  double* U = c.d_U.as<double>();
+  double* F = c.d_vals.as<double>() + m.nnzAll();
+  const double* K12 = c.d_vals.as<double>() + m.offB12();
+  double* R = c.d_R.as<double>();
+  double rhs1 = 0.0;
+  bool converged = false;
+  int it = 0;
+  while (it < o.max_iterations) {
+    it++;
+    st.newton_iterations++;
+    double t0 = nowMs();
+    hipCheck(hipMemsetAsync(R, 0, c.d_R.bytes, c.stream), "zero R");  // vecR.zero() (FESolver.cpp:445)
+    c.assemble();
+    newton_apply_bc(c.stream, (int)m.fixs.size(), c.d_fixEq.as<int>(), c.d_fixVal.as<double>(), (int)m.loads.size(),
+                    c.d_loadEq.as<int>(), c.d_loadVal.as<double>(), t, U, R);
+    newton_rhs(c.stream, c.n2, nC, F, R, U, c.d_Ucprev.as<double>(), c.d_k12tPtr.as<int>(), c.d_k12tPos.as<int>(),
+               c.d_k12tRow.as<int>(), K12, c.d_rhs.as<double>());
+    vec_sumsq(c.stream, c.n2, c.d_rhs.as<double>(), c.d_partial.as<double>(), c.d_scal.as<double>() + 0);
+    hipCheck(hipGetLastError(), "newton assemble");
+    const double rn = std::sqrt(readScalar(c, 0));
+    st.ms_assemble += nowMs() - t0;
+    if (it == 1) rhs1 = rn;
+    if (o.criterion == 1) {
+      st.last_criterion = rhs1 > 0 ? rn / rhs1 : 0.0;
+      if (!(rn == rn)) {
+        c.err = "The solution is diverged!";
+        return -6;
+      }
+      if (it > 1 && rn <= o.convergence * rhs1) {
+        converged = true;
+        st.newton_iterations--;  // the last pass only measured the residual
+        it--;
+        break;
+      }
+      if (rhs1 == 0.0) {
+        converged = true;
+        break;
+      }
+    }
+    // solve K.block(2) dU_sl = rhs (FESolver.cpp:457)
+    t0 = nowMs();
+    krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
+    double relres = 0;
+    bool brk = false;
+    const int kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), o.krylov_tol, o.krylov_max_iter, &relres,
+                                  &brk);
+    st.krylov_iterations += kit;
+    st.linear_solves++;
+    if (brk) st.krylov_breakdown = 1;
+    st.ms_solve += nowMs() - t0;
+    // restore DoF values from increments; Ucprev = Uc (FESolver.cpp:460-469)
+    newton_update(c.stream, nC, nU, nM, c.d_dU.as<double>(), U, c.d_Ucprev.as<double>());
+    if (o.criterion == 0) {  // calculateCriteria (FESolver.cpp:511-521)
+      vec_abssum(c.stream, nU, c.d_dU.as<double>(), c.d_partial.as<double>(), c.d_scal.as<double>() + 1);
+      hipCheck(hipGetLastError(), "criterion");
+      const double crit = readScalar(c, 1) / nU / o.convergence;
+      st.last_criterion = crit;
+      if (crit < o.convergence) {
+        converged = true;
+        break;
+      }
+      if (crit > 1.0e6 || crit != crit) {
+        c.err = "The solution is diverged!";
+        return -6;
+      }
+    } else {
+      hipCheck(hipStreamSynchronize(c.stream), "sync");
+    }
+  }
+  st.last_step_iterations = it;
+  if (!converged) {
+    c.err = "The solution is not converged with " + std::to_string(it) + " equilibrium iterations";
+    return -5;
+  }
+  st.load_steps++;
+  st.last_time = t;
+  return 0;
+}
+
+}  // namespace nla3d_amd
+
+using namespace nla3d_amd;
+
+static int newtonGuard(nla3d_gpu_t h, const std::function<int(Ctx&)>& fn) {
+  if (!h) return -1;
+  Ctx& c = *reinterpret_cast<Ctx*>(h);
+  try {
+    c.err.clear();
+    c.requireSolid();
+    hipCheck(hipSetDevice(c.device), "hipSetDevice");
+    if (!c.krylovReady) c.setupKrylov();
+    return fn(c);
+  } catch (const ApiError& e) {
+    c.err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    c.err = e.what();
+    return -1;
+  }
+}
+
+extern "C" {
+
+int nla3d_gpu_newton_defaults(nla3d_gpu_newton_opts* o) {
+  if (!o) return -1;
+  o->n_load_steps = 10;
+  o->max_iterations = 20;
+  o->criterion = 0;
+  o->convergence = 1.0e-3;
+  o->krylov_tol = 1.0e-12;
+  o->krylov_max_iter = 20000;
+  o->preconditioner = 0;
+  return 0;
+}
+
+int nla3d_gpu_solve_load_step(nla3d_gpu_t h, double t, const nla3d_gpu_newton_opts* opts,
+                              nla3d_gpu_newton_stats* stats) {
+  if (!opts || !stats) return -1;
+  return newtonGuard(h, [&](Ctx& c) {
+    const double t0 = nowMs();
+    int rc = newtonLoadStep(c, t, *opts, *stats);
+    stats->ms_total += nowMs() - t0;
+    return rc;
+  });
+}
+
+int nla3d_gpu_solve_nonlinear(nla3d_gpu_t h, const nla3d_gpu_newton_opts* opts, nla3d_gpu_newton_stats* stats) {
+  if (!opts || !stats || opts->n_load_steps < 1) return -1;
+  return newtonGuard(h, [&](Ctx& c) {
+    const double t0 = nowMs();
+    // TimeControl::nextStep (FESolver.cpp:34-56), currentTime from 0, delta = 1/N
+    const double endT = 1.0, delta = 1.0 / opts->n_load_steps;
+    double cur = 0.0;
+    int rc = 0;
+    while (cur < endT) {
+      double d = (cur + delta >= endT) ? endT - cur : ((cur + 2 * delta >= endT) ? (endT - cur) / 2.0 : delta);
+      cur += d;
+      rc = newtonLoadStep(c, cur, *opts, *stats);
+      if (rc) break;
+    }
+    stats->ms_total += nowMs() - t0;
+    return rc;
+  });
+}
+
+int nla3d_gpu_solve_block2(nla3d_gpu_t h, const double* rhs, double* x, double tol, int max_iter, int* iters,
+                           double* relres) {
+  if (!rhs || !x) return -1;
+  return newtonGuard(h, [&](Ctx& c) {
+    hipCheck(hipMemcpyAsync(c.d_rhs.p, rhs, (size_t)c.n2 * 8, hipMemcpyHostToDevice, c.stream), "rhs");
+    krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
+    double rr = 0;
+    bool brk = false;
+    int it = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), tol, max_iter, &rr, &brk);
+    hipCheck(hipMemcpyAsync(x, c.d_dU.p, (size_t)c.n2 * 8, hipMemcpyDeviceToHost, c.stream), "x");
+    hipCheck(hipStreamSynchronize(c.stream), "sync");
+    if (iters) *iters = it;
+    if (relres) *relres = rr;
+    if (brk) {
+      c.err = "MINRES breakdown (preconditioner not positive definite)";
+      return -5;
+    }
+    return 0;
+  });
+}
+
+}  // extern "C"

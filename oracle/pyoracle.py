@@ -471,3 +471,56 @@ class OracleModel:
                     for l, t in enumerate((ROTX, ROTY, ROTZ)):
                         coef = -(cD[0][l] * pq[0] + cD[1][l] * pq[1] + cD[2][l] * pq[2])
                         terms[2 + l] = (terms[2 + l][0], t, coef)
+
+    # ---------------- Newton-Raphson (NonlinearFESolver::solve, FESolver.cpp:407-521) ----------------
+    def dense_blocks(self, K):
+        """K values per block -> dense K12 (nC x n2) and symmetric K2 (n2 x n2)"""
+        nC, n2 = self.nC, self.nU + self.nMpc
+        K12 = np.zeros((nC, n2))
+        b = self.blocks[12]
+        for i in range(nC):
+            for k in range(int(b.iofeir[i]) - 1, int(b.iofeir[i + 1]) - 1):
+                K12[i, int(b.columns[k]) - 1] = K[12][k]
+        K2 = np.zeros((n2, n2))
+        b = self.blocks[2]
+        for i in range(n2):
+            for k in range(int(b.iofeir[i]) - 1, int(b.iofeir[i + 1]) - 1):
+                j = int(b.columns[k]) - 1
+                K2[i, j] = K2[j, i] = K[2][k]
+        return K12, K2
+
+    def newton_init(self):
+        self.U = np.zeros(self.nAll)
+        self.Ucprev = np.zeros(self.nC)
+
+    def solve_load_step(self, t, criterion=0, convergence=1e-3, max_iterations=20):
+        """one load step at normalised time t; dense FP64 solve stands in for PARDISO"""
+        nC, nU = self.nC, self.nU
+        U = self.U
+        rhs1 = None
+        for it in range(1, max_iterations + 1):
+            K, F = self.assemble(U)
+            R = np.zeros(self.nAll)
+            for node, dof, val in self.loads:  # applyBoundaryConditions (FESolver.cpp:229-247)
+                R[self.nodeEq[node - 1][dof] - 1] += val * t
+            for node, dof, val in self.fixes:
+                U[self.nodeEq[node - 1][dof] - 1] = val * t
+            dUc = U[:nC] - self.Ucprev
+            K12, K2 = self.dense_blocks(K)
+            rhs = F[nC:] + R[nC:] - K12.T @ dUc
+            rn = np.linalg.norm(rhs)
+            if it == 1:
+                rhs1 = rn
+            if criterion == 1 and it > 1 and rn <= convergence * rhs1:
+                return it - 1
+            dU = np.linalg.solve(K2, rhs)
+            U[nC:nC + nU] += dU[:nU]
+            U[nC + nU:] = dU[nU:]
+            self.Ucprev = U[:nC].copy()
+            if criterion == 0:
+                crit = np.abs(dU[:nU]).sum() / nU / convergence
+                if crit < convergence:
+                    return it
+                if crit > 1e6 or crit != crit:
+                    raise RuntimeError("The solution is diverged!")
+        raise RuntimeError("The solution is not converged")

@@ -22,4 +22,4 @@ def test_exports_match_header():
     assert len(names) >= 20
     for n in names:
         assert hasattr(L, n), n
-    assert set(nla3d_amd.ABI) | {"nla3d_gpu_last_error"} >= names
+    assert set(nla3d_amd.ABI) | set(nla3d_amd.ABI_FREE) | {"nla3d_gpu_last_error"} >= names

@@ -1,0 +1,108 @@
+"""GPU Krylov solve and Newton-Raphson loop against the oracle (dense FP64 solves).
+
+Linear solve: MINRES + block Jacobi to a 1e-13 preconditioned residual must match
+numpy's dense solve of the same block(2) to 1e-8 relative.  Newton: the converged
+displacements after every load step agree with the oracle's to 1e-8 relative
+(max-abs over the solution vector), with the same criterion and load schedule.
+"""
+import numpy as np
+import pytest
+
+import nla3d_amd
+from nla3d_amd import meshgen
+from oracle.pyoracle import OracleModel, load_steps
+
+pytestmark = pytest.mark.gpu
+
+
+def compression(n, mat=(3, [0.5, 0.1, 100.0]), top_uz=0.2):
+    m = nla3d_amd.GpuModel(0)
+    xyz, conn = meshgen.solid81_compression_case(m, n, material=mat, top_uz=top_uz)
+    m.init_solution()
+    z = xyz[:, 2]
+    fixes = [(int(nd), d, 0.0) for nd in np.nonzero(z == 0.0)[0] + 1 for d in range(3)]
+    fixes += [(int(nd), 2, top_uz) for nd in np.nonzero(z == z.max())[0] + 1]
+    o = OracleModel(xyz, [(1, c) for c in conn], material=mat, fixes=fixes).setup()
+    return m, o
+
+
+def rigid(nx, ny, nz, uz=0.15, rotz=0.349, mat=(3, [0.5, 0.1, 1000.0])):
+    m = nla3d_amd.GpuModel(0)
+    xyz, conn = meshgen.solid81_rigid_top_case(m, nx, ny, nz, 2.0, 2.0, 1.0, uz, rotz, material=mat)
+    m.init_solution()
+    master = len(xyz)
+    z = xyz[:-1, 2]
+    fixes = [(int(nd), d, 0.0) for nd in np.nonzero(z == 0.0)[0] + 1 for d in range(3)]
+    fixes += [(master, d, v) for d, v in zip(range(6), [0, 0, uz, 0, 0, rotz])]
+    top = list(np.nonzero(z == 1.0)[0] + 1)
+    o = OracleModel(xyz, [(1, c) for c in conn], material=mat, fixes=fixes,
+                    rigid_bodies=[(master, top, [0, 1, 2])]).setup()
+    return m, o
+
+
+def test_minres_matches_dense_solve(gpu_lib):
+    print("[gpu] minres vs dense", flush=True)
+    m, o = compression(3)
+    rng = np.random.default_rng(2)
+    U = 0.02 * rng.standard_normal(o.nAll)
+    m.set_solution(U)
+    m.assemble()
+    K, _ = o.assemble(U)
+    _, K2 = o.dense_blocks(K)
+    b = rng.standard_normal(K2.shape[0])
+    x, it, rr = m.solve_block2(b, tol=1e-13)
+    xr = np.linalg.solve(K2, b)
+    print("iterations", it, "relres", rr, flush=True)
+    assert np.abs(x - xr).max() <= 1e-8 * np.abs(xr).max()
+
+
+def test_minres_saddle_point_with_multipliers(gpu_lib):
+    print("[gpu] minres rigid-body saddle point", flush=True)
+    m, o = rigid(3, 3, 2)
+    U = np.zeros(o.nAll)
+    m.set_solution(U)
+    m.assemble()
+    K, _ = o.assemble(U)
+    _, K2 = o.dense_blocks(K)
+    b = np.random.default_rng(5).standard_normal(K2.shape[0])
+    x, it, rr = m.solve_block2(b, tol=1e-13)
+    xr = np.linalg.solve(K2, b)
+    assert np.abs(x - xr).max() <= 1e-8 * np.abs(xr).max()
+
+
+@pytest.mark.parametrize("criterion,conv", [(0, 1e-3), (1, 1e-10)])
+def test_newton_load_steps_match_oracle(gpu_lib, criterion, conv):
+    print(f"[gpu] newton compression criterion={criterion}", flush=True)
+    m, o = compression(2)
+    o.newton_init()
+    opts = m.newton_defaults(criterion=criterion, convergence=conv, krylov_tol=1e-14)
+    for t in load_steps(10):  # the reference schedule: 11 steps for N = 10
+        st = m.solve_load_step(t, opts)
+        it_ref = o.solve_load_step(t, criterion=criterion, convergence=conv)
+        U = m.get_solution()
+        err = np.abs(U - o.U).max() / np.abs(o.U).max()
+        assert err <= 1e-8, (t, err)
+        assert st.last_step_iterations == it_ref, (t, st.last_step_iterations, it_ref)
+
+
+def test_newton_rigid_body_matches_oracle(gpu_lib):
+    print("[gpu] newton rigid body", flush=True)
+    m, o = rigid(2, 2, 2)
+    o.newton_init()
+    opts = m.newton_defaults(criterion=1, convergence=1e-10, krylov_tol=1e-14)
+    for t in load_steps(4):
+        m.solve_load_step(t, opts)
+        o.solve_load_step(t, criterion=1, convergence=1e-10)
+        U = m.get_solution()
+        assert np.abs(U - o.U).max() <= 1e-8 * np.abs(o.U).max(), t
+
+
+def test_solve_nonlinear_full_schedule(gpu_lib):
+    print("[gpu] solve_nonlinear", flush=True)
+    m, o = compression(2)
+    st = m.solve_nonlinear(m.newton_defaults(n_load_steps=10, krylov_tol=1e-14))
+    assert st.load_steps == 11  # TimeControl quirk (FESolver.cpp:43-49)
+    o.newton_init()
+    for t in load_steps(10):
+        o.solve_load_step(t)
+    assert np.abs(m.get_solution() - o.U).max() <= 1e-8 * np.abs(o.U).max()
