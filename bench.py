@@ -1,25 +1,35 @@
 #!/usr/bin/env python3
-"""SOLID81 Newton-iteration hot path on one MI355X: elements/s of the fused
-update + buildK + deterministic scatter at the 3.5M-DoF shape.
+"""SOLID81 Newton-Raphson iteration on one MI355X at the 3.5M-DoF shape.
 
 Workload (synthetic, seeded): [0,2]x[0,2]x[0,1] box, 120x120x60 hexahedra
-(864,000 SOLID81 u/p elements, 893,101 nodes + 1 rigid-body master), interior
-nodes jittered by +-0.1 h (splitmix64, seed 20261203), Mooney-Rivlin
-C10=0.5 C01=0.1 K=1000, bottom face fixed, top face tied by a RigidBodyMpc to
-a master carrying +15% UZ and a 20 degree ROTZ.  The solution vector is a
-smooth state at half of that load (twist growing with z), so every Gauss
-point is in the nonlinear range.
+(864,000 SOLID81 u/p elements, 893,101 nodes + 1 rigid-body master;
+3,543,303 equations in K.block(2)), interior nodes jittered by +-0.1 h
+(splitmix64, seed 20261203), Mooney-Rivlin C10=0.5 C01=0.1 K=1000, bottom
+face fixed, top face tied by a RigidBodyMpc to a master node carrying +15% UZ
+and a 20 degree ROTZ, ramped over 20 load steps (TimeControl).
 
-A step is one FEStorage::assembleGlobalEqMatrices at the current solution
-(FEStorage.cpp:26-78): update() + buildK() of every element, scatter into the
-three CSR blocks and F, and the rigid-body MPC rows -- all on the device with
-the solution already resident in HBM.  value = elements / s over the timed
-steps.  The roofline entry is for the element kernel, from HIP events on the
-library's stream.  cpu_baseline times the reference assembly loop
-(oracle/nla3d_oracle.c: buildK + binary-search addValue, one core) on a
-bounded sample of the same model's elements.
+A step (mode "newton", the default) is one equilibrium iteration of
+NonlinearFESolver::solve (FESolver.cpp:443-490), all on the device with the
+model resident in HBM: update + buildK of every element, deterministic
+scatter into the three CSR blocks and F, rigid-body MPC rows, boundary
+conditions, rhs with the eliminated constrained DoFs, the block(2) solve
+(MINRES + block Jacobi to a 1e-9 relative preconditioned residual) and the
+increment update.  Newton runs to a 1e-8 relative residual; iterations that
+only confirm convergence (an assembly without a solve) are timed but not
+counted as steps.  value = elements / s = 864,000 / (Newton-iteration wall).
+Mode "assemble" times only the assembly (update + buildK + scatter + MPC).
+
+roofline: the dominant kernel of a Newton iteration, the Krylov SpMV over the
+full-symmetric block(2) CSR (HBM-bound), timed with HIP events on the
+library's stream; algorithmic bytes are those of a symmetric (upper) CSR
+SpMV.  The element kernel is reported beside it (FP64-bound).
+cpu_baseline: the reference assembly loop (oracle/nla3d_oracle.c: update +
+buildK + binary-search addValue, one core) on a bounded sample of the same
+model's elements, in elements/s; it excludes the reference's PARDISO solve
+(unavailable), so it is an upper bound on the CPU path's Newton-iteration rate.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -31,31 +41,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # Algorithmic FP64 work of one SOLID81 element per Newton iteration (update +
-# buildK, 8 Gauss points), counted on the reference formulation: per Gauss
-# point 8,420 flop (FMA = 2) -- see the breakdown in DESIGN.md.
+# buildK, 8 Gauss points) on the reference formulation: 8,420 flop per Gauss
+# point (FMA = 2).
 FLOP_PER_ELEMENT = 8 * 8420
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+METRIC = "SOLID81 Newton-iter wall @3.5M DoF, learned vs block-Jacobi; elements/s FP64"
 
 
-def synthetic_state(m, xyz, frac=0.5, uz=0.15, rotz=np.deg2rad(20.0)):
-    ne, _ = m.equation_numbers()
-    U = np.zeros(m.nAll)
-    X = xyz
-    z = X[:, 2]
-    th = frac * rotz * z
-    c, s = np.cos(th), np.sin(th)
-    dx, dy = X[:, 0] - 1.0, X[:, 1] - 1.0
-    disp = np.stack([c * dx - s * dy - dx, s * dx + c * dy - dy, frac * uz * z], axis=1)
-    for d in range(3):
-        eq = ne[:, d].astype(np.int64)
-        ok = eq > 0
-        U[eq[ok] - 1] = disp[ok, d]
-    # master node: UZ and ROTZ at the same load fraction
-    master = len(xyz) - 1
-    U[int(ne[master, 2]) - 1] = frac * uz
-    U[int(ne[master, 5]) - 1] = frac * rotz
-    return U
+def spmv_algorithmic_bytes(sizes):
+    """symmetric SpMV over block(2) stored once (upper CSR): values + column indices of
+    nnz(block2), row pointers, x read, y written"""
+    n = sizes["nUnknown"] + sizes["nMpc"]
+    return 12 * sizes["nnz2"] + 4 * (n + 1) + 16 * n
 
 
 def cpu_baseline(m, xyz, conn, U, n_sample):
@@ -74,22 +72,26 @@ def cpu_baseline(m, xyz, conn, U, n_sample):
     pyoracle.solid81_assemble_ref(Xe, Ue, pe, 3, [0.5, 0.1, 1000.0], eq, m.sizes["nConstrained"], *pats, m.nAll)
     dt = time.perf_counter() - t0
     return {"value": len(sel) / dt, "unit": "elements/s", "cores": 1, "kind": "port",
-            "sample": f"reference assembly loop (update+buildK+binary-search addValue) over the first {len(sel)} "
-                      f"elements of the same model, {dt:.1f} s, 1 thread"}
+            "sample": f"reference assembly loop (update + buildK + binary-search addValue) over the first "
+                      f"{len(sel)} elements of the same model: {dt:.1f} s on 1 thread; the PARDISO solve of a "
+                      f"reference Newton iteration is not included"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", choices=["newton", "assemble"], default="newton")
     ap.add_argument("--shape", type=str, default="120,120,60")
+    ap.add_argument("--load-steps", type=int, default=20)
+    ap.add_argument("--krylov-tol", type=float, default=1e-9)
     ap.add_argument("--cpu-sample", type=int, default=250000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
-                                   "sketch (DESIGN.md), no figure at several GPUs is produced"}))
+        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU sharding is a design "
+                                   "sketch, no figure at several GPUs is produced"}))
         sys.exit(2)
 
     import nla3d_amd
@@ -103,41 +105,97 @@ def main():
     # 100-entry row capacity, which would abort this model (SparseMatrix.cpp:84)
     m.set_max_in_row(0)
     sizes = m.init_solution()
-    U = synthetic_state(m, xyz)
-    m.set_solution(U)
     t_setup = time.perf_counter() - t_setup
     print(f"[bench] setup {t_setup:.1f}s sizes {sizes}", file=sys.stderr, flush=True)
     nE = sizes["nElements"]
+    n_eq = sizes["nUnknown"] + sizes["nMpc"]
 
-    for _ in range(args.warmup):
-        m.assemble()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        m.assemble()
-    t1 = time.perf_counter()
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    value = nE * args.steps / (t1 - t0)
+    extra = {}
+    opts = m.newton_defaults(criterion=1, convergence=1e-8, krylov_tol=args.krylov_tol, krylov_max_iter=100000)
+    if args.mode == "assemble":
+        for _ in range(args.warmup):
+            m.assemble()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            m.assemble()
+        t1 = time.perf_counter()
+        ms_step = (t1 - t0) * 1e3 / args.steps
+    else:
+        # walk the TimeControl schedule (FESolver.cpp:34-56), one equilibrium iteration per step
+        delta = 1.0 / args.load_steps
+        sched, cur = [], 0.0
+        while cur < 1.0:
+            cur += (1.0 - cur) if cur + delta >= 1.0 else ((1.0 - cur) / 2.0 if cur + 2 * delta >= 1.0 else delta)
+            sched.append(cur)
+        st = nla3d_amd.NewtonStats()
+        rhs1 = ctypes.c_double(0.0)
+        state = {"k": 0, "it": 0}
+
+        def one_solving_iteration():
+            while True:
+                state["it"] += 1
+                before = st.linear_solves
+                conv = m.newton_iteration(sched[state["k"]], state["it"], rhs1, opts, st)
+                if conv:
+                    state["k"] += 1
+                    state["it"] = 0
+                    if state["k"] >= len(sched):
+                        raise RuntimeError("load schedule exhausted")
+                if st.linear_solves > before:
+                    return st.last_krylov_iterations
+
+        for _ in range(args.warmup):
+            kit = one_solving_iteration()
+            print(f"[bench] warmup iteration: {kit} Krylov iterations", file=sys.stderr, flush=True)
+        kits = []
+        ms_asm0, ms_sol0 = st.ms_assemble, st.ms_solve
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            kits.append(one_solving_iteration())
+            print(f"[bench] iteration: {kits[-1]} Krylov iterations, criterion {st.last_criterion:.3e}",
+                  file=sys.stderr, flush=True)
+        t1 = time.perf_counter()
+        ms_step = (t1 - t0) * 1e3 / args.steps
+        extra = {"krylov_iterations_per_newton": float(np.mean(kits)),
+                 "ms_krylov_per_iteration": (st.ms_solve - ms_sol0) / max(1, sum(kits)),
+                 "ms_assemble_per_newton": (st.ms_assemble - ms_asm0) / args.steps,
+                 "load_step": state["k"] + 1, "preconditioner": "block-Jacobi"}
+    value = nE * 1e3 / ms_step
     print(f"[bench] {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
 
-    ms_elem, ms_scatter = m.time_assembly(max(5, args.steps))
-    achieved = FLOP_PER_ELEMENT * nE / (ms_elem * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                "kernel": "solid81_kernel", "kernel_ms": ms_elem, "scatter_ms": ms_scatter}
+    ms_elem, ms_scatter = m.time_assembly(5)
+    ms_spmv = m.time_spmv(20)
+    spmv_bytes = spmv_algorithmic_bytes(sizes)
+    spmv_gbs = spmv_bytes / (ms_spmv * 1e-3) / 1e9
+    elem_tf = FLOP_PER_ELEMENT * nE / (ms_elem * 1e-3) / 1e12
+    if args.mode == "newton":
+        roofline = {"bound": "hbm", "achieved": spmv_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": spmv_gbs / PEAK_HBM_GBS, "traffic": None, "kernel": "spmv_kernel (block(2) SpMV)",
+                    "kernel_ms": ms_spmv, "algorithmic_bytes": spmv_bytes}
+    else:
+        roofline = {"bound": "mfma", "achieved": elem_tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                    "frac": elem_tf / PEAK_FP64_TFLOPS, "traffic": None, "kernel": "solid81_kernel",
+                    "kernel_ms": ms_elem}
+    extra.update({"element_kernel_ms": ms_elem, "element_kernel_tflops": elem_tf,
+                  "element_kernel_frac_fp64": elem_tf / PEAK_FP64_TFLOPS, "scatter_kernel_ms": ms_scatter,
+                  "assembly_elements_per_s": nE / ((ms_elem + ms_scatter) * 1e-3),
+                  "spmv_ms": ms_spmv, "spmv_GBps": spmv_gbs})
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(m, xyz, conn, U, args.cpu_sample)
+        cpu = cpu_baseline(m, xyz, conn, m.get_solution(), args.cpu_sample)
 
-    n_dof = sizes["nUnknown"] + sizes["nMpc"]
     out = {
-        "metric": "SOLID81 Newton-iter wall @3.5M DoF, learned vs block-Jacobi; elements/s FP64",
-        "value": value, "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"SOLID81 u/p assembly (update+buildK+scatter+MPC), {nx}x{ny}x{nz} hex box, "
-                               f"{nE} elements, {n_dof} equations in block(2), Mooney-Rivlin K/C10=2000",
-                   "elements": nE, "equations": n_dof, "nnz_block2": sizes["nnz2"]},
+        "metric": METRIC, "value": value, "unit": "elements/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": (f"SOLID81 u/p Newton iteration (assemble + MINRES/block-Jacobi solve + update), "
+                                f"{nx}x{ny}x{nz} hex box, {nE} elements, {n_eq} equations in block(2), "
+                                f"Mooney-Rivlin K/C10=2000, rigid-body top (+15% UZ, 20 deg)")
+                   if args.mode == "newton" else
+                   (f"SOLID81 u/p assembly (update+buildK+scatter+MPC), {nx}x{ny}x{nz} hex box, {nE} elements, "
+                    f"{n_eq} equations in block(2)"),
+                   "mode": args.mode, "elements": nE, "equations": n_eq, "nnz_block2": sizes["nnz2"], **extra},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -125,6 +125,7 @@ typedef struct {
   int last_step_iterations;  /* iterations of the last load step */
   int linear_solves;
   long long krylov_iterations;  /* summed over all linear solves */
+  int last_krylov_iterations;   /* iterations of the last linear solve */
   int krylov_breakdown;      /* a MINRES solve hit an indefinite preconditioner */
   double last_criterion;     /* value of the convergence measure at exit */
   double last_time;          /* normalised time of the last completed step */
@@ -139,6 +140,13 @@ int nla3d_gpu_newton_defaults(nla3d_gpu_newton_opts* opts);
  * (vecU, previous constrained values) carries over between calls. */
 int nla3d_gpu_solve_load_step(nla3d_gpu_t h, double t, const nla3d_gpu_newton_opts* opts,
                               nla3d_gpu_newton_stats* stats);
+/* One equilibrium iteration (FESolver.cpp:443-490) at normalised time t:
+ * assemble, boundary conditions, rhs, Krylov solve, update, criterion.
+ * iteration is the 1-based count within the load step and *rhs1 carries the
+ * first iteration's ||rhs|| between calls.  *converged = 1 when the criterion
+ * holds (criterion 1: the state was already in equilibrium, nothing solved). */
+int nla3d_gpu_newton_iteration(nla3d_gpu_t h, double t, int iteration, double* rhs1,
+                               const nla3d_gpu_newton_opts* opts, nla3d_gpu_newton_stats* stats, int* converged);
 /* NonlinearFESolver::solve: TimeControl over [0, 1] with n_load_steps steps,
  * each solved by nla3d_gpu_solve_load_step. */
 int nla3d_gpu_solve_nonlinear(nla3d_gpu_t h, const nla3d_gpu_newton_opts* opts, nla3d_gpu_newton_stats* stats);
@@ -153,6 +161,9 @@ int nla3d_gpu_solve_block2(nla3d_gpu_t h, const double* rhs, double* x, double t
  * duration in ms of the element kernel and of the scatter kernel, measured with
  * HIP events on the handle's stream. */
 int nla3d_gpu_time_assembly(nla3d_gpu_t h, int iters, double* ms_element, double* ms_scatter);
+/* Mean duration in ms of the block(2) SpMV kernel of the Krylov solve (full-symmetric
+ * CSR, y = K2 x) over `iters` launches, HIP events on the handle's stream. */
+int nla3d_gpu_time_spmv(nla3d_gpu_t h, int iters, double* ms);
 
 #ifdef __cplusplus
 }

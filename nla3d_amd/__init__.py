@@ -34,7 +34,8 @@ class NewtonStats(ctypes.Structure):
     """nla3d_gpu_newton_stats (include/nla3d_gpu.h)"""
     _fields_ = [("load_steps", ctypes.c_int), ("newton_iterations", ctypes.c_int),
                 ("last_step_iterations", ctypes.c_int), ("linear_solves", ctypes.c_int),
-                ("krylov_iterations", ctypes.c_longlong), ("krylov_breakdown", ctypes.c_int),
+                ("krylov_iterations", ctypes.c_longlong), ("last_krylov_iterations", ctypes.c_int),
+                ("krylov_breakdown", ctypes.c_int),
                 ("last_criterion", ctypes.c_double), ("last_time", ctypes.c_double),
                 ("ms_assemble", ctypes.c_double), ("ms_solve", ctypes.c_double), ("ms_total", ctypes.c_double)]
 
@@ -71,8 +72,11 @@ ABI = {
     "nla3d_gpu_solid81_element_matrices": [_P, _DP],
     "nla3d_gpu_solid81_gp_state": [_P, _DP],
     "nla3d_gpu_time_assembly": [_P, ctypes.c_int, _DP, _DP],
+    "nla3d_gpu_time_spmv": [_P, ctypes.c_int, _DP],
     "nla3d_gpu_solve_load_step": [_P, ctypes.c_double, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats)],
     "nla3d_gpu_solve_nonlinear": [_P, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats)],
+    "nla3d_gpu_newton_iteration": [_P, ctypes.c_double, ctypes.c_int, _DP, ctypes.POINTER(NewtonOpts),
+                                   ctypes.POINTER(NewtonStats), _IP],
     "nla3d_gpu_solve_block2": [_P, _DP, _DP, ctypes.c_double, ctypes.c_int, _IP, _DP],
 }
 # functions that take no handle
@@ -246,6 +250,13 @@ class GpuModel:
         self._call("nla3d_gpu_solve_load_step", float(t), ctypes.byref(opts), ctypes.byref(stats))
         return stats
 
+    def newton_iteration(self, t, iteration, rhs1, opts, stats):
+        """one equilibrium iteration; rhs1 is a ctypes.c_double carried across the load step"""
+        conv = ctypes.c_int()
+        self._call("nla3d_gpu_newton_iteration", float(t), int(iteration), ctypes.byref(rhs1), ctypes.byref(opts),
+                   ctypes.byref(stats), ctypes.byref(conv))
+        return bool(conv.value)
+
     def solve_nonlinear(self, opts, stats=None):
         stats = stats or NewtonStats()
         self._call("nla3d_gpu_solve_nonlinear", ctypes.byref(opts), ctypes.byref(stats))
@@ -259,6 +270,11 @@ class GpuModel:
         self._call("nla3d_gpu_solve_block2", _ptr(rhs, ctypes.c_double), _ptr(x, ctypes.c_double), float(tol),
                    int(max_iter), ctypes.byref(it), ctypes.byref(rr))
         return x, it.value, rr.value
+
+    def time_spmv(self, iters):
+        a = ctypes.c_double()
+        self._call("nla3d_gpu_time_spmv", int(iters), ctypes.byref(a))
+        return a.value
 
     def time_assembly(self, iters):
         a = ctypes.c_double()

@@ -276,7 +276,10 @@ int nla3d_gpu_init_solution(nla3d_gpu_t h) {
     if (!(e = m.setConstrainedDofs()).empty()) throw ApiError(NLA3D_GPU_EINVAL, e);
     if (!(e = m.assignEquationNumbers()).empty()) throw ApiError(NLA3D_GPU_EINVAL, e);
     if (!(e = m.buildSparsity(c.maxInRow)).empty()) throw ApiError(NLA3D_GPU_EPATTERN, e);
-    if (c.device >= 0) c.setupDevice();
+    if (c.device >= 0) {
+      c.setupDevice();
+      c.setupKrylov();
+    }
     c.initialized = true;
   });
 }

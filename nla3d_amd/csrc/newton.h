@@ -13,6 +13,8 @@ void newton_rhs(hipStream_t s, int n2, int nC, const double* F, const double* R,
                 const int* cptr, const int* cpos, const int* crow, const double* K12, double* rhs);
 void newton_update(hipStream_t s, int nC, int nU, int nMpc, const double* dU, double* U, double* Ucprev);
 void vec_abssum(hipStream_t s, int n, const double* x, double* partial, double* out);
+void newton_lambda_vec(hipStream_t s, int nC, int nU, int nMpc, const double* U, double* v);
+void vec_diffsq(hipStream_t s, int n, const double* a, const double* b, double* partial, double* out);
 void vec_sumsq(hipStream_t s, int n, const double* x, double* partial, double* out);
 
 }  // namespace nla3d_amd

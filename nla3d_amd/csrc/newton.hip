@@ -87,6 +87,25 @@ __global__ __launch_bounds__(256) void finish_kernel(const double* __restrict__ 
   if (threadIdx.x == 0) *out = s;
 }
 
+// v = [0 (nU) ; U_l (nMpc)]
+__global__ __launch_bounds__(256) void lambda_vec_kernel(int nC, int nU, int nMpc, const double* __restrict__ U,
+                                                        double* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nU) v[i] = 0.0;
+  else if (i < nU + nMpc) v[i] = U[nC + i];
+}
+
+// partial sums of (a - b)^2
+__global__ __launch_bounds__(256) void diffsq_partial_kernel(int n, const double* __restrict__ a,
+                                                            const double* __restrict__ b,
+                                                            double* __restrict__ partial) {
+  __shared__ double sh[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double v = i < n ? a[i] - b[i] : 0.0;
+  const double s = bsum256(v * v, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 }  // namespace
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -111,6 +130,16 @@ void newton_update(hipStream_t s, int nC, int nU, int nMpc, const double* dU, do
 void vec_abssum(hipStream_t s, int n, const double* x, double* partial, double* out) {
   const int nb = cdiv(std::max(n, 1), 256);
   hipLaunchKernelGGL(abssum_partial_kernel, dim3(nb), dim3(256), 0, s, n, x, partial);
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, s, partial, nb, out);
+}
+
+void newton_lambda_vec(hipStream_t s, int nC, int nU, int nMpc, const double* U, double* v) {
+  hipLaunchKernelGGL(lambda_vec_kernel, dim3(cdiv(std::max(nU + nMpc, 1), 256)), dim3(256), 0, s, nC, nU, nMpc, U, v);
+}
+
+void vec_diffsq(hipStream_t s, int n, const double* a, const double* b, double* partial, double* out) {
+  const int nb = cdiv(std::max(n, 1), 256);
+  hipLaunchKernelGGL(diffsq_partial_kernel, dim3(nb), dim3(256), 0, s, n, a, b, partial);
   hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, s, partial, nb, out);
 }
 

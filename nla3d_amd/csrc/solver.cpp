@@ -248,82 +248,98 @@ static double readScalar(Ctx& c, int idx) {
   return v;
 }
 
-// one load step at normalised time t; returns 0 or an ABI error code (message in c.err)
-int newtonLoadStep(Ctx& c, double t, const nla3d_gpu_newton_opts& o, nla3d_gpu_newton_stats& st) {
+// One equilibrium iteration of the Newton loop at normalised time t
+// (FESolver.cpp:443-490).  it is the 1-based iteration number within the load
+// step; rhs1 carries ||rhs|| of iteration 1.  Returns 1 when the criterion
+// holds (for criterion 1 the state was already converged and nothing was
+// solved), 0 to continue, or a negative ABI error code.
+int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newton_opts& o,
+                    nla3d_gpu_newton_stats& st) {
   FEStorage& m = c.model;
   const int nC = (int)m.nConstrainedDofs, nU = (int)m.nUnknownDofs, nM = (int)m.nMpc();
   double* U = c.d_U.as<double>();
   double* F = c.d_vals.as<double>() + m.nnzAll();
   const double* K12 = c.d_vals.as<double>() + m.offB12();
   double* R = c.d_R.as<double>();
-  double rhs1 = 0.0;
-  bool converged = false;
-  int it = 0;
-  while (it < o.max_iterations) {
-    it++;
-    st.newton_iterations++;
-    double t0 = nowMs();
-    hipCheck(hipMemsetAsync(R, 0, c.d_R.bytes, c.stream), "zero R");  // vecR.zero() (FESolver.cpp:445)
-    c.assemble();
-    newton_apply_bc(c.stream, (int)m.fixs.size(), c.d_fixEq.as<int>(), c.d_fixVal.as<double>(), (int)m.loads.size(),
-                    c.d_loadEq.as<int>(), c.d_loadVal.as<double>(), t, U, R);
-    newton_rhs(c.stream, c.n2, nC, F, R, U, c.d_Ucprev.as<double>(), c.d_k12tPtr.as<int>(), c.d_k12tPos.as<int>(),
-               c.d_k12tRow.as<int>(), K12, c.d_rhs.as<double>());
+  double t0 = nowMs();
+  hipCheck(hipMemsetAsync(R, 0, c.d_R.bytes, c.stream), "zero R");  // vecR.zero() (FESolver.cpp:445)
+  c.assemble();
+  newton_apply_bc(c.stream, (int)m.fixs.size(), c.d_fixEq.as<int>(), c.d_fixVal.as<double>(), (int)m.loads.size(),
+                  c.d_loadEq.as<int>(), c.d_loadVal.as<double>(), t, U, R);
+  newton_rhs(c.stream, c.n2, nC, F, R, U, c.d_Ucprev.as<double>(), c.d_k12tPtr.as<int>(), c.d_k12tPos.as<int>(),
+             c.d_k12tRow.as<int>(), K12, c.d_rhs.as<double>());
+  // equilibrium residual: vecUl is replaced, not incremented (FESolver.cpp:461), so the
+  // multiplier forces K2 [0; U_l] are still in rhs and are taken out of the measure
+  if (nM > 0) {
+    double* v = c.d_mw[7].as<double>();
+    double* y = c.d_mw[2].as<double>();
+    newton_lambda_vec(c.stream, nC, nU, nM, U, v);
+    krylov_spmv(c.stream, c.krylovMatrix(), v, y);
+    vec_diffsq(c.stream, c.n2, c.d_rhs.as<double>(), y, c.d_partial.as<double>(), c.d_scal.as<double>() + 0);
+  } else {
     vec_sumsq(c.stream, c.n2, c.d_rhs.as<double>(), c.d_partial.as<double>(), c.d_scal.as<double>() + 0);
-    hipCheck(hipGetLastError(), "newton assemble");
-    const double rn = std::sqrt(readScalar(c, 0));
-    st.ms_assemble += nowMs() - t0;
-    if (it == 1) rhs1 = rn;
-    if (o.criterion == 1) {
-      st.last_criterion = rhs1 > 0 ? rn / rhs1 : 0.0;
-      if (!(rn == rn)) {
-        c.err = "The solution is diverged!";
-        return -6;
-      }
-      if (it > 1 && rn <= o.convergence * rhs1) {
-        converged = true;
-        st.newton_iterations--;  // the last pass only measured the residual
-        it--;
-        break;
-      }
-      if (rhs1 == 0.0) {
-        converged = true;
-        break;
-      }
-    }
-    // solve K.block(2) dU_sl = rhs (FESolver.cpp:457)
-    t0 = nowMs();
-    krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
-    double relres = 0;
-    bool brk = false;
-    const int kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), o.krylov_tol, o.krylov_max_iter, &relres,
-                                  &brk);
-    st.krylov_iterations += kit;
-    st.linear_solves++;
-    if (brk) st.krylov_breakdown = 1;
-    st.ms_solve += nowMs() - t0;
-    // restore DoF values from increments; Ucprev = Uc (FESolver.cpp:460-469)
-    newton_update(c.stream, nC, nU, nM, c.d_dU.as<double>(), U, c.d_Ucprev.as<double>());
-    if (o.criterion == 0) {  // calculateCriteria (FESolver.cpp:511-521)
-      vec_abssum(c.stream, nU, c.d_dU.as<double>(), c.d_partial.as<double>(), c.d_scal.as<double>() + 1);
-      hipCheck(hipGetLastError(), "criterion");
-      const double crit = readScalar(c, 1) / nU / o.convergence;
-      st.last_criterion = crit;
-      if (crit < o.convergence) {
-        converged = true;
-        break;
-      }
-      if (crit > 1.0e6 || crit != crit) {
-        c.err = "The solution is diverged!";
-        return -6;
-      }
-    } else {
-      hipCheck(hipStreamSynchronize(c.stream), "sync");
-    }
   }
-  st.last_step_iterations = it;
-  if (!converged) {
-    c.err = "The solution is not converged with " + std::to_string(it) + " equilibrium iterations";
+  hipCheck(hipGetLastError(), "newton assemble");
+  const double rn = std::sqrt(readScalar(c, 0));
+  st.ms_assemble += nowMs() - t0;
+  if (it == 1) rhs1 = rn;
+  if (!(rn == rn)) {
+    c.err = "The solution is diverged!";
+    return -6;
+  }
+  if (o.criterion == 1) {
+    st.last_criterion = rhs1 > 0 ? rn / rhs1 : 0.0;
+    if ((it > 1 && rn <= o.convergence * rhs1) || rhs1 == 0.0) return 1;
+  }
+  st.newton_iterations++;
+  // solve K.block(2) dU_sl = rhs (FESolver.cpp:457)
+  t0 = nowMs();
+  krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
+  double relres = 0;
+  bool brk = false;
+  const int kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), o.krylov_tol, o.krylov_max_iter, &relres,
+                                &brk);
+  st.krylov_iterations += kit;
+  st.linear_solves++;
+  st.last_krylov_iterations = kit;
+  if (brk) st.krylov_breakdown = 1;
+  // restore DoF values from increments; Ucprev = Uc (FESolver.cpp:460-469)
+  newton_update(c.stream, nC, nU, nM, c.d_dU.as<double>(), U, c.d_Ucprev.as<double>());
+  hipCheck(hipGetLastError(), "newton update");
+  if (o.criterion == 0) {  // calculateCriteria (FESolver.cpp:511-521)
+    vec_abssum(c.stream, nU, c.d_dU.as<double>(), c.d_partial.as<double>(), c.d_scal.as<double>() + 1);
+    hipCheck(hipGetLastError(), "criterion");
+    const double crit = readScalar(c, 1) / nU / o.convergence;
+    st.ms_solve += nowMs() - t0;
+    st.last_criterion = crit;
+    if (crit < o.convergence) return 1;
+    if (crit > 1.0e6 || crit != crit) {
+      c.err = "The solution is diverged!";
+      return -6;
+    }
+    return 0;
+  }
+  hipCheck(hipStreamSynchronize(c.stream), "sync");
+  st.ms_solve += nowMs() - t0;
+  return 0;
+}
+
+// one load step at normalised time t; returns 0 or an ABI error code (message in c.err)
+int newtonLoadStep(Ctx& c, double t, const nla3d_gpu_newton_opts& o, nla3d_gpu_newton_stats& st) {
+  double rhs1 = 0.0;
+  int it = 0, rc = 0;
+  const int before = st.newton_iterations;
+  while (it < o.max_iterations + (o.criterion == 1 ? 1 : 0)) {
+    it++;
+    rc = newtonIteration(c, t, it, rhs1, o, st);
+    if (rc != 0) break;
+    if (o.criterion == 1 && it == o.max_iterations + 1) break;
+  }
+  st.last_step_iterations = st.newton_iterations - before;
+  if (rc < 0) return rc;
+  if (rc != 1) {
+    c.err = "The solution is not converged with " + std::to_string(st.last_step_iterations) +
+            " equilibrium iterations";
     return -5;
   }
   st.load_steps++;
@@ -378,6 +394,18 @@ int nla3d_gpu_solve_load_step(nla3d_gpu_t h, double t, const nla3d_gpu_newton_op
   });
 }
 
+int nla3d_gpu_newton_iteration(nla3d_gpu_t h, double t, int iteration, double* rhs1,
+                               const nla3d_gpu_newton_opts* opts, nla3d_gpu_newton_stats* stats, int* converged) {
+  if (!opts || !stats || !rhs1 || iteration < 1) return -1;
+  return newtonGuard(h, [&](Ctx& c) {
+    const double t0 = nowMs();
+    int rc = newtonIteration(c, t, iteration, *rhs1, *opts, *stats);
+    stats->ms_total += nowMs() - t0;
+    if (converged) *converged = (rc == 1);
+    return rc < 0 ? rc : 0;
+  });
+}
+
 int nla3d_gpu_solve_nonlinear(nla3d_gpu_t h, const nla3d_gpu_newton_opts* opts, nla3d_gpu_newton_stats* stats) {
   if (!opts || !stats || opts->n_load_steps < 1) return -1;
   return newtonGuard(h, [&](Ctx& c) {
@@ -394,6 +422,28 @@ int nla3d_gpu_solve_nonlinear(nla3d_gpu_t h, const nla3d_gpu_newton_opts* opts, 
     }
     stats->ms_total += nowMs() - t0;
     return rc;
+  });
+}
+
+int nla3d_gpu_time_spmv(nla3d_gpu_t h, int iters, double* ms) {
+  if (iters < 1 || !ms) return -1;
+  return newtonGuard(h, [&](Ctx& c) {
+    hipEvent_t a, b;
+    hipCheck(hipEventCreate(&a), "event");
+    hipCheck(hipEventCreate(&b), "event");
+    double* x = c.d_mw[7].as<double>();
+    double* y = c.d_mw[6].as<double>();
+    krylov_spmv(c.stream, c.krylovMatrix(), x, y);  // warm-up
+    hipCheck(hipEventRecord(a, c.stream), "rec");
+    for (int i = 0; i < iters; i++) krylov_spmv(c.stream, c.krylovMatrix(), x, y);
+    hipCheck(hipEventRecord(b, c.stream), "rec");
+    hipCheck(hipEventSynchronize(b), "sync");
+    float t = 0;
+    hipCheck(hipEventElapsedTime(&t, a, b), "elapsed");
+    *ms = t / iters;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return 0;
   });
 }
 

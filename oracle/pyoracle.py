@@ -508,7 +508,11 @@ class OracleModel:
             dUc = U[:nC] - self.Ucprev
             K12, K2 = self.dense_blocks(K)
             rhs = F[nC:] + R[nC:] - K12.T @ dUc
-            rn = np.linalg.norm(rhs)
+            # equilibrium residual: lambda is a total, not an increment (FESolver.cpp:461), so the
+            # multiplier forces K2 [0; lambda] are still in rhs and are taken out of the measure
+            lam = np.zeros(rhs.size)
+            lam[nU:] = U[nC + nU:]
+            rn = np.linalg.norm(rhs - K2 @ lam)
             if it == 1:
                 rhs1 = rn
             if criterion == 1 and it > 1 and rn <= convergence * rhs1:
