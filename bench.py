@@ -82,7 +82,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["newton", "assemble"], default="newton")
+    ap.add_argument("--mode", choices=["newton", "assemble", "probe"], default="newton")
     ap.add_argument("--shape", type=str, default="120,120,60")
     ap.add_argument("--load-steps", type=int, default=20)
     ap.add_argument("--krylov-tol", type=float, default=1e-9)
@@ -109,6 +109,17 @@ def main():
     print(f"[bench] setup {t_setup:.1f}s sizes {sizes}", file=sys.stderr, flush=True)
     nE = sizes["nElements"]
     n_eq = sizes["nUnknown"] + sizes["nMpc"]
+
+    if args.mode == "probe":
+        # short run of each hot kernel for hardware-counter passes: 2 assemblies
+        # (element kernel + scatter), 3 SpMVs, 1 plain copy of 1 GiB
+        m.assemble()
+        m.assemble()
+        ms_spmv = m.time_spmv(3)
+        ms_copy = m.copy_calibration(1 << 30)
+        print(json.dumps({"mode": "probe", "spmv_ms": ms_spmv, "copy_bytes": 1 << 30, "copy_ms": ms_copy,
+                          "spmv_algorithmic_bytes": spmv_algorithmic_bytes(sizes), "sizes": sizes}))
+        return
 
     extra = {}
     opts = m.newton_defaults(criterion=1, convergence=1e-8, krylov_tol=args.krylov_tol, krylov_max_iter=100000)

@@ -164,6 +164,9 @@ int nla3d_gpu_time_assembly(nla3d_gpu_t h, int iters, double* ms_element, double
 /* Mean duration in ms of the block(2) SpMV kernel of the Krylov solve (full-symmetric
  * CSR, y = K2 x) over `iters` launches, HIP events on the handle's stream. */
 int nla3d_gpu_time_spmv(nla3d_gpu_t h, int iters, double* ms);
+/* One launch of a plain device copy of `bytes` bytes (read + write), timed with
+ * HIP events: the reference point for scaling hardware byte counters. */
+int nla3d_gpu_copy_calibration(nla3d_gpu_t h, uint64_t bytes, double* ms);
 
 #ifdef __cplusplus
 }

@@ -73,6 +73,7 @@ ABI = {
     "nla3d_gpu_solid81_gp_state": [_P, _DP],
     "nla3d_gpu_time_assembly": [_P, ctypes.c_int, _DP, _DP],
     "nla3d_gpu_time_spmv": [_P, ctypes.c_int, _DP],
+    "nla3d_gpu_copy_calibration": [_P, ctypes.c_uint64, _DP],
     "nla3d_gpu_solve_load_step": [_P, ctypes.c_double, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats)],
     "nla3d_gpu_solve_nonlinear": [_P, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats)],
     "nla3d_gpu_newton_iteration": [_P, ctypes.c_double, ctypes.c_int, _DP, ctypes.POINTER(NewtonOpts),
@@ -270,6 +271,11 @@ class GpuModel:
         self._call("nla3d_gpu_solve_block2", _ptr(rhs, ctypes.c_double), _ptr(x, ctypes.c_double), float(tol),
                    int(max_iter), ctypes.byref(it), ctypes.byref(rr))
         return x, it.value, rr.value
+
+    def copy_calibration(self, nbytes):
+        a = ctypes.c_double()
+        self._call("nla3d_gpu_copy_calibration", int(nbytes), ctypes.byref(a))
+        return a.value
 
     def time_spmv(self, iters):
         a = ctypes.c_double()

@@ -153,3 +153,18 @@ void launch_rigid_body_mpc(hipStream_t stream, const RigidBodyDev& rb, const dou
 }
 
 }  // namespace nla3d_amd
+
+namespace nla3d_amd {
+namespace {
+// plain streaming copy, used to calibrate hardware byte counters against a known size
+__global__ __launch_bounds__(256) void calibration_copy_kernel(size_t n, const double* __restrict__ a,
+                                                               double* __restrict__ b) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+}  // namespace
+
+void launch_calibration_copy(hipStream_t s, size_t n, const double* a, double* b) {
+  hipLaunchKernelGGL(calibration_copy_kernel, dim3(4096), dim3(256), 0, s, n, a, b);
+}
+}  // namespace nla3d_amd

@@ -426,4 +426,28 @@ int nla3d_gpu_time_assembly(nla3d_gpu_t h, int iters, double* ms_element, double
   });
 }
 
+int nla3d_gpu_copy_calibration(nla3d_gpu_t h, uint64_t bytes, double* ms) {
+  return guard(h, [&](Ctx& c) {
+    c.requireDevice();
+    if (bytes < 8 || !ms) throw ApiError(NLA3D_GPU_EINVAL, "bad arguments");
+    const size_t n = bytes / 8;
+    DevBuf a, b;
+    a.alloc(n * 8);
+    b.alloc(n * 8);
+    hipCheck(hipMemset(a.p, 0, n * 8), "memset");
+    hipEvent_t e0, e1;
+    hipCheck(hipEventCreate(&e0), "event");
+    hipCheck(hipEventCreate(&e1), "event");
+    hipCheck(hipEventRecord(e0, c.stream), "rec");
+    launch_calibration_copy(c.stream, n, a.as<double>(), b.as<double>());
+    hipCheck(hipEventRecord(e1, c.stream), "rec");
+    hipCheck(hipEventSynchronize(e1), "sync");
+    float t = 0;
+    hipCheck(hipEventElapsedTime(&t, e0, e1), "elapsed");
+    *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  });
+}
+
 }  // extern "C"

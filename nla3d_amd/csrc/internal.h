@@ -51,6 +51,9 @@ struct RigidBodyDev {
 void launch_rigid_body_mpc(hipStream_t stream, const RigidBodyDev& rb, const double* xyz, const int* slaveNode,
                            const int* slaveEq, const double* U, const int* tgt, const int* fTgt, double* vals);
 
+// plain copy b[i] = a[i], i < n (byte-counter calibration)
+void launch_calibration_copy(hipStream_t s, size_t n, const double* a, double* b);
+
 // Library slot k (0..349) of the SOLID81 kernel output -> index in the
 // reference assemble3 order (SOLID81.h:54-94).
 void solid81_slot_to_assemble3(int* map350);
