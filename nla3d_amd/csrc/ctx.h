@@ -65,7 +65,9 @@ struct Ctx {
 
   // ---- block(2) Krylov data (solver.cpp) ----
   int n2 = 0;                 // nUnknown + nMpc
-  DevBuf d_rowptr2, d_col2, d_diagpos2, d_full2;  // full-symmetric CSR of block 2
+  // full-symmetric block 2 in grouped CSR (krylov.h)
+  DevBuf d_sgRow0, d_sgSize, d_sgColPtr, d_sgCols, d_sgValPtr, d_rowGroup, d_diagpos2, d_full2;
+  int nSpmvGroups = 0;
   DevBuf d_mir2;              // int [2*nnz2]: full positions of each upper entry of block 2
   int64_t fullNnz2 = 0;
   DevBuf d_gStart, d_gInfo, d_gDiagPos, d_binv;  // block-Jacobi groups

@@ -5,13 +5,20 @@
 
 namespace nla3d_amd {
 
-// full-symmetric CSR of K.block(2), 0-based
+// K.block(2) in full-symmetric "grouped CSR": consecutive rows with identical
+// column lists (the 3 DoF rows of a node) form a group that stores its column
+// list once and its values row by row (group value block = size x L).
 struct KrylovMatrix {
   int n = 0;
-  const int* rowptr = nullptr;   // [n+1]
-  const int* col = nullptr;      // [nnz]
-  const double* val = nullptr;   // [nnz]
-  const int* diagpos = nullptr;  // [n], position of (i,i) or -1 (multiplier rows have none)
+  int nGroups = 0;
+  const int* gRow0 = nullptr;    // [nGroups] first row of the group
+  const int* gSize = nullptr;    // [nGroups] rows in the group, 1..3
+  const int* gColPtr = nullptr;  // [nGroups+1] start of the group's columns in gCols
+  const int* gCols = nullptr;    // 0-based column indices
+  const int* gValPtr = nullptr;  // [nGroups] start of the group's values in val
+  const double* val = nullptr;
+  const int* diagpos = nullptr;  // [n] position of (i,i) in val, or -1 (multiplier rows have none)
+  const int* rowGroup = nullptr; // [n] group of row i
 };
 
 // block-Jacobi groups: each row belongs to exactly one group
@@ -34,9 +41,12 @@ struct MinresState {
 
 struct MinresWork {
   double *X, *R1, *R2, *T, *Yp, *Z, *W1, *W2;
-  double* partial;  // per-block partial sums
+  double* partial;  // per-block partial sums (kKrylovBlocks of them)
   MinresState* st;
 };
+
+// fixed grid of the grid-stride Krylov kernels (also the number of reduction partials)
+constexpr int kKrylovBlocks = 2048;
 
 void krylov_precond_setup(hipStream_t s, const KrylovMatrix& M, const Precond& P);
 void krylov_spmv(hipStream_t s, const KrylovMatrix& M, const double* x, double* y);

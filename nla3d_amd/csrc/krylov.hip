@@ -50,26 +50,28 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // ---- preconditioner set-up: one thread per group --------------------------
 __global__ __launch_bounds__(256) void bj_setup_kernel(int nGroups, const int* __restrict__ gStart,
                                                       const int* __restrict__ gInfo,
-                                                      const int* __restrict__ gDiagPos,
-                                                      const int* __restrict__ rowptr, const int* __restrict__ col,
-                                                      const int* __restrict__ diagpos,
-                                                      const double* __restrict__ A, double* __restrict__ binv) {
+                                                      const int* __restrict__ gDiagPos, KrylovMatrix M,
+                                                      double* __restrict__ binv) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nGroups) return;
   const int info = gInfo[g];
   const int size = info & 3, kind = info >> 2;
+  const double* A = M.val;
   double* out = binv + (size_t)g * 9;
   for (int k = 0; k < 9; k++) out[k] = 0.0;
   if (kind == 1) {  // pressure / multiplier row: diagonal Schur estimate
     const int r = gStart[g];
+    const int sg = M.rowGroup[r];
+    const int c0 = M.gColPtr[sg], L = M.gColPtr[sg + 1] - c0;
+    const int vb = M.gValPtr[sg] + (r - M.gRow0[sg]) * L;
     double d = 0.0;
-    for (int k = rowptr[r]; k < rowptr[r + 1]; k++) {
-      const int c = col[k];
-      const double a = A[k];
+    for (int k = 0; k < L; k++) {
+      const int c = M.gCols[c0 + k];
+      const double a = A[vb + k];
       if (c == r) {
         d += fabs(a);
       } else {
-        const int dp = diagpos[c];
+        const int dp = M.diagpos[c];
         const double ajj = dp >= 0 ? fabs(A[dp]) : 0.0;
         if (ajj > 0.0) d += a * a / ajj;
       }
@@ -121,47 +123,80 @@ __global__ __launch_bounds__(256) void bj_setup_kernel(int nGroups, const int* _
     }
 }
 
-// ---- SpMV with the MINRES epilogue ----------------------------------------
+// ---- grouped SpMV -----------------------------------------------------------
+// LPG lanes per group, grid-stride over groups.  Each lane walks the group's
+// column list with stride LPG, reads x[col] once and multiplies it into the
+// (up to 3) rows of the group; the row sums are reduced over the LPG lanes.
+template <int LPG>
+__device__ __forceinline__ void group_rowsums(const KrylovMatrix& M, int g, int lane, const double* __restrict__ x,
+                                              double& a0, double& a1, double& a2, int& r0, int& sz) {
+  const int c0 = M.gColPtr[g], L = M.gColPtr[g + 1] - c0;
+  const int vb = M.gValPtr[g];
+  sz = M.gSize[g];
+  r0 = M.gRow0[g];
+  a0 = a1 = a2 = 0.0;
+  const int* __restrict__ cols = M.gCols + c0;
+  const double* __restrict__ v0 = M.val + vb;
+  if (sz == 3) {
+    for (int k = lane; k < L; k += LPG) {
+      const double xv = x[cols[k]];
+      a0 += v0[k] * xv;
+      a1 += v0[L + k] * xv;
+      a2 += v0[2 * L + k] * xv;
+    }
+  } else {
+    for (int k = lane; k < L; k += LPG) {
+      const double xv = x[cols[k]];
+      a0 += v0[k] * xv;
+      if (sz > 1) a1 += v0[L + k] * xv;
+    }
+  }
+  for (int o = LPG / 2; o > 0; o >>= 1) {
+    a0 += __shfl_xor(a0, o, LPG);
+    a1 += __shfl_xor(a1, o, LPG);
+    a2 += __shfl_xor(a2, o, LPG);
+  }
+}
+
 // T = s * A Yp - c1 * R1 ; partial alfa = s * sum Yp . T
-// LPR lanes per row; rows processed in order within a block.
-template <int LPR>
-__global__ __launch_bounds__(256) void spmv_minres_kernel(int n, const int* __restrict__ rowptr,
-                                                         const int* __restrict__ col, const double* __restrict__ A,
-                                                         const double* __restrict__ Yp,
+template <int LPG>
+__global__ __launch_bounds__(256) void spmv_minres_kernel(KrylovMatrix M, const double* __restrict__ Yp,
                                                          const double* __restrict__ R1, double* __restrict__ T,
                                                          MinresState* __restrict__ st, double* __restrict__ partial) {
   __shared__ double sh[4];
   if (st->converged) return;
   const double s = st->s, c1 = st->c_r1;
-  const int lane = threadIdx.x % LPR;
-  const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  const int lane = threadIdx.x % LPG;
+  const int gpb = blockDim.x / LPG;
   double contrib = 0.0;
-  if (row < n) {
-    double acc = 0.0;
-    for (int k = rowptr[row] + lane; k < rowptr[row + 1]; k += LPR) acc += A[k] * Yp[col[k]];
-    for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
-    if (lane == 0) {
+  for (int g = blockIdx.x * gpb + threadIdx.x / LPG; g < M.nGroups; g += gridDim.x * gpb) {
+    double a0, a1, a2;
+    int r0, sz;
+    group_rowsums<LPG>(M, g, lane, Yp, a0, a1, a2, r0, sz);
+    if (lane < sz) {
+      const int row = r0 + lane;
+      const double acc = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
       const double t = s * acc - c1 * R1[row];
       T[row] = t;
-      contrib = s * Yp[row] * t;
+      contrib += s * Yp[row] * t;
     }
   }
   const double b = block_sum<256>(contrib, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
-// plain y = A x (used for residual checks and K12-type products on block 2)
-template <int LPR>
-__global__ __launch_bounds__(256) void spmv_kernel(int n, const int* __restrict__ rowptr, const int* __restrict__ col,
-                                                  const double* __restrict__ A, const double* __restrict__ x,
+// plain y = A x
+template <int LPG>
+__global__ __launch_bounds__(256) void spmv_kernel(KrylovMatrix M, const double* __restrict__ x,
                                                   double* __restrict__ y) {
-  const int lane = threadIdx.x % LPR;
-  const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
-  if (row >= n) return;
-  double acc = 0.0;
-  for (int k = rowptr[row] + lane; k < rowptr[row + 1]; k += LPR) acc += A[k] * x[col[k]];
-  for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
-  if (lane == 0) y[row] = acc;
+  const int lane = threadIdx.x % LPG;
+  const int gpb = blockDim.x / LPG;
+  for (int g = blockIdx.x * gpb + threadIdx.x / LPG; g < M.nGroups; g += gridDim.x * gpb) {
+    double a0, a1, a2;
+    int r0, sz;
+    group_rowsums<LPG>(M, g, lane, x, a0, a1, a2, r0, sz);
+    if (lane < sz) y[r0 + lane] = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
+  }
 }
 
 // ---- fixed-order finish of a reduction + MINRES scalar steps --------------
@@ -230,7 +265,7 @@ __global__ __launch_bounds__(kRedThreads) void minres_scalar2_kernel(MinresState
   }
 }
 
-// T -= c_r2 * R2 ; Z = M T ; partial beta2 = T . Z   (one thread per group)
+// T -= c_r2 * R2 ; Z = M T ; partial beta2 = T . Z   (one thread per group, grid-stride)
 __global__ __launch_bounds__(256) void precond_kernel(int nGroups, const int* __restrict__ gStart,
                                                      const int* __restrict__ gInfo, const double* __restrict__ binv,
                                                      double* __restrict__ T, const double* __restrict__ R2,
@@ -239,9 +274,8 @@ __global__ __launch_bounds__(256) void precond_kernel(int nGroups, const int* __
   __shared__ double sh[4];
   if (st->converged) return;
   const double c2 = st->c_r2;
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
   double contrib = 0.0;
-  if (g < nGroups) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nGroups; g += gridDim.x * blockDim.x) {
     const int r0 = gStart[g], size = gInfo[g] & 3;
     double t[3] = {0, 0, 0};
     for (int i = 0; i < size; i++) {
@@ -272,7 +306,7 @@ __global__ __launch_bounds__(256) void minres_update_kernel(int n, const double*
   X[i] += st->phi * wn;
 }
 
-// start: R1 = R2 = b (x0 = 0), Z = M b, partial = b . Z
+// start: R1 = R2 = b (x0 = 0), Z = M b, partial = b . Z   (grid-stride)
 __global__ __launch_bounds__(256) void precond_start_kernel(int nGroups, const int* __restrict__ gStart,
                                                            const int* __restrict__ gInfo,
                                                            const double* __restrict__ binv,
@@ -280,9 +314,8 @@ __global__ __launch_bounds__(256) void precond_start_kernel(int nGroups, const i
                                                            double* __restrict__ R2, double* __restrict__ Z,
                                                            double* __restrict__ partial) {
   __shared__ double sh[4];
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
   double contrib = 0.0;
-  if (g < nGroups) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nGroups; g += gridDim.x * blockDim.x) {
     const int r0 = gStart[g], size = gInfo[g] & 3;
     double t[3] = {0, 0, 0};
     for (int i = 0; i < size; i++) {
@@ -358,13 +391,13 @@ static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 void krylov_precond_setup(hipStream_t s, const KrylovMatrix& M, const Precond& P) {
   hipLaunchKernelGGL(bj_setup_kernel, dim3(cdiv(P.nGroups, 256)), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo,
-                     P.gDiagPos, M.rowptr, M.col, M.diagpos, M.val, P.binv);
+                     P.gDiagPos, M, P.binv);
 }
 
+constexpr int kLPG = 16;
+
 void krylov_spmv(hipStream_t s, const KrylovMatrix& M, const double* x, double* y) {
-  constexpr int LPR = 8;
-  hipLaunchKernelGGL(spmv_kernel<LPR>, dim3(cdiv(M.n * LPR, 256)), dim3(256), 0, s, M.n, M.rowptr, M.col, M.val, x,
-                     y);
+  hipLaunchKernelGGL(spmv_kernel<kLPG>, dim3(kKrylovBlocks), dim3(256), 0, s, M, x, y);
 }
 
 void krylov_dot(hipStream_t s, int n, const double* a, const double* b, double* partial, double* out) {
@@ -374,25 +407,21 @@ void krylov_dot(hipStream_t s, int n, const double* a, const double* b, double* 
 }
 
 void minres_start(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, const double* b) {
-  const int nbG = cdiv(P.nGroups, 256);
-  hipLaunchKernelGGL(precond_start_kernel, dim3(nbG), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo, P.binv, b,
-                     w.R1, w.R2, w.Yp, w.partial);
-  hipLaunchKernelGGL(minres_init_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, nbG);
+  hipLaunchKernelGGL(precond_start_kernel, dim3(kKrylovBlocks), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo,
+                     P.binv, b, w.R1, w.R2, w.Yp, w.partial);
+  hipLaunchKernelGGL(minres_init_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, kKrylovBlocks);
   // x0 = 0, w = w2 = 0 (a fill kernel rather than memset nodes, so the sequence can be graph-captured)
   hipLaunchKernelGGL(zero3_kernel, dim3(cdiv(M.n, 256)), dim3(256), 0, s, M.n, w.X, w.W1, w.W2);
 }
 
 // one MINRES iteration; buffers rotate on the host (pure pointer swaps)
 void minres_iteration(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, double tol) {
-  constexpr int LPR = 8;
-  const int nbS = cdiv(M.n * LPR, 256);
-  const int nbG = cdiv(P.nGroups, 256);
-  hipLaunchKernelGGL(spmv_minres_kernel<LPR>, dim3(nbS), dim3(256), 0, s, M.n, M.rowptr, M.col, M.val, w.Yp, w.R1,
-                     w.T, w.st, w.partial);
-  hipLaunchKernelGGL(minres_scalar1_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, nbS);
-  hipLaunchKernelGGL(precond_kernel, dim3(nbG), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo, P.binv, w.T, w.R2,
-                     w.Z, w.st, w.partial);
-  hipLaunchKernelGGL(minres_scalar2_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, nbG, tol);
+  hipLaunchKernelGGL(spmv_minres_kernel<kLPG>, dim3(kKrylovBlocks), dim3(256), 0, s, M, w.Yp, w.R1, w.T, w.st,
+                     w.partial);
+  hipLaunchKernelGGL(minres_scalar1_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, kKrylovBlocks);
+  hipLaunchKernelGGL(precond_kernel, dim3(kKrylovBlocks), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo, P.binv, w.T,
+                     w.R2, w.Z, w.st, w.partial);
+  hipLaunchKernelGGL(minres_scalar2_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, kKrylovBlocks, tol);
   hipLaunchKernelGGL(minres_update_kernel, dim3(cdiv(M.n, 256)), dim3(256), 0, s, M.n, w.Yp, w.W1, w.W2, w.X, w.st);
   // rotations: R1 <- R2, R2 <- T, T <- old R1 ; Yp <- Z ; (W1, W2) <- (W2, Wn)
   double* t = w.R1;

@@ -18,7 +18,7 @@ void Ctx::setupKrylov() {
   n2 = (int)(m.nAll() - nC);
   const CsrPattern& U2 = m.block2;
   const int64_t nnzU = (int64_t)U2.nnz();
-  // full-symmetric CSR: row i = (entries (j,i), j<i, from the upper rows j) ++ (upper row i)
+  // full-symmetric rows: row i = (entries (j,i), j<i, from the upper rows j) ++ (upper row i)
   std::vector<int> cntLower(n2, 0);
   for (int i = 0; i < n2; i++)
     for (uint32_t k = U2.iofeir[i] - 1; k < U2.iofeir[i + 1] - 1; k++) {
@@ -28,8 +28,9 @@ void Ctx::setupKrylov() {
   std::vector<int64_t> rp(n2 + 1, 0);
   for (int i = 0; i < n2; i++) rp[i + 1] = rp[i] + cntLower[i] + (U2.iofeir[i + 1] - U2.iofeir[i]);
   fullNnz2 = rp[n2];
-  if (fullNnz2 >= (1ll << 31)) throw ApiError(-1, "block(2) too large for 32-bit CSR offsets");
-  std::vector<int> col(fullNnz2), mir(2 * nnzU), diagpos(n2, -1);
+  if (fullNnz2 >= (1ll << 31)) throw ApiError(-1, "block(2) too large for 32-bit offsets");
+  std::vector<int> col(fullNnz2);
+  std::vector<int64_t> mirFull(2 * nnzU);  // full (row-major) positions of each upper entry
   std::vector<int> fillLower(n2, 0);
   for (int i = 0; i < n2; i++) {
     const int64_t base = rp[i] + cntLower[i];
@@ -37,24 +38,16 @@ void Ctx::setupKrylov() {
       const int c = (int)U2.columns[k] - 1;
       const int64_t pu = base + (k - (U2.iofeir[i] - 1));
       col[pu] = c;
-      mir[2 * k] = (int)pu;
+      mirFull[2 * k] = pu;
       if (c == i) {
-        mir[2 * k + 1] = (int)pu;
-        diagpos[i] = (int)pu;
+        mirFull[2 * k + 1] = pu;
       } else {
         const int64_t pl = rp[c] + fillLower[c]++;
         col[pl] = i;  // rows are visited in order, so each row's lower part stays sorted
-        mir[2 * k + 1] = (int)pl;
+        mirFull[2 * k + 1] = pl;
       }
     }
   }
-  std::vector<int> rp32(rp.begin(), rp.end());
-  d_rowptr2.upload(rp32);
-  d_col2.upload(col);
-  d_diagpos2.upload(diagpos);
-  d_mir2.upload(mir);
-  d_full2.alloc((size_t)fullNnz2 * 8);
-  hipCheck(hipMemset(d_full2.p, 0, d_full2.bytes), "memset full2");
 
   // what each block(2) row is: element DoF (pressure), node DoF, or MPC multiplier
   std::vector<int8_t> rowKind(n2, -1);  // 0 node, 1 schur (element DoF / multiplier)
@@ -75,6 +68,51 @@ void Ctx::setupKrylov() {
       }
     }
   for (auto& mp : m.mpcs) rowKind[mp.eqNum - nC - 1] = 1;
+  auto sameNodeTriple = [&](int a, int b) {
+    return rowKind[a] == 0 && rowKind[b] == 0 && rowNode[a] == rowNode[b] && rowType[a] / 3 == rowType[b] / 3;
+  };
+
+  // SpMV groups: up to 3 consecutive DoF rows of one node with identical column lists
+  std::vector<int> sgRow0, sgSize, sgColPtr{0}, sgCols, sgValPtr, rowGroup(n2);
+  std::vector<int64_t> fullToVal(fullNnz2);
+  int64_t valTotal = 0;
+  for (int i = 0; i < n2;) {
+    const int64_t L = rp[i + 1] - rp[i];
+    int sz = 1;
+    while (sz < 3 && i + sz < n2 && sameNodeTriple(i, i + sz) && rp[i + sz + 1] - rp[i + sz] == L &&
+           std::equal(col.begin() + rp[i], col.begin() + rp[i + 1], col.begin() + rp[i + sz]))
+      sz++;
+    const int g = (int)sgRow0.size();
+    sgRow0.push_back(i);
+    sgSize.push_back(sz);
+    sgValPtr.push_back((int)valTotal);
+    for (int64_t k = 0; k < L; k++) sgCols.push_back(col[rp[i] + k]);
+    sgColPtr.push_back((int)sgCols.size());
+    for (int r = 0; r < sz; r++) {
+      rowGroup[i + r] = g;
+      for (int64_t k = 0; k < L; k++) fullToVal[rp[i + r] + k] = valTotal + r * L + k;
+    }
+    valTotal += sz * L;
+    i += sz;
+  }
+  std::vector<int> mir(2 * nnzU), diagpos(n2, -1);
+  for (int64_t k = 0; k < 2 * nnzU; k++) mir[k] = (int)fullToVal[mirFull[k]];
+  for (int i = 0; i < n2; i++)
+    for (int64_t k = rp[i]; k < rp[i + 1]; k++)
+      if (col[k] == i) diagpos[i] = (int)fullToVal[k];
+  nSpmvGroups = (int)sgRow0.size();
+  d_sgRow0.upload(sgRow0);
+  d_sgSize.upload(sgSize);
+  d_sgColPtr.upload(sgColPtr);
+  d_sgCols.upload(sgCols);
+  d_sgValPtr.upload(sgValPtr);
+  d_rowGroup.upload(rowGroup);
+  d_diagpos2.upload(diagpos);
+  d_mir2.upload(mir);
+  d_full2.alloc((size_t)valTotal * 8);
+  hipCheck(hipMemset(d_full2.p, 0, d_full2.bytes), "memset full2");
+
+  // block-Jacobi groups
   // a node DoF row without a diagonal entry (element-less node) is treated like a multiplier row
   for (int i = 0; i < n2; i++)
     if (rowKind[i] == 0 && diagpos[i] < 0) rowKind[i] = 1;
@@ -82,7 +120,7 @@ void Ctx::setupKrylov() {
   auto findPos = [&](int r, int c) -> int {
     auto b = col.begin() + rp[r], e = col.begin() + rp[r + 1];
     auto it = std::lower_bound(b, e, c);
-    return (it != e && *it == c) ? (int)(it - col.begin()) : -1;
+    return (it != e && *it == c) ? (int)fullToVal[it - col.begin()] : -1;
   };
   for (int i = 0; i < n2;) {
     if (rowKind[i] != 0) {
@@ -94,9 +132,7 @@ void Ctx::setupKrylov() {
     }
     // consecutive DoFs of the same node within one of the triples {UX,UY,UZ}, {ROTX,ROTY,ROTZ}
     int sz = 1;
-    while (sz < 3 && i + sz < n2 && rowKind[i + sz] == 0 && rowNode[i + sz] == rowNode[i] &&
-           rowType[i + sz] / 3 == rowType[i] / 3)
-      sz++;
+    while (sz < 3 && i + sz < n2 && sameNodeTriple(i, i + sz)) sz++;
     gStart.push_back(i);
     gInfo.push_back(sz);
     for (int a = 0; a < 3; a++)
@@ -152,7 +188,7 @@ void Ctx::setupKrylov() {
   d_rhs.alloc((size_t)n2 * 8);
   d_dU.alloc((size_t)m.nAll() * 8);
   for (auto& b : d_mw) b.alloc((size_t)n2 * 8);
-  const int nPart = std::max((n2 * 8 + 255) / 256, (nGroups + 255) / 256) + 16;
+  const int nPart = std::max(kKrylovBlocks, (std::max(n2, nGroups) + 255) / 256) + 16;
   d_partial.alloc((size_t)nPart * 8);
   d_mstate.alloc(sizeof(MinresState));
   d_scal.alloc(16 * 8);
@@ -173,10 +209,15 @@ void Ctx::setupKrylov() {
 KrylovMatrix Ctx::krylovMatrix() const {
   KrylovMatrix M;
   M.n = n2;
-  M.rowptr = d_rowptr2.as<int>();
-  M.col = d_col2.as<int>();
+  M.nGroups = nSpmvGroups;
+  M.gRow0 = d_sgRow0.as<int>();
+  M.gSize = d_sgSize.as<int>();
+  M.gColPtr = d_sgColPtr.as<int>();
+  M.gCols = d_sgCols.as<int>();
+  M.gValPtr = d_sgValPtr.as<int>();
   M.val = d_full2.as<double>();
   M.diagpos = d_diagpos2.as<int>();
+  M.rowGroup = d_rowGroup.as<int>();
   return M;
 }
 
