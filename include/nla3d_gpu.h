@@ -156,6 +156,35 @@ int nla3d_gpu_solve_nonlinear(nla3d_gpu_t h, const nla3d_gpu_newton_opts* opts, 
 int nla3d_gpu_solve_block2(nla3d_gpu_t h, const double* rhs, double* x, double tol, int max_iter, int* iters,
                            double* relres);
 
+/* ---- thermal path (QUADTH / SurfaceLINETH) -------------------------------- */
+/* Options of LinearTransientFESolver (FESolver.h:223-242) and of the CG solve. */
+typedef struct {
+  double alpha, delta;      /* Newmark parameters (0.25, 0.5: trapezoidal rule) */
+  double time0, time1;      /* time interval */
+  int n_timesteps;          /* numberOfTimesteps; the loop runs while curTime <= time1 (FESolver.cpp:604) */
+  double init_value;        /* initValue: every DoF of vecU at t = time0 (FESolver.cpp:577-579) */
+  int dirichlet_coupling;   /* 0: as the reference, the rhs has no -K12^T Uc term (FESolver.cpp:606-608);
+                               1: the coupling of prescribed temperatures is included */
+  double krylov_tol;        /* CG: preconditioned residual <= tol * ||b||_M */
+  int krylov_max_iter;
+} nla3d_gpu_transient_opts;
+
+/* FEStorage::assembleGlobalEqMatrices with transient matrices for QUADTH and
+ * SurfaceLINETH (QUADTH.cpp:22-125): K, C and F on the device. */
+int nla3d_gpu_assemble_thermal(nla3d_gpu_t h);
+/* C block values (block = 1, 12, 2), as nla3d_gpu_get_K */
+int nla3d_gpu_get_C(nla3d_gpu_t h, int block, double* values);
+/* LinearFESolver::solve (FESolver.cpp:337-397) with CG + Jacobi on K.block(2). */
+int nla3d_gpu_solve_linear(nla3d_gpu_t h, double krylov_tol, int krylov_max_iter, nla3d_gpu_newton_stats* stats);
+/* reference defaults: alpha 0.25, delta 0.5, [0, 1], 100 steps, init 0, no coupling; tol 1e-12 */
+int nla3d_gpu_transient_defaults(nla3d_gpu_transient_opts* opts);
+/* LinearTransientFESolver::solve, split for callers that read results per step:
+ * begin (initial field, assembly, BCs, Kmod = a1 C + K), then one call per time
+ * step (*done = 1 once curTime > time1), or the whole loop at once. */
+int nla3d_gpu_transient_begin(nla3d_gpu_t h, const nla3d_gpu_transient_opts* opts);
+int nla3d_gpu_transient_step(nla3d_gpu_t h, nla3d_gpu_newton_stats* stats, int* done);
+int nla3d_gpu_solve_transient(nla3d_gpu_t h, const nla3d_gpu_transient_opts* opts, nla3d_gpu_newton_stats* stats);
+
 /* ---- timing -------------------------------------------------------------- */
 /* Runs `iters` device assemblies on the resident solution and returns the mean
  * duration in ms of the element kernel and of the scatter kernel, measured with

@@ -30,6 +30,14 @@ class NewtonOpts(ctypes.Structure):
                 ("krylov_max_iter", ctypes.c_int), ("preconditioner", ctypes.c_int)]
 
 
+class TransientOpts(ctypes.Structure):
+    """nla3d_gpu_transient_opts (include/nla3d_gpu.h)"""
+    _fields_ = [("alpha", ctypes.c_double), ("delta", ctypes.c_double), ("time0", ctypes.c_double),
+                ("time1", ctypes.c_double), ("n_timesteps", ctypes.c_int), ("init_value", ctypes.c_double),
+                ("dirichlet_coupling", ctypes.c_int), ("krylov_tol", ctypes.c_double),
+                ("krylov_max_iter", ctypes.c_int)]
+
+
 class NewtonStats(ctypes.Structure):
     """nla3d_gpu_newton_stats (include/nla3d_gpu.h)"""
     _fields_ = [("load_steps", ctypes.c_int), ("newton_iterations", ctypes.c_int),
@@ -80,8 +88,17 @@ ABI = {
                                    ctypes.POINTER(NewtonStats), _IP],
     "nla3d_gpu_solve_block2": [_P, _DP, _DP, ctypes.c_double, ctypes.c_int, _IP, _DP],
 }
+ABI.update({
+    "nla3d_gpu_assemble_thermal": [_P],
+    "nla3d_gpu_get_C": [_P, ctypes.c_int, _DP],
+    "nla3d_gpu_solve_linear": [_P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(NewtonStats)],
+    "nla3d_gpu_transient_begin": [_P, ctypes.POINTER(TransientOpts)],
+    "nla3d_gpu_transient_step": [_P, ctypes.POINTER(NewtonStats), _IP],
+    "nla3d_gpu_solve_transient": [_P, ctypes.POINTER(TransientOpts), ctypes.POINTER(NewtonStats)],
+})
 # functions that take no handle
-ABI_FREE = {"nla3d_gpu_newton_defaults": [ctypes.POINTER(NewtonOpts)]}
+ABI_FREE = {"nla3d_gpu_newton_defaults": [ctypes.POINTER(NewtonOpts)],
+            "nla3d_gpu_transient_defaults": [ctypes.POINTER(TransientOpts)]}
 
 
 def lib():
@@ -276,6 +293,42 @@ class GpuModel:
         a = ctypes.c_double()
         self._call("nla3d_gpu_copy_calibration", int(nbytes), ctypes.byref(a))
         return a.value
+
+    # ---- thermal path ----
+    def assemble_thermal(self):
+        self._call("nla3d_gpu_assemble_thermal")
+
+    def get_C(self, block):
+        nnz = self.sizes[{1: "nnz1", 12: "nnz12", 2: "nnz2"}[block]]
+        v = np.zeros(max(nnz, 1))
+        self._call("nla3d_gpu_get_C", block, _ptr(v, ctypes.c_double))
+        return v[:nnz]
+
+    def solve_linear(self, tol=1e-12, max_iter=100000):
+        st = NewtonStats()
+        self._call("nla3d_gpu_solve_linear", float(tol), int(max_iter), ctypes.byref(st))
+        return st
+
+    @staticmethod
+    def transient_defaults(**kw):
+        o = TransientOpts()
+        lib().nla3d_gpu_transient_defaults(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+    def transient_begin(self, opts):
+        self._call("nla3d_gpu_transient_begin", ctypes.byref(opts))
+
+    def transient_step(self, stats):
+        done = ctypes.c_int()
+        self._call("nla3d_gpu_transient_step", ctypes.byref(stats), ctypes.byref(done))
+        return not done.value
+
+    def solve_transient(self, opts, stats=None):
+        stats = stats or NewtonStats()
+        self._call("nla3d_gpu_solve_transient", ctypes.byref(opts), ctypes.byref(stats))
+        return stats
 
     def time_spmv(self, iters):
         a = ctypes.c_double()

@@ -56,6 +56,8 @@ static void parallelFor(size_t n, const std::function<void(size_t, size_t)>& fn)
 // Build the SOLID81 (element, slot) -> target map and sort it by target
 // (stable in element order): the precomputed element-to-CSR index map of
 // the deterministic scatter.
+void setupThermal(Ctx& c);  // thermal.cpp
+
 void buildSolid81ScatterMap(Ctx& c) {
   FEStorage& m = c.model;
   const uint32_t nS = m.solidCount;
@@ -279,6 +281,7 @@ int nla3d_gpu_init_solution(nla3d_gpu_t h) {
     if (c.device >= 0) {
       c.setupDevice();
       c.setupKrylov();
+      setupThermal(c);
     }
     c.initialized = true;
   });

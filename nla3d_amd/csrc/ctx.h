@@ -2,6 +2,7 @@
 // buffers and the assembly pipeline.
 #pragma once
 
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -38,6 +39,8 @@ struct DevBuf {
     if (!v.empty()) hipCheck(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
   }
 };
+
+struct ThermalData;  // thermal.cpp
 
 struct Ctx {
   int device = 0;
@@ -82,6 +85,8 @@ struct Ctx {
   int nMpcMir = 0;
   MinresState* h_mstate = nullptr;  // pinned host mirror of the MINRES state
   bool krylovReady = false;
+
+  std::shared_ptr<ThermalData> thermal;  // QUADTH / SurfaceLINETH data (thermal.cpp)
 
   // rigid-body MPC collections
   struct RB {

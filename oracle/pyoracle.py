@@ -528,3 +528,95 @@ class OracleModel:
                 if crit > 1e6 or crit != crit:
                     raise RuntimeError("The solution is diverged!")
         raise RuntimeError("The solution is not converged")
+
+    # ---------------- thermal (QUADTH / SurfaceLINETH) ----------------
+    def assemble_thermal(self):
+        """K, C, F of FEStorage::assembleGlobalEqMatrices with transient matrices (FEStorage.cpp:26-78):
+        buildK / buildC of every element in element order (QUADTH.cpp:22-125)"""
+        for b in self.blocks.values():
+            b.values[:] = 0.0
+        Cvals = {b: np.zeros_like(blk.values) for b, blk in self.blocks.items()}
+        F = np.zeros(self.nAll)
+        for e, (kind, nodes) in enumerate(self.elements):
+            p = self.elem_params[e]
+            X = self.xyz[np.array(nodes) - 1]
+            eq = [self.nodeEq[n - 1][TEMP] for n in nodes]
+            if kind == QUADTH:
+                Ke, Fe, Ce = quadth_element(X, p[0], p[1], p[2], p[3])
+                nn = 4
+            elif kind == SURFACE_LINETH:
+                Ke, Fe = lineth_element(X, p[0], p[1], p[2], p[3])
+                Ce = None
+                nn = 2
+            else:
+                continue
+            k = 0
+            for i in range(nn):  # Element::assembleK (element.h:356-386)
+                for j in range(i, nn):
+                    self.addValueK(eq[i], eq[j], Ke[k])
+                    k += 1
+            for i in range(nn):
+                F[eq[i] - 1] += Fe[i]
+            if Ce is not None:  # Element::assembleC (element.h:306-328)
+                k = 0
+                for i in range(nn):
+                    for j in range(i, nn):
+                        a, b = sorted((eq[i], eq[j]))
+                        bi, pi = self._block(a)
+                        bj, pj = self._block(b)
+                        blk = 1 if (bi == bj == 1) else (2 if bi == bj == 2 else 12)
+                        idx = self.blocks[blk].getIndex(pi, pj)
+                        Cvals[blk][idx] += Ce[k]
+                        k += 1
+        return {b: blk.values.copy() for b, blk in self.blocks.items()}, Cvals, F
+
+    def _apply_bc(self, U, R, t):
+        for node, dof, val in self.loads:
+            R[self.nodeEq[node - 1][dof] - 1] += val * t
+        for node, dof, val in self.fixes:
+            U[self.nodeEq[node - 1][dof] - 1] = val * t
+
+    def solve_linear(self):
+        """LinearFESolver::solve (FESolver.cpp:337-397) with a dense solve; returns vecU"""
+        nC = self.nC
+        K, C, F = self.assemble_thermal()
+        U = np.zeros(self.nAll)
+        R = np.zeros(self.nAll)
+        self._apply_bc(U, R, 1.0)
+        K12, K2 = self.dense_blocks(K)
+        rhs = F[nC:] + R[nC:] - K12.T @ U[:nC]
+        U[nC:] = np.linalg.solve(K2, rhs)
+        return U
+
+    def solve_transient(self, time0=0.0, time1=1.0, n_timesteps=100, init_value=0.0, alpha=0.25, delta=0.5,
+                        dirichlet_coupling=False, probe=None):
+        """LinearTransientFESolver::solve (FESolver.cpp:532-642) with dense solves.
+        Returns (U after every step as a list of vectors if probe is None, else the probe eq history)."""
+        nC = self.nC
+        dt = (time1 - time0) / n_timesteps
+        cur = time0 + dt
+        a0 = 1.0 / (alpha * dt * dt); a1 = delta / (alpha * dt); a2 = 1.0 / (alpha * dt)
+        a3 = 1.0 / (2.0 * alpha) - 1.0; a4 = delta / alpha - 1.0; a5 = dt / 2.0 * (delta / alpha - 2.0)
+        a6 = dt * (1.0 - delta); a7 = delta * dt
+        U = np.full(self.nAll, init_value)
+        DU = np.zeros(self.nAll)
+        DDU = np.zeros(self.nAll)
+        K, Cv, F = self.assemble_thermal()
+        R = np.zeros(self.nAll)
+        self._apply_bc(U, R, 1.0)
+        K12, K2 = self.dense_blocks(K)
+        _, C2 = self.dense_blocks(Cv)
+        Kmod = a1 * C2 + K2  # M = 0 for QUADTH
+        out = []
+        while cur <= time1:
+            rhs = F[nC:] + R[nC:] + C2 @ (a1 * U[nC:] + a4 * DU[nC:] + a5 * DDU[nC:])
+            if dirichlet_coupling:
+                rhs = rhs - K12.T @ U[:nC]
+            Un = U.copy()
+            Un[nC:] = np.linalg.solve(Kmod, rhs)
+            DDUn = a0 * (Un - U) - a2 * DU - a3 * DDU
+            DUn = DU + a6 * DDUn + a7 * DDU
+            U, DU, DDU = Un, DUn, DDUn
+            out.append(U[probe] if probe is not None else U.copy())
+            cur += dt
+        return out
