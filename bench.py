@@ -56,6 +56,21 @@ def spmv_algorithmic_bytes(sizes):
     return 12 * sizes["nnz2"] + 4 * (n + 1) + 16 * n
 
 
+def profiled_traffic(kernel, shape):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary of this
+    workload (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy of known size);
+    None for another shape or when the summary is absent"""
+    path = os.path.join(ROOT, "profiles", "spmv_block2_120x120x60_hbm_traffic.txt")
+    if shape != (120, 120, 60) or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return float(parts[2])
+    return None
+
+
 def cpu_baseline(m, xyz, conn, U, n_sample):
     from oracle import pyoracle
     ne, ee = m.equation_numbers()
@@ -181,7 +196,8 @@ def main():
     elem_tf = FLOP_PER_ELEMENT * nE / (ms_elem * 1e-3) / 1e12
     if args.mode == "newton":
         roofline = {"bound": "hbm", "achieved": spmv_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": spmv_gbs / PEAK_HBM_GBS, "traffic": None, "kernel": "spmv_kernel (block(2) SpMV)",
+                    "frac": spmv_gbs / PEAK_HBM_GBS, "traffic": profiled_traffic("spmv_kernel<16>", (nx, ny, nz)),
+                    "kernel": "spmv_kernel<16> (block(2) SpMV)",
                     "kernel_ms": ms_spmv, "algorithmic_bytes": spmv_bytes}
     else:
         roofline = {"bound": "mfma", "achieved": elem_tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",

@@ -29,9 +29,11 @@ def heat(rho=0.0, c=0.0):
     m.add_elements(nla3d_amd.SURFACE_LINETH, lines, lp)
     for n, dof, v in d["loads"]:
         m.add_load(n, dof, v)
+    for n, dof, v in d["fixes"]:
+        m.add_fix(n, dof, v)
     m.init_solution()
     elements = [(2, q) for q in quads] + [(3, l) for l in lines]
-    o = OracleModel(d["nodes"], elements, loads=d["loads"], elem_params=qp + lp).setup()
+    o = OracleModel(d["nodes"], elements, loads=d["loads"], fixes=d["fixes"], elem_params=qp + lp).setup()
     return m, o, d
 
 
