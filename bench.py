@@ -101,6 +101,10 @@ def main():
     ap.add_argument("--shape", type=str, default="120,120,60")
     ap.add_argument("--load-steps", type=int, default=20)
     ap.add_argument("--krylov-tol", type=float, default=1e-9)
+    ap.add_argument("--preconditioner", choices=["learned", "block-jacobi"], default="learned")
+    ap.add_argument("--train-pairs", type=int, default=2000)
+    ap.add_argument("--train-k", type=int, default=100)
+    ap.add_argument("--train-lr", type=float, default=1e-4)
     ap.add_argument("--cpu-sample", type=int, default=250000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -156,6 +160,19 @@ def main():
         st = nla3d_amd.NewtonStats()
         rhs1 = ctypes.c_double(0.0)
         state = {"k": 0, "it": 0}
+        net = None
+        if args.preconditioner == "learned":
+            from nla3d_amd import learned
+            net = nla3d_amd.Mlp(0, seed=20261203)
+            val = learned.validation_system(24)
+            info = learned.train(net, n_pairs=args.train_pairs, k=args.train_k, lr=args.train_lr,
+                                 validation=[val], log=lambda msg: print(msg, file=sys.stderr, flush=True))
+            val[0].close()
+            m.attach_mlp(net)
+            opts.preconditioner = 1
+            extra["training"] = {"pairs": info["pairs"], "seconds": info["seconds"], "k": args.train_k,
+                                 "lr": args.train_lr, "validation_iterations": info["validation"],
+                                 "validation_system": "24x24x12 rigid-top box, first Newton iteration at t=0.5"}
 
         def one_solving_iteration():
             while True:
@@ -170,9 +187,20 @@ def main():
                 if st.linear_solves > before:
                     return st.last_krylov_iterations
 
-        for _ in range(args.warmup):
+        for w in range(args.warmup):
             kit = one_solving_iteration()
             print(f"[bench] warmup iteration: {kit} Krylov iterations", file=sys.stderr, flush=True)
+            if net is not None and w == 0:
+                # block Jacobi judges the learned corrections on the system just solved
+                from nla3d_amd import learned
+                it_bj, it_l, diff = learned.judge(m, net, m.get_rhs(), tol=args.krylov_tol, t=sched[state["k"]])
+                print(f"[bench] judge: block Jacobi {it_bj}, learned {it_l} Krylov iterations", file=sys.stderr,
+                      flush=True)
+                extra["judge"] = {"block_jacobi_iterations": it_bj, "learned_iterations": it_l,
+                                  "solution_difference": diff,
+                                  "system": "tangent and rhs of the first Newton iteration of load step 1"}
+                if it_l >= it_bj:
+                    opts.preconditioner = 0  # rejected: plain block Jacobi
         kits = []
         ms_asm0, ms_sol0 = st.ms_assemble, st.ms_solve
         t0 = time.perf_counter()
@@ -182,10 +210,12 @@ def main():
                   file=sys.stderr, flush=True)
         t1 = time.perf_counter()
         ms_step = (t1 - t0) * 1e3 / args.steps
-        extra = {"krylov_iterations_per_newton": float(np.mean(kits)),
+        extra.update({"krylov_iterations_per_newton": float(np.mean(kits)),
                  "ms_krylov_per_iteration": (st.ms_solve - ms_sol0) / max(1, sum(kits)),
                  "ms_assemble_per_newton": (st.ms_assemble - ms_asm0) / args.steps,
-                 "load_step": state["k"] + 1, "preconditioner": "block-Jacobi"}
+                 "load_step": state["k"] + 1,
+                 "preconditioner": "learned" if opts.preconditioner == 1 else "block-Jacobi",
+                 "precond_fallbacks": st.precond_fallbacks})
     value = nE * 1e3 / ms_step
     print(f"[bench] {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
 

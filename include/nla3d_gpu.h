@@ -116,7 +116,9 @@ typedef struct {
   double convergence;    /* c: convergenceCriteria (reference default 1e-3) or the residual ratio */
   double krylov_tol;     /* MINRES: preconditioned residual <= krylov_tol * initial */
   int krylov_max_iter;
-  int preconditioner;    /* 0: block Jacobi */
+  int preconditioner;    /* 0: block Jacobi; 1: block Jacobi with the learned node-block corrections of the
+                            network attached by nla3d_gpu_attach_mlp (a solve that breaks down or does not
+                            reach krylov_tol is repeated with plain block Jacobi) */
 } nla3d_gpu_newton_opts;
 
 typedef struct {
@@ -130,6 +132,7 @@ typedef struct {
   double last_criterion;     /* value of the convergence measure at exit */
   double last_time;          /* normalised time of the last completed step */
   double ms_assemble, ms_solve, ms_total;  /* wall time (host clock around device work) */
+  int precond_fallbacks;     /* learned-preconditioner solves repeated with block Jacobi */
 } nla3d_gpu_newton_stats;
 
 /* Fills opts with the reference defaults (20 iterations, 10 load steps,
@@ -155,6 +158,63 @@ int nla3d_gpu_solve_nonlinear(nla3d_gpu_t h, const nla3d_gpu_newton_opts* opts, 
  * length nUnknown + nMpc.  iters / relres (preconditioned estimate) may be NULL. */
 int nla3d_gpu_solve_block2(nla3d_gpu_t h, const double* rhs, double* x, double tol, int max_iter, int* iters,
                            double* relres);
+/* The right-hand side of block(2) of the last Newton iteration (FESolver.cpp:454-457:
+ * F_sl + R_sl - K12^T dU_c), host array of length nUnknown + nMpc; with the tangent
+ * still assembled it is the system that iteration solved. */
+int nla3d_gpu_get_rhs(nla3d_gpu_t h, double* rhs);
+/* As nla3d_gpu_solve_block2 with a choice of preconditioner: 0 block Jacobi, 1 learned
+ * corrections (attached network; t is the load-step time fed to it).  No fallback here:
+ * this is the entry that compares the two. */
+int nla3d_gpu_solve_block2_pc(nla3d_gpu_t h, const double* rhs, double* x, double tol, int max_iter,
+                              int preconditioner, double t, int* iters, double* relres);
+
+/* ---- learned block-Jacobi correction --------------------------------------
+ * No counterpart in the reference, whose only solver is PARDISO
+ * (EquationSolver.cpp:158-249): a per-node MLP 24-128-128-128-128-6 (GELU)
+ * maps features of a node's rows of the tangent to the lower-triangular
+ * factor C (exp on its diagonal) of a correction of that node's block-Jacobi
+ * block: P_i = L^-T C C^T L^-1 with K_ii = L L^T.  C = I (all outputs zero,
+ * the state of a new network) is block Jacobi.  Inference runs in BF16 on
+ * MFMA with FP32 accumulation; training runs FP32 with Adam on the loss
+ * mean_j log(phi_j / phi_{j-1}) of k steps of the preconditioned MINRES
+ * recurrence, differentiated in reverse through the recurrence. */
+typedef struct nla3d_gpu_mlp* nla3d_gpu_mlp_t;
+typedef struct {
+  float lr, beta1, beta2, eps;
+} nla3d_gpu_adam_opts;
+
+int nla3d_gpu_mlp_num_params(void); /* 53,510 FP32 values: W1 b1 W2 b2 W3 b3 W4 b4 W5 b5, W row-major [out][in] */
+/* A new network on `device`: Glorot-uniform hidden weights from splitmix64(seed), zero biases and
+ * a zero output layer. */
+int nla3d_gpu_mlp_create(nla3d_gpu_mlp_t* mlp, int device, unsigned long long seed);
+int nla3d_gpu_mlp_destroy(nla3d_gpu_mlp_t mlp);
+const char* nla3d_gpu_mlp_last_error(nla3d_gpu_mlp_t mlp);
+int nla3d_gpu_mlp_get_params(nla3d_gpu_mlp_t mlp, float* w, int n);
+int nla3d_gpu_mlp_set_params(nla3d_gpu_mlp_t mlp, const float* w, int n);
+/* input normalisation x' = (x - mean) * inv_std, 24 values each (default 0 and 1) */
+int nla3d_gpu_mlp_set_normalization(nla3d_gpu_mlp_t mlp, const float* mean, const float* inv_std);
+int nla3d_gpu_mlp_get_normalization(nla3d_gpu_mlp_t mlp, float* mean, float* inv_std);
+/* Y[n x 6] = MLP(X[n x 24]) on host arrays; bf16_mfma = 1: the BF16 MFMA inference kernel,
+ * 0: the FP32 training forward */
+int nla3d_gpu_mlp_forward(nla3d_gpu_mlp_t mlp, const float* X, int n, float* Y, int bf16_mfma);
+/* dW = d(sum(dY . Y))/dparams through the FP32 forward of X (host arrays; dW holds 53,510) */
+int nla3d_gpu_mlp_backward(nla3d_gpu_mlp_t mlp, const float* X, const float* dY, int n, float* dW);
+/* Attach (NULL: detach) the network used by preconditioner 1; not owned by the model. */
+int nla3d_gpu_attach_mlp(nla3d_gpu_t h, nla3d_gpu_mlp_t mlp);
+/* Features of every learned node (free UX UY UZ triple) of the current tangent:
+ * X[n x 24] and the node's first block(2) row (0-based).  *n_out = node count;
+ * with cap < that count only the count is returned. */
+int nla3d_gpu_node_features(nla3d_gpu_t h, double t, float* X, int* node_rows, int cap, int* n_out);
+/* The corrected 3x3 blocks (row-major, 9 per node) for outputs Y[n x 6] given by the caller. */
+int nla3d_gpu_learned_blocks(nla3d_gpu_t h, const float* Y, int n, double* blocks);
+/* Loss of k unrolled MINRES steps for rhs (host, length nUnknown + nMpc; NULL: the last
+ * Newton right-hand side) with the current tangent, and its gradient (53,510, may be NULL). */
+int nla3d_gpu_mlp_loss_grad(nla3d_gpu_mlp_t mlp, nla3d_gpu_t h, const double* rhs, int k, double t, double* loss,
+                            float* grad);
+void nla3d_gpu_adam_defaults(nla3d_gpu_adam_opts* opts); /* lr 1e-3, 0.9, 0.999, 1e-8 */
+/* One Adam step on that gradient. */
+int nla3d_gpu_mlp_train_step(nla3d_gpu_mlp_t mlp, nla3d_gpu_t h, const double* rhs, int k, double t,
+                             const nla3d_gpu_adam_opts* opts, double* loss);
 
 /* ---- thermal path (QUADTH / SurfaceLINETH) -------------------------------- */
 /* Options of LinearTransientFESolver (FESolver.h:223-242) and of the CG solve. */

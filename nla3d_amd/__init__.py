@@ -45,10 +45,16 @@ class NewtonStats(ctypes.Structure):
                 ("krylov_iterations", ctypes.c_longlong), ("last_krylov_iterations", ctypes.c_int),
                 ("krylov_breakdown", ctypes.c_int),
                 ("last_criterion", ctypes.c_double), ("last_time", ctypes.c_double),
-                ("ms_assemble", ctypes.c_double), ("ms_solve", ctypes.c_double), ("ms_total", ctypes.c_double)]
+                ("ms_assemble", ctypes.c_double), ("ms_solve", ctypes.c_double), ("ms_total", ctypes.c_double),
+                ("precond_fallbacks", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class AdamOpts(ctypes.Structure):
+    """nla3d_gpu_adam_opts (include/nla3d_gpu.h)"""
+    _fields_ = [("lr", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("eps", ctypes.c_float)]
 
 
 _P = ctypes.c_void_p
@@ -96,9 +102,32 @@ ABI.update({
     "nla3d_gpu_transient_step": [_P, ctypes.POINTER(NewtonStats), _IP],
     "nla3d_gpu_solve_transient": [_P, ctypes.POINTER(TransientOpts), ctypes.POINTER(NewtonStats)],
 })
+_FP = ctypes.POINTER(ctypes.c_float)
+# learned block-Jacobi correction (first argument: network or model handle)
+ABI.update({
+    "nla3d_gpu_mlp_destroy": [_P],
+    "nla3d_gpu_mlp_get_params": [_P, _FP, ctypes.c_int],
+    "nla3d_gpu_mlp_set_params": [_P, _FP, ctypes.c_int],
+    "nla3d_gpu_mlp_set_normalization": [_P, _FP, _FP],
+    "nla3d_gpu_mlp_get_normalization": [_P, _FP, _FP],
+    "nla3d_gpu_mlp_forward": [_P, _FP, ctypes.c_int, _FP, ctypes.c_int],
+    "nla3d_gpu_mlp_backward": [_P, _FP, _FP, ctypes.c_int, _FP],
+    "nla3d_gpu_attach_mlp": [_P, _P],
+    "nla3d_gpu_node_features": [_P, ctypes.c_double, _FP, _IP, ctypes.c_int, _IP],
+    "nla3d_gpu_learned_blocks": [_P, _FP, ctypes.c_int, _DP],
+    "nla3d_gpu_mlp_loss_grad": [_P, _P, _DP, ctypes.c_int, ctypes.c_double, _DP, _FP],
+    "nla3d_gpu_mlp_train_step": [_P, _P, _DP, ctypes.c_int, ctypes.c_double, ctypes.POINTER(AdamOpts), _DP],
+    "nla3d_gpu_get_rhs": [_P, _DP],
+    "nla3d_gpu_solve_block2_pc": [_P, _DP, _DP, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, _IP,
+                                  _DP],
+})
 # functions that take no handle
 ABI_FREE = {"nla3d_gpu_newton_defaults": [ctypes.POINTER(NewtonOpts)],
-            "nla3d_gpu_transient_defaults": [ctypes.POINTER(TransientOpts)]}
+            "nla3d_gpu_transient_defaults": [ctypes.POINTER(TransientOpts)],
+            "nla3d_gpu_mlp_num_params": [],
+            "nla3d_gpu_mlp_create": [ctypes.POINTER(_P), ctypes.c_int, ctypes.c_ulonglong],
+            "nla3d_gpu_adam_defaults": [ctypes.POINTER(AdamOpts)],
+            "nla3d_gpu_mlp_last_error": [_P]}
 
 
 def lib():
@@ -118,6 +147,7 @@ def lib():
             f.restype = ctypes.c_int
         L.nla3d_gpu_last_error.argtypes = [_P]
         L.nla3d_gpu_last_error.restype = ctypes.c_char_p
+        L.nla3d_gpu_mlp_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -280,6 +310,40 @@ class GpuModel:
         self._call("nla3d_gpu_solve_nonlinear", ctypes.byref(opts), ctypes.byref(stats))
         return stats
 
+    # ---- learned block-Jacobi correction ----
+    def attach_mlp(self, mlp):
+        self._call("nla3d_gpu_attach_mlp", mlp._h if mlp is not None else None)
+        self._mlp = mlp  # keep the network alive while attached
+
+    def node_features(self, t=1.0):
+        n = ctypes.c_int()
+        self._call("nla3d_gpu_node_features", float(t), None, None, 0, ctypes.byref(n))
+        X = np.zeros((max(n.value, 1), 24), dtype=np.float32)
+        rows = np.zeros(max(n.value, 1), dtype=np.int32)
+        self._call("nla3d_gpu_node_features", float(t), _ptr(X, ctypes.c_float), _ptr(rows, ctypes.c_int), n.value,
+                   ctypes.byref(n))
+        return X[:n.value], rows[:n.value]
+
+    def learned_blocks(self, Y):
+        Y = np.ascontiguousarray(Y, dtype=np.float32).reshape(-1, 6)
+        B = np.zeros((len(Y), 9))
+        self._call("nla3d_gpu_learned_blocks", _ptr(Y, ctypes.c_float), len(Y), _ptr(B, ctypes.c_double))
+        return B.reshape(-1, 3, 3)
+
+    def get_rhs(self):
+        r = np.zeros(self.sizes["nUnknown"] + self.sizes["nMpc"])
+        self._call("nla3d_gpu_get_rhs", _ptr(r, ctypes.c_double))
+        return r
+
+    def solve_block2_pc(self, rhs, tol=1e-12, max_iter=20000, preconditioner=0, t=1.0):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.zeros_like(rhs)
+        it = ctypes.c_int()
+        rr = ctypes.c_double()
+        self._call("nla3d_gpu_solve_block2_pc", _ptr(rhs, ctypes.c_double), _ptr(x, ctypes.c_double), float(tol),
+                   int(max_iter), int(preconditioner), float(t), ctypes.byref(it), ctypes.byref(rr))
+        return x, it.value, rr.value
+
     def solve_block2(self, rhs, tol=1e-12, max_iter=20000):
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)
         x = np.zeros_like(rhs)
@@ -340,3 +404,88 @@ class GpuModel:
         b = ctypes.c_double()
         self._call("nla3d_gpu_time_assembly", int(iters), ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+
+class Mlp:
+    """nla3d_gpu_mlp_t: the per-node network of the learned block-Jacobi correction."""
+    N_PARAMS = 53510
+
+    def __init__(self, device=0, seed=0):
+        self._L = lib()
+        self._h = _P()
+        rc = self._L.nla3d_gpu_mlp_create(ctypes.byref(self._h), device, int(seed))
+        if rc != 0:
+            raise Nla3dGpuError(rc, f"cannot create a network on device {device}")
+
+    def _call(self, name, *args):
+        rc = getattr(self._L, name)(self._h, *args)
+        if rc != 0:
+            raise Nla3dGpuError(rc, self._L.nla3d_gpu_mlp_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self._L.nla3d_gpu_mlp_destroy(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        w = np.zeros(self.N_PARAMS, dtype=np.float32)
+        self._call("nla3d_gpu_mlp_get_params", _ptr(w, ctypes.c_float), self.N_PARAMS)
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._call("nla3d_gpu_mlp_set_params", _ptr(w, ctypes.c_float), len(w))
+
+    def set_normalization(self, mean, inv_std):
+        m = np.ascontiguousarray(mean, dtype=np.float32)
+        s = np.ascontiguousarray(inv_std, dtype=np.float32)
+        self._call("nla3d_gpu_mlp_set_normalization", _ptr(m, ctypes.c_float), _ptr(s, ctypes.c_float))
+
+    def get_normalization(self):
+        m = np.zeros(24, dtype=np.float32)
+        s = np.zeros(24, dtype=np.float32)
+        self._call("nla3d_gpu_mlp_get_normalization", _ptr(m, ctypes.c_float), _ptr(s, ctypes.c_float))
+        return m, s
+
+    def forward(self, X, bf16=True):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 24)
+        Y = np.zeros((len(X), 6), dtype=np.float32)
+        self._call("nla3d_gpu_mlp_forward", _ptr(X, ctypes.c_float), len(X), _ptr(Y, ctypes.c_float), int(bf16))
+        return Y
+
+    def backward(self, X, dY):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 24)
+        dY = np.ascontiguousarray(dY, dtype=np.float32).reshape(-1, 6)
+        g = np.zeros(self.N_PARAMS, dtype=np.float32)
+        self._call("nla3d_gpu_mlp_backward", _ptr(X, ctypes.c_float), _ptr(dY, ctypes.c_float), len(X),
+                   _ptr(g, ctypes.c_float))
+        return g
+
+    def _rhs(self, rhs):
+        if rhs is None:
+            return None, None
+        r = np.ascontiguousarray(rhs, dtype=np.float64)
+        return r, _ptr(r, ctypes.c_double)
+
+    def loss_grad(self, model, rhs=None, k=40, t=1.0):
+        keep, rp = self._rhs(rhs)
+        loss = ctypes.c_double()
+        g = np.zeros(self.N_PARAMS, dtype=np.float32)
+        self._call("nla3d_gpu_mlp_loss_grad", model._h, rp, int(k), float(t), ctypes.byref(loss),
+                   _ptr(g, ctypes.c_float))
+        return loss.value, g
+
+    def train_step(self, model, rhs=None, k=40, t=1.0, lr=1e-3):
+        keep, rp = self._rhs(rhs)
+        o = AdamOpts()
+        self._L.nla3d_gpu_adam_defaults(ctypes.byref(o))
+        o.lr = lr
+        loss = ctypes.c_double()
+        self._call("nla3d_gpu_mlp_train_step", model._h, rp, int(k), float(t), ctypes.byref(o), ctypes.byref(loss))
+        return loss.value

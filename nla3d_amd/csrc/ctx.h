@@ -41,6 +41,8 @@ struct DevBuf {
 };
 
 struct ThermalData;  // thermal.cpp
+struct LearnedWork;  // learned.cpp
+struct Mlp;          // learned.cpp
 
 struct Ctx {
   int device = 0;
@@ -83,6 +85,13 @@ struct Ctx {
   DevBuf d_mw[8], d_partial, d_mstate, d_scal;
   DevBuf d_mpcMirIdx;         // block(2) upper indices of MPC entries
   int nMpcMir = 0;
+  // learned block-Jacobi correction (learned.cpp)
+  DevBuf d_lGroup, d_lRow0;   // [nLearned] block-Jacobi group and first row of each UX UY UZ node triple
+  DevBuf d_rowKind2;          // int8 [n2]: 0 node DoF, 1 element DoF, 2 multiplier
+  DevBuf d_rowNode2, d_rowPG; // int [n2]: node of a node-DoF row (-1 otherwise), block-Jacobi group of each row
+  int nLearned = 0;
+  std::shared_ptr<LearnedWork> lw;  // unrolled-MINRES and feature buffers
+  Mlp* mlp = nullptr;               // attached network (not owned)
   MinresState* h_mstate = nullptr;  // pinned host mirror of the MINRES state
   bool krylovReady = false;
 
@@ -122,5 +131,8 @@ struct Ctx {
   // solve block(2) x = b with the current K; b, x device pointers [n2]; returns iterations
   int minresSolve(const double* b, double* x, double tol, int maxit, double* relres_est, bool* breakdown);
 };
+
+// block-Jacobi blocks with the attached network's corrections (learned.cpp); t = load-step time
+void learnedPrecondSetup(Ctx& c, double t);
 
 }  // namespace nla3d_amd

@@ -140,6 +140,31 @@ void Ctx::setupKrylov() {
     i += sz;
   }
   nGroups = (int)gStart.size();
+  {  // learned-correction node list and row metadata
+    std::vector<signed char> kind2(n2, 0);
+    std::vector<int> node2(n2, -1), pg(n2, 0), lg, lr;
+    for (int i = 0; i < n2; i++) node2[i] = rowKind[i] == 0 ? (int)rowNode[i] : -1;
+    for (uint32_t e = 0; e < m.nElements(); e++)
+      for (int t = 0; t < NUM_DOF_TYPES; t++) {
+        uint32_t q = m.elemEq[(size_t)e * 8 + t];
+        if (q > nC) kind2[q - nC - 1] = 1;
+      }
+    for (auto& mp : m.mpcs) kind2[mp.eqNum - nC - 1] = 2;
+    for (int g = 0; g < nGroups; g++) {
+      const int r0 = gStart[g];
+      for (int a = 0; a < (gInfo[g] & 3); a++) pg[r0 + a] = g;
+      if (gInfo[g] == 3 && rowType[r0] == 0 && rowType[r0 + 1] == 1 && rowType[r0 + 2] == 2) {
+        lg.push_back(g);
+        lr.push_back(r0);
+      }
+    }
+    nLearned = (int)lg.size();
+    d_lGroup.upload(lg);
+    d_lRow0.upload(lr);
+    d_rowKind2.upload(kind2);
+    d_rowNode2.upload(node2);
+    d_rowPG.upload(pg);
+  }
   d_gStart.upload(gStart);
   d_gInfo.upload(gInfo);
   d_gDiagPos.upload(gDiag);
@@ -335,11 +360,19 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
   st.newton_iterations++;
   // solve K.block(2) dU_sl = rhs (FESolver.cpp:457)
   t0 = nowMs();
-  krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
+  if (o.preconditioner == 1) learnedPrecondSetup(c, t);
+  else krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
   double relres = 0;
   bool brk = false;
-  const int kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), o.krylov_tol, o.krylov_max_iter, &relres,
-                                &brk);
+  int kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), o.krylov_tol, o.krylov_max_iter, &relres, &brk);
+  if (o.preconditioner == 1 && (brk || relres > o.krylov_tol)) {
+    // the learned corrections failed this solve: block Jacobi from a fresh start
+    st.precond_fallbacks++;
+    st.krylov_iterations += kit;
+    krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
+    brk = false;
+    kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), o.krylov_tol, o.krylov_max_iter, &relres, &brk);
+  }
   st.krylov_iterations += kit;
   st.linear_solves++;
   st.last_krylov_iterations = kit;
@@ -505,6 +538,14 @@ int nla3d_gpu_solve_block2(nla3d_gpu_t h, const double* rhs, double* x, double t
       c.err = "MINRES breakdown (preconditioner not positive definite)";
       return -5;
     }
+    return 0;
+  });
+}
+
+int nla3d_gpu_get_rhs(nla3d_gpu_t h, double* rhs) {
+  if (!rhs) return -1;
+  return newtonGuard(h, [&](Ctx& c) {
+    hipCheck(hipMemcpy(rhs, c.d_rhs.p, (size_t)c.n2 * 8, hipMemcpyDeviceToHost), "rhs");
     return 0;
   });
 }
