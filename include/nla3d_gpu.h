@@ -162,6 +162,53 @@ int nla3d_gpu_solve_block2(nla3d_gpu_t h, const double* rhs, double* x, double t
  * F_sl + R_sl - K12^T dU_c), host array of length nUnknown + nMpc; with the tangent
  * still assembled it is the system that iteration solved. */
 int nla3d_gpu_get_rhs(nla3d_gpu_t h, double* rhs);
+
+/* ---- batched models ---------------------------------------------------------
+ * Many independent SOLID81 models (no MPCs) on one device, each handle set up as
+ * usual through nla3d_gpu_init_solution; the batch does not own them.  One
+ * Newton iteration of the batch = one persistent element-kernel launch over all
+ * models' elements and one gather launch over all their CSR targets, the
+ * per-model boundary conditions / rhs / norms / block-Jacobi set-up replayed
+ * from a hipGraph, then one persistent team-MINRES launch (each workgroup solves
+ * whole systems taken from a queue) plus whole-device solves for the largest
+ * systems.  Every model keeps its own Newton and Krylov convergence.  No
+ * counterpart in the reference, which solves one FEStorage at a time
+ * (FESolver.cpp:407-508). */
+typedef struct nla3d_gpu_batch* nla3d_gpu_batch_t;
+typedef struct {
+  int elem_grid;       /* workgroups of the persistent element kernel (0: one per element block) */
+  int gather_threads;  /* block size of the batched gather (multiple of 64, <= 1024) */
+  int team_threads;    /* team-MINRES workgroup size: 256, 512 or 1024 */
+  int team_lanes;      /* lanes per matrix row group in the team SpMV: 4, 8 or 16 */
+  int team_grid;       /* teams launched (0: 1024); at most one per queued system */
+  long long big_rows;  /* systems with more rows use the whole-device solver */
+} nla3d_gpu_batch_config;
+
+void nla3d_gpu_batch_default_config(nla3d_gpu_batch_config* cfg);
+int nla3d_gpu_batch_create(nla3d_gpu_batch_t* batch, const nla3d_gpu_t* models, int n);
+int nla3d_gpu_batch_destroy(nla3d_gpu_batch_t batch);
+const char* nla3d_gpu_batch_last_error(nla3d_gpu_batch_t batch);
+int nla3d_gpu_batch_get_config(nla3d_gpu_batch_t batch, nla3d_gpu_batch_config* cfg);
+int nla3d_gpu_batch_set_config(nla3d_gpu_batch_t batch, const nla3d_gpu_batch_config* cfg);
+/* Times candidate launch shapes on the batch itself at load time t (the assembly and
+ * the element/gather grids; team size and lanes, and the large-system split, on
+ * probe_iterations MINRES steps of every system) and keeps the fastest.  report
+ * (may be NULL) receives up to report_cap measured times in ms, in the order tried. */
+int nla3d_gpu_batch_autotune(nla3d_gpu_batch_t batch, double t, int probe_iterations, nla3d_gpu_batch_config* chosen,
+                             double* report, int report_cap);
+/* assembly + rhs + preconditioner of every model at time t (the pre-solve graph) */
+int nla3d_gpu_batch_assemble(nla3d_gpu_batch_t batch, double t);
+/* One Newton iteration (FESolver.cpp:443-490) of every model not yet converged in
+ * the load step; stats and converged hold one entry per model (converged: 1 done,
+ * 0 iterating, < 0 error code).  iteration 1 starts the load step. */
+int nla3d_gpu_batch_newton_iteration(nla3d_gpu_batch_t batch, double t, int iteration,
+                                     const nla3d_gpu_newton_opts* opts, nla3d_gpu_newton_stats* stats,
+                                     int* converged);
+int nla3d_gpu_batch_solve_load_step(nla3d_gpu_batch_t batch, double t, const nla3d_gpu_newton_opts* opts,
+                                    nla3d_gpu_newton_stats* stats, int* converged);
+/* TimeControl over [0, 1] with opts->n_load_steps steps for every model. */
+int nla3d_gpu_batch_solve_nonlinear(nla3d_gpu_batch_t batch, const nla3d_gpu_newton_opts* opts,
+                                    nla3d_gpu_newton_stats* stats, int* converged);
 /* As nla3d_gpu_solve_block2 with a choice of preconditioner: 0 block Jacobi, 1 learned
  * corrections (attached network; t is the load-step time fed to it).  No fallback here:
  * this is the entry that compares the two. */

@@ -57,6 +57,15 @@ class AdamOpts(ctypes.Structure):
     _fields_ = [("lr", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("eps", ctypes.c_float)]
 
 
+class BatchConfig(ctypes.Structure):
+    """nla3d_gpu_batch_config (include/nla3d_gpu.h)"""
+    _fields_ = [("elem_grid", ctypes.c_int), ("gather_threads", ctypes.c_int), ("team_threads", ctypes.c_int),
+                ("team_lanes", ctypes.c_int), ("team_grid", ctypes.c_int), ("big_rows", ctypes.c_longlong)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _P = ctypes.c_void_p
 _U32P = ctypes.POINTER(ctypes.c_uint32)
 _U64P = ctypes.POINTER(ctypes.c_uint64)
@@ -121,13 +130,29 @@ ABI.update({
     "nla3d_gpu_solve_block2_pc": [_P, _DP, _DP, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, _IP,
                                   _DP],
 })
+# batched models (first argument: batch handle)
+ABI.update({
+    "nla3d_gpu_batch_destroy": [_P],
+    "nla3d_gpu_batch_get_config": [_P, ctypes.POINTER(BatchConfig)],
+    "nla3d_gpu_batch_set_config": [_P, ctypes.POINTER(BatchConfig)],
+    "nla3d_gpu_batch_autotune": [_P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(BatchConfig), _DP, ctypes.c_int],
+    "nla3d_gpu_batch_assemble": [_P, ctypes.c_double],
+    "nla3d_gpu_batch_newton_iteration": [_P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(NewtonOpts),
+                                         ctypes.POINTER(NewtonStats), _IP],
+    "nla3d_gpu_batch_solve_load_step": [_P, ctypes.c_double, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats),
+                                        _IP],
+    "nla3d_gpu_batch_solve_nonlinear": [_P, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats), _IP],
+})
 # functions that take no handle
 ABI_FREE = {"nla3d_gpu_newton_defaults": [ctypes.POINTER(NewtonOpts)],
             "nla3d_gpu_transient_defaults": [ctypes.POINTER(TransientOpts)],
             "nla3d_gpu_mlp_num_params": [],
             "nla3d_gpu_mlp_create": [ctypes.POINTER(_P), ctypes.c_int, ctypes.c_ulonglong],
             "nla3d_gpu_adam_defaults": [ctypes.POINTER(AdamOpts)],
-            "nla3d_gpu_mlp_last_error": [_P]}
+            "nla3d_gpu_mlp_last_error": [_P],
+            "nla3d_gpu_batch_create": [ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.c_int],
+            "nla3d_gpu_batch_default_config": [ctypes.POINTER(BatchConfig)],
+            "nla3d_gpu_batch_last_error": [_P]}
 
 
 def lib():
@@ -148,6 +173,7 @@ def lib():
         L.nla3d_gpu_last_error.argtypes = [_P]
         L.nla3d_gpu_last_error.restype = ctypes.c_char_p
         L.nla3d_gpu_mlp_last_error.restype = ctypes.c_char_p
+        L.nla3d_gpu_batch_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -489,3 +515,75 @@ class Mlp:
         loss = ctypes.c_double()
         self._call("nla3d_gpu_mlp_train_step", model._h, rp, int(k), float(t), ctypes.byref(o), ctypes.byref(loss))
         return loss.value
+
+
+class Batch:
+    """nla3d_gpu_batch_t: independent SOLID81 models (initialised GpuModel handles on one
+    device) assembled and solved together; the models stay owned by the caller."""
+
+    def __init__(self, models):
+        self._L = lib()
+        self._h = _P()
+        self.models = list(models)
+        arr = (_P * len(self.models))(*[m._h.value for m in self.models])
+        rc = self._L.nla3d_gpu_batch_create(ctypes.byref(self._h), arr, len(self.models))
+        if rc != 0:
+            raise Nla3dGpuError(rc, "cannot create the batch (models must be initialised SOLID81 device handles "
+                                    "without MPCs, on one device)")
+        self.n = len(self.models)
+
+    def _call(self, name, *args):
+        rc = getattr(self._L, name)(self._h, *args)
+        if rc != 0:
+            raise Nla3dGpuError(rc, self._L.nla3d_gpu_batch_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self._L.nla3d_gpu_batch_destroy(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def config(self):
+        c = BatchConfig()
+        self._call("nla3d_gpu_batch_get_config", ctypes.byref(c))
+        return c
+
+    def set_config(self, **kw):
+        c = self.config()
+        for k, v in kw.items():
+            setattr(c, k, v)
+        self._call("nla3d_gpu_batch_set_config", ctypes.byref(c))
+
+    def autotune(self, t=0.1, probe_iterations=24):
+        c = BatchConfig()
+        rep = np.zeros(64)
+        self._call("nla3d_gpu_batch_autotune", float(t), int(probe_iterations), ctypes.byref(c),
+                   _ptr(rep, ctypes.c_double), len(rep))
+        return c, rep
+
+    def assemble(self, t):
+        self._call("nla3d_gpu_batch_assemble", float(t))
+
+    def new_stats(self):
+        return (NewtonStats * self.n)()
+
+    def newton_iteration(self, t, iteration, opts, stats, conv):
+        self._call("nla3d_gpu_batch_newton_iteration", float(t), int(iteration), ctypes.byref(opts), stats,
+                   _ptr(conv, ctypes.c_int))
+
+    def solve_load_step(self, t, opts, stats=None):
+        stats = stats if stats is not None else self.new_stats()
+        conv = np.zeros(self.n, dtype=np.int32)
+        self._call("nla3d_gpu_batch_solve_load_step", float(t), ctypes.byref(opts), stats, _ptr(conv, ctypes.c_int))
+        return stats, conv
+
+    def solve_nonlinear(self, opts, stats=None):
+        stats = stats if stats is not None else self.new_stats()
+        conv = np.zeros(self.n, dtype=np.int32)
+        self._call("nla3d_gpu_batch_solve_nonlinear", ctypes.byref(opts), stats, _ptr(conv, ctypes.c_int))
+        return stats, conv

@@ -51,7 +51,37 @@ __global__ __launch_bounds__(256) void mirror_kernel(int n, const double* __rest
   full[mir[2 * k + 1]] = v;
 }
 
+__global__ __launch_bounds__(1024) void gather_batched_kernel(int nModels, const GatherBatch* __restrict__ mods,
+                                                              const int* __restrict__ targetBegin, int nTargets) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nTargets) return;
+  int lo = 0, hi = nModels - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (targetBegin[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const GatherBatch& g = mods[lo];
+  const int u = t - targetBegin[lo];
+  const int s0 = g.seg[u], s1 = g.seg[u + 1];
+  double acc = 0.0;
+  for (int s = s0; s < s1; s++) acc += g.src_vals[g.src[s]];
+  g.out[u] = acc;
+  if (g.full && u >= g.mirBegin && u < g.mirEnd) {
+    const int k = u - g.mirBegin;
+    g.full[g.mir[2 * k]] = acc;
+    g.full[g.mir[2 * k + 1]] = acc;
+  }
+}
+
 }  // namespace
+
+void launch_gather_batched(hipStream_t stream, int threads, int nModels, const GatherBatch* mods,
+                           const int* targetBegin, int nTargets) {
+  if (nTargets <= 0) return;
+  hipLaunchKernelGGL(gather_batched_kernel, dim3((nTargets + threads - 1) / threads), dim3(threads), 0, stream,
+                     nModels, mods, targetBegin, nTargets);
+}
 
 void launch_segmented_gather(hipStream_t stream, int nTargets, const int* seg, const int* src,
                              const double* src_vals, double* out, int mirBegin, int mirEnd, const int* mir,

@@ -30,6 +30,37 @@ void launch_solid81(hipStream_t stream, int nE, const double* xyz, const int* co
                     const int* elem_eq, const double* U, const MaterialParams& mat, double* Ke,
                     double* gp_state);
 
+// One model of a batched assembly (device pointers of that model's buffers).
+struct Solid81Batch {
+  int nE;
+  const double* xyz;
+  const int* conn;
+  const int* node_eq;
+  const int* elem_eq;
+  const double* U;
+  MaterialParams mat;
+  double* Ke;
+};
+int solid81_blocks(int nE);  // 64-lane blocks of a model's elements
+// persistent grid-stride element kernel over the blocks of every model: blockBegin[m] is
+// model m's first block, nBlocks the total; grid = number of resident workgroups
+void launch_solid81_batched(hipStream_t stream, int grid, int nModels, const Solid81Batch* mods, const int* blockBegin,
+                            int nBlocks);
+
+struct GatherBatch {
+  int nTargets;
+  const int* seg;
+  const int* src;
+  const double* src_vals;
+  double* out;
+  int mirBegin, mirEnd;
+  const int* mir;
+  double* full;
+};
+// the segmented gather of every model in one launch; targetBegin[m] is model m's first target
+void launch_gather_batched(hipStream_t stream, int threads, int nModels, const GatherBatch* mods,
+                           const int* targetBegin, int nTargets);
+
 // Segmented reduction: out[t] = sum_{s in [seg[t], seg[t+1])} src_vals[src[s]], summed in s order
 // (starting from +0.0).  int32 offsets: the host checks the total count fits.
 // Targets in [mirBegin, mirEnd) are also stored at full[mir[2k]] and full[mir[2k+1]],

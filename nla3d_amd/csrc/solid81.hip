@@ -282,21 +282,25 @@ __device__ __forceinline__ void gauss_point(const double (*X)[3], const double (
   out[kOffKpp] = -1.0 / mat.k * w;
 }
 
-__global__ __launch_bounds__(64) void solid81_kernel(int nE, const double* __restrict__ xyz,
-                                                     const int* __restrict__ conn,
-                                                     const int* __restrict__ node_eq,
-                                                     const int* __restrict__ elem_eq,
-                                                     const double* __restrict__ Uall, MaterialParams mat,
-                                                     double* __restrict__ Ke, double* __restrict__ gp_state) {
-  __shared__ double sX[kEPB][8][3];
-  __shared__ double sU[kEPB][8][3];
-  __shared__ double sP[kEPB];
-  __shared__ double sG[kEPB][8][kGPW];
+struct Solid81Shared {
+  double X[kEPB][8][3];
+  double U[kEPB][8][3];
+  double P[kEPB];
+  double G[kEPB][8][kGPW];
+};
 
+// One 64-lane block of kEPB elements starting at element e0 (of nE): gather, Gauss
+// points (phase A), node-pair blocks (phase B).  Every lane of the block calls it; it
+// returns with a barrier passed, so the shared arrays can be reused by the next call.
+__device__ __forceinline__ void solid81_block(Solid81Shared& sh, int e0, int nE, const double* __restrict__ xyz,
+                                              const int* __restrict__ conn, const int* __restrict__ node_eq,
+                                              const int* __restrict__ elem_eq, const double* __restrict__ Uall,
+                                              const MaterialParams& mat, double* __restrict__ Ke,
+                                              double* __restrict__ gp_state) {
   const int lane = threadIdx.x;
   const int el = lane / kLPE;
   const int r = lane - el * kLPE;
-  const int e = blockIdx.x * kEPB + el;
+  const int e = e0 + el;
   const bool valid = (el < kEPB) && (e < nE);
 
   // phase 0: gather coordinates and the element's DoF values
@@ -305,87 +309,117 @@ __global__ __launch_bounds__(64) void solid81_kernel(int nE, const double* __res
       const int n = conn[(size_t)e * 8 + r];
 #pragma unroll
       for (int d = 0; d < 3; d++) {
-        sX[el][r][d] = xyz[(size_t)n * 3 + d];
+        sh.X[el][r][d] = xyz[(size_t)n * 3 + d];
         const int q = node_eq[(size_t)n * 3 + d];
-        sU[el][r][d] = (q >= 0) ? Uall[q] : 0.0;
+        sh.U[el][r][d] = (q >= 0) ? Uall[q] : 0.0;
       }
     } else {
       const int q = elem_eq[e];
-      sP[el] = (q >= 0) ? Uall[q] : 0.0;
+      sh.P[el] = (q >= 0) ? Uall[q] : 0.0;
     }
   }
   __syncthreads();
 
   // phase A: lane r evaluates Gauss point r
   if (valid && r < 8) {
-    gauss_point(sX[el], sU[el], sP[el], r, mat, sG[el][r],
+    gauss_point(sh.X[el], sh.U[el], sh.P[el], r, mat, sh.G[el][r],
                 gp_state ? gp_state + ((size_t)e * 8 + r) * kSolid81GpState : nullptr);
   }
   __syncthreads();
-  if (!valid) return;
 
   // phase B: lane (d,e) accumulates component (d,e) of every node-pair block
-  const int d = r / 3, ee = r - 3 * (r / 3);
-  int ia[3][3];
-#pragma unroll
-  for (int p = 0; p < 3; p++)
-#pragma unroll
-    for (int q = 0; q < 3; q++) ia[p][q] = pk9(3 * d + p, 3 * ee + q);
-  const int vsel = (ee == 0) ? kOffP : kOffQ;  // lanes e==2 compute a discarded copy of Kup
-
-  double kij[36];
-#pragma unroll
-  for (int i = 0; i < 36; i++) kij[i] = 0.0;
-  double kf[8];
-#pragma unroll
-  for (int i = 0; i < 8; i++) kf[i] = 0.0;
-  double ksc = 0.0;
-
-#pragma unroll 1
-  for (int g = 0; g < 8; g++) {
-    const double* G = sG[el][g];
-    double Ab[3][3];
+  if (valid) {
+    const int d = r / 3, ee = r - 3 * (r / 3);
+    int ia[3][3];
 #pragma unroll
     for (int p = 0; p < 3; p++)
 #pragma unroll
-      for (int q = 0; q < 3; q++) Ab[p][q] = G[ia[p][q]];
-    double N[8][3];
-#pragma unroll
-    for (int n = 0; n < 8; n++)
-#pragma unroll
-      for (int i = 0; i < 3; i++) N[n][i] = G[kOffN + 3 * n + i];
-    const double v0 = G[vsel + 3 * d + 0], v1 = G[vsel + 3 * d + 1], v2 = G[vsel + 3 * d + 2];
-    ksc += G[kOffFp + (r & 1)];
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const double w0 = Ab[0][0] * N[j][0] + Ab[0][1] * N[j][1] + Ab[0][2] * N[j][2];
-      const double w1 = Ab[1][0] * N[j][0] + Ab[1][1] * N[j][1] + Ab[1][2] * N[j][2];
-      const double w2 = Ab[2][0] * N[j][0] + Ab[2][1] * N[j][1] + Ab[2][2] * N[j][2];
-#pragma unroll
-      for (int i = 0; i <= j; i++) {
-        const int slot = (i < j) ? offpair(i, j) : 28 + i;
-        kij[slot] += N[i][0] * w0 + N[i][1] * w1 + N[i][2] * w2;
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < 8; i++) kf[i] += N[i][0] * v0 + N[i][1] * v1 + N[i][2] * v2;
-  }
+      for (int q = 0; q < 3; q++) ia[p][q] = pk9(3 * d + p, 3 * ee + q);
+    const int vsel = (ee == 0) ? kOffP : kOffQ;  // lanes e==2 compute a discarded copy of Kup
 
-  double* out = Ke + (size_t)e * kSolid81Slots;
+    double kij[36];
 #pragma unroll
-  for (int s = 0; s < 28; s++) out[s * 9 + r] = kij[s];
-  if (d <= ee) {
-    const int up = voigt(d, ee);
+    for (int i = 0; i < 36; i++) kij[i] = 0.0;
+    double kf[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) out[252 + 6 * i + up] = kij[28 + i];
+    for (int i = 0; i < 8; i++) kf[i] = 0.0;
+    double ksc = 0.0;
+
+#pragma unroll 1
+    for (int g = 0; g < 8; g++) {
+      const double* G = sh.G[el][g];
+      double Ab[3][3];
+#pragma unroll
+      for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) Ab[p][q] = G[ia[p][q]];
+      double N[8][3];
+#pragma unroll
+      for (int n = 0; n < 8; n++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) N[n][i] = G[kOffN + 3 * n + i];
+      const double v0 = G[vsel + 3 * d + 0], v1 = G[vsel + 3 * d + 1], v2 = G[vsel + 3 * d + 2];
+      ksc += G[kOffFp + (r & 1)];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const double w0 = Ab[0][0] * N[j][0] + Ab[0][1] * N[j][1] + Ab[0][2] * N[j][2];
+        const double w1 = Ab[1][0] * N[j][0] + Ab[1][1] * N[j][1] + Ab[1][2] * N[j][2];
+        const double w2 = Ab[2][0] * N[j][0] + Ab[2][1] * N[j][1] + Ab[2][2] * N[j][2];
+#pragma unroll
+        for (int i = 0; i <= j; i++) {
+          const int slot = (i < j) ? offpair(i, j) : 28 + i;
+          kij[slot] += N[i][0] * w0 + N[i][1] * w1 + N[i][2] * w2;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) kf[i] += N[i][0] * v0 + N[i][1] * v1 + N[i][2] * v2;
+    }
+
+    double* out = Ke + (size_t)e * kSolid81Slots;
+#pragma unroll
+    for (int s = 0; s < 28; s++) out[s * 9 + r] = kij[s];
+    if (d <= ee) {
+      const int up = voigt(d, ee);
+#pragma unroll
+      for (int i = 0; i < 8; i++) out[252 + 6 * i + up] = kij[28 + i];
+    }
+    if (ee < 2) {
+      const int base = (ee == 0) ? 325 : 300;
+#pragma unroll
+      for (int i = 0; i < 8; i++) out[base + 3 * i + d] = kf[i];
+    }
+    if (r == 0) out[349] = ksc;
+    if (r == 1) out[324] = ksc;
   }
-  if (ee < 2) {
-    const int base = (ee == 0) ? 325 : 300;
-#pragma unroll
-    for (int i = 0; i < 8; i++) out[base + 3 * i + d] = kf[i];
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(64) void solid81_kernel(int nE, const double* __restrict__ xyz,
+                                                     const int* __restrict__ conn,
+                                                     const int* __restrict__ node_eq,
+                                                     const int* __restrict__ elem_eq,
+                                                     const double* __restrict__ Uall, MaterialParams mat,
+                                                     double* __restrict__ Ke, double* __restrict__ gp_state) {
+  __shared__ Solid81Shared sh;
+  solid81_block(sh, blockIdx.x * kEPB, nE, xyz, conn, node_eq, elem_eq, Uall, mat, Ke, gp_state);
+}
+
+// Batched assembly: one persistent grid-stride launch over the element blocks of
+// every model of a batch; block b belongs to the model whose block range holds it.
+__global__ __launch_bounds__(64) void solid81_batched_kernel(int nModels, const Solid81Batch* __restrict__ mods,
+                                                             const int* __restrict__ blockBegin, int nBlocks) {
+  __shared__ Solid81Shared sh;
+  for (int b = blockIdx.x; b < nBlocks; b += gridDim.x) {
+    int lo = 0, hi = nModels - 1;  // last model with blockBegin <= b
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (blockBegin[mid] <= b) lo = mid;
+      else hi = mid - 1;
+    }
+    const Solid81Batch& md = mods[lo];
+    solid81_block(sh, (b - blockBegin[lo]) * kEPB, md.nE, md.xyz, md.conn, md.node_eq, md.elem_eq, md.U, md.mat,
+                  md.Ke, nullptr);
   }
-  if (r == 0) out[349] = ksc;
-  if (r == 1) out[324] = ksc;
 }
 
 }  // namespace
@@ -397,6 +431,14 @@ void launch_solid81(hipStream_t stream, int nE, const double* xyz, const int* co
   const int blocks = (nE + kEPB - 1) / kEPB;
   hipLaunchKernelGGL(solid81_kernel, dim3(blocks), dim3(64), 0, stream, nE, xyz, conn, node_eq, elem_eq, U,
                      mat, Ke, gp_state);
+}
+
+int solid81_blocks(int nE) { return (nE + kEPB - 1) / kEPB; }
+
+void launch_solid81_batched(hipStream_t stream, int grid, int nModels, const Solid81Batch* mods, const int* blockBegin,
+                            int nBlocks) {
+  if (nBlocks <= 0) return;
+  hipLaunchKernelGGL(solid81_batched_kernel, dim3(grid), dim3(64), 0, stream, nModels, mods, blockBegin, nBlocks);
 }
 
 void solid81_slot_to_assemble3(int* map) {

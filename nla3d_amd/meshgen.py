@@ -108,3 +108,26 @@ def solid81_rigid_top_case(model, nx, ny, nz, lx, ly, lz, uz, rotz, jitter=0.1, 
     for d in range(6):
         model.add_fix(master, d, vals[d])
     return xyz_all, conn
+
+
+def batch_models(count, seed=DEFAULT_SEED, n_range=(4, 32), device=0, make=None):
+    """Seeded independent compression cubes for the batched path: n^3 hexahedra with n
+    log-uniform in n_range, Mooney-Rivlin C10 ~ U[0.2, 1], C01 ~ U[0, 0.3],
+    K/C10 ~ U[50, 2000], top UZ strain ~ U[5%, 30%]; interior jitter 0.1 h.
+    Returns a list of (model, params) with the models initialised."""
+    import nla3d_amd
+    rng = np.random.default_rng(seed)
+    out = []
+    lo, hi = np.log(n_range[0]), np.log(n_range[1])
+    for i in range(count):
+        n = int(round(np.exp(rng.uniform(lo, hi))))
+        c10 = rng.uniform(0.2, 1.0)
+        c01 = rng.uniform(0.0, 0.3)
+        bulk = c10 * rng.uniform(50.0, 2000.0)
+        strain = rng.uniform(0.05, 0.30)
+        m = (make or nla3d_amd.GpuModel)(device)
+        solid81_compression_case(m, n, jitter=0.1, seed=seed + 7919 * (i + 1), material=(3, (c10, c01, bulk)),
+                                 top_uz=strain)
+        m.init_solution()
+        out.append((m, {"n": n, "c10": c10, "c01": c01, "bulk": bulk, "strain": strain}))
+    return out
