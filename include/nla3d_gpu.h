@@ -181,7 +181,7 @@ typedef struct {
   int team_threads;    /* team-MINRES workgroup size: 256, 512 or 1024 */
   int team_lanes;      /* lanes per matrix row group in the team SpMV: 4, 8 or 16 */
   int team_grid;       /* teams launched (0: 1024); at most one per queued system */
-  long long big_rows;  /* systems with more rows use the whole-device solver */
+  long long big_rows;  /* systems with more rows use the whole-device solver (always above 65,536 rows) */
 } nla3d_gpu_batch_config;
 
 void nla3d_gpu_batch_default_config(nla3d_gpu_batch_config* cfg);

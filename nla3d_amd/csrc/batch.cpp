@@ -31,6 +31,9 @@
 
 namespace nla3d_amd {
 
+// a system larger than this never goes to the team launch (one workgroup per system)
+constexpr long long kTeamMaxRows = 65536;
+
 struct Batch {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -109,7 +112,7 @@ static std::vector<double> readNorms(Batch& b) {
 static void solveModels(Batch& b, const std::vector<int>& ids, double tol, int maxit, std::vector<int>& its,
                         std::vector<int>& flags, std::vector<double>& relres) {
   std::vector<int> team, big;
-  for (int k : ids) ((long long)b.n2[k] > b.cfg.big_rows ? big : team).push_back(k);
+  for (int k : ids) ((long long)b.n2[k] > std::min(b.cfg.big_rows, kTeamMaxRows) ? big : team).push_back(k);
   std::sort(team.begin(), team.end(), [&](int a, int c) { return b.n2[a] > b.n2[c]; });  // largest first
   // the right-hand sides and preconditioners were written on the batch stream
   hipEvent_t ready = nullptr;
@@ -317,7 +320,7 @@ static void autotune(Batch& b, double t, int probeIters, std::vector<double>& re
   std::vector<int> its(b.m.size()), flags(b.m.size());
   std::vector<double> rr(b.m.size());
   const long long keepBig = b.cfg.big_rows;
-  b.cfg.big_rows = (1ll << 62);
+  b.cfg.big_rows = kTeamMaxRows;
   best = 1e30f;
   int bt = 256, bl = 8;
   for (int th : {256, 512, 1024})
@@ -339,8 +342,13 @@ static void autotune(Batch& b, double t, int probeIters, std::vector<double>& re
   // how many of the largest models leave the team launch for the whole-device solver
   std::vector<int> order(all);
   std::sort(order.begin(), order.end(), [&](int a, int c) { return b.n2[a] > b.n2[c]; });
+  // Krylov iteration counts grow like the mesh edge (~ rows^(1/3)); the probe ran a fixed
+  // number of iterations, so each side's probe time is weighted by the edge of its
+  // largest system (team launch: the makespan is set by its largest member) or of each
+  // system (whole-device solves run one after another)
+  auto edge = [&](int k) { return std::cbrt((double)b.n2[k]); };
   std::vector<double> gpuMs;
-  const int maxBig = std::min<int>(32, (int)order.size() - 1);
+  const int maxBig = std::min<int>(64, (int)order.size() - 1);
   for (int i = 0; i < maxBig; i++) {
     Ctx& c = *b.m[order[i]];
     const double t0 = nowMsB();
@@ -351,20 +359,21 @@ static void autotune(Batch& b, double t, int probeIters, std::vector<double>& re
   }
   double bestCost = 1e300;
   long long bestRows = (1ll << 62);
-  for (int k : {0, 1, 2, 4, 8, 16, 32}) {
+  for (int k : {0, 1, 2, 4, 8, 16, 32, 48, 64}) {
     if (k > maxBig) break;
+    if (k < maxBig && b.n2[order[k]] > kTeamMaxRows) continue;  // too large for one workgroup
     std::vector<int> rest(order.begin() + k, order.end());
-    b.cfg.big_rows = (1ll << 62);
+    b.cfg.big_rows = kTeamMaxRows;
     const double t0 = nowMsB();
     solveModels(b, rest, 0.0, probeIters, its, flags, rr);
-    double cost = nowMsB() - t0;
+    double cost = (nowMsB() - t0) * edge(order[k]);
     double bigCost = 0.0;
-    for (int i = 0; i < k; i++) bigCost += gpuMs[i];
+    for (int i = 0; i < k; i++) bigCost += gpuMs[i] * edge(order[i]);
     cost = std::max(cost, bigCost);  // the two run concurrently
     report.push_back(cost);
     if (cost < bestCost) {
       bestCost = cost;
-      bestRows = k == 0 ? (1ll << 62) : (long long)b.n2[order[k]];
+      bestRows = k == 0 ? kTeamMaxRows : (long long)b.n2[order[k]];
     }
   }
   b.cfg.big_rows = bestRows;
