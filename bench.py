@@ -184,12 +184,64 @@ def batch_mode(args):
     print(json.dumps(out))
 
 
+def thermal_mode(args):
+    """QUADTH transient: 1024 x 1024 quads on the unit square (1,050,625 nodes),
+    conductivity exp(N(0, 0.5^2)) per element, rho c = 1, T = 1 on the left edge,
+    SurfaceLINETH h = 10, T_inf = 0 on the right edge, T0 = 0; Newmark (alpha 0.25,
+    delta 0.5) steps of dt = 1e-3 over [0, 0.2], each solved by Jacobi-CG to 1e-10.
+    The prescribed edge temperature enters the rhs (dirichlet_coupling = 1): the
+    reference's transient rhs omits it (FESolver.cpp:606-608), which would leave the
+    field at zero.  A step is one time step."""
+    import nla3d_amd
+    from nla3d_amd import meshgen
+    n = 1024
+    t0 = time.perf_counter()
+    m = nla3d_amd.GpuModel(0)
+    meshgen.quadth_transient_case(m, n)
+    sizes = m.init_solution()
+    opts = m.transient_defaults(time1=0.2, n_timesteps=200, init_value=0.0, dirichlet_coupling=1, krylov_tol=1e-10,
+                                krylov_max_iter=200000)
+    m.transient_begin(opts)
+    print(f"[bench] setup {time.perf_counter() - t0:.1f}s sizes {sizes}", file=sys.stderr, flush=True)
+    st = nla3d_amd.NewtonStats()
+    for _ in range(args.warmup):
+        m.transient_step(st)
+    k0, s0 = st.krylov_iterations, st.ms_solve
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m.transient_step(st)
+        print(f"[bench] step {st.load_steps}: {st.last_krylov_iterations} CG iterations", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    its = st.krylov_iterations - k0
+    ms_step = wall * 1e3 / args.steps
+    n2 = sizes["nUnknown"]
+    nnz = sizes["nnz2"]
+    # one Jacobi-CG iteration over the upper CSR of block(2): values + columns, row pointers, and
+    # the vector passes p, q (SpMV); x, r read/write, p, q, z (update); z, p read, p write (direction)
+    it_bytes = 12 * nnz + 4 * (n2 + 1) + 16 * n2 + 64 * n2 + 24 * n2
+    ms_it = (st.ms_solve - s0) / max(its, 1)
+    gbs = it_bytes / (ms_it * 1e-3) / 1e9
+    out = {
+        "metric": "QUADTH transient time step (Jacobi-CG to 1e-10) at 1024^2 quads; steps/s FP64",
+        "value": 1e3 / ms_step, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"QUADTH transient, {n}x{n} quads, {sizes['nNodes']} nodes, {n2} unknowns, "
+                               f"nnz(block2 upper) {nnz}, dt 1e-3", "mode": "thermal",
+                   "cg_iterations_per_step": its / args.steps, "ms_per_cg_iteration": ms_it},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+                     "traffic": None, "kernel": "one Jacobi-CG iteration (5 launches)", "algorithmic_bytes": it_bytes},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["newton", "assemble", "probe", "batch"], default="newton")
+    ap.add_argument("--mode", choices=["newton", "assemble", "probe", "batch", "thermal"], default="newton")
     ap.add_argument("--batch-models", type=int, default=256)
     ap.add_argument("--shape", type=str, default="120,120,60")
     ap.add_argument("--load-steps", type=int, default=20)
@@ -215,6 +267,8 @@ def main():
 
     if args.mode == "batch":
         return batch_mode(args)
+    if args.mode == "thermal":
+        return thermal_mode(args)
     nx, ny, nz = (int(v) for v in args.shape.split(","))
     t_setup = time.perf_counter()
     m = nla3d_amd.GpuModel(0)

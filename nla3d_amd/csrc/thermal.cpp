@@ -175,7 +175,8 @@ static CgWork cgWork(Ctx& c, ThermalData& T) {
 static int cgSolve(Ctx& c, ThermalData& T, const double* b, double* x, double tol, int maxit,
                    nla3d_gpu_newton_stats& st) {
   CgWork w = cgWork(c, T);
-  const int it = cg_solve(c.stream, c.krylovMatrix(), c.precond(), w, b, x, tol, maxit, T.hst);
+  const double avgCols = c.n2 > 0 ? (double)c.fullNnz2 / c.n2 : 0.0;
+  const int it = cg_solve(c.stream, c.krylovMatrix(), c.precond(), w, b, x, tol, maxit, T.hst, avgCols);
   if (it < 0) throw ApiError(-2, "HIP error in CG");
   st.linear_solves++;
   st.krylov_iterations += it;

@@ -26,9 +26,10 @@ void launch_quadth(hipStream_t s, int n, const int* conn, const double* xyz, con
 void launch_lineth(hipStream_t s, int n, const int* conn, const double* xyz, const double* par, double* out);
 void launch_combine_mirror(hipStream_t s, int n, double a, const double* A, double b, const double* B, const int* mir,
                            double* full);
-// preconditioned CG on M x = b from the x passed in; returns iterations (negative on HIP error)
+// preconditioned CG on M x = b from the x passed in; returns iterations (negative on HIP error).
+// avgCols: mean entries per row of the full matrix (picks the lanes per row group of the SpMV)
 int cg_solve(hipStream_t s, const KrylovMatrix& M, const Precond& P, CgWork& w, const double* b, double* x, double tol,
-             int maxit, CgState* hstate);
+             int maxit, CgState* hstate, double avgCols);
 void launch_tr_comb(hipStream_t s, int n, int off, double a, const double* U, double b, const double* DU, double c,
                     const double* DDU, double* v);
 void launch_tr_rhs(hipStream_t s, int n2, int nC, const double* F, const double* R, const double* y, const double* U,

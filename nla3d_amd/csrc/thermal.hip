@@ -366,9 +366,9 @@ void launch_combine_mirror(hipStream_t s, int n, double a, const double* A, doub
     hipLaunchKernelGGL(combine_mirror_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, A, b, B, mir, full);
 }
 
-int cg_solve(hipStream_t s, const KrylovMatrix& M, const Precond& P, CgWork& w, const double* b, double* x, double tol,
-             int maxit, CgState* hstate) {
-  constexpr int LPG = 16;
+template <int LPG>
+static int cg_solve_t(hipStream_t s, const KrylovMatrix& M, const Precond& P, CgWork& w, const double* b, double* x,
+                      double tol, int maxit, CgState* hstate) {
   const int nb = kKrylovBlocks;
   // r = b - A x0
   hipLaunchKernelGGL(cg_spmv_kernel<LPG>, dim3(nb), dim3(256), 0, s, M, x, w.q, w.st0, w.partial);
@@ -391,6 +391,14 @@ int cg_solve(hipStream_t s, const KrylovMatrix& M, const Precond& P, CgWork& w, 
     if (hstate->converged || it >= maxit) break;
   }
   return hstate->itn;
+}
+
+int cg_solve(hipStream_t s, const KrylovMatrix& M, const Precond& P, CgWork& w, const double* b, double* x, double tol,
+             int maxit, CgState* hstate, double avgCols) {
+  // lanes per row group sized to the row length (scalar thermal rows hold about 9 entries)
+  if (avgCols <= 12.0) return cg_solve_t<4>(s, M, P, w, b, x, tol, maxit, hstate);
+  if (avgCols <= 40.0) return cg_solve_t<8>(s, M, P, w, b, x, tol, maxit, hstate);
+  return cg_solve_t<16>(s, M, P, w, b, x, tol, maxit, hstate);
 }
 
 void launch_tr_comb(hipStream_t s, int n, int off, double a, const double* U, double b, const double* DU, double c,

@@ -131,3 +131,46 @@ def batch_models(count, seed=DEFAULT_SEED, n_range=(4, 32), device=0, make=None)
         m.init_solution()
         out.append((m, {"n": n, "c10": c10, "c01": c01, "bulk": bulk, "strain": strain}))
     return out
+
+
+def quad_grid(nx, ny, lx=1.0, ly=1.0):
+    """Structured 4-node quadrilaterals of [0,lx]x[0,ly] (counter-clockwise).
+    Returns (xyz[nN,3] with z = 0, conn[nE,4] 1-based, node grid indices (i, j))."""
+    i = np.arange(nx + 1)
+    j = np.arange(ny + 1)
+    J, I = np.meshgrid(j, i, indexing="ij")
+    I = I.ravel()
+    J = J.ravel()
+    xyz = np.stack([I * (lx / nx), J * (ly / ny), np.zeros(I.size)], axis=1)
+    EJ, EI = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    EI = EI.ravel()
+    EJ = EJ.ravel()
+
+    def nid(a, b):
+        return 1 + a + (nx + 1) * b
+
+    conn = np.stack([nid(EI, EJ), nid(EI + 1, EJ), nid(EI + 1, EJ + 1), nid(EI, EJ + 1)], axis=1).astype(np.uint32)
+    return xyz, conn, (I, J)
+
+
+def quadth_transient_case(model, n, seed=DEFAULT_SEED, h=10.0, t_left=1.0):
+    """Unit square of n x n QUADTH elements, conductivity exp(N(0, 0.5^2)) per element
+    (Box-Muller on splitmix64 uniforms), rho c = 1, T = t_left fixed on the left edge,
+    SurfaceLINETH convection (h, T_inf = 0) on the right edge."""
+    xyz, conn, (I, J) = quad_grid(n, n)
+    idx = np.arange(len(conn), dtype=np.uint64)
+    u1 = uniform01(seed, np.uint64(2) * idx)
+    u2 = uniform01(seed, np.uint64(2) * idx + np.uint64(1))
+    z = np.sqrt(-2.0 * np.log(np.maximum(u1, 1e-300))) * np.cos(2.0 * np.pi * u2)
+    k = np.exp(0.5 * z)
+    qp = np.stack([k, np.ones_like(k), np.ones_like(k), np.zeros_like(k)], axis=1)
+    right = np.nonzero(I == n)[0] + 1
+    right = right[np.argsort(J[right - 1])]
+    lines = np.stack([right[:-1], right[1:]], axis=1).astype(np.uint32)
+    lp = np.tile([0.0, h, 0.0, 0.0], (len(lines), 1))
+    model.set_nodes(xyz)
+    model.add_elements(2, conn, qp)
+    model.add_elements(3, lines, lp)
+    for node in np.nonzero(I == 0)[0] + 1:
+        model.add_fix(int(node), 7, t_left)
+    return xyz, conn, k
