@@ -158,6 +158,11 @@ int nla3d_gpu_solve_nonlinear(nla3d_gpu_t h, const nla3d_gpu_newton_opts* opts, 
  * length nUnknown + nMpc.  iters / relres (preconditioned estimate) may be NULL. */
 int nla3d_gpu_solve_block2(nla3d_gpu_t h, const double* rhs, double* x, double tol, int max_iter, int* iters,
                            double* relres);
+/* vecR (length nDofs + nMpc, FEStorage::getR): the constrained-DoF reactions of the last
+ * Newton iteration, vecRc = K11 dUc + K12 dUsl - Fc (FESolver.cpp:464-467), and the
+ * applied loads at the other equations; FEStorage::getReaction(node, dof) is
+ * R[eq - 1] for the DoF's equation number. */
+int nla3d_gpu_get_R(nla3d_gpu_t h, double* R);
 /* The right-hand side of block(2) of the last Newton iteration (FESolver.cpp:454-457:
  * F_sl + R_sl - K12^T dU_c), host array of length nUnknown + nMpc; with the tangent
  * still assembled it is the system that iteration solved. */

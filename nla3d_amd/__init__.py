@@ -127,6 +127,7 @@ ABI.update({
     "nla3d_gpu_mlp_loss_grad": [_P, _P, _DP, ctypes.c_int, ctypes.c_double, _DP, _FP],
     "nla3d_gpu_mlp_train_step": [_P, _P, _DP, ctypes.c_int, ctypes.c_double, ctypes.POINTER(AdamOpts), _DP],
     "nla3d_gpu_get_rhs": [_P, _DP],
+    "nla3d_gpu_get_R": [_P, _DP],
     "nla3d_gpu_solve_block2_pc": [_P, _DP, _DP, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, _IP,
                                   _DP],
 })
@@ -355,6 +356,20 @@ class GpuModel:
         B = np.zeros((len(Y), 9))
         self._call("nla3d_gpu_learned_blocks", _ptr(Y, ctypes.c_float), len(Y), _ptr(B, ctypes.c_double))
         return B.reshape(-1, 3, 3)
+
+    def get_R(self):
+        """vecR: reactions at the constrained equations, applied loads elsewhere"""
+        r = np.zeros(self.nAll)
+        self._call("nla3d_gpu_get_R", _ptr(r, ctypes.c_double))
+        return r
+
+    def reaction(self, node, dof, R=None):
+        """FEStorage::getReaction(node, dof) (FEStorage.cpp:105-119)"""
+        ne, _ = self.equation_numbers()
+        eq = int(ne[node - 1, dof])
+        if eq == 0:
+            return 0.0
+        return float((self.get_R() if R is None else R)[eq - 1])
 
     def get_rhs(self):
         r = np.zeros(self.sizes["nUnknown"] + self.sizes["nMpc"])

@@ -211,6 +211,7 @@ static int batchNewtonIteration(Batch& b, double t, int it, const nla3d_gpu_newt
     st[k].krylov_iterations += its[k];
     st[k].last_krylov_iterations = its[k];
     if (flags[k] == 2) st[k].krylov_breakdown = 1;
+    computeReactions(c, b.stream);
     newton_update(b.stream, (int)md.nConstrainedDofs, (int)md.nUnknownDofs, (int)md.nMpc(), c.d_dU.as<double>(),
                   c.d_U.as<double>(), c.d_Ucprev.as<double>());
     if (o.criterion == 0)

@@ -78,6 +78,8 @@ struct Ctx {
   DevBuf d_gStart, d_gInfo, d_gDiagPos, d_binv;  // block-Jacobi groups
   int nGroups = 0;
   DevBuf d_k12tPtr, d_k12tPos, d_k12tRow;       // block(1,2) by columns: K12^T products
+  DevBuf d_k11Ptr, d_k11Col, d_k11Pos;          // block(1) full-symmetric rows (reactions)
+  DevBuf d_k12rPtr, d_k12rCol;                  // block(1,2) by rows (reactions)
   // Newton / Krylov vectors
   DevBuf d_R;                 // vecR [nAll]
   DevBuf d_Ucprev, d_fixEq, d_fixVal, d_loadEq, d_loadVal;
@@ -132,6 +134,8 @@ struct Ctx {
   int minresSolve(const double* b, double* x, double tol, int maxit, double* relres_est, bool* breakdown);
 };
 
+// constrained-DoF reactions of the last solve into vecR (solver.cpp), on `stream` (null: the handle's)
+void computeReactions(Ctx& c, hipStream_t stream = nullptr);
 // block-Jacobi blocks with the attached network's corrections (learned.cpp); t = load-step time
 void learnedPrecondSetup(Ctx& c, double t);
 

@@ -12,6 +12,10 @@ void newton_apply_bc(hipStream_t s, int nFix, const int* fixEq, const double* fi
 void newton_rhs(hipStream_t s, int n2, int nC, const double* F, const double* R, const double* U, const double* Ucprev,
                 const int* cptr, const int* cpos, const int* crow, const double* K12, double* rhs);
 void newton_update(hipStream_t s, int nC, int nU, int nMpc, const double* dU, double* U, double* Ucprev);
+// constrained-DoF reactions into R[0, nC): K11 dUc + K12 dUsl - Fc
+void newton_reactions(hipStream_t s, int nC, const int* k11Ptr, const int* k11Col, const int* k11Pos, const double* K11,
+                      const int* k12Ptr, const int* k12Col, const double* K12, const double* U, const double* Ucprev,
+                      const double* dUsl, const double* F, double* R);
 void vec_abssum(hipStream_t s, int n, const double* x, double* partial, double* out);
 void newton_lambda_vec(hipStream_t s, int nC, int nU, int nMpc, const double* U, double* v);
 void vec_diffsq(hipStream_t s, int n, const double* a, const double* b, double* partial, double* out);

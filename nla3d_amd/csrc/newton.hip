@@ -52,6 +52,26 @@ __global__ __launch_bounds__(256) void rhs_kernel(int n2, int nC, const double* 
   rhs[j] = v - acc;
 }
 
+// vecRc = K11 dUc + K12 dUsl - Fc (FESolver.cpp:464-467), dUc = Uc - Ucprev; K11 through
+// full-symmetric row lists (positions into its upper values), K12 by rows
+__global__ __launch_bounds__(256) void reaction_kernel(int nC, const int* __restrict__ k11Ptr,
+                                                      const int* __restrict__ k11Col, const int* __restrict__ k11Pos,
+                                                      const double* __restrict__ K11, const int* __restrict__ k12Ptr,
+                                                      const int* __restrict__ k12Col, const double* __restrict__ K12,
+                                                      const double* __restrict__ U, const double* __restrict__ Ucprev,
+                                                      const double* __restrict__ dUsl, const double* __restrict__ F,
+                                                      double* __restrict__ R) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nC) return;
+  double r = 0.0;
+  for (int k = k11Ptr[i]; k < k11Ptr[i + 1]; k++) {
+    const int j = k11Col[k];
+    r += K11[k11Pos[k]] * (U[j] - Ucprev[j]);
+  }
+  for (int k = k12Ptr[i]; k < k12Ptr[i + 1]; k++) r += K12[k] * dUsl[k12Col[k]];
+  R[i] = r - F[i];
+}
+
 // vecUs += dUs ; vecUl = dUl ; Ucprev = Uc (FESolver.cpp:460-469)
 __global__ __launch_bounds__(256) void update_kernel(int nC, int nU, int nMpc, const double* __restrict__ dU,
                                                     double* __restrict__ U, double* __restrict__ Ucprev) {
@@ -120,6 +140,14 @@ void newton_rhs(hipStream_t s, int n2, int nC, const double* F, const double* R,
                 const int* cptr, const int* cpos, const int* crow, const double* K12, double* rhs) {
   hipLaunchKernelGGL(rhs_kernel, dim3(cdiv(n2, 256)), dim3(256), 0, s, n2, nC, F, R, U, Ucprev, cptr, cpos, crow, K12,
                      rhs);
+}
+
+void newton_reactions(hipStream_t s, int nC, const int* k11Ptr, const int* k11Col, const int* k11Pos, const double* K11,
+                      const int* k12Ptr, const int* k12Col, const double* K12, const double* U, const double* Ucprev,
+                      const double* dUsl, const double* F, double* R) {
+  if (nC <= 0) return;
+  hipLaunchKernelGGL(reaction_kernel, dim3(cdiv(nC, 256)), dim3(256), 0, s, nC, k11Ptr, k11Col, k11Pos, K11, k12Ptr,
+                     k12Col, K12, U, Ucprev, dUsl, F, R);
 }
 
 void newton_update(hipStream_t s, int nC, int nU, int nMpc, const double* dU, double* U, double* Ucprev) {

@@ -189,6 +189,36 @@ void Ctx::setupKrylov() {
   d_k12tPtr.upload(cptr);
   d_k12tPos.upload(cpos);
   d_k12tRow.upload(crow);
+  {  // reactions (FESolver.cpp:464-467): block(1) as full-symmetric rows, block(1,2) by rows
+    const CsrPattern& B1 = m.block1;
+    std::vector<int> cnt(nC + 1, 0);
+    for (uint32_t i = 0; i < nC; i++)
+      for (uint32_t k = B1.iofeir[i] - 1; k < B1.iofeir[i + 1] - 1; k++) {
+        const uint32_t j = B1.columns[k] - 1;
+        cnt[i + 1]++;
+        if (j != i) cnt[j + 1]++;
+      }
+    for (uint32_t i = 0; i < nC; i++) cnt[i + 1] += cnt[i];
+    std::vector<int> fcol(cnt[nC]), fpos(cnt[nC]), fill(cnt.begin(), cnt.end() - 1);
+    for (uint32_t i = 0; i < nC; i++)
+      for (uint32_t k = B1.iofeir[i] - 1; k < B1.iofeir[i + 1] - 1; k++) {
+        const uint32_t j = B1.columns[k] - 1;
+        fcol[fill[i]] = (int)j;
+        fpos[fill[i]++] = (int)k;
+        if (j != i) {
+          fcol[fill[j]] = (int)i;
+          fpos[fill[j]++] = (int)k;
+        }
+      }
+    d_k11Ptr.upload(cnt);
+    d_k11Col.upload(fcol);
+    d_k11Pos.upload(fpos);
+    std::vector<int> rptr(nC + 1), rcol(B12.nnz());
+    for (uint32_t i = 0; i <= nC; i++) rptr[i] = (int)B12.iofeir[i] - 1;
+    for (uint64_t k = 0; k < B12.nnz(); k++) rcol[k] = (int)B12.columns[k] - 1;
+    d_k12rPtr.upload(rptr);
+    d_k12rCol.upload(rcol);
+  }
 
   // boundary conditions as equation lists (FESolver::applyBoundaryConditions, FESolver.cpp:229-247)
   std::vector<int> fe, le;
@@ -314,6 +344,15 @@ static double readScalar(Ctx& c, int idx) {
   return v;
 }
 
+// vecRc = K11 dUc + K12 dUsl - Fc after the solve, before Ucprev takes Uc (FESolver.cpp:464-467)
+void computeReactions(Ctx& c, hipStream_t stream) {
+  FEStorage& m = c.model;
+  newton_reactions(stream ? stream : c.stream, (int)m.nConstrainedDofs, c.d_k11Ptr.as<int>(), c.d_k11Col.as<int>(),
+                   c.d_k11Pos.as<int>(), c.d_vals.as<double>(), c.d_k12rPtr.as<int>(), c.d_k12rCol.as<int>(),
+                   c.d_vals.as<double>() + m.offB12(), c.d_U.as<double>(), c.d_Ucprev.as<double>(),
+                   c.d_dU.as<double>(), c.d_vals.as<double>() + m.nnzAll(), c.d_R.as<double>());
+}
+
 // One equilibrium iteration of the Newton loop at normalised time t
 // (FESolver.cpp:443-490).  it is the 1-based iteration number within the load
 // step; rhs1 carries ||rhs|| of iteration 1.  Returns 1 when the criterion
@@ -377,6 +416,7 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
   st.linear_solves++;
   st.last_krylov_iterations = kit;
   if (brk) st.krylov_breakdown = 1;
+  computeReactions(c);
   // restore DoF values from increments; Ucprev = Uc (FESolver.cpp:460-469)
   newton_update(c.stream, nC, nU, nM, c.d_dU.as<double>(), U, c.d_Ucprev.as<double>());
   hipCheck(hipGetLastError(), "newton update");
@@ -538,6 +578,14 @@ int nla3d_gpu_solve_block2(nla3d_gpu_t h, const double* rhs, double* x, double t
       c.err = "MINRES breakdown (preconditioner not positive definite)";
       return -5;
     }
+    return 0;
+  });
+}
+
+int nla3d_gpu_get_R(nla3d_gpu_t h, double* R) {
+  if (!R) return -1;
+  return newtonGuard(h, [&](Ctx& c) {
+    hipCheck(hipMemcpy(R, c.d_R.p, c.d_R.bytes, hipMemcpyDeviceToHost), "R");
     return 0;
   });
 }

@@ -489,6 +489,23 @@ class OracleModel:
                 K2[i, j] = K2[j, i] = K[2][k]
         return K12, K2
 
+    def dense_k11(self, K):
+        """symmetric K11 (nC x nC) from the upper CSR of block 1"""
+        nC = self.nC
+        K11 = np.zeros((nC, nC))
+        b = self.blocks[1]
+        for i in range(nC):
+            for k in range(int(b.iofeir[i]) - 1, int(b.iofeir[i + 1]) - 1):
+                j = int(b.columns[k]) - 1
+                K11[i, j] = K11[j, i] = K[1][k]
+        return K11
+
+    def reaction(self, node, dof):
+        """FEStorage::getReaction(node, dof) (FEStorage.cpp:105-119): vecR at the DoF's equation"""
+        if dof not in self.nodeMask[node - 1]:
+            return 0.0
+        return self.R[self.nodeEq[node - 1][dof] - 1]
+
     def newton_init(self):
         self.U = np.zeros(self.nAll)
         self.Ucprev = np.zeros(self.nC)
@@ -520,6 +537,9 @@ class OracleModel:
             dU = np.linalg.solve(K2, rhs)
             U[nC:nC + nU] += dU[:nU]
             U[nC + nU:] = dU[nU:]
+            # constrained-DoF reactions: vecRc = K11 dUc + K12 dUsl - Fc (FESolver.cpp:464-467)
+            self.R = R
+            self.R[:nC] = self.dense_k11(K) @ dUc + K12 @ dU - F[:nC]
             self.Ucprev = U[:nC].copy()
             if criterion == 0:
                 crit = np.abs(dU[:nU]).sum() / nU / convergence
