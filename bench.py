@@ -246,7 +246,8 @@ def main():
     ap.add_argument("--shape", type=str, default="120,120,60")
     ap.add_argument("--load-steps", type=int, default=20)
     ap.add_argument("--krylov-tol", type=float, default=1e-9)
-    ap.add_argument("--preconditioner", choices=["learned", "block-jacobi"], default="learned")
+    ap.add_argument("--preconditioner", choices=["schur-multigrid", "learned", "block-jacobi"],
+                    default="schur-multigrid")
     ap.add_argument("--train-pairs", type=int, default=2000)
     ap.add_argument("--train-k", type=int, default=100)
     ap.add_argument("--train-lr", type=float, default=1e-4)
@@ -314,7 +315,9 @@ def main():
         rhs1 = ctypes.c_double(0.0)
         state = {"k": 0, "it": 0}
         net = None
-        if args.preconditioner == "learned":
+        base_pc = 2 if args.preconditioner == "schur-multigrid" else 0
+        opts.preconditioner = base_pc
+        if args.preconditioner in ("learned", "schur-multigrid"):
             from nla3d_amd import learned
             net = nla3d_amd.Mlp(0, seed=20261203)
             val = learned.validation_system(24)
@@ -322,7 +325,7 @@ def main():
                                  validation=[val], log=lambda msg: print(msg, file=sys.stderr, flush=True))
             val[0].close()
             m.attach_mlp(net)
-            opts.preconditioner = 1
+            opts.preconditioner = base_pc + 1  # 1: learned, 3: learned + Schur multigrid
             extra["training"] = {"pairs": info["pairs"], "seconds": info["seconds"], "k": args.train_k,
                                  "lr": args.train_lr, "validation_iterations": info["validation"],
                                  "validation_system": "24x24x12 rigid-top box, first Newton iteration at t=0.5"}
@@ -344,17 +347,30 @@ def main():
             kit = one_solving_iteration()
             print(f"[bench] warmup iteration: {kit} Krylov iterations", file=sys.stderr, flush=True)
             if net is not None and w == 0:
-                # block Jacobi judges the learned corrections on the system just solved
-                from nla3d_amd import learned
-                it_bj, it_l, diff = learned.judge(m, net, m.get_rhs(), tol=args.krylov_tol, t=sched[state["k"]])
-                print(f"[bench] judge: block Jacobi {it_bj}, learned {it_l} Krylov iterations", file=sys.stderr,
-                      flush=True)
-                extra["judge"] = {"block_jacobi_iterations": it_bj, "learned_iterations": it_l,
-                                  "solution_difference": diff,
+                # the method without the network judges the learned corrections on the system just
+                # solved; plain block Jacobi is solved beside them for reference
+                rhs = m.get_rhs()
+                tt = sched[state["k"]]
+                m.attach_mlp(net)
+                x0, it_bj, _ = m.solve_block2_pc(rhs, args.krylov_tol, 100000, preconditioner=0, t=tt)
+                its = {0: it_bj}
+                if base_pc == 2:
+                    x2, its[2], _ = m.solve_block2_pc(rhs, args.krylov_tol, 100000, preconditioner=2, t=tt)
+                xl, its[base_pc + 1], _ = m.solve_block2_pc(rhs, args.krylov_tol, 100000,
+                                                            preconditioner=base_pc + 1, t=tt)
+                it_without, it_with = its[base_pc], its[base_pc + 1]
+                print(f"[bench] judge: block Jacobi {it_bj}, " +
+                      (f"Schur multigrid {its[2]}, " if base_pc == 2 else "") +
+                      f"learned {it_with} Krylov iterations", file=sys.stderr, flush=True)
+                extra["judge"] = {"block_jacobi_iterations": it_bj,
+                                  "without_network_iterations": it_without, "learned_iterations": it_with,
+                                  "solution_difference": float(np.abs(xl - x0).max() / np.abs(x0).max()),
+                                  "margin": args.judge_margin,
                                   "system": "tangent and rhs of the first Newton iteration of load step 1"}
-                extra["judge"]["margin"] = args.judge_margin
-                if it_l > (1.0 - args.judge_margin) * it_bj:
-                    opts.preconditioner = 0  # rejected: plain block Jacobi
+                if base_pc == 2:
+                    extra["judge"]["schur_multigrid_iterations"] = its[2]
+                if it_with > (1.0 - args.judge_margin) * it_without:
+                    opts.preconditioner = base_pc  # rejected: the method without the network
         kits = []
         ms_asm0, ms_sol0 = st.ms_assemble, st.ms_solve
         t0 = time.perf_counter()
@@ -368,7 +384,9 @@ def main():
                  "ms_krylov_per_iteration": (st.ms_solve - ms_sol0) / max(1, sum(kits)),
                  "ms_assemble_per_newton": (st.ms_assemble - ms_asm0) / args.steps,
                  "load_step": state["k"] + 1,
-                 "preconditioner": "learned" if opts.preconditioner == 1 else "block-Jacobi",
+                 "preconditioner": {0: "block-Jacobi", 1: "learned block-Jacobi",
+                                    2: "block-Jacobi + Schur multigrid",
+                                    3: "learned block-Jacobi + Schur multigrid"}[opts.preconditioner],
                  "precond_fallbacks": st.precond_fallbacks})
     value = nE * 1e3 / ms_step
     print(f"[bench] {ms_step:.3f} ms/step", file=sys.stderr, flush=True)

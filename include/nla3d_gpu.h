@@ -117,8 +117,12 @@ typedef struct {
   double krylov_tol;     /* MINRES: preconditioned residual <= krylov_tol * initial */
   int krylov_max_iter;
   int preconditioner;    /* 0: block Jacobi; 1: block Jacobi with the learned node-block corrections of the
-                            network attached by nla3d_gpu_attach_mlp (a solve that breaks down or does not
-                            reach krylov_tol is repeated with plain block Jacobi) */
+                            network attached by nla3d_gpu_attach_mlp; 2: block Jacobi on the node DoFs and one
+                            V-cycle of aggregation multigrid on the sparse Schur approximation
+                            S = B^T D^-1 B - C for the pressure and multiplier rows; 3: 1 and 2 together
+                            (S is formed from the corrected node blocks).  A solve of 1, 2 or 3 that
+                            breaks down or does not reach krylov_tol is repeated with plain block Jacobi
+                            (stats.precond_fallbacks) */
 } nla3d_gpu_newton_opts;
 
 typedef struct {
@@ -214,11 +218,23 @@ int nla3d_gpu_batch_solve_load_step(nla3d_gpu_batch_t batch, double t, const nla
 /* TimeControl over [0, 1] with opts->n_load_steps steps for every model. */
 int nla3d_gpu_batch_solve_nonlinear(nla3d_gpu_batch_t batch, const nla3d_gpu_newton_opts* opts,
                                     nla3d_gpu_newton_stats* stats, int* converged);
-/* As nla3d_gpu_solve_block2 with a choice of preconditioner: 0 block Jacobi, 1 learned
- * corrections (attached network; t is the load-step time fed to it).  No fallback here:
- * this is the entry that compares the two. */
+/* As nla3d_gpu_solve_block2 with a choice of preconditioner (the values of
+ * nla3d_gpu_newton_opts.preconditioner; t is the load-step time fed to an attached
+ * network).  No fallback here: this is the entry that compares them. */
 int nla3d_gpu_solve_block2_pc(nla3d_gpu_t h, const double* rhs, double* x, double tol, int max_iter,
                               int preconditioner, double t, int* iters, double* relres);
+
+/* ---- Schur-row multigrid (preconditioner 2) ---------------------------------
+ * No counterpart in the reference.  Inspection entries: sizes of the hierarchy
+ * (sizes[l] = rows of level l, up to cap); S of level 0 for the current tangent
+ * (qrow[nq] = block(2) row of each Schur row; CSR rowPtr[nq+1], col[nnz0], val[nnz0]);
+ * the aggregate of each row of a level and its power-iteration estimate of
+ * lambda_max(D^-1 A); one V-cycle z = B r on host vectors of length nq (refresh = 1
+ * recomputes the hierarchy's values from the current tangent first). */
+int nla3d_gpu_schur_info(nla3d_gpu_t h, int* available, int* nq, int* nnz0, int* levels, int* sizes, int cap);
+int nla3d_gpu_schur_matrix(nla3d_gpu_t h, int* qrow, int* rowPtr, int* col, double* val);
+int nla3d_gpu_schur_level(nla3d_gpu_t h, int level, int* agg, double* lmax);
+int nla3d_gpu_schur_apply(nla3d_gpu_t h, const double* r, double* z, int refresh);
 
 /* ---- learned block-Jacobi correction --------------------------------------
  * No counterpart in the reference, whose only solver is PARDISO

@@ -127,6 +127,10 @@ ABI.update({
     "nla3d_gpu_mlp_loss_grad": [_P, _P, _DP, ctypes.c_int, ctypes.c_double, _DP, _FP],
     "nla3d_gpu_mlp_train_step": [_P, _P, _DP, ctypes.c_int, ctypes.c_double, ctypes.POINTER(AdamOpts), _DP],
     "nla3d_gpu_get_rhs": [_P, _DP],
+    "nla3d_gpu_schur_info": [_P, _IP, _IP, _IP, _IP, _IP, ctypes.c_int],
+    "nla3d_gpu_schur_matrix": [_P, _IP, _IP, _IP, _DP],
+    "nla3d_gpu_schur_level": [_P, ctypes.c_int, _IP, _DP],
+    "nla3d_gpu_schur_apply": [_P, _DP, _DP, ctypes.c_int],
     "nla3d_gpu_get_R": [_P, _DP],
     "nla3d_gpu_solve_block2_pc": [_P, _DP, _DP, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, _IP,
                                   _DP],
@@ -356,6 +360,38 @@ class GpuModel:
         B = np.zeros((len(Y), 9))
         self._call("nla3d_gpu_learned_blocks", _ptr(Y, ctypes.c_float), len(Y), _ptr(B, ctypes.c_double))
         return B.reshape(-1, 3, 3)
+
+    # ---- Schur-row multigrid ----
+    def schur_info(self):
+        av, nq, nnz, lv = (ctypes.c_int() for _ in range(4))
+        sizes = np.zeros(16, dtype=np.int32)
+        self._call("nla3d_gpu_schur_info", ctypes.byref(av), ctypes.byref(nq), ctypes.byref(nnz), ctypes.byref(lv),
+                   _ptr(sizes, ctypes.c_int), 16)
+        return {"available": bool(av.value), "nq": nq.value, "nnz0": nnz.value, "levels": lv.value,
+                "sizes": sizes[:lv.value].tolist()}
+
+    def schur_matrix(self):
+        """(qrow, scipy-style CSR (rowPtr, col, val)) of S on the Schur rows for the current tangent"""
+        inf = self.schur_info()
+        qrow = np.zeros(inf["nq"], dtype=np.int32)
+        rp = np.zeros(inf["nq"] + 1, dtype=np.int32)
+        col = np.zeros(inf["nnz0"], dtype=np.int32)
+        val = np.zeros(inf["nnz0"])
+        self._call("nla3d_gpu_schur_matrix", _ptr(qrow, ctypes.c_int), _ptr(rp, ctypes.c_int), _ptr(col, ctypes.c_int),
+                   _ptr(val, ctypes.c_double))
+        return qrow, rp, col, val
+
+    def schur_level(self, level, n):
+        agg = np.zeros(max(n, 1), dtype=np.int32)
+        lm = ctypes.c_double()
+        self._call("nla3d_gpu_schur_level", int(level), _ptr(agg, ctypes.c_int), ctypes.byref(lm))
+        return agg[:n], lm.value
+
+    def schur_apply(self, r, refresh=False):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros_like(r)
+        self._call("nla3d_gpu_schur_apply", _ptr(r, ctypes.c_double), _ptr(z, ctypes.c_double), int(refresh))
+        return z
 
     def get_R(self):
         """vecR: reactions at the constrained equations, applied loads elsewhere"""

@@ -42,6 +42,7 @@ struct DevBuf {
 
 struct ThermalData;  // thermal.cpp
 struct LearnedWork;  // learned.cpp
+struct SchurAmg;     // schur.cpp
 struct Mlp;          // learned.cpp
 
 struct Ctx {
@@ -131,9 +132,19 @@ struct Ctx {
   KrylovMatrix krylovMatrix() const;
   Precond precond();
   // solve block(2) x = b with the current K; b, x device pointers [n2]; returns iterations
-  int minresSolve(const double* b, double* x, double tol, int maxit, double* relres_est, bool* breakdown);
+  int minresSolve(const double* b, double* x, double tol, int maxit, double* relres_est, bool* breakdown,
+                  const PrecondHook* hook = nullptr);
+  // multigrid on the Schur rows (schur.cpp)
+  std::shared_ptr<SchurAmg> schur;
 };
 
+// multigrid on the Schur rows (schur.cpp): structure from the full block(2) pattern and the
+// block-Jacobi groups; refresh for the current tangent (block Jacobi current); the MINRES stage
+void setupSchur(Ctx& c, const std::vector<int64_t>& rp, const std::vector<int>& col,
+                const std::vector<int64_t>& fullToVal, const std::vector<int>& gStart, const std::vector<int>& gInfo);
+bool schurUsable(Ctx& c, std::string* why);
+bool schurRefresh(Ctx& c, std::string* why);
+PrecondHook schurHook(Ctx& c);
 // constrained-DoF reactions of the last solve into vecR (solver.cpp), on `stream` (null: the handle's)
 void computeReactions(Ctx& c, hipStream_t stream = nullptr);
 // block-Jacobi blocks with the attached network's corrections (learned.cpp); t = load-step time

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
+
 namespace nla3d_amd {
 
 // K.block(2) in full-symmetric "grouped CSR": consecutive rows with identical
@@ -51,7 +53,15 @@ constexpr int kKrylovBlocks = 2048;
 void krylov_precond_setup(hipStream_t s, const KrylovMatrix& M, const Precond& P);
 void krylov_spmv(hipStream_t s, const KrylovMatrix& M, const double* x, double* y);
 void krylov_dot(hipStream_t s, int n, const double* a, const double* b, double* partial, double* out);
-void minres_start(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, const double* b);
-void minres_iteration(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, double tol);
+// A preconditioner stage after the block-Jacobi pass: given r and z = P_bj r (device, n), it
+// may replace part of z and must then rewrite the kKrylovBlocks partials of r . z (the
+// layout of the block-Jacobi pass); every kernel it launches must do nothing once
+// st->converged is set.
+using PrecondHook = std::function<void(hipStream_t, const double* r, double* z, double* partial,
+                                       const MinresState* st)>;
+void minres_start(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, const double* b,
+                  const PrecondHook* hook = nullptr);
+void minres_iteration(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, double tol,
+                      const PrecondHook* hook = nullptr);
 
 }  // namespace nla3d_amd

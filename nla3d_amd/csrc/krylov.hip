@@ -406,21 +406,25 @@ void krylov_dot(hipStream_t s, int n, const double* a, const double* b, double* 
   hipLaunchKernelGGL(finish_to_kernel, dim3(1), dim3(kRedThreads), 0, s, partial, nb, out);
 }
 
-void minres_start(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, const double* b) {
+void minres_start(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, const double* b,
+                  const PrecondHook* hook) {
   hipLaunchKernelGGL(precond_start_kernel, dim3(kKrylovBlocks), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo,
                      P.binv, b, w.R1, w.R2, w.Yp, w.partial);
+  if (hook) (*hook)(s, w.R1, w.Yp, w.partial, nullptr);
   hipLaunchKernelGGL(minres_init_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, kKrylovBlocks);
   // x0 = 0, w = w2 = 0 (a fill kernel rather than memset nodes, so the sequence can be graph-captured)
   hipLaunchKernelGGL(zero3_kernel, dim3(cdiv(M.n, 256)), dim3(256), 0, s, M.n, w.X, w.W1, w.W2);
 }
 
 // one MINRES iteration; buffers rotate on the host (pure pointer swaps)
-void minres_iteration(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, double tol) {
+void minres_iteration(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, double tol,
+                      const PrecondHook* hook) {
   hipLaunchKernelGGL(spmv_minres_kernel<kLPG>, dim3(kKrylovBlocks), dim3(256), 0, s, M, w.Yp, w.R1, w.T, w.st,
                      w.partial);
   hipLaunchKernelGGL(minres_scalar1_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, kKrylovBlocks);
   hipLaunchKernelGGL(precond_kernel, dim3(kKrylovBlocks), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo, P.binv, w.T,
                      w.R2, w.Z, w.st, w.partial);
+  if (hook) (*hook)(s, w.T, w.Z, w.partial, w.st);
   hipLaunchKernelGGL(minres_scalar2_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, kKrylovBlocks, tol);
   hipLaunchKernelGGL(minres_update_kernel, dim3(cdiv(M.n, 256)), dim3(256), 0, s, M.n, w.Yp, w.W1, w.W2, w.X, w.st);
   // rotations: R1 <- R2, R2 <- T, T <- old R1 ; Yp <- Z ; (W1, W2) <- (W2, Wn)

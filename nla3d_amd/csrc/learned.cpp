@@ -613,14 +613,22 @@ int nla3d_gpu_solve_block2_pc(nla3d_gpu_t h, const double* rhs, double* x, doubl
     hipCheck(hipSetDevice(c.device), "hipSetDevice");
     requireLearnable(c);
     if (!rhs || !x) throw ApiError(-1, "null vectors");
-    if (preconditioner == 1) learnedPrecondSetup(c, t);
-    else if (preconditioner == 0) krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
-    else throw ApiError(-1, "preconditioner must be 0 (block Jacobi) or 1 (learned)");
+    PrecondHook hook;
+    if (preconditioner < 0 || preconditioner > 3)
+      throw ApiError(-1, "preconditioner must be 0 (block Jacobi), 1 (learned), 2 (Schur multigrid) or 3 (both)");
+    if (preconditioner == 1 || preconditioner == 3) learnedPrecondSetup(c, t);
+    else krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
+    const bool mg = preconditioner >= 2;
+    if (mg) {
+      std::string why;
+      if (!schurUsable(c, &why) || !schurRefresh(c, &why)) throw ApiError(-3, "Schur multigrid: " + why);
+      hook = schurHook(c);
+    }
     double* b = c.d_mw[7].as<double>();
     hipCheck(hipMemcpyAsync(b, rhs, (size_t)c.n2 * 8, hipMemcpyHostToDevice, c.stream), "rhs");
     double rr = 0;
     bool brk = false;
-    const int it = c.minresSolve(b, c.d_dU.as<double>(), tol, max_iter, &rr, &brk);
+    const int it = c.minresSolve(b, c.d_dU.as<double>(), tol, max_iter, &rr, &brk, mg ? &hook : nullptr);
     hipCheck(hipMemcpy(x, c.d_dU.p, (size_t)c.n2 * 8, hipMemcpyDeviceToHost), "x");
     if (iterations) *iterations = it;
     if (relres) *relres = rr;

@@ -1,0 +1,524 @@
+// Host side of the Schur-row multigrid preconditioner (see schur.h): structure set-up
+// (once per model), the refresh of values with every tangent, the V-cycle launch
+// sequence, its MINRES hook and the ABI entries that expose S and one V-cycle.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <vector>
+
+#include "../../include/nla3d_gpu.h"
+#include "ctx.h"
+#include "learned.h"
+#include "schur.h"
+
+namespace nla3d_amd {
+
+constexpr int kCoarsest = 256;   // at most this many rows are inverted densely
+constexpr int kMaxCoarse = 2048; // coarsening that stalls above this disables the preconditioner
+constexpr int kMaxQPerNode = 64;
+
+struct AmgLevel {
+  int n = 0, nnz = 0, nc = 0;
+  std::vector<int> h_rowPtr, h_col;
+  DevBuf rowPtr, col, val, dinv, agg, memPtr, mem, gSeg, gSrc;
+  DevBuf r, x, res, da, db, t, start;
+  double lmax = 1.0;
+  AmgCsr csr() const { return AmgCsr{n, rowPtr.as<int>(), col.as<int>(), val.as<double>()}; }
+};
+
+struct SchurAmg {
+  bool available = false;
+  std::string why;
+  int nq = 0;
+  std::vector<int> h_qrow;
+  DevBuf qrow;
+  int nNodes = 0, nPairs = 0, nContrib = 0;
+  DevBuf pairNode, nodeGroup, nodeSize, qPtr, cPtr, posB, contrib;
+  DevBuf sSeg, sSrc, sDiag, cPos;
+  std::vector<std::unique_ptr<AmgLevel>> L;
+  DevBuf cinv;
+  int steps = 3;
+  bool fresh = false;
+};
+
+// stable counting sort of (target, source) pairs by target: seg[t]..seg[t+1] lists the sources
+static void countingMap(int nTargets, const std::vector<int>& target, std::vector<int>& seg, std::vector<int>& src) {
+  seg.assign(nTargets + 1, 0);
+  for (int t : target) seg[t + 1]++;
+  for (int t = 0; t < nTargets; t++) seg[t + 1] += seg[t];
+  src.assign(target.size(), 0);
+  std::vector<int> fill(seg.begin(), seg.end() - 1);
+  for (size_t k = 0; k < target.size(); k++) src[fill[target[k]]++] = (int)k;
+}
+
+static int findIn(const std::vector<int>& a, int b, int e, int v) {
+  auto it = std::lower_bound(a.begin() + b, a.begin() + e, v);
+  return (it != a.begin() + e && *it == v) ? (int)(it - a.begin()) : -1;
+}
+
+// greedy distance-one aggregation of a CSR graph
+static int aggregate(const std::vector<int>& rp, const std::vector<int>& cl, std::vector<int>& agg) {
+  const int n = (int)rp.size() - 1;
+  agg.assign(n, -1);
+  int nc = 0;
+  for (int i = 0; i < n; i++) {
+    if (agg[i] >= 0) continue;
+    bool free = true;
+    for (int k = rp[i]; k < rp[i + 1] && free; k++)
+      if (agg[cl[k]] >= 0) free = false;
+    if (!free) continue;
+    for (int k = rp[i]; k < rp[i + 1]; k++) agg[cl[k]] = nc;
+    agg[i] = nc++;
+  }
+  for (int i = 0; i < n; i++) {
+    if (agg[i] >= 0) continue;
+    for (int k = rp[i]; k < rp[i + 1]; k++)
+      if (agg[cl[k]] >= 0) {
+        agg[i] = agg[cl[k]];
+        break;
+      }
+    if (agg[i] < 0) agg[i] = nc++;
+  }
+  return nc;
+}
+
+static uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+static void allocLevel(AmgLevel& A, bool coarsest) {
+  A.rowPtr.upload(A.h_rowPtr);
+  A.col.upload(A.h_col);
+  A.val.alloc((size_t)std::max(A.nnz, 1) * 8);
+  const size_t nb = (size_t)std::max(A.n, 1) * 8;
+  A.r.alloc(nb);
+  A.x.alloc(nb);
+  if (coarsest) return;
+  A.dinv.alloc(nb);
+  A.res.alloc(nb);
+  A.da.alloc(nb);
+  A.db.alloc(nb);
+  A.t.alloc(nb);
+  std::vector<double> st(A.n);
+  for (int i = 0; i < A.n; i++) st[i] = (double)(mix64((uint64_t)i) >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+  A.start.upload(st);
+}
+
+void setupSchur(Ctx& c, const std::vector<int64_t>& rp, const std::vector<int>& col,
+                const std::vector<int64_t>& fullToVal, const std::vector<int>& gStart, const std::vector<int>& gInfo) {
+  auto S = std::make_shared<SchurAmg>();
+  c.schur = S;
+  const int n2 = c.n2;
+  std::vector<int> qOf(n2, -1);
+  for (size_t g = 0; g < gStart.size(); g++)
+    if ((gInfo[g] >> 2) == 1) {
+      qOf[gStart[g]] = (int)S->h_qrow.size();
+      S->h_qrow.push_back(gStart[g]);
+    }
+  S->nq = (int)S->h_qrow.size();
+  if (S->nq == 0) {
+    S->why = "no Schur rows";
+    return;
+  }
+  for (int r : S->h_qrow)
+    for (int64_t k = rp[r]; k < rp[r + 1]; k++)
+      if (col[k] != r && qOf[col[k]] >= 0) {
+        S->why = "Schur rows coupled to each other (element-less node DoFs)";
+        return;
+      }
+  auto fullPos = [&](int r, int cc) -> int {
+    auto b = col.begin() + rp[r], e = col.begin() + rp[r + 1];
+    auto it = std::lower_bound(b, e, cc);
+    return (it != e && *it == cc) ? (int)fullToVal[it - col.begin()] : -1;
+  };
+  // node groups with their Schur columns
+  std::vector<int> nodeGroup, nodeSize, qPtr{0}, qlist, posB, pairNode;
+  for (size_t g = 0; g < gStart.size(); g++) {
+    if ((gInfo[g] >> 2) != 0) continue;
+    const int r0 = gStart[g], sz = gInfo[g] & 3;
+    std::vector<int> qs;
+    for (int a = 0; a < sz; a++)
+      for (int64_t k = rp[r0 + a]; k < rp[r0 + a + 1]; k++)
+        if (qOf[col[k]] >= 0) qs.push_back(qOf[col[k]]);
+    if (qs.empty()) continue;
+    std::sort(qs.begin(), qs.end());
+    qs.erase(std::unique(qs.begin(), qs.end()), qs.end());
+    if ((int)qs.size() > kMaxQPerNode) {
+      S->why = "a node couples to too many Schur rows";
+      return;
+    }
+    const int node = (int)nodeGroup.size();
+    nodeGroup.push_back((int)g);
+    nodeSize.push_back(sz);
+    for (int q : qs) {
+      qlist.push_back(q);
+      pairNode.push_back(node);
+      for (int r = 0; r < 3; r++) posB.push_back(r < sz ? fullPos(r0 + r, S->h_qrow[q]) : -1);
+    }
+    qPtr.push_back((int)qlist.size());
+  }
+  const int nNodes = (int)nodeGroup.size();
+  // pattern of S: all Schur-column pairs of a node, and the diagonal
+  std::vector<std::vector<int>> srow(S->nq);
+  for (int q = 0; q < S->nq; q++) srow[q].push_back(q);
+  for (int nd = 0; nd < nNodes; nd++)
+    for (int a = qPtr[nd]; a < qPtr[nd + 1]; a++)
+      for (int b = qPtr[nd]; b < qPtr[nd + 1]; b++) srow[qlist[a]].push_back(qlist[b]);
+  auto L0 = std::make_unique<AmgLevel>();
+  L0->n = S->nq;
+  L0->h_rowPtr.assign(S->nq + 1, 0);
+  for (int q = 0; q < S->nq; q++) {
+    auto& v = srow[q];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    L0->h_rowPtr[q + 1] = L0->h_rowPtr[q] + (int)v.size();
+  }
+  if ((int64_t)L0->h_rowPtr[S->nq] >= (1ll << 31)) throw ApiError(-1, "Schur matrix too large");
+  L0->nnz = L0->h_rowPtr[S->nq];
+  L0->h_col.reserve(L0->nnz);
+  for (int q = 0; q < S->nq; q++) L0->h_col.insert(L0->h_col.end(), srow[q].begin(), srow[q].end());
+  std::vector<std::vector<int>>().swap(srow);
+  // contributions -> S entries
+  std::vector<int> cPtr(nNodes + 1, 0), target;
+  for (int nd = 0; nd < nNodes; nd++) {
+    const int m = qPtr[nd + 1] - qPtr[nd];
+    cPtr[nd + 1] = cPtr[nd] + m * m;
+  }
+  target.reserve(cPtr[nNodes]);
+  for (int nd = 0; nd < nNodes; nd++)
+    for (int a = qPtr[nd]; a < qPtr[nd + 1]; a++) {
+      const int qa = qlist[a];
+      for (int b = qPtr[nd]; b < qPtr[nd + 1]; b++)
+        target.push_back(findIn(L0->h_col, L0->h_rowPtr[qa], L0->h_rowPtr[qa + 1], qlist[b]));
+    }
+  std::vector<int> seg, src;
+  countingMap(L0->nnz, target, seg, src);
+  std::vector<int>().swap(target);
+  std::vector<int> sDiag(S->nq), cPos(S->nq);
+  for (int q = 0; q < S->nq; q++) {
+    sDiag[q] = findIn(L0->h_col, L0->h_rowPtr[q], L0->h_rowPtr[q + 1], q);
+    cPos[q] = fullPos(S->h_qrow[q], S->h_qrow[q]);
+  }
+  // aggregation levels
+  S->L.push_back(std::move(L0));
+  for (;;) {
+    AmgLevel& F = *S->L.back();
+    if (F.n <= kCoarsest) break;
+    std::vector<int> agg;
+    const int nc = aggregate(F.h_rowPtr, F.h_col, agg);
+    if (nc > 0.85 * F.n) {
+      if (F.n > kMaxCoarse) {
+        S->why = "aggregation stalled";
+        S->L.clear();
+        return;
+      }
+      break;
+    }
+    F.nc = nc;
+    std::vector<std::vector<int>> crow(nc), members(nc);
+    for (int i = 0; i < F.n; i++) {
+      members[agg[i]].push_back(i);
+      for (int k = F.h_rowPtr[i]; k < F.h_rowPtr[i + 1]; k++) crow[agg[i]].push_back(agg[F.h_col[k]]);
+    }
+    auto C = std::make_unique<AmgLevel>();
+    C->n = nc;
+    C->h_rowPtr.assign(nc + 1, 0);
+    for (int I = 0; I < nc; I++) {
+      auto& v = crow[I];
+      std::sort(v.begin(), v.end());
+      v.erase(std::unique(v.begin(), v.end()), v.end());
+      C->h_rowPtr[I + 1] = C->h_rowPtr[I] + (int)v.size();
+    }
+    C->nnz = C->h_rowPtr[nc];
+    for (int I = 0; I < nc; I++) C->h_col.insert(C->h_col.end(), crow[I].begin(), crow[I].end());
+    std::vector<int> tgt(F.nnz);
+    for (int i = 0; i < F.n; i++)
+      for (int k = F.h_rowPtr[i]; k < F.h_rowPtr[i + 1]; k++) {
+        const int I = agg[i], J = agg[F.h_col[k]];
+        tgt[k] = findIn(C->h_col, C->h_rowPtr[I], C->h_rowPtr[I + 1], J);
+      }
+    std::vector<int> gseg, gsrc, mptr(nc + 1, 0), mem;
+    countingMap(C->nnz, tgt, gseg, gsrc);
+    for (int I = 0; I < nc; I++) {
+      mptr[I + 1] = mptr[I] + (int)members[I].size();
+      mem.insert(mem.end(), members[I].begin(), members[I].end());
+    }
+    F.agg.upload(agg);
+    F.memPtr.upload(mptr);
+    F.mem.upload(mem);
+    F.gSeg.upload(gseg);
+    F.gSrc.upload(gsrc);
+    S->L.push_back(std::move(C));
+  }
+  if (S->L.back()->n > kMaxCoarse) {
+    S->why = "coarsest level too large";
+    S->L.clear();
+    return;
+  }
+  for (size_t l = 0; l < S->L.size(); l++) allocLevel(*S->L[l], l + 1 == S->L.size());
+  const int ncst = S->L.back()->n;
+  S->cinv.alloc((size_t)ncst * ncst * 8);
+  S->qrow.upload(S->h_qrow);
+  S->nNodes = nNodes;
+  S->nPairs = (int)qlist.size();
+  S->nContrib = cPtr[nNodes];
+  S->pairNode.upload(pairNode);
+  S->nodeGroup.upload(nodeGroup);
+  S->nodeSize.upload(nodeSize);
+  S->qPtr.upload(qPtr);
+  S->cPtr.upload(cPtr);
+  S->posB.upload(posB);
+  S->contrib.alloc((size_t)std::max(S->nContrib, 1) * 8);
+  S->sSeg.upload(seg);
+  S->sSrc.upload(src);
+  S->sDiag.upload(sDiag);
+  S->cPos.upload(cPos);
+  S->available = true;
+}
+
+static double hostDot(Ctx& c, int n, const double* a, const double* b) {
+  krylov_dot(c.stream, n, a, b, c.d_partial.as<double>(), c.d_scal.as<double>() + 9);
+  double v;
+  hipCheck(hipMemcpyAsync(&v, c.d_scal.as<double>() + 9, 8, hipMemcpyDeviceToHost, c.stream), "dot");
+  hipCheck(hipStreamSynchronize(c.stream), "dot sync");
+  return v;
+}
+
+// values of every level for the current tangent (block-Jacobi blocks must be current);
+// returns false (with the reason) when the coarsest matrix is not positive definite
+bool schurRefresh(Ctx& c, std::string* why) {
+  SchurAmg& S = *c.schur;
+  hipStream_t s = c.stream;
+  const double* kval = c.d_full2.as<double>();
+  AmgLevel& L0 = *S.L[0];
+  launch_schur_contrib(s, S.nPairs, S.pairNode.as<int>(), S.nodeGroup.as<int>(), S.nodeSize.as<int>(),
+                       S.qPtr.as<int>(), S.cPtr.as<int>(), S.posB.as<int>(), kval, c.d_binv.as<double>(),
+                       S.contrib.as<double>());
+  launch_gather_sum(s, L0.nnz, S.sSeg.as<int>(), S.sSrc.as<int>(), S.contrib.as<double>(), L0.val.as<double>());
+  launch_schur_sub_c(s, S.nq, S.sDiag.as<int>(), S.cPos.as<int>(), kval, L0.val.as<double>());
+  for (size_t l = 0; l + 1 < S.L.size(); l++) {
+    AmgLevel& A = *S.L[l];
+    launch_inv_diag(s, A.csr(), A.dinv.as<double>());
+    // largest eigenvalue of D^-1 A by power iteration (12 steps from a fixed pseudo-random start)
+    double* x = A.da.as<double>();
+    double* y = A.db.as<double>();
+    hipCheck(hipMemcpyAsync(x, A.start.p, (size_t)A.n * 8, hipMemcpyDeviceToDevice, s), "start");
+    double lam = 1.0;
+    for (int it = 0; it < 13; it++) {
+      const double nx = std::sqrt(hostDot(c, A.n, x, x));
+      if (!(nx > 0.0)) break;
+      launch_dinv_spmv(s, A.csr(), A.dinv.as<double>(), x, 1.0 / nx, y);
+      std::swap(x, y);
+      if (it == 12) lam = std::sqrt(hostDot(c, A.n, x, x));
+    }
+    A.lmax = lam;
+    AmgLevel& C = *S.L[l + 1];
+    launch_gather_sum(s, C.nnz, A.gSeg.as<int>(), A.gSrc.as<int>(), A.val.as<double>(), C.val.as<double>());
+  }
+  hipCheck(hipGetLastError(), "schur refresh");
+  // coarsest: explicit inverse by Cholesky on the host
+  AmgLevel& Z = *S.L.back();
+  const int n = Z.n;
+  std::vector<double> v(Z.nnz), A((size_t)n * n, 0.0);
+  hipCheck(hipMemcpyAsync(v.data(), Z.val.p, v.size() * 8, hipMemcpyDeviceToHost, s), "coarse");
+  hipCheck(hipStreamSynchronize(s), "sync");
+  for (int i = 0; i < n; i++)
+    for (int k = Z.h_rowPtr[i]; k < Z.h_rowPtr[i + 1]; k++) A[(size_t)i * n + Z.h_col[k]] = v[k];
+  for (int j = 0; j < n; j++) {
+    double d = A[(size_t)j * n + j];
+    for (int k = 0; k < j; k++) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+    if (!(d > 0.0)) {
+      if (why) *why = "coarsest Schur matrix not positive definite";
+      S.fresh = false;
+      return false;
+    }
+    const double ljj = std::sqrt(d);
+    A[(size_t)j * n + j] = ljj;
+    for (int i = j + 1; i < n; i++) {
+      double t = A[(size_t)i * n + j];
+      for (int k = 0; k < j; k++) t -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+      A[(size_t)i * n + j] = t / ljj;
+    }
+  }
+  // inverse = L^-T L^-1, column by column
+  std::vector<double> inv((size_t)n * n), e(n);
+  for (int j = 0; j < n; j++) {
+    std::fill(e.begin(), e.end(), 0.0);
+    e[j] = 1.0;
+    for (int i = 0; i < n; i++) {  // forward: L y = e
+      double t = e[i];
+      for (int k = 0; k < i; k++) t -= A[(size_t)i * n + k] * e[k];
+      e[i] = t / A[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {  // backward: L^T x = y
+      double t = e[i];
+      for (int k = i + 1; k < n; k++) t -= A[(size_t)k * n + i] * e[k];
+      e[i] = t / A[(size_t)i * n + i];
+    }
+    for (int i = 0; i < n; i++) inv[(size_t)i * n + j] = e[i];
+  }
+  hipCheck(hipMemcpyAsync(S.cinv.p, inv.data(), inv.size() * 8, hipMemcpyHostToDevice, s), "cinv");
+  hipCheck(hipStreamSynchronize(s), "sync");
+  S.fresh = true;
+  return true;
+}
+
+// Chebyshev coefficients on [lmax/10, 1.05 lmax] (Jacobi-preconditioned, steps terms)
+struct ChebCoef {
+  double c0;
+  double c1[8], c2[8];
+};
+static ChebCoef chebCoef(double lmax, int steps) {
+  const double hi = 1.05 * lmax, lo = lmax / 10.0;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+  ChebCoef k{};
+  k.c0 = 1.0 / theta;
+  double rho = 1.0 / sigma;
+  for (int i = 1; i < steps && i < 8; i++) {
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    k.c1[i] = rn * rho;
+    k.c2[i] = 2.0 * rn / delta;
+    rho = rn;
+  }
+  return k;
+}
+
+static void vcycle(SchurAmg& S, hipStream_t s, size_t l) {
+  AmgLevel& A = *S.L[l];
+  if (l + 1 == S.L.size()) {
+    launch_dense_matvec(s, A.n, S.cinv.as<double>(), A.r.as<double>(), A.x.as<double>());
+    return;
+  }
+  const ChebCoef k = chebCoef(A.lmax, S.steps);
+  const AmgCsr M = A.csr();
+  double* dinv = A.dinv.as<double>();
+  double* din = A.da.as<double>();
+  double* dout = A.db.as<double>();
+  launch_cheb_start(s, A.n, A.r.as<double>(), dinv, k.c0, A.res.as<double>(), din, A.x.as<double>());
+  for (int i = 1; i < S.steps; i++) {
+    launch_cheb_step(s, M, din, dinv, k.c1[i], k.c2[i], A.res.as<double>(), dout, A.x.as<double>());
+    std::swap(din, dout);
+  }
+  AmgLevel& C = *S.L[l + 1];
+  launch_residual_restrict(s, M, A.res.as<double>(), din, A.t.as<double>(), A.nc, A.memPtr.as<int>(),
+                           A.mem.as<int>(), C.r.as<double>(), A.x.as<double>());
+  vcycle(S, s, l + 1);
+  launch_prolong(s, A.n, A.agg.as<int>(), C.x.as<double>(), A.x.as<double>());
+  din = A.da.as<double>();
+  dout = A.db.as<double>();
+  launch_cheb_start_x(s, M, A.r.as<double>(), dinv, k.c0, A.res.as<double>(), din, A.x.as<double>());
+  for (int i = 1; i < S.steps; i++) {
+    launch_cheb_step(s, M, din, dinv, k.c1[i], k.c2[i], A.res.as<double>(), dout, A.x.as<double>());
+    std::swap(din, dout);
+  }
+  launch_axpby(s, A.n, 1.0, din, 1.0, A.x.as<double>(), A.x.as<double>());
+}
+
+// the MINRES stage: z[Schur rows] = V-cycle(r[Schur rows]); partials of r . z
+PrecondHook schurHook(Ctx& c) {
+  std::shared_ptr<SchurAmg> S = c.schur;
+  const int n2 = c.n2;
+  return [S, n2](hipStream_t s, const double* r, double* z, double* partial, const MinresState* st) {
+    AmgLevel& L0 = *S->L[0];
+    launch_q_gather(s, S->nq, S->qrow.as<int>(), r, st, L0.r.as<double>());
+    vcycle(*S, s, 0);
+    launch_q_scatter(s, S->nq, S->qrow.as<int>(), L0.x.as<double>(), st, z);
+    launch_dot_partials(s, n2, r, z, st, partial);
+  };
+}
+
+bool schurUsable(Ctx& c, std::string* why) {
+  if (!c.schur || !c.schur->available) {
+    if (why) *why = c.schur ? c.schur->why : "Krylov data not set up";
+    return false;
+  }
+  return true;
+}
+
+}  // namespace nla3d_amd
+
+using namespace nla3d_amd;
+
+static int schurGuard(nla3d_gpu_t h, const std::function<void(Ctx&)>& fn) {
+  if (!h) return NLA3D_GPU_EINVAL;
+  Ctx& c = *reinterpret_cast<Ctx*>(h);
+  try {
+    c.err.clear();
+    c.requireSolid();
+    hipCheck(hipSetDevice(c.device), "hipSetDevice");
+    if (!c.krylovReady) c.setupKrylov();
+    fn(c);
+    return NLA3D_GPU_OK;
+  } catch (const ApiError& e) {
+    c.err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    c.err = e.what();
+    return NLA3D_GPU_EINVAL;
+  }
+}
+
+static void refreshOrThrow(Ctx& c) {
+  std::string why;
+  if (!schurUsable(c, &why)) throw ApiError(-3, "Schur multigrid unavailable: " + why);
+  krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
+  if (!schurRefresh(c, &why)) throw ApiError(-6, "Schur multigrid: " + why);
+}
+
+extern "C" {
+
+int nla3d_gpu_schur_info(nla3d_gpu_t h, int* available, int* nq, int* nnz0, int* levels, int* sizes, int cap) {
+  return schurGuard(h, [&](Ctx& c) {
+    SchurAmg* S = c.schur.get();
+    if (available) *available = S && S->available ? 1 : 0;
+    if (nq) *nq = S ? S->nq : 0;
+    if (nnz0) *nnz0 = S && S->available ? S->L[0]->nnz : 0;
+    if (levels) *levels = S && S->available ? (int)S->L.size() : 0;
+    if (sizes && S && S->available)
+      for (int l = 0; l < (int)S->L.size() && l < cap; l++) sizes[l] = S->L[l]->n;
+  });
+}
+
+int nla3d_gpu_schur_matrix(nla3d_gpu_t h, int* qrow, int* rowPtr, int* col, double* val) {
+  return schurGuard(h, [&](Ctx& c) {
+    refreshOrThrow(c);
+    SchurAmg& S = *c.schur;
+    AmgLevel& L0 = *S.L[0];
+    if (qrow) std::memcpy(qrow, S.h_qrow.data(), S.h_qrow.size() * 4);
+    if (rowPtr) std::memcpy(rowPtr, L0.h_rowPtr.data(), L0.h_rowPtr.size() * 4);
+    if (col) std::memcpy(col, L0.h_col.data(), L0.h_col.size() * 4);
+    if (val) hipCheck(hipMemcpy(val, L0.val.p, (size_t)L0.nnz * 8, hipMemcpyDeviceToHost), "S");
+  });
+}
+
+int nla3d_gpu_schur_level(nla3d_gpu_t h, int level, int* agg, double* lmax) {
+  return schurGuard(h, [&](Ctx& c) {
+    SchurAmg* S = c.schur.get();
+    if (!S || !S->available || level < 0 || level >= (int)S->L.size()) throw ApiError(-1, "no such level");
+    AmgLevel& A = *S->L[level];
+    if (lmax) *lmax = A.lmax;
+    if (agg && level + 1 < (int)S->L.size())
+      hipCheck(hipMemcpy(agg, A.agg.p, (size_t)A.n * 4, hipMemcpyDeviceToHost), "agg");
+  });
+}
+
+int nla3d_gpu_schur_apply(nla3d_gpu_t h, const double* r, double* z, int refresh) {
+  return schurGuard(h, [&](Ctx& c) {
+    if (!r || !z) throw ApiError(-1, "null vectors");
+    if (refresh || !c.schur || !c.schur->fresh) refreshOrThrow(c);
+    SchurAmg& S = *c.schur;
+    AmgLevel& L0 = *S.L[0];
+    hipCheck(hipMemcpyAsync(L0.r.p, r, (size_t)S.nq * 8, hipMemcpyHostToDevice, c.stream), "r");
+    vcycle(S, c.stream, 0);
+    hipCheck(hipGetLastError(), "vcycle");
+    hipCheck(hipMemcpyAsync(z, L0.x.p, (size_t)S.nq * 8, hipMemcpyDeviceToHost, c.stream), "z");
+    hipCheck(hipStreamSynchronize(c.stream), "sync");
+  });
+}
+
+}  // extern "C"

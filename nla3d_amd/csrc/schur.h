@@ -1,0 +1,56 @@
+// Multigrid preconditioner for the Schur rows of block(2) (schur.hip, schur.cpp).
+//
+// The rows of block(2) that block Jacobi treats by a diagonal Schur estimate (element
+// pressures, MPC multipliers) are preconditioned instead by one V-cycle of
+// aggregation multigrid on the sparse Schur approximation
+//     S = B^T D^-1 B - C,
+// B the couplings of those rows to the node DoFs, D^-1 the block-Jacobi inverse node
+// blocks, C their own block (the pressure rows' -V/k).  S is SPD when the node blocks
+// are.  Levels: piecewise-constant aggregation of S's graph (greedy, distance one),
+// Galerkin coarse matrices P^T A P, Jacobi-Chebyshev smoothing (the same polynomial
+// before and after, so the cycle is symmetric), an explicit dense inverse on the
+// coarsest level.  All structure is built once per model; values, smoother intervals
+// (power iteration) and the coarsest inverse are refreshed with every tangent.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "krylov.h"
+
+namespace nla3d_amd {
+
+// device view of one level's CSR (full symmetric)
+struct AmgCsr {
+  int n;
+  const int* rowPtr;
+  const int* col;
+  const double* val;
+};
+
+// level-0 assembly of S: one thread per (node, Schur column a) writes B(:,a)^T D^-1 B(:,b)
+void launch_schur_contrib(hipStream_t s, int nPairs, const int* pairNode, const int* nodeGroup, const int* nodeSize,
+                          const int* qPtr, const int* cPtr, const int* posB, const double* kval, const double* binv,
+                          double* contrib);
+void launch_schur_sub_c(hipStream_t s, int nq, const int* sDiag, const int* cPos, const double* kval, double* sval);
+void launch_gather_sum(hipStream_t s, int n, const int* seg, const int* src, const double* in, double* out);
+void launch_inv_diag(hipStream_t s, const AmgCsr& A, double* dinv);
+
+// V-cycle pieces (c0, c1, c2: Chebyshev coefficients of a step)
+void launch_cheb_start(hipStream_t s, int n, const double* r, const double* dinv, double c0, double* res, double* d,
+                       double* x);
+void launch_cheb_start_x(hipStream_t s, const AmgCsr& A, const double* r, const double* dinv, double c0, double* res,
+                         double* d, double* x);
+void launch_cheb_step(hipStream_t s, const AmgCsr& A, const double* din, const double* dinv, double c1, double c2,
+                      double* res, double* dout, double* x);
+void launch_residual_restrict(hipStream_t s, const AmgCsr& A, const double* res, const double* dlast, double* t,
+                              int nc, const int* memPtr, const int* mem, double* rc, double* x);
+void launch_prolong(hipStream_t s, int n, const int* agg, const double* xc, double* x);
+void launch_dense_matvec(hipStream_t s, int n, const double* Ainv, const double* r, double* x);
+void launch_q_gather(hipStream_t s, int nq, const int* qrow, const double* v, const MinresState* st, double* out);
+void launch_q_scatter(hipStream_t s, int nq, const int* qrow, const double* v, const MinresState* st, double* out);
+void launch_dot_partials(hipStream_t s, int n, const double* a, const double* b, const MinresState* st,
+                         double* partial);
+// y = scale * D^-1 A x (power iteration)
+void launch_dinv_spmv(hipStream_t s, const AmgCsr& A, const double* dinv, const double* x, double scale, double* y);
+
+}  // namespace nla3d_amd

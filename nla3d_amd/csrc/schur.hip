@@ -1,0 +1,283 @@
+// Kernels of the Schur-row multigrid preconditioner (see schur.h).
+#include <hip/hip_runtime.h>
+
+#include "schur.h"
+
+namespace nla3d_amd {
+namespace {
+
+inline int cdiv(long long a, int b) { return (int)((a + b - 1) / b); }
+constexpr int kLanes = 8;  // lanes per CSR row (S rows hold about 27 entries)
+constexpr int kGrid = kKrylovBlocks;
+
+// one row of A times x, spread over kLanes lanes and reduced (every lane returns it)
+__device__ __forceinline__ double row_dot(const AmgCsr& A, int i, int lane, const double* __restrict__ x) {
+  double s = 0.0;
+  for (int k = A.rowPtr[i] + lane; k < A.rowPtr[i + 1]; k += kLanes) s += A.val[k] * x[A.col[k]];
+  for (int o = kLanes / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kLanes);
+  return s;
+}
+
+// contribution B(:,a)^T D^-1 B(:,b) of node n to S(q_a, q_b), one thread per (node, a)
+__global__ __launch_bounds__(256) void schur_contrib_kernel(int nPairs, const int* __restrict__ pairNode,
+                                                          const int* __restrict__ nodeGroup,
+                                                          const int* __restrict__ nodeSize,
+                                                          const int* __restrict__ qPtr, const int* __restrict__ cPtr,
+                                                          const int* __restrict__ posB, const double* __restrict__ kval,
+                                                          const double* __restrict__ binv, double* __restrict__ contrib) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nPairs) return;
+  const int n = pairNode[k];
+  const int q0 = qPtr[n], m = qPtr[n + 1] - q0, a = k - q0, sz = nodeSize[n];
+  const double* D = binv + (size_t)nodeGroup[n] * 9;
+  double ba[3] = {0, 0, 0};
+  for (int r = 0; r < sz; r++) {
+    const int p = posB[(size_t)(q0 + a) * 3 + r];
+    ba[r] = p >= 0 ? kval[p] : 0.0;
+  }
+  double wa[3] = {0, 0, 0};  // D^-1 B(:,a)
+  for (int r = 0; r < sz; r++)
+    for (int c = 0; c < sz; c++) wa[r] += D[r * 3 + c] * ba[c];
+  double* out = contrib + cPtr[n] + (size_t)a * m;
+  for (int b = 0; b < m; b++) {
+    double s = 0.0;
+    for (int r = 0; r < sz; r++) {
+      const int p = posB[(size_t)(q0 + b) * 3 + r];
+      if (p >= 0) s += wa[r] * kval[p];
+    }
+    out[b] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void schur_sub_c_kernel(int nq, const int* __restrict__ sDiag,
+                                                        const int* __restrict__ cPos, const double* __restrict__ kval,
+                                                        double* __restrict__ sval) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  if (cPos[q] >= 0) sval[sDiag[q]] -= kval[cPos[q]];
+}
+
+__global__ __launch_bounds__(256) void gather_sum_kernel(int n, const int* __restrict__ seg,
+                                                       const int* __restrict__ src, const double* __restrict__ in,
+                                                       double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double acc = 0.0;
+  for (int s = seg[t]; s < seg[t + 1]; s++) acc += in[src[s]];
+  out[t] = acc;
+}
+
+__global__ __launch_bounds__(256) void inv_diag_kernel(AmgCsr A, double* __restrict__ dinv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  double d = 0.0;
+  for (int k = A.rowPtr[i]; k < A.rowPtr[i + 1]; k++)
+    if (A.col[k] == i) d = A.val[k];
+  dinv[i] = d > 0.0 ? 1.0 / d : 0.0;
+}
+
+// res = r ; d = c0 D^-1 r ; x = 0
+__global__ __launch_bounds__(256) void cheb_start_kernel(int n, const double* __restrict__ r,
+                                                       const double* __restrict__ dinv, double c0,
+                                                       double* __restrict__ res, double* __restrict__ d,
+                                                       double* __restrict__ x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double ri = r[i];
+    res[i] = ri;
+    d[i] = c0 * dinv[i] * ri;
+    x[i] = 0.0;
+  }
+}
+
+// res = r - A x ; d = c0 D^-1 res   (x is read, not written)
+__global__ __launch_bounds__(256) void cheb_start_x_kernel(AmgCsr A, const double* __restrict__ r,
+                                                         const double* __restrict__ dinv, double c0,
+                                                         double* __restrict__ res, double* __restrict__ d,
+                                                         const double* __restrict__ x) {
+  const int lane = threadIdx.x % kLanes, rpb = blockDim.x / kLanes;
+  for (int i = blockIdx.x * rpb + threadIdx.x / kLanes; i < A.n; i += gridDim.x * rpb) {
+    const double ax = row_dot(A, i, lane, x);
+    if (lane == 0) {
+      const double ri = r[i] - ax;
+      res[i] = ri;
+      d[i] = c0 * dinv[i] * ri;
+    }
+  }
+}
+
+// res -= A din ; dout = c1 din + c2 D^-1 res ; x += din
+__global__ __launch_bounds__(256) void cheb_step_kernel(AmgCsr A, const double* __restrict__ din,
+                                                      const double* __restrict__ dinv, double c1, double c2,
+                                                      double* __restrict__ res, double* __restrict__ dout,
+                                                      double* __restrict__ x) {
+  const int lane = threadIdx.x % kLanes, rpb = blockDim.x / kLanes;
+  for (int i = blockIdx.x * rpb + threadIdx.x / kLanes; i < A.n; i += gridDim.x * rpb) {
+    const double ad = row_dot(A, i, lane, din);
+    if (lane == 0) {
+      const double ri = res[i] - ad;
+      res[i] = ri;
+      const double di = din[i];
+      dout[i] = c1 * di + c2 * dinv[i] * ri;
+      x[i] += di;
+    }
+  }
+}
+
+// t = res - A dlast (the residual of x + dlast) ; x += dlast
+__global__ __launch_bounds__(256) void residual_kernel(AmgCsr A, const double* __restrict__ res,
+                                                     const double* __restrict__ dlast, double* __restrict__ t,
+                                                     double* __restrict__ x) {
+  const int lane = threadIdx.x % kLanes, rpb = blockDim.x / kLanes;
+  for (int i = blockIdx.x * rpb + threadIdx.x / kLanes; i < A.n; i += gridDim.x * rpb) {
+    const double ad = row_dot(A, i, lane, dlast);
+    if (lane == 0) {
+      t[i] = res[i] - ad;
+      x[i] += dlast[i];
+    }
+  }
+}
+
+// rc[I] = sum of t over the members of aggregate I, in member order
+__global__ __launch_bounds__(256) void restrict_kernel(int nc, const int* __restrict__ memPtr,
+                                                     const int* __restrict__ mem, const double* __restrict__ t,
+                                                     double* __restrict__ rc) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nc) return;
+  double s = 0.0;
+  for (int k = memPtr[I]; k < memPtr[I + 1]; k++) s += t[mem[k]];
+  rc[I] = s;
+}
+
+__global__ __launch_bounds__(256) void prolong_kernel(int n, const int* __restrict__ agg, const double* __restrict__ xc,
+                                                    double* __restrict__ x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += xc[agg[i]];
+}
+
+__global__ __launch_bounds__(256) void dense_matvec_kernel(int n, const double* __restrict__ Ainv,
+                                                         const double* __restrict__ r, double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = 0; j < n; j++) s += Ainv[(size_t)i * n + j] * r[j];
+  x[i] = s;
+}
+
+__global__ __launch_bounds__(256) void q_gather_kernel(int nq, const int* __restrict__ qrow,
+                                                     const double* __restrict__ v, const MinresState* __restrict__ st,
+                                                     double* __restrict__ out) {
+  if (st && st->converged) return;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) out[q] = v[qrow[q]];
+}
+
+__global__ __launch_bounds__(256) void q_scatter_kernel(int nq, const int* __restrict__ qrow,
+                                                      const double* __restrict__ v,
+                                                      const MinresState* __restrict__ st, double* __restrict__ out) {
+  if (st && st->converged) return;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) out[qrow[q]] = v[q];
+}
+
+__device__ __forceinline__ double wsum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// partial[b] = sum over block b's grid-stride rows of a . b, the layout of the MINRES partials
+__global__ __launch_bounds__(256) void dot_partials_kernel(int n, const double* __restrict__ a,
+                                                         const double* __restrict__ b,
+                                                         const MinresState* __restrict__ st,
+                                                         double* __restrict__ partial) {
+  __shared__ double sh[4];
+  if (st && st->converged) return;
+  double v = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v += a[i] * b[i];
+  v = wsum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ __launch_bounds__(256) void dinv_spmv_kernel(AmgCsr A, const double* __restrict__ dinv,
+                                                      const double* __restrict__ x, double scale,
+                                                      double* __restrict__ y) {
+  const int lane = threadIdx.x % kLanes, rpb = blockDim.x / kLanes;
+  for (int i = blockIdx.x * rpb + threadIdx.x / kLanes; i < A.n; i += gridDim.x * rpb) {
+    const double ax = row_dot(A, i, lane, x);
+    if (lane == 0) y[i] = scale * dinv[i] * ax;
+  }
+}
+
+inline int rowGrid(int n) { return std::min(kGrid, std::max(1, cdiv(n, 256 / kLanes))); }
+inline int elemGrid(int n) { return std::min(kGrid, std::max(1, cdiv(n, 256))); }
+
+}  // namespace
+
+void launch_schur_contrib(hipStream_t s, int nPairs, const int* pairNode, const int* nodeGroup, const int* nodeSize,
+                          const int* qPtr, const int* cPtr, const int* posB, const double* kval, const double* binv,
+                          double* contrib) {
+  if (nPairs <= 0) return;
+  hipLaunchKernelGGL(schur_contrib_kernel, dim3(cdiv(nPairs, 256)), dim3(256), 0, s, nPairs, pairNode, nodeGroup,
+                     nodeSize, qPtr, cPtr, posB, kval, binv, contrib);
+}
+
+void launch_schur_sub_c(hipStream_t s, int nq, const int* sDiag, const int* cPos, const double* kval, double* sval) {
+  if (nq <= 0) return;
+  hipLaunchKernelGGL(schur_sub_c_kernel, dim3(cdiv(nq, 256)), dim3(256), 0, s, nq, sDiag, cPos, kval, sval);
+}
+
+void launch_gather_sum(hipStream_t s, int n, const int* seg, const int* src, const double* in, double* out) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(gather_sum_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, seg, src, in, out);
+}
+
+void launch_inv_diag(hipStream_t s, const AmgCsr& A, double* dinv) {
+  if (A.n <= 0) return;
+  hipLaunchKernelGGL(inv_diag_kernel, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, dinv);
+}
+
+void launch_cheb_start(hipStream_t s, int n, const double* r, const double* dinv, double c0, double* res, double* d,
+                       double* x) {
+  hipLaunchKernelGGL(cheb_start_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, r, dinv, c0, res, d, x);
+}
+
+void launch_cheb_start_x(hipStream_t s, const AmgCsr& A, const double* r, const double* dinv, double c0, double* res,
+                         double* d, double* x) {
+  hipLaunchKernelGGL(cheb_start_x_kernel, dim3(rowGrid(A.n)), dim3(256), 0, s, A, r, dinv, c0, res, d, x);
+}
+
+void launch_cheb_step(hipStream_t s, const AmgCsr& A, const double* din, const double* dinv, double c1, double c2,
+                      double* res, double* dout, double* x) {
+  hipLaunchKernelGGL(cheb_step_kernel, dim3(rowGrid(A.n)), dim3(256), 0, s, A, din, dinv, c1, c2, res, dout, x);
+}
+
+void launch_residual_restrict(hipStream_t s, const AmgCsr& A, const double* res, const double* dlast, double* t,
+                              int nc, const int* memPtr, const int* mem, double* rc, double* x) {
+  hipLaunchKernelGGL(residual_kernel, dim3(rowGrid(A.n)), dim3(256), 0, s, A, res, dlast, t, x);
+  hipLaunchKernelGGL(restrict_kernel, dim3(cdiv(nc, 256)), dim3(256), 0, s, nc, memPtr, mem, t, rc);
+}
+
+void launch_prolong(hipStream_t s, int n, const int* agg, const double* xc, double* x) {
+  hipLaunchKernelGGL(prolong_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, agg, xc, x);
+}
+
+void launch_dense_matvec(hipStream_t s, int n, const double* Ainv, const double* r, double* x) {
+  hipLaunchKernelGGL(dense_matvec_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, Ainv, r, x);
+}
+
+void launch_q_gather(hipStream_t s, int nq, const int* qrow, const double* v, const MinresState* st, double* out) {
+  hipLaunchKernelGGL(q_gather_kernel, dim3(elemGrid(nq)), dim3(256), 0, s, nq, qrow, v, st, out);
+}
+
+void launch_q_scatter(hipStream_t s, int nq, const int* qrow, const double* v, const MinresState* st, double* out) {
+  hipLaunchKernelGGL(q_scatter_kernel, dim3(elemGrid(nq)), dim3(256), 0, s, nq, qrow, v, st, out);
+}
+
+void launch_dot_partials(hipStream_t s, int n, const double* a, const double* b, const MinresState* st,
+                         double* partial) {
+  hipLaunchKernelGGL(dot_partials_kernel, dim3(kGrid), dim3(256), 0, s, n, a, b, st, partial);
+}
+
+void launch_dinv_spmv(hipStream_t s, const AmgCsr& A, const double* dinv, const double* x, double scale, double* y) {
+  hipLaunchKernelGGL(dinv_spmv_kernel, dim3(rowGrid(A.n)), dim3(256), 0, s, A, dinv, x, scale, y);
+}
+
+}  // namespace nla3d_amd

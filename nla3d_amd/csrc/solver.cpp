@@ -140,6 +140,7 @@ void Ctx::setupKrylov() {
     i += sz;
   }
   nGroups = (int)gStart.size();
+  setupSchur(*this, rp, col, fullToVal, gStart, gInfo);
   {  // learned-correction node list and row metadata
     std::vector<signed char> kind2(n2, 0);
     std::vector<int> node2(n2, -1), pg(n2, 0), lg, lr;
@@ -286,7 +287,8 @@ Precond Ctx::precond() {
   return P;
 }
 
-int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* relres_est, bool* breakdown) {
+int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* relres_est, bool* breakdown,
+                     const PrecondHook* hook) {
   KrylovMatrix M = krylovMatrix();
   Precond P = precond();
   MinresWork w;
@@ -300,12 +302,12 @@ int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* 
   w.W2 = d_mw[6].as<double>();
   w.partial = d_partial.as<double>();
   w.st = d_mstate.as<MinresState>();
-  minres_start(stream, M, P, w, b);
+  minres_start(stream, M, P, w, b, hook);
   hipCheck(hipGetLastError(), "minres start");
   const int check = 16;
   int it = 0;
   for (;;) {
-    for (int k = 0; k < check && it < maxit; k++, it++) minres_iteration(stream, M, P, w, tol);
+    for (int k = 0; k < check && it < maxit; k++, it++) minres_iteration(stream, M, P, w, tol, hook);
     hipCheck(hipGetLastError(), "minres iteration");
     hipCheck(hipMemcpyAsync(h_mstate, w.st, sizeof(MinresState), hipMemcpyDeviceToHost, stream), "state");
     hipCheck(hipStreamSynchronize(stream), "minres sync");
@@ -399,12 +401,22 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
   st.newton_iterations++;
   // solve K.block(2) dU_sl = rhs (FESolver.cpp:457)
   t0 = nowMs();
-  if (o.preconditioner == 1) learnedPrecondSetup(c, t);
+  const bool learnedPc = o.preconditioner == 1 || o.preconditioner == 3;
+  if (learnedPc) learnedPrecondSetup(c, t);
   else krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
+  PrecondHook hook;
+  bool useSchur = false;
+  if (o.preconditioner == 2 || o.preconditioner == 3) {
+    std::string why;
+    useSchur = schurUsable(c, &why) && schurRefresh(c, &why);
+    if (useSchur) hook = schurHook(c);
+    else st.precond_fallbacks++;  // plain block Jacobi for this solve
+  }
   double relres = 0;
   bool brk = false;
-  int kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), o.krylov_tol, o.krylov_max_iter, &relres, &brk);
-  if (o.preconditioner == 1 && (brk || relres > o.krylov_tol)) {
+  int kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), o.krylov_tol, o.krylov_max_iter, &relres, &brk,
+                          useSchur ? &hook : nullptr);
+  if ((learnedPc || useSchur) && (brk || relres > o.krylov_tol)) {
     // the learned corrections failed this solve: block Jacobi from a fresh start
     st.precond_fallbacks++;
     st.krylov_iterations += kit;

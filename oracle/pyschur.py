@@ -1,0 +1,83 @@
+"""CPU oracle of the Schur-row multigrid preconditioner (test infrastructure only).
+
+No counterpart in the reference (its solver is PARDISO).  Restates the definitions
+the device path implements:
+
+* schur_matrix: S = B^T D^-1 B - C on the rows that block Jacobi treats by a scalar
+  Schur estimate (element pressures, multipliers), B their couplings to the node
+  rows, D^-1 the block-Jacobi inverse node blocks, C their own block;
+* vcycle: one V-cycle over given aggregation maps with Jacobi-Chebyshev smoothing
+  on [lmax/10, 1.05 lmax], the same polynomial before and after, exact solve on
+  the coarsest level.
+
+Pinned by: a one-level hierarchy gives S^-1 exactly; the cycle is a symmetric
+positive-definite operator; the smoother alone reduces the energy-norm error; S is
+symmetric positive definite on the models of the tests.
+"""
+import numpy as np
+import scipy.sparse as sp
+
+
+def schur_matrix(K2, groups, bj):
+    """(q rows, S as a dense array) from the full block(2) matrix, the block-Jacobi groups
+    [(r0, size, kind)] and their inverse blocks [(r0, size, B)]"""
+    q = np.array([r0 for r0, sz, kind in groups if kind == 1], dtype=np.int64)
+    urows, blocks = [], []
+    for (r0, sz, kind), (_, _, B) in zip(groups, bj):
+        if kind == 0:
+            urows.extend(range(r0, r0 + sz))
+            blocks.append(B)
+    Dinv = sp.block_diag(blocks).tocsr()
+    K = K2.tocsr()
+    Bq = K[urows][:, q]
+    C = K[q][:, q]
+    return q, (Bq.T @ Dinv @ Bq - C).toarray()
+
+
+def cheb_coef(lmax, steps):
+    hi, lo = 1.05 * lmax, lmax / 10.0
+    theta, delta = 0.5 * (hi + lo), 0.5 * (hi - lo)
+    sigma = theta / delta
+    c0 = 1.0 / theta
+    rho = 1.0 / sigma
+    c1, c2 = [0.0], [0.0]
+    for _ in range(1, steps):
+        rn = 1.0 / (2.0 * sigma - rho)
+        c1.append(rn * rho)
+        c2.append(2.0 * rn / delta)
+        rho = rn
+    return c0, c1, c2
+
+
+def smooth(A, dinv, r, x, lmax, steps):
+    """Jacobi-Chebyshev: x + p(D^-1 A) D^-1 (r - A x)"""
+    c0, c1, c2 = cheb_coef(lmax, steps)
+    res = r - A @ x
+    d = c0 * dinv * res
+    for i in range(1, steps):
+        x = x + d
+        res = res - A @ d
+        d = c1[i] * d + c2[i] * dinv * res
+    return x + d
+
+
+def hierarchy(S, aggs):
+    """levels [A_0, A_1, ...] from the aggregate maps of every level but the coarsest"""
+    mats = [sp.csr_matrix(S)]
+    for agg in aggs:
+        n, nc = len(agg), int(agg.max()) + 1
+        P = sp.csr_matrix((np.ones(n), (np.arange(n), agg)), shape=(n, nc))
+        mats.append((P.T @ mats[-1] @ P).tocsr())
+    return mats
+
+
+def vcycle(mats, aggs, lmaxs, r, steps=3, level=0):
+    A = mats[level]
+    if level == len(mats) - 1:
+        return np.linalg.solve(A.toarray(), r)
+    dinv = 1.0 / A.diagonal()
+    x = smooth(A, dinv, r, np.zeros_like(r), lmaxs[level], steps)
+    agg = aggs[level]
+    rc = np.bincount(agg, weights=r - A @ x, minlength=mats[level + 1].shape[0])
+    x = x + vcycle(mats, aggs, lmaxs, rc, steps, level + 1)[agg]
+    return smooth(A, dinv, r, x, lmaxs[level], steps)

@@ -1,0 +1,40 @@
+"""Pins of the Schur-multigrid oracle (oracle/pyschur.py) on the CPU."""
+import numpy as np
+import scipy.sparse as sp
+
+from oracle import pyschur as ps
+
+
+def laplacian(n):
+    T = sp.diags([-np.ones(n - 1), 2 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1])
+    I = sp.identity(n)
+    return (sp.kron(T, I) + sp.kron(I, T) + 0.01 * sp.identity(n * n)).tocsr()
+
+
+def test_one_level_is_exact_inverse():
+    A = laplacian(6)
+    r = np.random.default_rng(0).standard_normal(36)
+    z = ps.vcycle([A], [], [], r)
+    assert np.abs(A @ z - r).max() <= 1e-12 * np.abs(r).max()
+
+
+def test_vcycle_is_symmetric_positive_definite():
+    A = laplacian(8)
+    agg = np.arange(64) // 4
+    mats = ps.hierarchy(A, [agg])
+    lmax = float(np.max(np.linalg.eigvals((sp.diags(1 / A.diagonal()) @ A).toarray()).real))
+    B = np.stack([ps.vcycle(mats, [agg], [lmax], e) for e in np.eye(64)], axis=1)
+    assert np.abs(B - B.T).max() <= 1e-12 * np.abs(B).max()
+    assert np.linalg.eigvalsh(0.5 * (B + B.T)).min() > 0
+
+
+def test_smoother_reduces_energy_error():
+    A = laplacian(8).toarray()
+    rng = np.random.default_rng(1)
+    x_true = rng.standard_normal(64)
+    r = A @ x_true
+    dinv = 1 / np.diag(A)
+    lmax = float(np.max(np.linalg.eigvals(np.diag(dinv) @ A).real))
+    x = ps.smooth(A, dinv, r, np.zeros(64), lmax, 3)
+    e0, e1 = x_true, x_true - x
+    assert e1 @ A @ e1 < e0 @ A @ e0
