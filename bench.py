@@ -248,6 +248,7 @@ def main():
     ap.add_argument("--krylov-tol", type=float, default=1e-9)
     ap.add_argument("--preconditioner", choices=["schur-multigrid", "learned", "block-jacobi"],
                     default="schur-multigrid")
+    ap.add_argument("--no-train", action="store_true", help="skip the network (and its judge)")
     ap.add_argument("--train-pairs", type=int, default=2000)
     ap.add_argument("--train-k", type=int, default=100)
     ap.add_argument("--train-lr", type=float, default=1e-4)
@@ -317,12 +318,13 @@ def main():
         net = None
         base_pc = 2 if args.preconditioner == "schur-multigrid" else 0
         opts.preconditioner = base_pc
-        if args.preconditioner in ("learned", "schur-multigrid"):
+        if args.preconditioner in ("learned", "schur-multigrid") and not args.no_train:
             from nla3d_amd import learned
             net = nla3d_amd.Mlp(0, seed=20261203)
             val = learned.validation_system(24)
             info = learned.train(net, n_pairs=args.train_pairs, k=args.train_k, lr=args.train_lr,
-                                 validation=[val], log=lambda msg: print(msg, file=sys.stderr, flush=True))
+                                 validation=[val], log=lambda msg: print(msg, file=sys.stderr, flush=True),
+                                 multigrid=base_pc == 2)
             val[0].close()
             m.attach_mlp(net)
             opts.preconditioner = base_pc + 1  # 1: learned, 3: learned + Schur multigrid

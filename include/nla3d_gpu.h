@@ -120,7 +120,7 @@ typedef struct {
                             network attached by nla3d_gpu_attach_mlp; 2: block Jacobi on the node DoFs and one
                             V-cycle of aggregation multigrid on the sparse Schur approximation
                             S = B^T D^-1 B - C for the pressure and multiplier rows; 3: 1 and 2 together
-                            (S is formed from the corrected node blocks).  A solve of 1, 2 or 3 that
+                            (S is formed from the plain block-Jacobi blocks).  A solve of 1, 2 or 3 that
                             breaks down or does not reach krylov_tol is repeated with plain block Jacobi
                             (stats.precond_fallbacks) */
 } nla3d_gpu_newton_opts;
@@ -277,11 +277,13 @@ int nla3d_gpu_node_features(nla3d_gpu_t h, double t, float* X, int* node_rows, i
 int nla3d_gpu_learned_blocks(nla3d_gpu_t h, const float* Y, int n, double* blocks);
 /* Loss of k unrolled MINRES steps for rhs (host, length nUnknown + nMpc; NULL: the last
  * Newton right-hand side) with the current tangent, and its gradient (53,510, may be NULL). */
-int nla3d_gpu_mlp_loss_grad(nla3d_gpu_mlp_t mlp, nla3d_gpu_t h, const double* rhs, int k, double t, double* loss,
-                            float* grad);
+int nla3d_gpu_mlp_loss_grad(nla3d_gpu_mlp_t mlp, nla3d_gpu_t h, const double* rhs, int k, double t, int multigrid,
+                            double* loss, float* grad);
 void nla3d_gpu_adam_defaults(nla3d_gpu_adam_opts* opts); /* lr 1e-3, 0.9, 0.999, 1e-8 */
-/* One Adam step on that gradient. */
-int nla3d_gpu_mlp_train_step(nla3d_gpu_mlp_t mlp, nla3d_gpu_t h, const double* rhs, int k, double t,
+/* One Adam step on that gradient.  multigrid = 1 (both entries): the unrolled MINRES uses
+ * preconditioner 3's operator, the Schur-row multigrid (built from the plain block-Jacobi
+ * blocks) beside the corrected node blocks; 0: preconditioner 1's. */
+int nla3d_gpu_mlp_train_step(nla3d_gpu_mlp_t mlp, nla3d_gpu_t h, const double* rhs, int k, double t, int multigrid,
                              const nla3d_gpu_adam_opts* opts, double* loss);
 
 /* ---- thermal path (QUADTH / SurfaceLINETH) -------------------------------- */

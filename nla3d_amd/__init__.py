@@ -124,8 +124,9 @@ ABI.update({
     "nla3d_gpu_attach_mlp": [_P, _P],
     "nla3d_gpu_node_features": [_P, ctypes.c_double, _FP, _IP, ctypes.c_int, _IP],
     "nla3d_gpu_learned_blocks": [_P, _FP, ctypes.c_int, _DP],
-    "nla3d_gpu_mlp_loss_grad": [_P, _P, _DP, ctypes.c_int, ctypes.c_double, _DP, _FP],
-    "nla3d_gpu_mlp_train_step": [_P, _P, _DP, ctypes.c_int, ctypes.c_double, ctypes.POINTER(AdamOpts), _DP],
+    "nla3d_gpu_mlp_loss_grad": [_P, _P, _DP, ctypes.c_int, ctypes.c_double, ctypes.c_int, _DP, _FP],
+    "nla3d_gpu_mlp_train_step": [_P, _P, _DP, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(AdamOpts),
+                                 _DP],
     "nla3d_gpu_get_rhs": [_P, _DP],
     "nla3d_gpu_schur_info": [_P, _IP, _IP, _IP, _IP, _IP, ctypes.c_int],
     "nla3d_gpu_schur_matrix": [_P, _IP, _IP, _IP, _DP],
@@ -550,21 +551,22 @@ class Mlp:
         r = np.ascontiguousarray(rhs, dtype=np.float64)
         return r, _ptr(r, ctypes.c_double)
 
-    def loss_grad(self, model, rhs=None, k=40, t=1.0):
+    def loss_grad(self, model, rhs=None, k=40, t=1.0, multigrid=False):
         keep, rp = self._rhs(rhs)
         loss = ctypes.c_double()
         g = np.zeros(self.N_PARAMS, dtype=np.float32)
-        self._call("nla3d_gpu_mlp_loss_grad", model._h, rp, int(k), float(t), ctypes.byref(loss),
+        self._call("nla3d_gpu_mlp_loss_grad", model._h, rp, int(k), float(t), int(multigrid), ctypes.byref(loss),
                    _ptr(g, ctypes.c_float))
         return loss.value, g
 
-    def train_step(self, model, rhs=None, k=40, t=1.0, lr=1e-3):
+    def train_step(self, model, rhs=None, k=40, t=1.0, lr=1e-3, multigrid=False):
         keep, rp = self._rhs(rhs)
         o = AdamOpts()
         self._L.nla3d_gpu_adam_defaults(ctypes.byref(o))
         o.lr = lr
         loss = ctypes.c_double()
-        self._call("nla3d_gpu_mlp_train_step", model._h, rp, int(k), float(t), ctypes.byref(o), ctypes.byref(loss))
+        self._call("nla3d_gpu_mlp_train_step", model._h, rp, int(k), float(t), int(multigrid), ctypes.byref(o),
+                   ctypes.byref(loss))
         return loss.value
 
 

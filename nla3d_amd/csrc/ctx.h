@@ -149,5 +149,7 @@ PrecondHook schurHook(Ctx& c);
 void computeReactions(Ctx& c, hipStream_t stream = nullptr);
 // block-Jacobi blocks with the attached network's corrections (learned.cpp); t = load-step time
 void learnedPrecondSetup(Ctx& c, double t);
+// the attached network's corrections over the current block-Jacobi blocks (learned.cpp)
+void learnedCorrect(Ctx& c, double t);
 
 }  // namespace nla3d_amd

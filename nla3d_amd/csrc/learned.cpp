@@ -134,10 +134,9 @@ static void backwardF32(Mlp& net, hipStream_t s, int n, const float* dY) {
   }
 }
 
-// block-Jacobi blocks, then the network's corrections over the learned nodes
+// the network's corrections over the learned nodes (block-Jacobi blocks must be current)
 static void learnedPreconditioner(Ctx& c, Mlp& net, double t) {
   LearnedWork& lw = work(c);
-  krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
   if (c.nLearned == 0) return;
   nodeFeatures(c, t, lw.X.as<float>());
   net.pack(c.stream);
@@ -148,10 +147,15 @@ static void learnedPreconditioner(Ctx& c, Mlp& net, double t) {
   hipCheck(hipGetLastError(), "learned preconditioner");
 }
 
-void learnedPrecondSetup(Ctx& c, double t) {
+void learnedCorrect(Ctx& c, double t) {
   if (!c.mlp) throw ApiError(-3, "no network attached (nla3d_gpu_attach_mlp)");
   if (c.mlp->device != c.device) throw ApiError(-1, "network and model are on different devices");
   learnedPreconditioner(c, *c.mlp, t);
+}
+
+void learnedPrecondSetup(Ctx& c, double t) {
+  krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
+  learnedCorrect(c, t);
 }
 
 // ---- unrolled MINRES --------------------------------------------------------
@@ -195,7 +199,7 @@ static double hostDot(Ctx& c, int n, const double* a, const double* b) {
 
 // loss and dL/dY for rhs b (device, n2) with the corrected preconditioner of Y;
 // returns false when the recurrence broke down (sample unusable)
-static bool unrolledLossGrad(Ctx& c, int k, const double* b, double& loss) {
+static bool unrolledLossGrad(Ctx& c, int k, const double* b, double& loss, const PrecondHook* hook) {
   LearnedWork& lw = work(c);
   const int n = c.n2;
   const size_t nv = (size_t)n;
@@ -217,10 +221,16 @@ static bool unrolledLossGrad(Ctx& c, int k, const double* b, double& loss) {
   const KrylovMatrix M = c.krylovMatrix();
   const Precond P = c.precond();
   hipStream_t s = c.stream;
+  // the preconditioner: (corrected) block Jacobi, then the optional stage on the Schur rows
+  // (a fixed symmetric operator, so its adjoint is itself)
+  auto applyP = [&](const double* r, double* z) {
+    launch_bj_apply(s, P, r, z);
+    if (hook) (*hook)(s, r, z, c.d_partial.as<double>(), nullptr);
+  };
 
   std::vector<double> alpha(k), beta(k + 1);
   hipCheck(hipMemcpyAsync(R(0), b, nv * 8, hipMemcpyDeviceToDevice, s), "rhs");
-  launch_bj_apply(s, P, R(0), Yv(0));
+  applyP(R(0), Yv(0));
   beta[0] = std::sqrt(hostDot(c, n, R(0), Yv(0)));
   if (!(beta[0] > 0.0)) return false;
   for (int j = 0; j < k; j++) {
@@ -229,7 +239,7 @@ static bool unrolledLossGrad(Ctx& c, int k, const double* b, double& loss) {
     if (j >= 1) launch_axpby(s, n, 1.0, T, -beta[j] / beta[j - 1], R(j - 1), T);
     alpha[j] = hostDot(c, n, V, T);
     launch_axpby(s, n, 1.0, T, -alpha[j] / beta[j], R(j), R(j + 1));
-    launch_bj_apply(s, P, R(j + 1), Yv(j + 1));
+    applyP(R(j + 1), Yv(j + 1));
     const double bb = hostDot(c, n, R(j + 1), Yv(j + 1));
     if (!(bb > 0.0)) return false;
     beta[j + 1] = std::sqrt(bb);
@@ -261,7 +271,7 @@ static bool unrolledLossGrad(Ctx& c, int k, const double* b, double& loss) {
     launch_axpby(s, n, cb, R(j + 1), 1.0, Yb(j + 1), Yb(j + 1));
     // Y_{j+1} = P R_{j+1}
     launch_outer_acc(s, c.nLearned, c.d_lRow0.as<int>(), Yb(j + 1), R(j + 1), lw.Pbar.as<double>());
-    launch_bj_apply(s, P, Yb(j + 1), W);
+    applyP(Yb(j + 1), W);
     launch_axpby(s, n, 1.0, W, 1.0, Rb(j + 1), Rb(j + 1));
     // R_{j+1} = T3 - d R_j, d = alpha_j / beta_j
     const double d = alpha[j] / beta[j];
@@ -299,17 +309,23 @@ static bool unrolledLossGrad(Ctx& c, int k, const double* b, double& loss) {
 }
 
 // loss and parameter gradient (net.g) for one (tangent, rhs) sample
-static bool lossGrad(Mlp& net, Ctx& c, int k, double t, const double* rhsDev, double& loss) {
+static bool lossGrad(Mlp& net, Ctx& c, int k, double t, const double* rhsDev, double& loss, bool multigrid) {
   requireLearnable(c);
   if (c.nLearned == 0) throw ApiError(-3, "model has no free UX UY UZ node triples");
   if (k < 1 || k > 1024) throw ApiError(-1, "k must be in [1, 1024]");
   LearnedWork& lw = work(c);
   krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
+  PrecondHook hook;
+  if (multigrid) {  // S from the plain block-Jacobi blocks, as preconditioner 3 builds it
+    std::string why;
+    if (!schurUsable(c, &why) || !schurRefresh(c, &why)) throw ApiError(-3, "Schur multigrid: " + why);
+    hook = schurHook(c);
+  }
   nodeFeatures(c, t, lw.X.as<float>());
   const float* Y = forwardF32(net, c.stream, c.nLearned, lw.X.as<float>());
   launch_learned_blocks(c.stream, c.nLearned, c.d_lGroup.as<int>(), c.precond(), c.krylovMatrix(), Y,
                         lw.Rinv.as<double>(), lw.ok.as<int>());
-  if (!unrolledLossGrad(c, k, rhsDev, loss)) return false;
+  if (!unrolledLossGrad(c, k, rhsDev, loss, multigrid ? &hook : nullptr)) return false;
   launch_learned_grad(c.stream, c.nLearned, lw.Rinv.as<double>(), lw.ok.as<int>(), Y, lw.Pbar.as<double>(),
                       lw.dY.as<float>());
   backwardF32(net, c.stream, c.nLearned, lw.dY.as<float>());
@@ -566,13 +582,14 @@ static const double* rhsOnDevice(Ctx& c, const double* rhs) {
   return d;
 }
 
-int nla3d_gpu_mlp_loss_grad(nla3d_gpu_mlp_t mh, nla3d_gpu_t h, const double* rhs, int k, double t, double* loss,
-                            float* grad) {
+int nla3d_gpu_mlp_loss_grad(nla3d_gpu_mlp_t mh, nla3d_gpu_t h, const double* rhs, int k, double t, int multigrid,
+                            double* loss, float* grad) {
   return pairGuard(mh, h, [&](Mlp& m, Ctx& c) {
     if (!loss) throw ApiError(-1, "null loss");
     requireLearnable(c);
     double L = 0.0;
-    if (!lossGrad(m, c, k, t, rhsOnDevice(c, rhs), L)) throw ApiError(-6, "Lanczos recurrence broke down");
+    if (!lossGrad(m, c, k, t, rhsOnDevice(c, rhs), L, multigrid != 0))
+      throw ApiError(-6, "Lanczos recurrence broke down");
     *loss = L;
     if (grad) hipCheck(hipMemcpyAsync(grad, m.g.p, (size_t)kMlpParams * 4, hipMemcpyDeviceToHost, c.stream), "grad");
     hipCheck(hipStreamSynchronize(c.stream), "sync");
@@ -587,13 +604,14 @@ void nla3d_gpu_adam_defaults(nla3d_gpu_adam_opts* o) {
   o->eps = 1e-8f;
 }
 
-int nla3d_gpu_mlp_train_step(nla3d_gpu_mlp_t mh, nla3d_gpu_t h, const double* rhs, int k, double t,
+int nla3d_gpu_mlp_train_step(nla3d_gpu_mlp_t mh, nla3d_gpu_t h, const double* rhs, int k, double t, int multigrid,
                              const nla3d_gpu_adam_opts* o, double* loss) {
   return pairGuard(mh, h, [&](Mlp& m, Ctx& c) {
     if (!o || !loss) throw ApiError(-1, "null options or loss");
     requireLearnable(c);
     double L = 0.0;
-    if (!lossGrad(m, c, k, t, rhsOnDevice(c, rhs), L)) throw ApiError(-6, "Lanczos recurrence broke down");
+    if (!lossGrad(m, c, k, t, rhsOnDevice(c, rhs), L, multigrid != 0))
+      throw ApiError(-6, "Lanczos recurrence broke down");
     m.step++;
     launch_adam(c.stream, kMlpParams, o->lr, o->beta1, o->beta2, o->eps, m.step, m.g.as<float>(), m.w.as<float>(),
                 m.m.as<float>(), m.v.as<float>());
@@ -616,14 +634,14 @@ int nla3d_gpu_solve_block2_pc(nla3d_gpu_t h, const double* rhs, double* x, doubl
     PrecondHook hook;
     if (preconditioner < 0 || preconditioner > 3)
       throw ApiError(-1, "preconditioner must be 0 (block Jacobi), 1 (learned), 2 (Schur multigrid) or 3 (both)");
-    if (preconditioner == 1 || preconditioner == 3) learnedPrecondSetup(c, t);
-    else krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
+    krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
     const bool mg = preconditioner >= 2;
-    if (mg) {
+    if (mg) {  // S from the plain block-Jacobi blocks
       std::string why;
       if (!schurUsable(c, &why) || !schurRefresh(c, &why)) throw ApiError(-3, "Schur multigrid: " + why);
       hook = schurHook(c);
     }
+    if (preconditioner == 1 || preconditioner == 3) learnedCorrect(c, t);
     double* b = c.d_mw[7].as<double>();
     hipCheck(hipMemcpyAsync(b, rhs, (size_t)c.n2 * 8, hipMemcpyHostToDevice, c.stream), "rhs");
     double rr = 0;

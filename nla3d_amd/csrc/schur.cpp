@@ -20,12 +20,14 @@ constexpr int kMaxCoarse = 2048; // coarsening that stalls above this disables t
 constexpr int kMaxQPerNode = 64;
 
 struct AmgLevel {
-  int n = 0, nnz = 0, nc = 0;
+  int n = 0, nnz = 0, nc = 0, nSlots = 0;
   std::vector<int> h_rowPtr, h_col;
   DevBuf rowPtr, col, val, dinv, agg, memPtr, mem, gSeg, gSrc;
+  DevBuf sOff, sCol, sPos, sVal;  // the smoother's sliced-ELLPACK copy
   DevBuf r, x, res, da, db, t, start;
   double lmax = 1.0;
   AmgCsr csr() const { return AmgCsr{n, rowPtr.as<int>(), col.as<int>(), val.as<double>()}; }
+  AmgSell sell() const { return AmgSell{n, sOff.as<int>(), sCol.as<int>(), sVal.as<float>()}; }
 };
 
 struct SchurAmg {
@@ -99,6 +101,33 @@ static void allocLevel(AmgLevel& A, bool coarsest) {
   A.r.alloc(nb);
   A.x.alloc(nb);
   if (coarsest) return;
+  // sliced ELLPACK: 64-row slices, each as wide as its longest row
+  const int nSl = (A.n + 63) / 64;
+  std::vector<int> off(nSl + 1, 0);
+  for (int sl = 0; sl < nSl; sl++) {
+    int w = 0;
+    for (int i = sl * 64; i < std::min(A.n, sl * 64 + 64); i++) w = std::max(w, A.h_rowPtr[i + 1] - A.h_rowPtr[i]);
+    if ((int64_t)off[sl] + 64ll * w >= (1ll << 31)) throw ApiError(-1, "Schur level too large for its smoother copy");
+    off[sl + 1] = off[sl] + 64 * w;
+  }
+  A.nSlots = off[nSl];
+  std::vector<int> scol(A.nSlots), spos(A.nSlots);
+  for (int sl = 0; sl < nSl; sl++) {
+    const int w = (off[sl + 1] - off[sl]) / 64;
+    for (int l = 0; l < 64; l++) {
+      const int i = sl * 64 + l;
+      for (int k = 0; k < w; k++) {
+        const int j = off[sl] + k * 64 + l;
+        const bool real = i < A.n && k < A.h_rowPtr[i + 1] - A.h_rowPtr[i];
+        scol[j] = real ? A.h_col[A.h_rowPtr[i] + k] : (i < A.n ? i : 0);
+        spos[j] = real ? A.h_rowPtr[i] + k : -1;
+      }
+    }
+  }
+  A.sOff.upload(off);
+  A.sCol.upload(scol);
+  A.sPos.upload(spos);
+  A.sVal.alloc((size_t)std::max(A.nSlots, 1) * 4);
   A.dinv.alloc(nb);
   A.res.alloc(nb);
   A.da.alloc(nb);
@@ -303,6 +332,7 @@ bool schurRefresh(Ctx& c, std::string* why) {
   launch_schur_sub_c(s, S.nq, S.sDiag.as<int>(), S.cPos.as<int>(), kval, L0.val.as<double>());
   for (size_t l = 0; l + 1 < S.L.size(); l++) {
     AmgLevel& A = *S.L[l];
+    launch_sell_fill(s, A.nSlots, A.sPos.as<int>(), A.val.as<double>(), A.sVal.as<float>());
     launch_inv_diag(s, A.csr(), A.dinv.as<double>());
     // largest eigenvalue of D^-1 A by power iteration (12 steps from a fixed pseudo-random start)
     double* x = A.da.as<double>();
@@ -395,7 +425,7 @@ static void vcycle(SchurAmg& S, hipStream_t s, size_t l) {
     return;
   }
   const ChebCoef k = chebCoef(A.lmax, S.steps);
-  const AmgCsr M = A.csr();
+  const AmgSell M = A.sell();
   double* dinv = A.dinv.as<double>();
   double* din = A.da.as<double>();
   double* dout = A.db.as<double>();

@@ -7,9 +7,9 @@
 // B the couplings of those rows to the node DoFs, D^-1 the block-Jacobi inverse node
 // blocks, C their own block (the pressure rows' -V/k).  S is SPD when the node blocks
 // are.  Levels: piecewise-constant aggregation of S's graph (greedy, distance one),
-// Galerkin coarse matrices P^T A P, Jacobi-Chebyshev smoothing (the same polynomial
-// before and after, so the cycle is symmetric), an explicit dense inverse on the
-// coarsest level.  All structure is built once per model; values, smoother intervals
+// Galerkin coarse matrices P^T A P (FP64), Jacobi-Chebyshev smoothing (the same
+// polynomial before and after, so the cycle is symmetric) whose products use an FP32
+// sliced-ELLPACK copy of each level, an explicit dense FP64 inverse on the coarsest level.  All structure is built once per model; values, smoother intervals
 // (power iteration) and the coarsest inverse are refreshed with every tangent.
 #pragma once
 
@@ -27,6 +27,17 @@ struct AmgCsr {
   const double* val;
 };
 
+// the smoother's copy of a level's matrix: sliced ELLPACK over 64-row slices with FP32
+// values.  Row i = 64 s + l holds (off[s+1] - off[s]) / 64 slots at off[s] + k * 64 + l;
+// padding slots hold value 0 and column i.  The values are the FP64 matrix rounded once per
+// tangent, so the smoother stays one fixed symmetric operator.
+struct AmgSell {
+  int n;
+  const int* off;
+  const int* col;
+  const float* val;
+};
+
 // level-0 assembly of S: one thread per (node, Schur column a) writes B(:,a)^T D^-1 B(:,b)
 void launch_schur_contrib(hipStream_t s, int nPairs, const int* pairNode, const int* nodeGroup, const int* nodeSize,
                           const int* qPtr, const int* cPtr, const int* posB, const double* kval, const double* binv,
@@ -35,14 +46,17 @@ void launch_schur_sub_c(hipStream_t s, int nq, const int* sDiag, const int* cPos
 void launch_gather_sum(hipStream_t s, int n, const int* seg, const int* src, const double* in, double* out);
 void launch_inv_diag(hipStream_t s, const AmgCsr& A, double* dinv);
 
+// sellVal[j] = csrVal[pos[j]] rounded to FP32 (0 where pos[j] < 0)
+void launch_sell_fill(hipStream_t s, int nSlots, const int* pos, const double* csrVal, float* sellVal);
+
 // V-cycle pieces (c0, c1, c2: Chebyshev coefficients of a step)
 void launch_cheb_start(hipStream_t s, int n, const double* r, const double* dinv, double c0, double* res, double* d,
                        double* x);
-void launch_cheb_start_x(hipStream_t s, const AmgCsr& A, const double* r, const double* dinv, double c0, double* res,
-                         double* d, double* x);
-void launch_cheb_step(hipStream_t s, const AmgCsr& A, const double* din, const double* dinv, double c1, double c2,
+void launch_cheb_start_x(hipStream_t s, const AmgSell& A, const double* r, const double* dinv, double c0, double* res,
+                         double* d, const double* x);
+void launch_cheb_step(hipStream_t s, const AmgSell& A, const double* din, const double* dinv, double c1, double c2,
                       double* res, double* dout, double* x);
-void launch_residual_restrict(hipStream_t s, const AmgCsr& A, const double* res, const double* dlast, double* t,
+void launch_residual_restrict(hipStream_t s, const AmgSell& A, const double* res, const double* dlast, double* t,
                               int nc, const int* memPtr, const int* mem, double* rc, double* x);
 void launch_prolong(hipStream_t s, int n, const int* agg, const double* xc, double* x);
 void launch_dense_matvec(hipStream_t s, int n, const double* Ainv, const double* r, double* x);

@@ -18,6 +18,27 @@ __device__ __forceinline__ double row_dot(const AmgCsr& A, int i, int lane, cons
   return s;
 }
 
+// one row of the sliced-ELLPACK copy times x, one thread per row: the 64 rows of a slice
+// are the 64 lanes of one wave, so every load of values and columns is coalesced and the
+// loop trip count (the slice width) is uniform over the wave
+__device__ __forceinline__ double sell_dot(const AmgSell& A, int i, const double* __restrict__ x) {
+  const int b = A.off[i >> 6], w = (A.off[(i >> 6) + 1] - b) >> 6;
+  const float* __restrict__ v = A.val + b + (i & 63);
+  const int* __restrict__ c = A.col + b + (i & 63);
+  double acc = 0.0;
+#pragma unroll 4
+  for (int k = 0; k < w; k++) acc += (double)v[k * 64] * x[c[k * 64]];
+  return acc;
+}
+
+__global__ __launch_bounds__(256) void sell_fill_kernel(int nSlots, const int* __restrict__ pos,
+                                                      const double* __restrict__ csrVal, float* __restrict__ val) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nSlots) return;
+  const int p = pos[j];
+  val[j] = p >= 0 ? (float)csrVal[p] : 0.0f;
+}
+
 // contribution B(:,a)^T D^-1 B(:,b) of node n to S(q_a, q_b), one thread per (node, a)
 __global__ __launch_bounds__(256) void schur_contrib_kernel(int nPairs, const int* __restrict__ pairNode,
                                                           const int* __restrict__ nodeGroup,
@@ -90,51 +111,39 @@ __global__ __launch_bounds__(256) void cheb_start_kernel(int n, const double* __
 }
 
 // res = r - A x ; d = c0 D^-1 res   (x is read, not written)
-__global__ __launch_bounds__(256) void cheb_start_x_kernel(AmgCsr A, const double* __restrict__ r,
+__global__ __launch_bounds__(256) void cheb_start_x_kernel(AmgSell A, const double* __restrict__ r,
                                                          const double* __restrict__ dinv, double c0,
                                                          double* __restrict__ res, double* __restrict__ d,
                                                          const double* __restrict__ x) {
-  const int lane = threadIdx.x % kLanes, rpb = blockDim.x / kLanes;
-  for (int i = blockIdx.x * rpb + threadIdx.x / kLanes; i < A.n; i += gridDim.x * rpb) {
-    const double ax = row_dot(A, i, lane, x);
-    if (lane == 0) {
-      const double ri = r[i] - ax;
-      res[i] = ri;
-      d[i] = c0 * dinv[i] * ri;
-    }
-  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  const double ri = r[i] - sell_dot(A, i, x);
+  res[i] = ri;
+  d[i] = c0 * dinv[i] * ri;
 }
 
 // res -= A din ; dout = c1 din + c2 D^-1 res ; x += din
-__global__ __launch_bounds__(256) void cheb_step_kernel(AmgCsr A, const double* __restrict__ din,
+__global__ __launch_bounds__(256) void cheb_step_kernel(AmgSell A, const double* __restrict__ din,
                                                       const double* __restrict__ dinv, double c1, double c2,
                                                       double* __restrict__ res, double* __restrict__ dout,
                                                       double* __restrict__ x) {
-  const int lane = threadIdx.x % kLanes, rpb = blockDim.x / kLanes;
-  for (int i = blockIdx.x * rpb + threadIdx.x / kLanes; i < A.n; i += gridDim.x * rpb) {
-    const double ad = row_dot(A, i, lane, din);
-    if (lane == 0) {
-      const double ri = res[i] - ad;
-      res[i] = ri;
-      const double di = din[i];
-      dout[i] = c1 * di + c2 * dinv[i] * ri;
-      x[i] += di;
-    }
-  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  const double ri = res[i] - sell_dot(A, i, din);
+  res[i] = ri;
+  const double di = din[i];
+  dout[i] = c1 * di + c2 * dinv[i] * ri;
+  x[i] += di;
 }
 
 // t = res - A dlast (the residual of x + dlast) ; x += dlast
-__global__ __launch_bounds__(256) void residual_kernel(AmgCsr A, const double* __restrict__ res,
+__global__ __launch_bounds__(256) void residual_kernel(AmgSell A, const double* __restrict__ res,
                                                      const double* __restrict__ dlast, double* __restrict__ t,
                                                      double* __restrict__ x) {
-  const int lane = threadIdx.x % kLanes, rpb = blockDim.x / kLanes;
-  for (int i = blockIdx.x * rpb + threadIdx.x / kLanes; i < A.n; i += gridDim.x * rpb) {
-    const double ad = row_dot(A, i, lane, dlast);
-    if (lane == 0) {
-      t[i] = res[i] - ad;
-      x[i] += dlast[i];
-    }
-  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  t[i] = res[i] - sell_dot(A, i, dlast);
+  x[i] += dlast[i];
 }
 
 // rc[I] = sum of t over the members of aggregate I, in member order
@@ -153,13 +162,15 @@ __global__ __launch_bounds__(256) void prolong_kernel(int n, const int* __restri
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += xc[agg[i]];
 }
 
+// x = Ainv r, one 64-lane wave per row (4 rows per block)
 __global__ __launch_bounds__(256) void dense_matvec_kernel(int n, const double* __restrict__ Ainv,
                                                          const double* __restrict__ r, double* __restrict__ x) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (i >= n) return;
   double s = 0.0;
-  for (int j = 0; j < n; j++) s += Ainv[(size_t)i * n + j] * r[j];
-  x[i] = s;
+  for (int j = lane; j < n; j += 64) s += Ainv[(size_t)i * n + j] * r[j];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) x[i] = s;
 }
 
 __global__ __launch_bounds__(256) void q_gather_kernel(int nq, const int* __restrict__ qrow,
@@ -239,19 +250,24 @@ void launch_cheb_start(hipStream_t s, int n, const double* r, const double* dinv
   hipLaunchKernelGGL(cheb_start_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, r, dinv, c0, res, d, x);
 }
 
-void launch_cheb_start_x(hipStream_t s, const AmgCsr& A, const double* r, const double* dinv, double c0, double* res,
-                         double* d, double* x) {
-  hipLaunchKernelGGL(cheb_start_x_kernel, dim3(rowGrid(A.n)), dim3(256), 0, s, A, r, dinv, c0, res, d, x);
+void launch_sell_fill(hipStream_t s, int nSlots, const int* pos, const double* csrVal, float* sellVal) {
+  if (nSlots <= 0) return;
+  hipLaunchKernelGGL(sell_fill_kernel, dim3(cdiv(nSlots, 256)), dim3(256), 0, s, nSlots, pos, csrVal, sellVal);
 }
 
-void launch_cheb_step(hipStream_t s, const AmgCsr& A, const double* din, const double* dinv, double c1, double c2,
+void launch_cheb_start_x(hipStream_t s, const AmgSell& A, const double* r, const double* dinv, double c0, double* res,
+                         double* d, const double* x) {
+  hipLaunchKernelGGL(cheb_start_x_kernel, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, r, dinv, c0, res, d, x);
+}
+
+void launch_cheb_step(hipStream_t s, const AmgSell& A, const double* din, const double* dinv, double c1, double c2,
                       double* res, double* dout, double* x) {
-  hipLaunchKernelGGL(cheb_step_kernel, dim3(rowGrid(A.n)), dim3(256), 0, s, A, din, dinv, c1, c2, res, dout, x);
+  hipLaunchKernelGGL(cheb_step_kernel, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, din, dinv, c1, c2, res, dout, x);
 }
 
-void launch_residual_restrict(hipStream_t s, const AmgCsr& A, const double* res, const double* dlast, double* t,
+void launch_residual_restrict(hipStream_t s, const AmgSell& A, const double* res, const double* dlast, double* t,
                               int nc, const int* memPtr, const int* mem, double* rc, double* x) {
-  hipLaunchKernelGGL(residual_kernel, dim3(rowGrid(A.n)), dim3(256), 0, s, A, res, dlast, t, x);
+  hipLaunchKernelGGL(residual_kernel, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, res, dlast, t, x);
   hipLaunchKernelGGL(restrict_kernel, dim3(cdiv(nc, 256)), dim3(256), 0, s, nc, memPtr, mem, t, rc);
 }
 
@@ -260,7 +276,7 @@ void launch_prolong(hipStream_t s, int n, const int* agg, const double* xc, doub
 }
 
 void launch_dense_matvec(hipStream_t s, int n, const double* Ainv, const double* r, double* x) {
-  hipLaunchKernelGGL(dense_matvec_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, Ainv, r, x);
+  hipLaunchKernelGGL(dense_matvec_kernel, dim3(cdiv(n, 4)), dim3(256), 0, s, n, Ainv, r, x);
 }
 
 void launch_q_gather(hipStream_t s, int nq, const int* qrow, const double* v, const MinresState* st, double* out) {

@@ -1,6 +1,7 @@
 // Block(2) Krylov data set-up, the MINRES driver, and the Newton-Raphson
 // loop of NonlinearFESolver::solve (FESolver.cpp:407-521) on the device.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -304,15 +305,41 @@ int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* 
   w.st = d_mstate.as<MinresState>();
   minres_start(stream, M, P, w, b, hook);
   hipCheck(hipGetLastError(), "minres start");
-  const int check = 16;
+  // The host rotates the work-vector pointers with period 6, so 12 iterations form a
+  // launch sequence with fixed arguments: capture it once per solve and replay it;
+  // iterations after convergence do nothing (every kernel tests the state).
+  // NLA3D_GPU_GRAPHS=0 launches every iteration directly (the same kernels and arguments),
+  // for tools that do not follow graph launches.
+  static const bool graphs = [] {
+    const char* e = std::getenv("NLA3D_GPU_GRAPHS");
+    return !(e && e[0] == '0');
+  }();
+  const int period = 12;
+  hipGraphExec_t exec = nullptr;
+  if (graphs && maxit >= 4 * period) {
+    hipGraph_t g = nullptr;
+    MinresWork w0 = w;
+    hipCheck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
+    for (int k = 0; k < period; k++) minres_iteration(stream, M, P, w, tol, hook);
+    hipCheck(hipStreamEndCapture(stream, &g), "end capture");
+    hipCheck(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0), "instantiate");
+    (void)hipGraphDestroy(g);
+    if (w.R1 != w0.R1 || w.Yp != w0.Yp || w.W1 != w0.W1) throw ApiError(-2, "MINRES rotation is not periodic");
+  }
   int it = 0;
   for (;;) {
-    for (int k = 0; k < check && it < maxit; k++, it++) minres_iteration(stream, M, P, w, tol, hook);
+    if (exec && it + period <= maxit) {
+      hipCheck(hipGraphLaunch(exec, stream), "graph launch");
+      it += period;
+    } else {
+      for (int k = 0; k < period && it < maxit; k++, it++) minres_iteration(stream, M, P, w, tol, hook);
+    }
     hipCheck(hipGetLastError(), "minres iteration");
     hipCheck(hipMemcpyAsync(h_mstate, w.st, sizeof(MinresState), hipMemcpyDeviceToHost, stream), "state");
     hipCheck(hipStreamSynchronize(stream), "minres sync");
     if (h_mstate->converged || it >= maxit) break;
   }
+  if (exec) (void)hipGraphExecDestroy(exec);
   if (relres_est) *relres_est = h_mstate->beta1 > 0 ? h_mstate->phibar / h_mstate->beta1 : 0.0;
   if (breakdown) *breakdown = h_mstate->breakdown != 0;
   return h_mstate->itn;
@@ -402,16 +429,16 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
   // solve K.block(2) dU_sl = rhs (FESolver.cpp:457)
   t0 = nowMs();
   const bool learnedPc = o.preconditioner == 1 || o.preconditioner == 3;
-  if (learnedPc) learnedPrecondSetup(c, t);
-  else krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
+  krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
   PrecondHook hook;
   bool useSchur = false;
-  if (o.preconditioner == 2 || o.preconditioner == 3) {
+  if (o.preconditioner == 2 || o.preconditioner == 3) {  // S from the plain block-Jacobi blocks
     std::string why;
     useSchur = schurUsable(c, &why) && schurRefresh(c, &why);
     if (useSchur) hook = schurHook(c);
     else st.precond_fallbacks++;  // plain block Jacobi for this solve
   }
+  if (learnedPc) learnedCorrect(c, t);
   double relres = 0;
   bool brk = false;
   int kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), o.krylov_tol, o.krylov_max_iter, &relres, &brk,

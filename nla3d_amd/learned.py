@@ -57,13 +57,14 @@ def set_normalization_from(net, model, t=0.5):
 
 
 def train(net, n_pairs=2000, k=100, lr=1e-4, seed=20261203, sizes=(8, 10, 12, 16), load_steps=4,
-          random_rhs_per_tangent=0, device=0, validation=(), check_every=250, log=None):
+          random_rhs_per_tangent=0, device=0, validation=(), check_every=250, log=None, multigrid=False):
     """Adam steps over n_pairs (tangent, rhs) pairs harvested from Newton iterations of
     seeded training models.
 
     validation: (model, rhs, t) systems judged every check_every pairs by full MINRES
     solves to 1e-9; the parameters with the fewest summed iterations are kept, the
-    untrained network (exactly block Jacobi) being the first candidate.  Returns a dict
+    untrained network (exactly block Jacobi) being the first candidate.  multigrid: train
+    and judge under the Schur-row multigrid (preconditioner 3 against 2).  Returns a dict
     with the loss history and the validation record."""
     rng = np.random.default_rng(seed)
     opts = nla3d_amd.GpuModel.newton_defaults(criterion=1, convergence=1e-6, krylov_tol=1e-8,
@@ -73,7 +74,7 @@ def train(net, n_pairs=2000, k=100, lr=1e-4, seed=20261203, sizes=(8, 10, 12, 16
 
     def evaluate():
         nonlocal best
-        its = [judge(vm, net, vr, t=vt)[1] for vm, vr, vt in validation]
+        its = [judge(vm, net, vr, t=vt, base=2 if multigrid else 0)[1] for vm, vr, vt in validation]
         record.append((pairs, its))
         if best is None or sum(its) < best[0]:
             best = (sum(its), net.get_params())
@@ -107,7 +108,7 @@ def train(net, n_pairs=2000, k=100, lr=1e-4, seed=20261203, sizes=(8, 10, 12, 16
                     for q in range(1 + random_rhs_per_tangent):
                         rhs = None if q == 0 else rng.standard_normal(n2)
                         try:
-                            losses.append(net.train_step(m, rhs, k=k, t=t, lr=lr))
+                            losses.append(net.train_step(m, rhs, k=k, t=t, lr=lr, multigrid=multigrid))
                         except nla3d_amd.Nla3dGpuError:
                             continue  # Krylov space exhausted on a tiny model: skip the sample
                         pairs += 1
@@ -128,10 +129,11 @@ def train(net, n_pairs=2000, k=100, lr=1e-4, seed=20261203, sizes=(8, 10, 12, 16
             "best_validation_iterations": best[0] if best else None}
 
 
-def judge(model, net, rhs, tol=1e-9, max_iter=100000, t=1.0):
-    """solve K.block(2) x = rhs with block Jacobi and with the learned corrections;
-    returns (iterations bj, iterations learned, max |x_bj - x_learned| / max |x_bj|)"""
+def judge(model, net, rhs, tol=1e-9, max_iter=100000, t=1.0, base=0):
+    """solve K.block(2) x = rhs without (preconditioner base: 0 block Jacobi, 2 with the
+    Schur-row multigrid) and with the learned corrections (base + 1); returns
+    (iterations without, iterations with, max |x_without - x_with| / max |x_without|)"""
     model.attach_mlp(net)
-    x0, it0, _ = model.solve_block2_pc(rhs, tol, max_iter, preconditioner=0, t=t)
-    x1, it1, _ = model.solve_block2_pc(rhs, tol, max_iter, preconditioner=1, t=t)
+    x0, it0, _ = model.solve_block2_pc(rhs, tol, max_iter, preconditioner=base, t=t)
+    x1, it1, _ = model.solve_block2_pc(rhs, tol, max_iter, preconditioner=base + 1, t=t)
     return it0, it1, float(np.abs(x0 - x1).max() / max(np.abs(x0).max(), 1e-300))

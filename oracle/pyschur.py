@@ -8,7 +8,9 @@ the device path implements:
   rows, D^-1 the block-Jacobi inverse node blocks, C their own block;
 * vcycle: one V-cycle over given aggregation maps with Jacobi-Chebyshev smoothing
   on [lmax/10, 1.05 lmax], the same polynomial before and after, exact solve on
-  the coarsest level.
+  the coarsest level.  The smoother's and the residual's products use each level's
+  matrix with its entries rounded to FP32 (products and sums in FP64), the diagonal
+  scaling and the coarse matrices the FP64 values.
 
 Pinned by: a one-level hierarchy gives S^-1 exactly; the cycle is a symmetric
 positive-definite operator; the smoother alone reduces the energy-norm error; S is
@@ -71,13 +73,21 @@ def hierarchy(S, aggs):
     return mats
 
 
+def rounded_f32(A):
+    """A with its stored entries rounded to FP32 (kept as FP64)"""
+    A = sp.csr_matrix(A, dtype=np.float64, copy=True)
+    A.data = A.data.astype(np.float32).astype(np.float64)
+    return A
+
+
 def vcycle(mats, aggs, lmaxs, r, steps=3, level=0):
     A = mats[level]
     if level == len(mats) - 1:
         return np.linalg.solve(A.toarray(), r)
     dinv = 1.0 / A.diagonal()
-    x = smooth(A, dinv, r, np.zeros_like(r), lmaxs[level], steps)
+    As = rounded_f32(A)
+    x = smooth(As, dinv, r, np.zeros_like(r), lmaxs[level], steps)
     agg = aggs[level]
-    rc = np.bincount(agg, weights=r - A @ x, minlength=mats[level + 1].shape[0])
+    rc = np.bincount(agg, weights=r - As @ x, minlength=mats[level + 1].shape[0])
     x = x + vcycle(mats, aggs, lmaxs, rc, steps, level + 1)[agg]
-    return smooth(A, dinv, r, x, lmaxs[level], steps)
+    return smooth(As, dinv, r, x, lmaxs[level], steps)

@@ -141,6 +141,54 @@ def test_unrolled_minres_loss_and_gradient(gpu_lib):
     assert np.all(np.abs(g[idx] - fd) <= 1e-3 * np.abs(fd).max() + 1e-2 * np.abs(fd))
 
 
+def test_unrolled_minres_gradient_under_schur_multigrid(gpu_lib):
+    """preconditioner 3's operator: corrected node blocks, the V-cycle (taken from the device
+    column by column, a fixed symmetric matrix) on the Schur rows, block Jacobi elsewhere;
+    loss against the oracle's recurrence on that matrix, gradient against its finite differences"""
+    print("[gpu] unrolled MINRES loss gradient, multigrid", flush=True)
+    m = rigid_model(4, 4, 2)
+    o = oracle_inputs(m)
+    rng = np.random.default_rng(17)
+    net = nla3d_amd.Mlp(0, seed=4)
+    w = random_params(rng, out_scale=0.2)
+    net.set_params(w)
+    X, rows = m.node_features(0.5)
+    mean, std = X.mean(0), X.std(0)
+    istd = np.where(std > 1e-6, 1.0 / np.maximum(std, 1e-6), 1.0).astype(np.float32)
+    net.set_normalization(mean.astype(np.float32), istd)
+    k = 12
+    rhs = rng.standard_normal(o["n2"])
+    loss, g = net.loss_grad(m, rhs, k=k, t=0.5, multigrid=True)
+    qrow, _, _, _ = m.schur_matrix()
+    nq = len(qrow)
+    V = np.column_stack([m.schur_apply(np.eye(nq)[:, j]) for j in range(nq)])
+    assert np.abs(V - V.T).max() <= 1e-10 * np.abs(V).max()
+    q = qrow.astype(np.int64)
+
+    def oracle_loss(wv):
+        Y, _ = pl.mlp_forward(wv, X.astype(np.float64), mean.astype(np.float32), istd)
+        P = pl.corrected_blocks(o["K2"], o["lrows"], Y)
+        M = pl.precond_matrix(o["n2"], o["bj"], o["lrows"], P).toarray()
+        M[np.ix_(q, q)] = V
+        return pl.minres_log_rate(o["K2"], M, rhs, k)
+
+    w64 = w.astype(np.float64)
+    Lo = oracle_loss(w64)
+    assert abs(loss - Lo) <= 1e-6 * max(1.0, abs(Lo))
+    idx = np.concatenate([rng.choice(np.arange(pl.OFF_W[l], pl.OFF_B[l] + pl.SHAPES[l][0]), 3, replace=False)
+                          for l in range(5)])
+    fd = []
+    for j in idx:
+        h = 1e-4
+        wp, wm = w64.copy(), w64.copy()
+        wp[j] += h
+        wm[j] -= h
+        fd.append((oracle_loss(wp) - oracle_loss(wm)) / (2 * h))
+    fd = np.array(fd)
+    print("grad", g[idx], "fd", fd, flush=True)
+    assert np.all(np.abs(g[idx] - fd) <= 1e-3 * np.abs(fd).max() + 1e-2 * np.abs(fd))
+
+
 def test_training_gain_and_block_jacobi_judge(gpu_lib):
     """train on seeded small models, checkpoint by the block-Jacobi judge on one
     validation system, and measure on a held-out model of another size and seed"""
