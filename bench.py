@@ -390,6 +390,9 @@ def main():
                                     2: "block-Jacobi + Schur multigrid",
                                     3: "learned block-Jacobi + Schur multigrid"}[opts.preconditioner],
                  "precond_fallbacks": st.precond_fallbacks})
+        if opts.preconditioner >= 2:
+            si = m.schur_info()
+            extra["schur_multigrid"] = {"rows": si["nq"], "nnz_level0": si["nnz0"], "levels": si["sizes"]}
     value = nE * 1e3 / ms_step
     print(f"[bench] {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
 
