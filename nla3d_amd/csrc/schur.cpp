@@ -18,6 +18,17 @@ namespace nla3d_amd {
 constexpr int kCoarsest = 256;   // at most this many rows are inverted densely
 constexpr int kMaxCoarse = 2048; // coarsening that stalls above this disables the preconditioner
 constexpr int kMaxQPerNode = 64;
+// Jacobi-Chebyshev smoothing on [lmax / div, 1.05 lmax] with `steps` terms.  The finest level
+// gets a long polynomial over a wide interval: S's spectrum holds many small eigenvalues
+// (checkerboard pressure modes, whose smooth envelopes no piecewise-constant aggregate
+// represents), so a short smoother leaves them all to the coarse grid.  Coarser levels
+// keep a short one.
+constexpr int kSteps0 = 12;
+constexpr double kLoDiv0 = 400.0;
+constexpr int kStepsC = 3;
+constexpr double kLoDivC = 10.0;
+constexpr int kPowerSteps = 40;     // power iteration for lmax of D^-1 A
+constexpr double kLmaxSafety = 1.05;  // the estimate approaches lmax from below
 
 struct AmgLevel {
   int n = 0, nnz = 0, nc = 0, nSlots = 0;
@@ -41,7 +52,6 @@ struct SchurAmg {
   DevBuf sSeg, sSrc, sDiag, cPos;
   std::vector<std::unique_ptr<AmgLevel>> L;
   DevBuf cinv;
-  int steps = 3;
   bool fresh = false;
 };
 
@@ -334,19 +344,20 @@ bool schurRefresh(Ctx& c, std::string* why) {
     AmgLevel& A = *S.L[l];
     launch_sell_fill(s, A.nSlots, A.sPos.as<int>(), A.val.as<double>(), A.sVal.as<float>());
     launch_inv_diag(s, A.csr(), A.dinv.as<double>());
-    // largest eigenvalue of D^-1 A by power iteration (12 steps from a fixed pseudo-random start)
+    // largest eigenvalue of D^-1 A by power iteration from a fixed pseudo-random start,
+    // raised by a safety factor (the Chebyshev polynomial grows quickly above its interval)
     double* x = A.da.as<double>();
     double* y = A.db.as<double>();
     hipCheck(hipMemcpyAsync(x, A.start.p, (size_t)A.n * 8, hipMemcpyDeviceToDevice, s), "start");
     double lam = 1.0;
-    for (int it = 0; it < 13; it++) {
+    for (int it = 0; it < kPowerSteps; it++) {
       const double nx = std::sqrt(hostDot(c, A.n, x, x));
       if (!(nx > 0.0)) break;
       launch_dinv_spmv(s, A.csr(), A.dinv.as<double>(), x, 1.0 / nx, y);
       std::swap(x, y);
-      if (it == 12) lam = std::sqrt(hostDot(c, A.n, x, x));
+      if (it == kPowerSteps - 1) lam = std::sqrt(hostDot(c, A.n, x, x));
     }
-    A.lmax = lam;
+    A.lmax = kLmaxSafety * lam;
     AmgLevel& C = *S.L[l + 1];
     launch_gather_sum(s, C.nnz, A.gSeg.as<int>(), A.gSrc.as<int>(), A.val.as<double>(), C.val.as<double>());
   }
@@ -398,18 +409,19 @@ bool schurRefresh(Ctx& c, std::string* why) {
   return true;
 }
 
-// Chebyshev coefficients on [lmax/10, 1.05 lmax] (Jacobi-preconditioned, steps terms)
+// Chebyshev coefficients on [lmax/div, 1.05 lmax] (Jacobi-preconditioned, steps terms)
 struct ChebCoef {
   double c0;
-  double c1[8], c2[8];
+  double c1[kSteps0], c2[kSteps0];
 };
-static ChebCoef chebCoef(double lmax, int steps) {
-  const double hi = 1.05 * lmax, lo = lmax / 10.0;
+static_assert(kStepsC <= kSteps0, "coefficient arrays sized by the longest smoother");
+static ChebCoef chebCoef(double lmax, int steps, double div) {
+  const double hi = 1.05 * lmax, lo = lmax / div;
   const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
   ChebCoef k{};
   k.c0 = 1.0 / theta;
   double rho = 1.0 / sigma;
-  for (int i = 1; i < steps && i < 8; i++) {
+  for (int i = 1; i < steps && i < kSteps0; i++) {
     const double rn = 1.0 / (2.0 * sigma - rho);
     k.c1[i] = rn * rho;
     k.c2[i] = 2.0 * rn / delta;
@@ -424,13 +436,14 @@ static void vcycle(SchurAmg& S, hipStream_t s, size_t l) {
     launch_dense_matvec(s, A.n, S.cinv.as<double>(), A.r.as<double>(), A.x.as<double>());
     return;
   }
-  const ChebCoef k = chebCoef(A.lmax, S.steps);
+  const int steps = l == 0 ? kSteps0 : kStepsC;
+  const ChebCoef k = chebCoef(A.lmax, steps, l == 0 ? kLoDiv0 : kLoDivC);
   const AmgSell M = A.sell();
   double* dinv = A.dinv.as<double>();
   double* din = A.da.as<double>();
   double* dout = A.db.as<double>();
   launch_cheb_start(s, A.n, A.r.as<double>(), dinv, k.c0, A.res.as<double>(), din, A.x.as<double>());
-  for (int i = 1; i < S.steps; i++) {
+  for (int i = 1; i < steps; i++) {
     launch_cheb_step(s, M, din, dinv, k.c1[i], k.c2[i], A.res.as<double>(), dout, A.x.as<double>());
     std::swap(din, dout);
   }
@@ -442,7 +455,7 @@ static void vcycle(SchurAmg& S, hipStream_t s, size_t l) {
   din = A.da.as<double>();
   dout = A.db.as<double>();
   launch_cheb_start_x(s, M, A.r.as<double>(), dinv, k.c0, A.res.as<double>(), din, A.x.as<double>());
-  for (int i = 1; i < S.steps; i++) {
+  for (int i = 1; i < steps; i++) {
     launch_cheb_step(s, M, din, dinv, k.c1[i], k.c2[i], A.res.as<double>(), dout, A.x.as<double>());
     std::swap(din, dout);
   }

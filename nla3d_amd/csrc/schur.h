@@ -8,8 +8,9 @@
 // blocks, C their own block (the pressure rows' -V/k).  S is SPD when the node blocks
 // are.  Levels: piecewise-constant aggregation of S's graph (greedy, distance one),
 // Galerkin coarse matrices P^T A P (FP64), Jacobi-Chebyshev smoothing (the same
-// polynomial before and after, so the cycle is symmetric) whose products use an FP32
-// sliced-ELLPACK copy of each level, an explicit dense FP64 inverse on the coarsest level.  All structure is built once per model; values, smoother intervals
+// polynomial before and after, so the cycle is symmetric; a long one on the finest level)
+// whose products use an FP32 sliced-ELLPACK copy of each level, an explicit dense FP64
+// inverse on the coarsest level.  All structure is built once per model; values, smoother intervals
 // (power iteration) and the coarsest inverse are refreshed with every tangent.
 #pragma once
 

@@ -7,8 +7,9 @@ the device path implements:
   Schur estimate (element pressures, multipliers), B their couplings to the node
   rows, D^-1 the block-Jacobi inverse node blocks, C their own block;
 * vcycle: one V-cycle over given aggregation maps with Jacobi-Chebyshev smoothing
-  on [lmax/10, 1.05 lmax], the same polynomial before and after, exact solve on
-  the coarsest level.  The smoother's and the residual's products use each level's
+  on [lmax/div, 1.05 lmax], the same polynomial before and after (finest level:
+  12 terms, div 400; coarser levels: 3 terms, div 10), exact solve on the
+  coarsest level.  The smoother's and the residual's products use each level's
   matrix with its entries rounded to FP32 (products and sums in FP64), the diagonal
   scaling and the coarse matrices the FP64 values.
 
@@ -36,8 +37,12 @@ def schur_matrix(K2, groups, bj):
     return q, (Bq.T @ Dinv @ Bq - C).toarray()
 
 
-def cheb_coef(lmax, steps):
-    hi, lo = 1.05 * lmax, lmax / 10.0
+STEPS0, DIV0 = 12, 400.0   # finest level
+STEPSC, DIVC = 3, 10.0     # coarser levels
+
+
+def cheb_coef(lmax, steps, div=10.0):
+    hi, lo = 1.05 * lmax, lmax / div
     theta, delta = 0.5 * (hi + lo), 0.5 * (hi - lo)
     sigma = theta / delta
     c0 = 1.0 / theta
@@ -51,9 +56,9 @@ def cheb_coef(lmax, steps):
     return c0, c1, c2
 
 
-def smooth(A, dinv, r, x, lmax, steps):
+def smooth(A, dinv, r, x, lmax, steps, div=10.0):
     """Jacobi-Chebyshev: x + p(D^-1 A) D^-1 (r - A x)"""
-    c0, c1, c2 = cheb_coef(lmax, steps)
+    c0, c1, c2 = cheb_coef(lmax, steps, div)
     res = r - A @ x
     d = c0 * dinv * res
     for i in range(1, steps):
@@ -80,14 +85,15 @@ def rounded_f32(A):
     return A
 
 
-def vcycle(mats, aggs, lmaxs, r, steps=3, level=0):
+def vcycle(mats, aggs, lmaxs, r, level=0):
     A = mats[level]
     if level == len(mats) - 1:
         return np.linalg.solve(A.toarray(), r)
+    steps, div = (STEPS0, DIV0) if level == 0 else (STEPSC, DIVC)
     dinv = 1.0 / A.diagonal()
     As = rounded_f32(A)
-    x = smooth(As, dinv, r, np.zeros_like(r), lmaxs[level], steps)
+    x = smooth(As, dinv, r, np.zeros_like(r), lmaxs[level], steps, div)
     agg = aggs[level]
     rc = np.bincount(agg, weights=r - As @ x, minlength=mats[level + 1].shape[0])
-    x = x + vcycle(mats, aggs, lmaxs, rc, steps, level + 1)[agg]
-    return smooth(As, dinv, r, x, lmaxs[level], steps)
+    x = x + vcycle(mats, aggs, lmaxs, rc, level + 1)[agg]
+    return smooth(As, dinv, r, x, lmaxs[level], steps, div)

@@ -38,3 +38,25 @@ def test_smoother_reduces_energy_error():
     x = ps.smooth(A, dinv, r, np.zeros(64), lmax, 3)
     e0, e1 = x_true, x_true - x
     assert e1 @ A @ e1 < e0 @ A @ e0
+
+
+def test_chebyshev_error_polynomial_is_the_minimax_one():
+    """On A = diag(mu) (D = I) the smoother maps an error e to (1 - mu p(mu)) e; the
+    three-term recurrence must reproduce the scaled Chebyshev polynomial
+    T_k((theta - mu) / delta) / T_k(theta / delta) on [lo, hi] = [lmax/div, 1.05 lmax]:
+    its closed form, |.| <= 1 / T_k(theta/delta) inside the interval and < 1 below it
+    (for both the finest-level and the coarse-level settings)"""
+    lmax = 2.0
+    mu = np.linspace(1e-6, 1.05 * lmax, 4001)
+    for steps, div in ((ps.STEPS0, ps.DIV0), (ps.STEPSC, ps.DIVC)):
+        e = ps.smooth(np.diag(mu), np.ones_like(mu), np.zeros_like(mu), np.ones_like(mu), lmax, steps, div)
+        hi, lo = 1.05 * lmax, lmax / div
+        theta, delta = 0.5 * (hi + lo), 0.5 * (hi - lo)
+        T = lambda x: np.cosh(steps * np.arccosh(x)) if np.ndim(x) == 0 else np.where(
+            np.abs(x) <= 1, np.cos(steps * np.arccos(np.clip(x, -1, 1))),
+            np.sign(x) ** steps * np.cosh(steps * np.arccosh(np.abs(x))))
+        closed = T((theta - mu) / delta) / T(theta / delta)
+        assert np.abs(e - closed).max() <= 1e-10
+        inside = mu >= lo
+        assert np.abs(e[inside]).max() <= 1.0 / T(theta / delta) + 1e-12
+        assert np.abs(e[~inside]).max() < 1.0
