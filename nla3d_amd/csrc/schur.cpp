@@ -2,6 +2,7 @@
 // (once per model), the refresh of values with every tangent, the V-cycle launch
 // sequence, its MINRES hook and the ABI entries that expose S and one V-cycle.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -23,7 +24,7 @@ constexpr int kMaxQPerNode = 64;
 // (checkerboard pressure modes, whose smooth envelopes no piecewise-constant aggregate
 // represents), so a short smoother leaves them all to the coarse grid.  Coarser levels
 // keep a short one.
-constexpr int kSteps0 = 12;
+constexpr int kSteps0 = 16;
 constexpr double kLoDiv0 = 400.0;
 constexpr int kStepsC = 3;
 constexpr double kLoDivC = 10.0;
@@ -34,11 +35,16 @@ struct AmgLevel {
   int n = 0, nnz = 0, nc = 0, nSlots = 0;
   std::vector<int> h_rowPtr, h_col;
   DevBuf rowPtr, col, val, dinv, agg, memPtr, mem, gSeg, gSrc;
-  DevBuf sOff, sCol, sPos, sVal;  // the smoother's sliced-ELLPACK copy
+  DevBuf sOff, sCol, sCol16, sBase, sPos, sVal;  // the smoother's sliced-ELLPACK copy
+  bool narrow = false;
   DevBuf r, x, res, da, db, t, start;
   double lmax = 1.0;
   AmgCsr csr() const { return AmgCsr{n, rowPtr.as<int>(), col.as<int>(), val.as<double>()}; }
-  AmgSell sell() const { return AmgSell{n, sOff.as<int>(), sCol.as<int>(), sVal.as<float>()}; }
+  AmgSell sell() const {
+    return AmgSell{n, sOff.as<int>(), narrow ? nullptr : sCol.as<int>(),
+                   narrow ? sCol16.as<unsigned short>() : nullptr, narrow ? sBase.as<int>() : nullptr,
+                   sVal.as<float>()};
+  }
 };
 
 struct SchurAmg {
@@ -121,6 +127,21 @@ static void allocLevel(AmgLevel& A, bool coarsest) {
     off[sl + 1] = off[sl] + 64 * w;
   }
   A.nSlots = off[nSl];
+  // base column of each slice; 16-bit offsets when every slice spans < 65,536 columns
+  std::vector<int> base(nSl, 0);
+  A.narrow = true;
+  for (int sl = 0; sl < nSl; sl++) {
+    int lo = INT_MAX, hi = -1;
+    for (int i = sl * 64; i < std::min(A.n, sl * 64 + 64); i++)
+      for (int k = A.h_rowPtr[i]; k < A.h_rowPtr[i + 1]; k++) {
+        lo = std::min(lo, A.h_col[k]);
+        hi = std::max(hi, A.h_col[k]);
+      }
+    if (hi < 0) lo = hi = sl * 64;
+    base[sl] = lo;
+    if (hi - lo > 65535) A.narrow = false;
+  }
+  // padding: value 0 at the slice's base column (a valid index)
   std::vector<int> scol(A.nSlots), spos(A.nSlots);
   for (int sl = 0; sl < nSl; sl++) {
     const int w = (off[sl + 1] - off[sl]) / 64;
@@ -129,13 +150,21 @@ static void allocLevel(AmgLevel& A, bool coarsest) {
       for (int k = 0; k < w; k++) {
         const int j = off[sl] + k * 64 + l;
         const bool real = i < A.n && k < A.h_rowPtr[i + 1] - A.h_rowPtr[i];
-        scol[j] = real ? A.h_col[A.h_rowPtr[i] + k] : (i < A.n ? i : 0);
+        scol[j] = real ? A.h_col[A.h_rowPtr[i] + k] : base[sl];
         spos[j] = real ? A.h_rowPtr[i] + k : -1;
       }
     }
   }
   A.sOff.upload(off);
-  A.sCol.upload(scol);
+  if (A.narrow) {
+    std::vector<unsigned short> c16(A.nSlots);
+    for (int sl = 0; sl < nSl; sl++)
+      for (int j = off[sl]; j < off[sl + 1]; j++) c16[j] = (unsigned short)(scol[j] - base[sl]);
+    A.sCol16.upload(c16);
+    A.sBase.upload(base);
+  } else {
+    A.sCol.upload(scol);
+  }
   A.sPos.upload(spos);
   A.sVal.alloc((size_t)std::max(A.nSlots, 1) * 4);
   A.dinv.alloc(nb);

@@ -30,12 +30,16 @@ struct AmgCsr {
 
 // the smoother's copy of a level's matrix: sliced ELLPACK over 64-row slices with FP32
 // values.  Row i = 64 s + l holds (off[s+1] - off[s]) / 64 slots at off[s] + k * 64 + l;
-// padding slots hold value 0 and column i.  The values are the FP64 matrix rounded once per
-// tangent, so the smoother stays one fixed symmetric operator.
+// padding slots hold value 0.  Columns are 16-bit offsets from the slice's base column
+// (col16, base) when every slice spans fewer than 65,536 columns, else full indices (col).
+// The values are the FP64 matrix rounded once per tangent, so the smoother stays one fixed
+// symmetric operator.
 struct AmgSell {
   int n;
   const int* off;
-  const int* col;
+  const int* col;                // full column indices (col16 == nullptr)
+  const unsigned short* col16;   // offsets from base[s]
+  const int* base;               // [slices] smallest column of the slice
   const float* val;
 };
 

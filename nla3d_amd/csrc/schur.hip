@@ -21,13 +21,23 @@ __device__ __forceinline__ double row_dot(const AmgCsr& A, int i, int lane, cons
 // one row of the sliced-ELLPACK copy times x, one thread per row: the 64 rows of a slice
 // are the 64 lanes of one wave, so every load of values and columns is coalesced and the
 // loop trip count (the slice width) is uniform over the wave
+// (kNarrow: 16-bit column offsets from the slice's base column)
+template <bool kNarrow>
 __device__ __forceinline__ double sell_dot(const AmgSell& A, int i, const double* __restrict__ x) {
-  const int b = A.off[i >> 6], w = (A.off[(i >> 6) + 1] - b) >> 6;
+  const int sl = i >> 6;
+  const int b = A.off[sl], w = (A.off[sl + 1] - b) >> 6;
   const float* __restrict__ v = A.val + b + (i & 63);
-  const int* __restrict__ c = A.col + b + (i & 63);
   double acc = 0.0;
+  if constexpr (kNarrow) {
+    const unsigned short* __restrict__ c = A.col16 + b + (i & 63);
+    const double* __restrict__ xb = x + A.base[sl];
 #pragma unroll 4
-  for (int k = 0; k < w; k++) acc += (double)v[k * 64] * x[c[k * 64]];
+    for (int k = 0; k < w; k++) acc += (double)v[k * 64] * xb[c[k * 64]];
+  } else {
+    const int* __restrict__ c = A.col + b + (i & 63);
+#pragma unroll 4
+    for (int k = 0; k < w; k++) acc += (double)v[k * 64] * x[c[k * 64]];
+  }
   return acc;
 }
 
@@ -111,25 +121,27 @@ __global__ __launch_bounds__(256) void cheb_start_kernel(int n, const double* __
 }
 
 // res = r - A x ; d = c0 D^-1 res   (x is read, not written)
+template <bool kNarrow>
 __global__ __launch_bounds__(256) void cheb_start_x_kernel(AmgSell A, const double* __restrict__ r,
                                                          const double* __restrict__ dinv, double c0,
                                                          double* __restrict__ res, double* __restrict__ d,
                                                          const double* __restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
-  const double ri = r[i] - sell_dot(A, i, x);
+  const double ri = r[i] - sell_dot<kNarrow>(A, i, x);
   res[i] = ri;
   d[i] = c0 * dinv[i] * ri;
 }
 
 // res -= A din ; dout = c1 din + c2 D^-1 res ; x += din
+template <bool kNarrow>
 __global__ __launch_bounds__(256) void cheb_step_kernel(AmgSell A, const double* __restrict__ din,
                                                       const double* __restrict__ dinv, double c1, double c2,
                                                       double* __restrict__ res, double* __restrict__ dout,
                                                       double* __restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
-  const double ri = res[i] - sell_dot(A, i, din);
+  const double ri = res[i] - sell_dot<kNarrow>(A, i, din);
   res[i] = ri;
   const double di = din[i];
   dout[i] = c1 * di + c2 * dinv[i] * ri;
@@ -137,12 +149,13 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(AmgSell A, const double*
 }
 
 // t = res - A dlast (the residual of x + dlast) ; x += dlast
+template <bool kNarrow>
 __global__ __launch_bounds__(256) void residual_kernel(AmgSell A, const double* __restrict__ res,
                                                      const double* __restrict__ dlast, double* __restrict__ t,
                                                      double* __restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
-  t[i] = res[i] - sell_dot(A, i, dlast);
+  t[i] = res[i] - sell_dot<kNarrow>(A, i, dlast);
   x[i] += dlast[i];
 }
 
@@ -257,17 +270,28 @@ void launch_sell_fill(hipStream_t s, int nSlots, const int* pos, const double* c
 
 void launch_cheb_start_x(hipStream_t s, const AmgSell& A, const double* r, const double* dinv, double c0, double* res,
                          double* d, const double* x) {
-  hipLaunchKernelGGL(cheb_start_x_kernel, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, r, dinv, c0, res, d, x);
+  if (A.col16)
+    hipLaunchKernelGGL(cheb_start_x_kernel<true>, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, r, dinv, c0, res, d, x);
+  else
+    hipLaunchKernelGGL(cheb_start_x_kernel<false>, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, r, dinv, c0, res, d, x);
 }
 
 void launch_cheb_step(hipStream_t s, const AmgSell& A, const double* din, const double* dinv, double c1, double c2,
                       double* res, double* dout, double* x) {
-  hipLaunchKernelGGL(cheb_step_kernel, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, din, dinv, c1, c2, res, dout, x);
+  if (A.col16)
+    hipLaunchKernelGGL(cheb_step_kernel<true>, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, din, dinv, c1, c2, res, dout,
+                       x);
+  else
+    hipLaunchKernelGGL(cheb_step_kernel<false>, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, din, dinv, c1, c2, res, dout,
+                       x);
 }
 
 void launch_residual_restrict(hipStream_t s, const AmgSell& A, const double* res, const double* dlast, double* t,
                               int nc, const int* memPtr, const int* mem, double* rc, double* x) {
-  hipLaunchKernelGGL(residual_kernel, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, res, dlast, t, x);
+  if (A.col16)
+    hipLaunchKernelGGL(residual_kernel<true>, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, res, dlast, t, x);
+  else
+    hipLaunchKernelGGL(residual_kernel<false>, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, res, dlast, t, x);
   hipLaunchKernelGGL(restrict_kernel, dim3(cdiv(nc, 256)), dim3(256), 0, s, nc, memPtr, mem, t, rc);
 }
 

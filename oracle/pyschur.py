@@ -8,7 +8,7 @@ the device path implements:
   rows, D^-1 the block-Jacobi inverse node blocks, C their own block;
 * vcycle: one V-cycle over given aggregation maps with Jacobi-Chebyshev smoothing
   on [lmax/div, 1.05 lmax], the same polynomial before and after (finest level:
-  12 terms, div 400; coarser levels: 3 terms, div 10), exact solve on the
+  16 terms, div 400; coarser levels: 3 terms, div 10), exact solve on the
   coarsest level.  The smoother's and the residual's products use each level's
   matrix with its entries rounded to FP32 (products and sums in FP64), the diagonal
   scaling and the coarse matrices the FP64 values.
@@ -37,7 +37,7 @@ def schur_matrix(K2, groups, bj):
     return q, (Bq.T @ Dinv @ Bq - C).toarray()
 
 
-STEPS0, DIV0 = 12, 400.0   # finest level
+STEPS0, DIV0 = 16, 400.0   # finest level
 STEPSC, DIVC = 3, 10.0     # coarser levels
 
 
