@@ -103,8 +103,10 @@ def cpu_baseline(m, xyz, conn, U, n_sample):
 def batch_mode(args):
     """Batched SOLID81: independent seeded cubes (n^3 hexahedra, n log-uniform in [4, 32],
     Mooney-Rivlin C10 ~ U[0.2, 1], C01 ~ U[0, 0.3], K/C10 ~ U[50, 2000], top UZ strain
-    ~ U[5%, 30%] over 10 load steps), block-Jacobi MINRES per model with its own
-    convergence.  A step is one batched Newton iteration of every unconverged model."""
+    ~ U[5%, 30%] over 10 load steps), MINRES per model with its own convergence: block
+    Jacobi in the team launch, block Jacobi + Schur-row multigrid (the default
+    --preconditioner) for the models solved on the whole device.  A step is one batched
+    Newton iteration of every unconverged model."""
     import nla3d_amd
     from nla3d_amd import meshgen
     t0 = time.perf_counter()
@@ -121,6 +123,8 @@ def batch_mode(args):
     print(f"[bench] autotuned in {t_tune:.1f}s: {cfg.as_dict()}", file=sys.stderr, flush=True)
     opts = nla3d_amd.GpuModel.newton_defaults(criterion=1, convergence=1e-8, krylov_tol=args.krylov_tol,
                                               krylov_max_iter=30000, n_load_steps=10, max_iterations=15)
+    pc = 2 if args.preconditioner == "schur-multigrid" else 0
+    opts.preconditioner = pc
     delta = 0.1
     sched, cur = [], 0.0
     while cur < 1.0:  # TimeControl (FESolver.cpp:34-56)
@@ -152,8 +156,11 @@ def batch_mode(args):
     for _ in range(args.steps):
         t1 = time.perf_counter()
         solved.append(one_step())
+        top = sorted(range(len(stats)), key=lambda k: -stats[k].last_krylov_iterations)[:4]
         print(f"[bench] batched iteration: {solved[-1]} models solved, {time.perf_counter() - t1:.2f}s, "
-              f"Krylov iterations max {max(s.last_krylov_iterations for s in stats)}", file=sys.stderr, flush=True)
+              f"most Krylov iterations (iterations, rows, whole-device): "
+              f"{[(stats[k].last_krylov_iterations, int(rows[k]), bool(rows[k] > cfg.big_rows)) for k in top]}",
+              file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     ms_step = wall * 1e3 / args.steps
     # sequential estimate: whole-device single-model solves of a seeded sample of the current systems
@@ -164,7 +171,7 @@ def batch_mode(args):
         m = models[i][0]
         r = m.get_rhs()
         t1 = time.perf_counter()
-        m.solve_block2(r, args.krylov_tol, 100000)
+        m.solve_block2_pc(r, args.krylov_tol, 100000, preconditioner=pc)
         t_seq += time.perf_counter() - t1
     seq_est = t_seq / len(sample) * len(models)
     out = {
@@ -175,6 +182,8 @@ def batch_mode(args):
         "config": {"workload": f"{len(models)} seeded SOLID81 cubes, n^3 elements with n log-uniform in [4, 32], "
                                f"{int(nel.sum())} elements, {int(rows.sum())} unknowns in total",
                    "mode": "batch", "models_solved_per_step": solved, "autotuned": cfg.as_dict(),
+                   "preconditioner": "block-Jacobi (team launch), block-Jacobi + Schur multigrid (whole device)"
+                                     if pc == 2 else "block-Jacobi",
                    "setup_s": t_setup, "autotune_s": t_tune,
                    "sequential_solve_estimate_s": seq_est,
                    "sequential_sample": f"whole-device single-model MINRES of {len(sample)} of the models' current "

@@ -136,7 +136,8 @@ typedef struct {
   double last_criterion;     /* value of the convergence measure at exit */
   double last_time;          /* normalised time of the last completed step */
   double ms_assemble, ms_solve, ms_total;  /* wall time (host clock around device work) */
-  int precond_fallbacks;     /* learned-preconditioner solves repeated with block Jacobi */
+  int precond_fallbacks;     /* solves with preconditioner 1-3 repeated with block Jacobi, or run with it
+                                because the multigrid was unavailable */
 } nla3d_gpu_newton_stats;
 
 /* Fills opts with the reference defaults (20 iterations, 10 load steps,
@@ -202,14 +203,20 @@ int nla3d_gpu_batch_set_config(nla3d_gpu_batch_t batch, const nla3d_gpu_batch_co
 /* Times candidate launch shapes on the batch itself at load time t (the assembly and
  * the element/gather grids; team size and lanes, and the large-system split, on
  * probe_iterations MINRES steps of every system) and keeps the fastest.  report
- * (may be NULL) receives up to report_cap measured times in ms, in the order tried. */
+ * (may be NULL) receives up to report_cap measured times in ms, in the order tried.
+ * Every model's solution vector U is left as it was. */
 int nla3d_gpu_batch_autotune(nla3d_gpu_batch_t batch, double t, int probe_iterations, nla3d_gpu_batch_config* chosen,
                              double* report, int report_cap);
-/* assembly + rhs + preconditioner of every model at time t (the pre-solve graph) */
+/* assembly + rhs + preconditioner of every model at time t (the pre-solve graph); the
+ * prescribed values at t are written into each model's U (as the first Newton iteration
+ * of a load step does) */
 int nla3d_gpu_batch_assemble(nla3d_gpu_batch_t batch, double t);
 /* One Newton iteration (FESolver.cpp:443-490) of every model not yet converged in
  * the load step; stats and converged hold one entry per model (converged: 1 done,
- * 0 iterating, < 0 error code).  iteration 1 starts the load step. */
+ * 0 iterating, < 0 error code).  iteration 1 starts the load step.  Preconditioner:
+ * the team launch runs block Jacobi; the models solved on the whole device (above
+ * the configuration's big_rows) add the Schur-row multigrid when opts->preconditioner
+ * is 2 or 3 (the learned corrections are not used in a batch). */
 int nla3d_gpu_batch_newton_iteration(nla3d_gpu_batch_t batch, double t, int iteration,
                                      const nla3d_gpu_newton_opts* opts, nla3d_gpu_newton_stats* stats,
                                      int* converged);

@@ -109,8 +109,12 @@ static std::vector<double> readNorms(Batch& b) {
 
 // solve block(2) of the models in `ids` (their rhs and preconditioners are current);
 // iterations and flags per model into its/flags
+// pc >= 2: the whole-device solves add the Schur-row multigrid (a model where it is
+// unavailable or whose solve fails with it is solved again with block Jacobi, counted in
+// fallbacks[k]); the team launch always runs block Jacobi
 static void solveModels(Batch& b, const std::vector<int>& ids, double tol, int maxit, std::vector<int>& its,
-                        std::vector<int>& flags, std::vector<double>& relres) {
+                        std::vector<int>& flags, std::vector<double>& relres, int pc = 0,
+                        std::vector<int>* fallbacks = nullptr) {
   std::vector<int> team, big;
   for (int k : ids) ((long long)b.n2[k] > std::min(b.cfg.big_rows, kTeamMaxRows) ? big : team).push_back(k);
   std::sort(team.begin(), team.end(), [&](int a, int c) { return b.n2[a] > b.n2[c]; });  // largest first
@@ -149,7 +153,20 @@ static void solveModels(Batch& b, const std::vector<int>& ids, double tol, int m
     hipCheck(hipStreamWaitEvent(c.stream, ready, 0), "wait");
     double rr = 0;
     bool brk = false;
-    its[k] = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), tol, maxit, &rr, &brk);
+    PrecondHook hook;
+    bool mg = false;
+    if (pc >= 2) {
+      std::string why;
+      mg = schurUsable(c, &why) && schurRefresh(c, &why);
+      if (mg) hook = schurHook(c);
+      else if (fallbacks) (*fallbacks)[k]++;
+    }
+    its[k] = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), tol, maxit, &rr, &brk, mg ? &hook : nullptr);
+    if (mg && (brk || rr > tol)) {  // block Jacobi from a fresh start
+      if (fallbacks) (*fallbacks)[k]++;
+      brk = false;
+      its[k] += c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), tol, maxit, &rr, &brk);
+    }
     flags[k] = brk ? 2 : 0;
     relres[k] = rr;
   }
@@ -200,9 +217,9 @@ static int batchNewtonIteration(Batch& b, double t, int it, const nla3d_gpu_newt
   for (int k : active) st[k].ms_assemble += tAsm / std::max<size_t>(1, active.size());
   if (active.empty()) return 0;
   t0 = nowMsB();
-  std::vector<int> its(n, 0), flags(n, 0);
+  std::vector<int> its(n, 0), flags(n, 0), fb(n, 0);
   std::vector<double> relres(n, 0.0);
-  solveModels(b, active, o.krylov_tol, o.krylov_max_iter, its, flags, relres);
+  solveModels(b, active, o.krylov_tol, o.krylov_max_iter, its, flags, relres, o.preconditioner, &fb);
   for (int k : active) {
     Ctx& c = *b.m[k];
     FEStorage& md = c.model;
@@ -210,6 +227,7 @@ static int batchNewtonIteration(Batch& b, double t, int it, const nla3d_gpu_newt
     st[k].linear_solves++;
     st[k].krylov_iterations += its[k];
     st[k].last_krylov_iterations = its[k];
+    st[k].precond_fallbacks += fb[k];
     if (flags[k] == 2) st[k].krylov_breakdown = 1;
     computeReactions(c, b.stream);
     newton_update(b.stream, (int)md.nConstrainedDofs, (int)md.nUnknownDofs, (int)md.nMpc(), c.d_dU.as<double>(),
@@ -286,6 +304,14 @@ static float timeIt(hipStream_t s, int reps, F&& fn) {
 static void autotune(Batch& b, double t, int probeIters, std::vector<double>& report) {
   hipStream_t s = b.stream;
   b.dropGraph();
+  // the probes apply the prescribed values at t to U (fix_kernel); every model's U is
+  // put back afterwards, so a Newton iteration after tuning starts from the caller's state
+  std::vector<DevBuf> savedU(b.m.size());
+  for (size_t k = 0; k < b.m.size(); k++) {
+    Ctx& c = *b.m[k];
+    savedU[k].alloc(c.d_U.bytes);
+    hipCheck(hipMemcpyAsync(savedU[k].p, c.d_U.p, c.d_U.bytes, hipMemcpyDeviceToDevice, s), "save U");
+  }
   preSolveChain(b, t);  // current matrices, right-hand sides and preconditioners
   hipCheck(hipStreamSynchronize(s), "sync");
   // element kernel: persistent grid
@@ -379,6 +405,11 @@ static void autotune(Batch& b, double t, int probeIters, std::vector<double>& re
   }
   b.cfg.big_rows = bestRows;
   (void)keepBig;
+  for (size_t k = 0; k < b.m.size(); k++) {
+    Ctx& c = *b.m[k];
+    hipCheck(hipMemcpyAsync(c.d_U.p, savedU[k].p, c.d_U.bytes, hipMemcpyDeviceToDevice, s), "restore U");
+  }
+  hipCheck(hipStreamSynchronize(s), "restore sync");
   b.dropGraph();
 }
 
