@@ -54,6 +54,9 @@ sys.path.insert(0, ROOT)
 FLOP_PER_ELEMENT = 8 * 8420
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+# finest-level smoother steps (cheb_step_kernel) per V-cycle, i.e. per MINRES iteration:
+# 15 before and 15 after the coarse correction (a 16-term Chebyshev polynomial each)
+SMOOTHER_PRODUCTS_PER_CYCLE = 30
 METRIC = "SOLID81 Newton-iter wall @3.5M DoF, learned vs block-Jacobi; elements/s FP64"
 
 
@@ -68,7 +71,7 @@ def profiled_traffic(kernel, shape):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary of this
     workload (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy of known size);
     None for another shape or when the summary is absent"""
-    path = os.path.join(ROOT, "profiles", "spmv_block2_120x120x60_hbm_traffic.txt")
+    path = os.path.join(ROOT, "profiles", "spmv_and_schur_smoother_120x120x60_hbm_traffic.txt")
     if shape != (120, 120, 60) or not os.path.exists(path):
         return None
     with open(path) as f:
@@ -295,12 +298,15 @@ def main():
 
     if args.mode == "probe":
         # short run of each hot kernel for hardware-counter passes: 2 assemblies
-        # (element kernel + scatter), 3 SpMVs, 1 plain copy of 1 GiB
+        # (element kernel + scatter), 3 SpMVs, 8 (+1 warm-up) finest-level smoother steps
+        # of the Schur multigrid, 1 plain copy of 1 GiB
         m.assemble()
         m.assemble()
         ms_spmv = m.time_spmv(3)
+        ms_sm, sm_bytes = m.time_smoother(8)
         ms_copy = m.copy_calibration(1 << 30)
-        print(json.dumps({"mode": "probe", "spmv_ms": ms_spmv, "copy_bytes": 1 << 30, "copy_ms": ms_copy,
+        print(json.dumps({"mode": "probe", "spmv_ms": ms_spmv, "smoother_step_ms": ms_sm,
+                          "smoother_step_bytes": sm_bytes, "copy_bytes": 1 << 30, "copy_ms": ms_copy,
                           "spmv_algorithmic_bytes": spmv_algorithmic_bytes(sizes), "sizes": sizes}))
         return
 
@@ -410,7 +416,23 @@ def main():
     spmv_bytes = spmv_algorithmic_bytes(sizes)
     spmv_gbs = spmv_bytes / (ms_spmv * 1e-3) / 1e9
     elem_tf = FLOP_PER_ELEMENT * nE / (ms_elem * 1e-3) / 1e12
-    if args.mode == "newton":
+    smoother = None
+    if args.mode == "newton" and opts.preconditioner >= 2:
+        ms_sm, sm_bytes = m.time_smoother(32)
+        sm_gbs = sm_bytes / (ms_sm * 1e-3) / 1e9
+        smoother = {"ms": ms_sm, "bytes": sm_bytes, "GBps": sm_gbs,
+                    "launches_per_krylov_iteration": SMOOTHER_PRODUCTS_PER_CYCLE}
+        extra.update({"smoother_step_ms": ms_sm, "smoother_step_GBps": sm_gbs,
+                      "smoother_step_bytes": sm_bytes})
+    if args.mode == "newton" and smoother is not None and \
+            smoother["ms"] * smoother["launches_per_krylov_iteration"] > ms_spmv:
+        # with the multigrid, the finest-level smoother steps take the largest share
+        roofline = {"bound": "hbm", "achieved": smoother["GBps"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": smoother["GBps"] / PEAK_HBM_GBS,
+                    "traffic": profiled_traffic("cheb_step_kernel<true>", (nx, ny, nz)),
+                    "kernel": "cheb_step_kernel<true> (finest-level Chebyshev smoother step of the Schur multigrid)",
+                    "kernel_ms": smoother["ms"], "algorithmic_bytes": smoother["bytes"]}
+    elif args.mode == "newton":
         roofline = {"bound": "hbm", "achieved": spmv_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": spmv_gbs / PEAK_HBM_GBS, "traffic": profiled_traffic("spmv_kernel<16>", (nx, ny, nz)),
                     "kernel": "spmv_kernel<16> (block(2) SpMV)",
@@ -422,7 +444,8 @@ def main():
     extra.update({"element_kernel_ms": ms_elem, "element_kernel_tflops": elem_tf,
                   "element_kernel_frac_fp64": elem_tf / PEAK_FP64_TFLOPS, "scatter_kernel_ms": ms_scatter,
                   "assembly_elements_per_s": nE / ((ms_elem + ms_scatter) * 1e-3),
-                  "spmv_ms": ms_spmv, "spmv_GBps": spmv_gbs})
+                  "spmv_ms": ms_spmv, "spmv_GBps": spmv_gbs,
+                  "spmv_traffic": profiled_traffic("spmv_kernel<16>", (nx, ny, nz))})
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -132,6 +132,7 @@ ABI.update({
     "nla3d_gpu_schur_matrix": [_P, _IP, _IP, _IP, _DP],
     "nla3d_gpu_schur_level": [_P, ctypes.c_int, _IP, _DP],
     "nla3d_gpu_schur_apply": [_P, _DP, _DP, ctypes.c_int],
+    "nla3d_gpu_time_smoother": [_P, ctypes.c_int, _DP, ctypes.POINTER(ctypes.c_longlong)],
     "nla3d_gpu_get_R": [_P, _DP],
     "nla3d_gpu_solve_block2_pc": [_P, _DP, _DP, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, _IP,
                                   _DP],
@@ -471,6 +472,13 @@ class GpuModel:
         stats = stats or NewtonStats()
         self._call("nla3d_gpu_solve_transient", ctypes.byref(opts), ctypes.byref(stats))
         return stats
+
+    def time_smoother(self, iters):
+        """(ms per finest-level smoother step, bytes it moves)"""
+        a = ctypes.c_double()
+        nb = ctypes.c_longlong()
+        self._call("nla3d_gpu_time_smoother", int(iters), ctypes.byref(a), ctypes.byref(nb))
+        return a.value, nb.value
 
     def time_spmv(self, iters):
         a = ctypes.c_double()

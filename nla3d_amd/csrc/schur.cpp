@@ -579,6 +579,52 @@ int nla3d_gpu_schur_level(nla3d_gpu_t h, int level, int* agg, double* lmax) {
   });
 }
 
+int nla3d_gpu_time_smoother(nla3d_gpu_t h, int iters, double* ms, long long* bytes) {
+  return schurGuard(h, [&](Ctx& c) {
+    if (iters < 1 || !ms) throw ApiError(-1, "bad arguments");
+    if (!c.schur || !c.schur->fresh) refreshOrThrow(c);
+    SchurAmg& S = *c.schur;
+    if (S.L.size() < 2) throw ApiError(-3, "the hierarchy has no smoothed level");
+    AmgLevel& A = *S.L[0];
+    const ChebCoef k = chebCoef(A.lmax, kSteps0, kLoDiv0);
+    const AmgSell M = A.sell();
+    hipStream_t s = c.stream;
+    double* din = A.da.as<double>();
+    double* dout = A.db.as<double>();
+    // a finite start (the power-iteration start vector), then the finest level's steps
+    hipCheck(hipMemcpyAsync(A.r.p, A.start.p, (size_t)A.n * 8, hipMemcpyDeviceToDevice, s), "start");
+    launch_cheb_start(s, A.n, A.r.as<double>(), A.dinv.as<double>(), k.c0, A.res.as<double>(), din,
+                      A.x.as<double>());
+    launch_cheb_step(s, M, din, A.dinv.as<double>(), k.c1[1], k.c2[1], A.res.as<double>(), dout,
+                     A.x.as<double>());  // warm-up
+    std::swap(din, dout);
+    hipEvent_t e0, e1;
+    hipCheck(hipEventCreate(&e0), "event");
+    hipCheck(hipEventCreate(&e1), "event");
+    hipCheck(hipEventRecord(e0, s), "rec");
+    for (int i = 0; i < iters; i++) {
+      const int j = 1 + i % (kSteps0 - 1);
+      launch_cheb_step(s, M, din, A.dinv.as<double>(), k.c1[j], k.c2[j], A.res.as<double>(), dout,
+                       A.x.as<double>());
+      std::swap(din, dout);
+    }
+    hipCheck(hipEventRecord(e1, s), "rec");
+    hipCheck(hipEventSynchronize(e1), "sync");
+    float t = 0;
+    hipCheck(hipEventElapsedTime(&t, e0, e1), "elapsed");
+    *ms = t / iters;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (bytes) {
+      // stored matrix (FP32 values, column offsets or indices, slice offsets and bases) and
+      // seven vectors of the step: din, res, D^-1, x read; res, dout, x written
+      const long long nSl = (A.n + 63) / 64;
+      *bytes = (long long)A.nSlots * (4 + (A.narrow ? 2 : 4)) + (nSl + 1) * 4 + (A.narrow ? nSl * 4 : 0) +
+               7ll * A.n * 8;
+    }
+  });
+}
+
 int nla3d_gpu_schur_apply(nla3d_gpu_t h, const double* r, double* z, int refresh) {
   return schurGuard(h, [&](Ctx& c) {
     if (!r || !z) throw ApiError(-1, "null vectors");

@@ -111,3 +111,13 @@ def test_minres_and_newton_with_schur_multigrid(gpu_lib):
     print(f"Newton Krylov iterations: block Jacobi {k0}, Schur multigrid {k2}, fallbacks {fb}", flush=True)
     assert fb == 0
     assert np.abs(u2 - u0).max() <= 1e-8 * np.abs(u0).max()
+
+
+def test_smoother_timing_entry(gpu_lib):
+    print("[gpu] smoother timing entry", flush=True)
+    m = rigid_model(8, 8, 4)  # large enough for a smoothed level below the dense coarsest one
+    ms, nbytes = m.time_smoother(4)
+    info = m.schur_info()
+    assert ms > 0.0
+    # at least the FP32 values of every entry and seven vectors
+    assert nbytes >= 4 * info["nnz0"] + 7 * 8 * info["nq"]
