@@ -120,13 +120,13 @@ def batch_mode(args):
     rows = np.array([m.sizes["nUnknown"] for m, _ in models])
     print(f"[bench] {len(models)} models, {nel.sum()} elements, {rows.sum()} equations, setup {t_setup:.1f}s",
           file=sys.stderr, flush=True)
+    pc = 2 if args.preconditioner == "schur-multigrid" else 0
     t0 = time.perf_counter()
-    cfg, rep = batch.autotune(t=0.1, probe_iterations=16)
+    cfg, rep = batch.autotune(t=0.1, probe_iterations=16, preconditioner=pc)
     t_tune = time.perf_counter() - t0
     print(f"[bench] autotuned in {t_tune:.1f}s: {cfg.as_dict()}", file=sys.stderr, flush=True)
     opts = nla3d_amd.GpuModel.newton_defaults(criterion=1, convergence=1e-8, krylov_tol=args.krylov_tol,
                                               krylov_max_iter=30000, n_load_steps=10, max_iterations=15)
-    pc = 2 if args.preconditioner == "schur-multigrid" else 0
     opts.preconditioner = pc
     delta = 0.1
     sched, cur = [], 0.0

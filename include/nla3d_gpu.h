@@ -202,10 +202,13 @@ int nla3d_gpu_batch_get_config(nla3d_gpu_batch_t batch, nla3d_gpu_batch_config* 
 int nla3d_gpu_batch_set_config(nla3d_gpu_batch_t batch, const nla3d_gpu_batch_config* cfg);
 /* Times candidate launch shapes on the batch itself at load time t (the assembly and
  * the element/gather grids; team size and lanes, and the large-system split, on
- * probe_iterations MINRES steps of every system) and keeps the fastest.  report
- * (may be NULL) receives up to report_cap measured times in ms, in the order tried.
- * Every model's solution vector U is left as it was. */
-int nla3d_gpu_batch_autotune(nla3d_gpu_batch_t batch, double t, int probe_iterations, nla3d_gpu_batch_config* chosen,
+ * probe_iterations MINRES steps of every system) and keeps the fastest.  preconditioner:
+ * the value the Newton options will carry (2 or 3: the whole-device probes include the
+ * Schur multigrid and its refresh).  report (may be NULL) receives up to report_cap
+ * measured times or costs in ms, in the order tried.  Every model's solution vector U is
+ * left as it was. */
+int nla3d_gpu_batch_autotune(nla3d_gpu_batch_t batch, double t, int probe_iterations, int preconditioner,
+                             nla3d_gpu_batch_config* chosen,
                              double* report, int report_cap);
 /* assembly + rhs + preconditioner of every model at time t (the pre-solve graph); the
  * prescribed values at t are written into each model's U (as the first Newton iteration

@@ -142,7 +142,8 @@ ABI.update({
     "nla3d_gpu_batch_destroy": [_P],
     "nla3d_gpu_batch_get_config": [_P, ctypes.POINTER(BatchConfig)],
     "nla3d_gpu_batch_set_config": [_P, ctypes.POINTER(BatchConfig)],
-    "nla3d_gpu_batch_autotune": [_P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(BatchConfig), _DP, ctypes.c_int],
+    "nla3d_gpu_batch_autotune": [_P, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(BatchConfig), _DP,
+                                 ctypes.c_int],
     "nla3d_gpu_batch_assemble": [_P, ctypes.c_double],
     "nla3d_gpu_batch_newton_iteration": [_P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(NewtonOpts),
                                          ctypes.POINTER(NewtonStats), _IP],
@@ -620,10 +621,10 @@ class Batch:
             setattr(c, k, v)
         self._call("nla3d_gpu_batch_set_config", ctypes.byref(c))
 
-    def autotune(self, t=0.1, probe_iterations=24):
+    def autotune(self, t=0.1, probe_iterations=24, preconditioner=0):
         c = BatchConfig()
         rep = np.zeros(64)
-        self._call("nla3d_gpu_batch_autotune", float(t), int(probe_iterations), ctypes.byref(c),
+        self._call("nla3d_gpu_batch_autotune", float(t), int(probe_iterations), int(preconditioner), ctypes.byref(c),
                    _ptr(rep, ctypes.c_double), len(rep))
         return c, rep
 

@@ -16,7 +16,10 @@
 // kernel's persistent grid, the gather's block size, the team size and lanes per
 // matrix group, and how many of the largest models leave the team launch.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <exception>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -112,6 +115,9 @@ static std::vector<double> readNorms(Batch& b) {
 // pc >= 2: the whole-device solves add the Schur-row multigrid (a model where it is
 // unavailable or whose solve fails with it is solved again with block Jacobi, counted in
 // fallbacks[k]); the team launch always runs block Jacobi
+// host threads driving whole-device solves at once (the team launch uses a fourth queue)
+constexpr int kBigWorkers = 3;
+
 static void solveModels(Batch& b, const std::vector<int>& ids, double tol, int maxit, std::vector<int>& its,
                         std::vector<int>& flags, std::vector<double>& relres, int pc = 0,
                         std::vector<int>* fallbacks = nullptr) {
@@ -147,28 +153,57 @@ static void solveModels(Batch& b, const std::vector<int>& ids, double tol, int m
                        b.d_team.as<TeamSolve>(), b.d_queue.as<int>(), b.d_counter.as<int>());
     hipCheck(hipGetLastError(), "team launch");
   }
-  // the largest systems: whole-device MINRES, each on its own stream, while the team launch runs
-  for (int k : big) {
+  // the largest systems: whole-device MINRES, each on its own model's stream, while the team
+  // launch runs; kBigWorkers host threads take them largest first, so up to that many solves
+  // (each too small to fill the device) overlap.  Every model is touched by one thread only.
+  auto solveBig = [&](int k) {
     Ctx& c = *b.m[k];
+    hipCheck(hipSetDevice(c.device), "hipSetDevice");
     hipCheck(hipStreamWaitEvent(c.stream, ready, 0), "wait");
     double rr = 0;
     bool brk = false;
     PrecondHook hook;
     bool mg = false;
+    int fb = 0;
     if (pc >= 2) {
       std::string why;
       mg = schurUsable(c, &why) && schurRefresh(c, &why);
       if (mg) hook = schurHook(c);
-      else if (fallbacks) (*fallbacks)[k]++;
+      else fb++;
     }
     its[k] = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), tol, maxit, &rr, &brk, mg ? &hook : nullptr);
     if (mg && (brk || rr > tol)) {  // block Jacobi from a fresh start
-      if (fallbacks) (*fallbacks)[k]++;
+      fb++;
       brk = false;
       its[k] += c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), tol, maxit, &rr, &brk);
     }
     flags[k] = brk ? 2 : 0;
     relres[k] = rr;
+    if (fallbacks) (*fallbacks)[k] += fb;
+  };
+  std::sort(big.begin(), big.end(), [&](int a, int c) { return b.n2[a] > b.n2[c]; });
+  const int nThreads = std::min<int>(kBigWorkers, (int)big.size());
+  if (nThreads <= 1) {
+    for (int k : big) solveBig(k);
+  } else {
+    std::atomic<int> next{0};
+    std::vector<std::exception_ptr> errs(nThreads);
+    std::vector<std::thread> pool;
+    for (int w = 0; w < nThreads; w++)
+      pool.emplace_back([&, w] {
+        try {
+          for (int i = next++; i < (int)big.size(); i = next++) solveBig(big[i]);
+        } catch (...) {
+          errs[w] = std::current_exception();
+          next = (int)big.size();  // the other workers stop after their current model
+        }
+      });
+    for (auto& t : pool) t.join();
+    for (auto& e : errs)
+      if (e) {
+        if (ready) (void)hipEventDestroy(ready);
+        std::rethrow_exception(e);
+      }
   }
   if (ready) (void)hipEventDestroy(ready);
   if (!team.empty()) {
@@ -301,7 +336,7 @@ static float timeIt(hipStream_t s, int reps, F&& fn) {
   return ms / reps;
 }
 
-static void autotune(Batch& b, double t, int probeIters, std::vector<double>& report) {
+static void autotune(Batch& b, double t, int probeIters, int pc, std::vector<double>& report) {
   hipStream_t s = b.stream;
   b.dropGraph();
   // the probes apply the prescribed values at t to U (fix_kernel); every model's U is
@@ -366,37 +401,53 @@ static void autotune(Batch& b, double t, int probeIters, std::vector<double>& re
     }
   b.cfg.team_threads = bt;
   b.cfg.team_lanes = bl;
-  // how many of the largest models leave the team launch for the whole-device solver
+  // how many of the largest models leave the team launch for the whole-device solver.
+  // Each side's probe time per iteration is scaled by an expected iteration count that
+  // grows with the mesh edge e = rows^(1/3): block Jacobi ~ 100 e, block Jacobi + Schur
+  // multigrid ~ 150 e^(1/3) (measured on SOLID81 boxes from 32^3 to 120x120x60).  The team
+  // launch's makespan is set by its largest member; the whole-device solves run
+  // kBigWorkers at a time, each preceded (multigrid) by its refresh.
   std::vector<int> order(all);
   std::sort(order.begin(), order.end(), [&](int a, int c) { return b.n2[a] > b.n2[c]; });
-  // Krylov iteration counts grow like the mesh edge (~ rows^(1/3)); the probe ran a fixed
-  // number of iterations, so each side's probe time is weighted by the edge of its
-  // largest system (team launch: the makespan is set by its largest member) or of each
-  // system (whole-device solves run one after another)
   auto edge = [&](int k) { return std::cbrt((double)b.n2[k]); };
-  std::vector<double> gpuMs;
-  const int maxBig = std::min<int>(64, (int)order.size() - 1);
+  auto itersBJ = [&](int k) { return 100.0 * edge(k); };
+  auto itersMG = [&](int k) { return 150.0 * std::cbrt(edge(k)); };
+  std::vector<double> devMs;  // expected whole-device solve time of order[i]
+  const int maxBig = std::min<int>(128, (int)order.size() - 1);
   for (int i = 0; i < maxBig; i++) {
     Ctx& c = *b.m[order[i]];
-    const double t0 = nowMsB();
-    double r = 0;
+    double r = 0, tRefresh = 0;
     bool brk = false;
-    c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), 0.0, probeIters, &r, &brk);
-    gpuMs.push_back(nowMsB() - t0);
+    PrecondHook hook;
+    bool mg = false;
+    if (pc >= 2) {
+      std::string why;
+      const double t0 = nowMsB();
+      mg = schurUsable(c, &why) && schurRefresh(c, &why);
+      tRefresh = nowMsB() - t0;
+      if (mg) hook = schurHook(c);
+    }
+    const double t0 = nowMsB();
+    c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), 0.0, probeIters, &r, &brk, mg ? &hook : nullptr);
+    const double perIt = (nowMsB() - t0) / probeIters;
+    devMs.push_back(tRefresh + perIt * (mg ? itersMG(order[i]) : itersBJ(order[i])));
   }
   double bestCost = 1e300;
   long long bestRows = (1ll << 62);
-  for (int k : {0, 1, 2, 4, 8, 16, 32, 48, 64}) {
+  for (int k : {0, 1, 2, 4, 8, 16, 24, 32, 48, 64, 96, 128}) {
     if (k > maxBig) break;
     if (k < maxBig && b.n2[order[k]] > kTeamMaxRows) continue;  // too large for one workgroup
     std::vector<int> rest(order.begin() + k, order.end());
     b.cfg.big_rows = kTeamMaxRows;
     const double t0 = nowMsB();
     solveModels(b, rest, 0.0, probeIters, its, flags, rr);
-    double cost = (nowMsB() - t0) * edge(order[k]);
-    double bigCost = 0.0;
-    for (int i = 0; i < k; i++) bigCost += gpuMs[i] * edge(order[i]);
-    cost = std::max(cost, bigCost);  // the two run concurrently
+    double cost = (nowMsB() - t0) / probeIters * itersBJ(order[k]);
+    double bigSum = 0.0, bigMax = 0.0;
+    for (int i = 0; i < k; i++) {
+      bigSum += devMs[i];
+      bigMax = std::max(bigMax, devMs[i]);
+    }
+    cost = std::max(cost, std::max(bigMax, bigSum / kBigWorkers));  // the two run concurrently
     report.push_back(cost);
     if (cost < bestCost) {
       bestCost = cost;
@@ -543,12 +594,12 @@ int nla3d_gpu_batch_set_config(nla3d_gpu_batch_t h, const nla3d_gpu_batch_config
   });
 }
 
-int nla3d_gpu_batch_autotune(nla3d_gpu_batch_t h, double t, int probe_iterations, nla3d_gpu_batch_config* chosen,
-                             double* report, int report_cap) {
+int nla3d_gpu_batch_autotune(nla3d_gpu_batch_t h, double t, int probe_iterations, int preconditioner,
+                             nla3d_gpu_batch_config* chosen, double* report, int report_cap) {
   return batchGuard(h, [&](Batch& b) {
     if (probe_iterations < 1) throw ApiError(-1, "probe_iterations must be positive");
     std::vector<double> rep;
-    autotune(b, t, probe_iterations, rep);
+    autotune(b, t, probe_iterations, preconditioner, rep);
     if (chosen) *chosen = b.cfg;
     if (report)
       for (int i = 0; i < (int)rep.size() && i < report_cap; i++) report[i] = rep[i];

@@ -98,7 +98,7 @@ def test_autotune_leaves_models_unchanged(gpu_lib):
     A, B = pair(5, 8, (3, 9))
     before = [m.get_solution().copy() for m, _ in A]
     batch = nla3d_amd.Batch([m for m, _ in A])
-    batch.autotune(t=0.5, probe_iterations=8)
+    batch.autotune(t=0.5, probe_iterations=8, preconditioner=2)  # multigrid probes on the whole-device side
     for (m, _), u0 in zip(A, before):
         assert np.array_equal(m.get_solution(), u0)
     opts = nla3d_amd.GpuModel.newton_defaults(criterion=1, convergence=1e-9, krylov_tol=1e-12, krylov_max_iter=20000,
