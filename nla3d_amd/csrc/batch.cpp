@@ -37,9 +37,13 @@ namespace nla3d_amd {
 // a system larger than this never goes to the team launch (one workgroup per system)
 constexpr long long kTeamMaxRows = 65536;
 
+// host threads driving whole-device solves at once (the team launch uses a fourth queue)
+constexpr int kBigWorkers = 3;
+
 struct Batch {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t workers[kBigWorkers] = {};  // the whole-device solves run on these
   std::string err;
   std::vector<Ctx*> m;
   std::vector<int> n2;
@@ -54,6 +58,8 @@ struct Batch {
   ~Batch() {
     if (pre) (void)hipGraphExecDestroy(pre);
     if (stream) (void)hipStreamDestroy(stream);
+    for (hipStream_t w : workers)
+      if (w) (void)hipStreamDestroy(w);
   }
   void dropGraph() {
     if (pre) (void)hipGraphExecDestroy(pre);
@@ -115,9 +121,6 @@ static std::vector<double> readNorms(Batch& b) {
 // pc >= 2: the whole-device solves add the Schur-row multigrid (a model where it is
 // unavailable or whose solve fails with it is solved again with block Jacobi, counted in
 // fallbacks[k]); the team launch always runs block Jacobi
-// host threads driving whole-device solves at once (the team launch uses a fourth queue)
-constexpr int kBigWorkers = 3;
-
 static void solveModels(Batch& b, const std::vector<int>& ids, double tol, int maxit, std::vector<int>& its,
                         std::vector<int>& flags, std::vector<double>& relres, int pc = 0,
                         std::vector<int>* fallbacks = nullptr) {
@@ -156,9 +159,17 @@ static void solveModels(Batch& b, const std::vector<int>& ids, double tol, int m
   // the largest systems: whole-device MINRES, each on its own model's stream, while the team
   // launch runs; kBigWorkers host threads take them largest first, so up to that many solves
   // (each too small to fill the device) overlap.  Every model is touched by one thread only.
-  auto solveBig = [&](int k) {
+  // worker w drives its models on b.workers[w] (the model's own stream is put back after
+  // its solve, which ends synchronised), so concurrent solves use distinct streams
+  auto solveBig = [&](int k, int w) {
     Ctx& c = *b.m[k];
     hipCheck(hipSetDevice(c.device), "hipSetDevice");
+    struct StreamSwap {
+      Ctx& c;
+      hipStream_t own;
+      ~StreamSwap() { c.stream = own; }
+    } swap{c, c.stream};
+    c.stream = b.workers[w];
     hipCheck(hipStreamWaitEvent(c.stream, ready, 0), "wait");
     double rr = 0;
     bool brk = false;
@@ -184,7 +195,7 @@ static void solveModels(Batch& b, const std::vector<int>& ids, double tol, int m
   std::sort(big.begin(), big.end(), [&](int a, int c) { return b.n2[a] > b.n2[c]; });
   const int nThreads = std::min<int>(kBigWorkers, (int)big.size());
   if (nThreads <= 1) {
-    for (int k : big) solveBig(k);
+    for (int k : big) solveBig(k, 0);
   } else {
     std::atomic<int> next{0};
     std::vector<std::exception_ptr> errs(nThreads);
@@ -192,7 +203,7 @@ static void solveModels(Batch& b, const std::vector<int>& ids, double tol, int m
     for (int w = 0; w < nThreads; w++)
       pool.emplace_back([&, w] {
         try {
-          for (int i = next++; i < (int)big.size(); i = next++) solveBig(big[i]);
+          for (int i = next++; i < (int)big.size(); i = next++) solveBig(big[i], w);
         } catch (...) {
           errs[w] = std::current_exception();
           next = (int)big.size();  // the other workers stop after their current model
@@ -519,6 +530,7 @@ int nla3d_gpu_batch_create(nla3d_gpu_batch_t* out, const nla3d_gpu_t* models, in
     }
     hipCheck(hipSetDevice(b->device), "hipSetDevice");
     hipCheck(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
+    for (hipStream_t& w : b->workers) hipCheck(hipStreamCreateWithFlags(&w, hipStreamNonBlocking), "stream");
     std::vector<Solid81Batch> s81(n);
     std::vector<GatherBatch> ga(n);
     std::vector<int> bb(n), tb(n);
