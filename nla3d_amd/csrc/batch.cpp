@@ -37,8 +37,10 @@ namespace nla3d_amd {
 // a system larger than this never goes to the team launch (one workgroup per system)
 constexpr long long kTeamMaxRows = 65536;
 
-// host threads driving whole-device solves at once (the team launch uses a fourth queue)
-constexpr int kBigWorkers = 3;
+// host threads driving whole-device solves at once, each on its own stream (6 measured
+// best at the default 4 hardware queues per process: 3 gave 5.5 s, 6 and 10 gave 4.2 s on
+// the 256-model batch)
+constexpr int kBigWorkers = 6;
 
 struct Batch {
   int device = 0;
