@@ -82,6 +82,66 @@ def profiled_traffic(kernel, shape):
     return None
 
 
+def c3_like_block2(mm):
+    """K.block(2) of the C3 box at mesh 2mm x 2mm x mm (same material, rigid-body top) at the
+    first tangent (U = 0), assembled by the oracle's reference loop: the element blocks,
+    and the multiplier rows with their slave coefficient 1 (the master's DoFs are
+    prescribed, so its coefficients sit in block(1,2)).  Returns a CSC matrix."""
+    import scipy.sparse as sp
+    import nla3d_amd
+    from nla3d_amd import meshgen
+    from oracle import pyoracle
+    m = nla3d_amd.GpuModel(-1)
+    xyz, conn = meshgen.solid81_rigid_top_case(m, 2 * mm, 2 * mm, mm, 2.0, 2.0, 1.0, 0.15, 0.349,
+                                               material=(3, (0.5, 0.1, 1000.0)))
+    m.set_max_in_row(0)
+    m.init_solution()
+    ne, ee = m.equation_numbers()
+    nE, nC, nU = len(conn), m.sizes["nConstrained"], m.sizes["nUnknown"]
+    eq = np.zeros((nE, 25), dtype=np.uint32)
+    eq[:, :24] = ne[conn - 1][:, :, :3].reshape(nE, 24)
+    eq[:, 24] = ee[:, 6]
+    U = np.zeros(m.nAll)
+    Kb, _ = pyoracle.solid81_assemble_ref(xyz[conn - 1].reshape(nE, 24), U[eq[:, :24].astype(np.int64) - 1],
+                                          U[eq[:, 24].astype(np.int64) - 1], 3, [0.5, 0.1, 1000.0], eq, nC,
+                                          m.pattern(1), m.pattern(12), m.pattern(2), m.nAll)
+    io, cols = m.pattern(2)
+    n2 = len(io) - 1
+    rows = np.repeat(np.arange(n2), np.diff(io.astype(np.int64)))
+    A = sp.csr_matrix((Kb[2], (rows, cols.astype(np.int64) - 1)), shape=(n2, n2))
+    A = A + sp.triu(A, 1).T
+    top = np.nonzero(xyz[:-1, 2] == 1.0)[0] + 1
+    r = [int(ne[node - 1, d]) - nC - 1 for node in top for d in range(3)]
+    B = sp.csr_matrix((np.ones(len(r)), (r, nU + np.arange(len(r)))), shape=(n2, n2))
+    return (A + B + B.T).tocsc()
+
+
+def cpu_solve_fit(n2_target, budget_s):
+    """The direct solve of K.block(2) that a reference Newton iteration performs (PARDISO
+    there, absent here): SuperLU (scipy, one thread, COLAMD ordering) factor + solve on
+    C3-like boxes of growing size until the budget is spent, t = a n2^b fitted on the three
+    largest and extrapolated to n2_target (labelled as extrapolated)."""
+    import scipy.sparse.linalg as sla
+    sizes, secs = [], []
+    spent = 0.0
+    for mm in (3, 4, 5, 6, 7, 8, 9, 10, 12):
+        K2 = c3_like_block2(mm)
+        rhs = np.random.default_rng(mm).standard_normal(K2.shape[0])
+        t0 = time.perf_counter()
+        sla.splu(K2).solve(rhs)
+        dt = time.perf_counter() - t0
+        sizes.append(K2.shape[0])
+        secs.append(dt)
+        spent += dt
+        print(f"[bench] cpu solve sample 2x{mm}: n2 = {K2.shape[0]}, {dt:.2f} s", file=sys.stderr, flush=True)
+        if spent + 6.0 * dt > budget_s:  # the next size costs several times this one
+            break
+    ln_n, ln_t = np.log(sizes[-3:]), np.log(secs[-3:])
+    b, ln_a = np.polyfit(ln_n, ln_t, 1)
+    return {"sizes": sizes, "seconds": secs, "exponent": float(b),
+            "extrapolated_seconds": float(np.exp(ln_a) * n2_target ** b), "target_n2": int(n2_target)}
+
+
 def cpu_baseline(m, xyz, conn, U, n_sample):
     from oracle import pyoracle
     ne, ee = m.equation_numbers()
@@ -97,10 +157,18 @@ def cpu_baseline(m, xyz, conn, U, n_sample):
     t0 = time.perf_counter()
     pyoracle.solid81_assemble_ref(Xe, Ue, pe, 3, [0.5, 0.1, 1000.0], eq, m.sizes["nConstrained"], *pats, m.nAll)
     dt = time.perf_counter() - t0
-    return {"value": len(sel) / dt, "unit": "elements/s", "cores": 1, "kind": "port",
-            "sample": f"reference assembly loop (update + buildK + binary-search addValue) over the first "
-                      f"{len(sel)} elements of the same model: {dt:.1f} s on 1 thread; the PARDISO solve of a "
-                      f"reference Newton iteration is not included"}
+    asm_rate = len(sel) / dt
+    n2 = m.sizes["nUnknown"] + m.sizes["nMpc"]
+    fit = cpu_solve_fit(n2, 15.0)
+    t_iter = len(conn) / asm_rate + fit["extrapolated_seconds"]
+    return {"value": len(conn) / t_iter, "unit": "elements/s", "cores": 1, "kind": "port",
+            "assembly_elements_per_s": asm_rate, "solve": fit,
+            "sample": f"one Newton iteration on 1 thread = the reference assembly loop (update + buildK + "
+                      f"binary-search addValue) timed over the first {len(sel)} elements of this model "
+                      f"({dt:.1f} s, {asm_rate:.0f} elements/s, scaled to {len(conn)}) + the direct solve of "
+                      f"K.block(2) (SuperLU, one thread) timed on C3-like boxes up to n2 = {fit['sizes'][-1]} "
+                      f"and extrapolated as n2^{fit['exponent']:.2f} to n2 = {n2}: "
+                      f"{fit['extrapolated_seconds']:.3g} s (extrapolated)"}
 
 
 def batch_mode(args):
