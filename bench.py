@@ -296,8 +296,9 @@ def thermal_mode(args):
     ms_step = wall * 1e3 / args.steps
     n2 = sizes["nUnknown"]
     nnz = sizes["nnz2"]
-    # one Jacobi-CG iteration over the upper CSR of block(2): values + columns, row pointers, and
-    # the vector passes p, q (SpMV); x, r read/write, p, q, z (update); z, p read, p write (direction)
+    # one Jacobi-CG iteration over the upper CSR of block(2) (the symmetric matrix stored once):
+    # values + columns, row pointers, and the vector passes p, q (SpMV); x, r read/write, p, q,
+    # z (update); z, p read, p write (direction) -- the algorithmic figure, whatever the layout
     it_bytes = 12 * nnz + 4 * (n2 + 1) + 16 * n2 + 64 * n2 + 24 * n2
     ms_it = (st.ms_solve - s0) / max(its, 1)
     gbs = it_bytes / (ms_it * 1e-3) / 1e9
@@ -310,7 +311,9 @@ def thermal_mode(args):
                                f"nnz(block2 upper) {nnz}, dt 1e-3", "mode": "thermal",
                    "cg_iterations_per_step": its / args.steps, "ms_per_cg_iteration": ms_it},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
-                     "traffic": None, "kernel": "one Jacobi-CG iteration (5 launches)", "algorithmic_bytes": it_bytes},
+                     "traffic": None, "kernel": "one Jacobi-CG iteration (4 launches: sliced-ELLPACK SpMV that also forms "
+                                                          "the search direction, a one-block scalar step, the Jacobi update, "
+                                                          "a one-block scalar step)", "algorithmic_bytes": it_bytes},
         "cpu_baseline": None,
     }
     print(json.dumps(out))

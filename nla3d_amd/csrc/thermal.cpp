@@ -2,7 +2,9 @@
 // LinearFESolver::solve (FESolver.cpp:337-397) and
 // LinearTransientFESolver::solve (FESolver.cpp:532-642) with CG in place of
 // the direct solve.
+#include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <functional>
 #include <mutex>
@@ -33,8 +35,12 @@ struct ThermalData {
   int curStep = 0;
   bool running = false;
   DevBuf DU, DDU, v, y, x;
-  DevBuf cgw[4], cgPart2, cgSt, cgSt0;
+  DevBuf cgw[5], cgPart2, cgSt, cgSt0;
   CgState* hst = nullptr;
+  // scalar-row CG copy (built on the first solve when every row is its own group)
+  bool sellTried = false, sellOk = false;
+  DevBuf sOff, sBase, sCol16, sCol, sPos, sVal, sDinv;
+  CgSell sell;
   ~ThermalData() {
     if (hst) (void)hipHostFree(hst);
   }
@@ -150,6 +156,81 @@ void assembleThermal(Ctx& c) {
   T.assembled = true;
 }
 
+// the sliced-ELLPACK copy of a matrix whose rows are all single-row groups (QUADTH: one
+// TEMP DoF per node), from the grouped full-symmetric structure
+static void buildSell(Ctx& c, ThermalData& T) {
+  T.sellTried = true;
+  const int n = c.n2, ng = c.nSpmvGroups;
+  if (n == 0 || ng != n || c.nGroups != n) return;
+  auto down = [](const DevBuf& d, size_t count) {
+    std::vector<int> v(count);
+    hipCheck(hipMemcpy(v.data(), d.p, count * 4, hipMemcpyDeviceToHost), "download");
+    return v;
+  };
+  const std::vector<int> size = down(c.d_sgSize, ng), row0 = down(c.d_sgRow0, ng), cptr = down(c.d_sgColPtr, ng + 1),
+                         vptr = down(c.d_sgValPtr, ng), info = down(c.d_gInfo, n);
+  for (int g = 0; g < ng; g++)
+    if (size[g] != 1 || row0[g] != g || (info[g] & 3) != 1) return;
+  const std::vector<int> cols = down(c.d_sgCols, (size_t)cptr[ng]);
+  const int nSl = (n + 63) / 64;
+  std::vector<int> off(nSl + 1, 0), base(nSl, 0);
+  bool narrow = true;
+  for (int sl = 0; sl < nSl; sl++) {
+    int w = 0, lo = INT_MAX, hi = -1;
+    for (int i = sl * 64; i < std::min(n, sl * 64 + 64); i++) {
+      w = std::max(w, cptr[i + 1] - cptr[i]);
+      for (int k = cptr[i]; k < cptr[i + 1]; k++) {
+        lo = std::min(lo, cols[k]);
+        hi = std::max(hi, cols[k]);
+      }
+    }
+    if (hi < 0) lo = hi = sl * 64;
+    base[sl] = lo;
+    if (hi - lo > 65535) narrow = false;
+    if ((int64_t)off[sl] + 64ll * w >= (1ll << 31)) return;
+    off[sl + 1] = off[sl] + 64 * w;
+  }
+  const int nSlots = off[nSl];
+  std::vector<int> scol(nSlots), spos(nSlots);
+  for (int sl = 0; sl < nSl; sl++) {
+    const int w = (off[sl + 1] - off[sl]) / 64;
+    for (int l = 0; l < 64; l++) {
+      const int i = sl * 64 + l;
+      for (int k = 0; k < w; k++) {
+        const int j = off[sl] + k * 64 + l;
+        const bool real = i < n && k < cptr[i + 1] - cptr[i];
+        scol[j] = real ? cols[cptr[i] + k] : base[sl];
+        spos[j] = real ? vptr[i] + k : -1;
+      }
+    }
+  }
+  T.sOff.upload(off);
+  T.sPos.upload(spos);
+  if (narrow) {
+    std::vector<unsigned short> c16(nSlots);
+    for (int sl = 0; sl < nSl; sl++)
+      for (int j = off[sl]; j < off[sl + 1]; j++) c16[j] = (unsigned short)(scol[j] - base[sl]);
+    T.sCol16.upload(c16);
+    T.sBase.upload(base);
+  } else {
+    T.sCol.upload(scol);
+  }
+  T.sVal.alloc((size_t)std::max(nSlots, 1) * 8);
+  T.sDinv.alloc((size_t)n * 8);
+  CgSell& A = T.sell;
+  A.n = n;
+  A.nSlices = nSl;
+  A.nSlots = nSlots;
+  A.off = T.sOff.as<int>();
+  A.base = narrow ? T.sBase.as<int>() : nullptr;
+  A.col16 = narrow ? T.sCol16.as<unsigned short>() : nullptr;
+  A.col = narrow ? nullptr : T.sCol.as<int>();
+  A.pos = T.sPos.as<int>();
+  A.val = T.sVal.as<double>();
+  A.dinv = T.sDinv.as<double>();
+  T.sellOk = true;
+}
+
 static CgWork cgWork(Ctx& c, ThermalData& T) {
   const size_t n = (size_t)c.n2 * 8;
   for (auto& b : T.cgw) b.alloc(n);
@@ -165,6 +246,7 @@ static CgWork cgWork(Ctx& c, ThermalData& T) {
   w.z = T.cgw[1].as<double>();
   w.p = T.cgw[2].as<double>();
   w.q = T.cgw[3].as<double>();
+  w.p2 = T.cgw[4].as<double>();
   w.partial = c.d_partial.as<double>();
   w.partial2 = T.cgPart2.as<double>();
   w.st = T.cgSt.as<CgState>();
@@ -175,8 +257,10 @@ static CgWork cgWork(Ctx& c, ThermalData& T) {
 static int cgSolve(Ctx& c, ThermalData& T, const double* b, double* x, double tol, int maxit,
                    nla3d_gpu_newton_stats& st) {
   CgWork w = cgWork(c, T);
+  if (!T.sellTried) buildSell(c, T);
   const double avgCols = c.n2 > 0 ? (double)c.fullNnz2 / c.n2 : 0.0;
-  const int it = cg_solve(c.stream, c.krylovMatrix(), c.precond(), w, b, x, tol, maxit, T.hst, avgCols);
+  const int it = cg_solve(c.stream, c.krylovMatrix(), c.precond(), w, b, x, tol, maxit, T.hst, avgCols,
+                          T.sellOk ? &T.sell : nullptr);
   if (it < 0) throw ApiError(-2, "HIP error in CG");
   st.linear_solves++;
   st.krylov_iterations += it;

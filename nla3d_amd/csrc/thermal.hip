@@ -12,6 +12,8 @@
 // TEMP DoFs the blocks are 1x1, i.e. Jacobi).  As for MINRES, the scalar
 // recurrences run on the device and inner products are reduced in a fixed
 // order.
+#include <algorithm>
+
 #include "internal.h"
 #include "krylov.h"
 #include "thermal.h"
@@ -340,6 +342,84 @@ __global__ __launch_bounds__(256) void tr_update_kernel(int nAll, int nC, const 
   DDU[i] = ddun;
 }
 
+// ---- scalar-row CG on the sliced-ELLPACK copy --------------------------------
+__global__ __launch_bounds__(256) void cg_sell_fill_kernel(int nSlots, const int* __restrict__ pos,
+                                                         const double* __restrict__ full, double* __restrict__ val,
+                                                         int nGroups, const int* __restrict__ gStart,
+                                                         const double* __restrict__ binv, double* __restrict__ dinv) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nSlots) {
+    const int p = pos[j];
+    val[j] = p >= 0 ? full[p] : 0.0;
+  }
+  if (j < nGroups) dinv[gStart[j]] = binv[(size_t)j * 9];
+}
+
+// pnew = z + beta pold (rows of this slice), q = A pnew (the gathered entries of pnew are
+// formed on the fly from z and pold), partial pnew . q; one thread per row, one wave per
+// 64-row slice, grid-stride over slices on the fixed grid
+template <bool kNarrow>
+__global__ __launch_bounds__(256) void cg_sell_spmv_kernel(CgSell A, const double* __restrict__ z,
+                                                          const double* __restrict__ pold, double* __restrict__ pnew,
+                                                          double* __restrict__ q, const CgState* __restrict__ st,
+                                                          double* __restrict__ partial) {
+  __shared__ double sh[4];
+  if (st->converged) return;
+  const double beta = st->beta;
+  const int lane = threadIdx.x & 63;
+  double contrib = 0.0;
+  for (int sl = blockIdx.x * 4 + (threadIdx.x >> 6); sl < A.nSlices; sl += gridDim.x * 4) {
+    const int i = sl * 64 + lane;
+    if (i >= A.n) continue;
+    const int b = A.off[sl], w = (A.off[sl + 1] - b) >> 6;
+    const double* __restrict__ v = A.val + b + lane;
+    double acc = 0.0;
+    if constexpr (kNarrow) {
+      const unsigned short* __restrict__ c = A.col16 + b + lane;
+      const int cb = A.base[sl];
+      for (int k = 0; k < w; k++) {
+        const int cc = cb + c[k * 64];
+        acc += v[k * 64] * (z[cc] + beta * pold[cc]);
+      }
+    } else {
+      const int* __restrict__ c = A.col + b + lane;
+      for (int k = 0; k < w; k++) {
+        const int cc = c[k * 64];
+        acc += v[k * 64] * (z[cc] + beta * pold[cc]);
+      }
+    }
+    const double pi = z[i] + beta * pold[i];
+    pnew[i] = pi;
+    q[i] = acc;
+    contrib += pi * acc;
+  }
+  const double s = bsum(contrib, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// x += alpha p ; r -= alpha q ; z = D^-1 r ; partial r . z
+__global__ __launch_bounds__(256) void cg_jacobi_update_kernel(int n, const double* __restrict__ dinv,
+                                                              double* __restrict__ x, double* __restrict__ r,
+                                                              const double* __restrict__ p,
+                                                              const double* __restrict__ q, double* __restrict__ z,
+                                                              const CgState* __restrict__ st,
+                                                              double* __restrict__ partial) {
+  __shared__ double sh[4];
+  if (st->converged) return;
+  const double a = st->alpha;
+  double contrib = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    x[i] += a * p[i];
+    const double t = r[i] - a * q[i];
+    r[i] = t;
+    const double zz = dinv[i] * t;
+    z[i] = zz;
+    contrib += t * zz;
+  }
+  const double s = bsum(contrib, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 __global__ __launch_bounds__(256) void fill_kernel(int n, double v, double* __restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = v;
@@ -393,8 +473,46 @@ static int cg_solve_t(hipStream_t s, const KrylovMatrix& M, const Precond& P, Cg
   return hstate->itn;
 }
 
+// scalar rows: two kernels per iteration besides the two one-block scalar steps (the
+// search direction is formed inside the SpMV, the Jacobi pass reads one inverse per row)
+static int cg_solve_sell(hipStream_t s, const KrylovMatrix& M, const Precond& P, CgWork& w, const double* b, double* x,
+                         double tol, int maxit, CgState* hstate, const CgSell& A) {
+  const int nb = kKrylovBlocks;
+  hipLaunchKernelGGL(cg_sell_fill_kernel, dim3(cdiv(std::max(A.nSlots, P.nGroups), 256)), dim3(256), 0, s, A.nSlots,
+                     A.pos, M.val, A.val, P.nGroups, P.gStart, P.binv, A.dinv);
+  // r = b - A x0, z = M r, p = z (the grouped SpMV with the always-active state)
+  hipLaunchKernelGGL(cg_spmv_kernel<4>, dim3(nb), dim3(256), 0, s, M, x, w.q, w.st0, w.partial);
+  hipLaunchKernelGGL(cg_start_kernel, dim3(nb), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo, P.binv, b, w.q, w.r,
+                     w.z, w.p, w.partial, w.partial2);
+  hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(256), 0, s, w.st, w.partial, w.partial2, nb, tol);
+  double* pa = w.p;   // direction of the previous iteration (beta = 0 at the first)
+  double* pb = w.p2;  // this iteration's direction
+  int it = 0;
+  const int check = 16;
+  for (;;) {
+    for (int k = 0; k < check && it < maxit; k++, it++) {
+      if (A.col16)
+        hipLaunchKernelGGL(cg_sell_spmv_kernel<true>, dim3(nb), dim3(256), 0, s, A, w.z, pa, pb, w.q, w.st,
+                           w.partial);
+      else
+        hipLaunchKernelGGL(cg_sell_spmv_kernel<false>, dim3(nb), dim3(256), 0, s, A, w.z, pa, pb, w.q, w.st,
+                           w.partial);
+      hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(256), 0, s, w.st, w.partial, nb);
+      hipLaunchKernelGGL(cg_jacobi_update_kernel, dim3(nb), dim3(256), 0, s, A.n, A.dinv, x, w.r, pb, w.q, w.z, w.st,
+                         w.partial);
+      hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(256), 0, s, w.st, w.partial, nb, tol);
+      std::swap(pa, pb);
+    }
+    if (hipMemcpyAsync(hstate, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    if (hstate->converged || it >= maxit) break;
+  }
+  return hstate->itn;
+}
+
 int cg_solve(hipStream_t s, const KrylovMatrix& M, const Precond& P, CgWork& w, const double* b, double* x, double tol,
-             int maxit, CgState* hstate, double avgCols) {
+             int maxit, CgState* hstate, double avgCols, const CgSell* sell) {
+  if (sell) return cg_solve_sell(s, M, P, w, b, x, tol, maxit, hstate, *sell);
   // lanes per row group sized to the row length (scalar thermal rows hold about 9 entries)
   if (avgCols <= 12.0) return cg_solve_t<4>(s, M, P, w, b, x, tol, maxit, hstate);
   if (avgCols <= 40.0) return cg_solve_t<8>(s, M, P, w, b, x, tol, maxit, hstate);
