@@ -62,6 +62,12 @@ int nla3d_gpu_set_material(nla3d_gpu_t h, int type, const double* MC, int n);
 /* FESolver::addFix / addLoad (FESolver.cpp:250-259) */
 int nla3d_gpu_add_fix(nla3d_gpu_t h, uint32_t node, int dof, double value);
 int nla3d_gpu_add_load(nla3d_gpu_t h, uint32_t node, int dof, double value);
+/* FEStorage::addMpc (FEStorage.cpp:175-179) with an Mpc (Mpc.h:30-36): the linear
+ * constraint sum_i coefs[i] * U(nodes[i], dofs[i]) = b, e.g. an ANSYS CE record.  Its
+ * multiplier equation is numbered after the DoFs, before the rigid bodies' rows, in the
+ * order of the calls; every term's DoF must belong to an element. */
+int nla3d_gpu_add_mpc(nla3d_gpu_t h, int n_terms, const uint32_t* nodes, const int* dofs, const double* coefs,
+                      double b);
 /* RigidBodyMpc (Mpc.h:49-60): masterNode, slaveNodes, dofs */
 int nla3d_gpu_add_rigid_body_mpc(nla3d_gpu_t h, uint32_t master, uint32_t n_slaves, const uint32_t* slaves,
                                  int n_dofs, const int* dofs);

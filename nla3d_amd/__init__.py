@@ -81,6 +81,7 @@ ABI = {
     "nla3d_gpu_set_material": [_P, ctypes.c_int, _DP, ctypes.c_int],
     "nla3d_gpu_add_fix": [_P, ctypes.c_uint32, ctypes.c_int, ctypes.c_double],
     "nla3d_gpu_add_load": [_P, ctypes.c_uint32, ctypes.c_int, ctypes.c_double],
+    "nla3d_gpu_add_mpc": [_P, ctypes.c_int, _U32P, _IP, _DP, ctypes.c_double],
     "nla3d_gpu_add_rigid_body_mpc": [_P, ctypes.c_uint32, ctypes.c_uint32, _U32P, ctypes.c_int, _IP],
     "nla3d_gpu_set_max_in_row": [_P, ctypes.c_uint32],
     "nla3d_gpu_init_solution": [_P],
@@ -248,6 +249,14 @@ class GpuModel:
 
     def add_load(self, node, dof, value):
         self._call("nla3d_gpu_add_load", int(node), int(dof), float(value))
+
+    def add_mpc(self, terms, b=0.0):
+        """FEStorage::addMpc: sum coef * U(node, dof) = b over terms [(node, dof, coef), ...]"""
+        n = np.ascontiguousarray([t[0] for t in terms], dtype=np.uint32)
+        d = np.ascontiguousarray([t[1] for t in terms], dtype=np.int32)
+        c = np.ascontiguousarray([t[2] for t in terms], dtype=np.float64)
+        self._call("nla3d_gpu_add_mpc", len(n), _ptr(n, ctypes.c_uint32), _ptr(d, ctypes.c_int),
+                   _ptr(c, ctypes.c_double), float(b))
 
     def add_rigid_body_mpc(self, master, slaves, dofs=(UX, UY, UZ)):
         s = np.ascontiguousarray(slaves, dtype=np.uint32)

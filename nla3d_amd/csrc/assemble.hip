@@ -93,6 +93,23 @@ void launch_segmented_gather(hipStream_t stream, int nTargets, const int* seg, c
                      src_vals, out, mirBegin, mirEnd, mir, full);
 }
 
+__global__ __launch_bounds__(256) void ce_mpc_kernel(int nRows, const int* __restrict__ termPtr,
+                                                    const int* __restrict__ tgt, const double* __restrict__ coef,
+                                                    const int* __restrict__ fTgt, const double* __restrict__ b,
+                                                    double* __restrict__ vals) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nRows) return;
+  for (int k = termPtr[r]; k < termPtr[r + 1]; k++) vals[tgt[k]] += coef[k];  // addValueMPC accumulates
+  vals[fTgt[r]] = b[r];
+}
+
+void launch_ce_mpc(hipStream_t stream, int nRows, const int* termPtr, const int* tgt, const double* coef,
+                   const int* fTgt, const double* b, double* vals) {
+  if (nRows <= 0) return;
+  hipLaunchKernelGGL(ce_mpc_kernel, dim3((nRows + 255) / 256), dim3(256), 0, stream, nRows, termPtr, tgt, coef, fTgt,
+                     b, vals);
+}
+
 void launch_mirror(hipStream_t stream, int n, const double* upper2, const int* idx, const int* mir, double* full) {
   if (n <= 0) return;
   hipLaunchKernelGGL(mirror_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, upper2, idx, mir, full);

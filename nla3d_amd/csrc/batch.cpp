@@ -239,6 +239,19 @@ static int batchNewtonIteration(Batch& b, double t, int it, const nla3d_gpu_newt
                                 nla3d_gpu_newton_stats* st, int* conv) {
   const size_t n = b.m.size();
   double t0 = nowMsB();
+  // criterion 1 may end a model's load step without a solve here: keep its last reactions
+  const bool mayStop = o.criterion == 1 && it > 1;
+  auto saveR = [&](size_t k, bool restore) {
+    Ctx& c = *b.m[k];
+    const size_t bytes = (size_t)c.model.nConstrainedDofs * 8;
+    if (bytes)
+      hipCheck(hipMemcpyAsync(restore ? c.d_R.p : c.d_Rsave.p, restore ? c.d_Rsave.p : c.d_R.p, bytes,
+                              hipMemcpyDeviceToDevice, b.stream),
+               "reactions");
+  };
+  if (mayStop)
+    for (size_t k = 0; k < n; k++)
+      if (!b.done[k]) saveR(k, false);
   runPreSolve(b, t);
   std::vector<double> nrm = readNorms(b);
   std::vector<int> active;
@@ -254,6 +267,7 @@ static int batchNewtonIteration(Batch& b, double t, int it, const nla3d_gpu_newt
     if (o.criterion == 1) {
       st[k].last_criterion = b.rhs1[k] > 0 ? rn / b.rhs1[k] : 0.0;
       if ((it > 1 && rn <= o.convergence * b.rhs1[k]) || b.rhs1[k] == 0.0) {
+        if (mayStop) saveR(k, true);
         b.done[k] = 1;
         conv[k] = 1;
         continue;

@@ -262,6 +262,21 @@ int nla3d_gpu_add_rigid_body_mpc(nla3d_gpu_t h, uint32_t master, uint32_t n_slav
   });
 }
 
+int nla3d_gpu_add_mpc(nla3d_gpu_t h, int n_terms, const uint32_t* nodes, const int* dofs, const double* coefs,
+                      double b) {
+  return guard(h, [&](Ctx& c) {
+    if (c.initialized) throw ApiError(NLA3D_GPU_ESTATE, "model already initialised");
+    if (n_terms < 1 || !nodes || !dofs || !coefs) throw ApiError(NLA3D_GPU_EINVAL, "an MPC needs terms");  // FEStorage.cpp:176-177
+    Mpc mp;
+    mp.b = b;
+    for (int i = 0; i < n_terms; i++) {
+      if (dofs[i] < 0 || dofs[i] >= NUM_DOF_TYPES) throw ApiError(NLA3D_GPU_EINVAL, "bad MPC degree of freedom");
+      mp.eq.push_back({nodes[i], dofs[i], coefs[i]});
+    }
+    c.model.ceMpcs.push_back(mp);
+  });
+}
+
 int nla3d_gpu_set_max_in_row(nla3d_gpu_t h, uint32_t max_in_row) {
   return guard(h, [&](Ctx& c) {
     if (c.initialized) throw ApiError(NLA3D_GPU_ESTATE, "model already initialised");

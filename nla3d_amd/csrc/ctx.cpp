@@ -33,6 +33,29 @@ void Ctx::setupDevice() {
   d_vals.alloc((size_t)(m.nnzAll() + m.nAll()) * 8);
   hipCheck(hipMemset(d_vals.p, 0, d_vals.bytes), "memset vals");
 
+  // constant MPC rows (the first m.ceMpcs.size() entries of m.mpcs)
+  {
+    std::vector<int> ptr{0}, tgt, ft;
+    std::vector<double> coef, bb;
+    for (size_t r = 0; r < m.ceMpcs.size(); r++) {
+      const Mpc& mp = m.mpcs[r];
+      for (const MpcTerm& t : mp.eq) {
+        const int64_t k = m.findEntry(m.nodeDofEq(t.node, t.dof), mp.eqNum);
+        if (k < 0) throw ApiError(-4, "MPC entry absent from the pattern");
+        tgt.push_back((int)k);
+        coef.push_back(t.coef);
+      }
+      ptr.push_back((int)tgt.size());
+      ft.push_back((int)(m.nnzAll() + mp.eqNum - 1));
+      bb.push_back(mp.b);
+    }
+    nCeRows = (int)m.ceMpcs.size();
+    ceTermPtr.upload(ptr);
+    ceTgt.upload(tgt);
+    ceCoef.upload(coef);
+    ceFTgt.upload(ft);
+    ceB.upload(bb);
+  }
   // rigid-body MPC rows: targets of (term DoF, MPC eq) and of F[MPC eq]
   rbs.clear();
   rbs.resize(m.rigidBodies.size());
@@ -78,6 +101,8 @@ void Ctx::scatter() {
 }
 
 void Ctx::applyMpc() {
+  launch_ce_mpc(stream, nCeRows, ceTermPtr.as<int>(), ceTgt.as<int>(), ceCoef.as<double>(), ceFTgt.as<int>(),
+                ceB.as<double>(), d_vals.as<double>());
   for (auto& rb : rbs) {
     launch_rigid_body_mpc(stream, rb.dev, d_xyz.as<double>(), rb.slaveNode.as<int>(), rb.slaveEq.as<int>(),
                           d_U.as<double>(), rb.tgt.as<int>(), rb.fTgt.as<int>(), d_vals.as<double>());

@@ -83,6 +83,7 @@ struct Ctx {
   DevBuf d_k12rPtr, d_k12rCol;                  // block(1,2) by rows (reactions)
   // Newton / Krylov vectors
   DevBuf d_R;                 // vecR [nAll]
+  DevBuf d_Rsave;             // reactions of the last solve while a criterion-1 check runs
   DevBuf d_Ucprev, d_fixEq, d_fixVal, d_loadEq, d_loadVal;
   DevBuf d_rhs, d_dU;         // rhs [n2], increment [nAll]
   DevBuf d_mw[8], d_partial, d_mstate, d_scal;
@@ -106,6 +107,9 @@ struct Ctx {
     DevBuf slaveNode, slaveEq, tgt, fTgt;
   };
   std::vector<RB> rbs;
+  // constant MPC rows: per row its terms' value positions and coefficients, F position, b
+  int nCeRows = 0;
+  DevBuf ceTermPtr, ceTgt, ceCoef, ceFTgt, ceB;
 
   ~Ctx() {
     if (h_mstate) (void)hipHostFree(h_mstate);

@@ -70,6 +70,10 @@ void launch_segmented_gather(hipStream_t stream, int nTargets, const int* seg, c
                              double* full);
 // full[mir[2k]] = full[mir[2k+1]] = upper2[k] for k = idx[0..n)
 void launch_mirror(hipStream_t stream, int n, const double* upper2, const int* idx, const int* mir, double* full);
+// constant MPC rows (FEStorage.cpp:51-61): row r adds coef[k] at vals[tgt[k]] for its terms
+// k in [termPtr[r], termPtr[r+1]) in order, and sets vals[fTgt[r]] = b[r]
+void launch_ce_mpc(hipStream_t stream, int nRows, const int* termPtr, const int* tgt, const double* coef,
+                   const int* fTgt, const double* b, double* vals);
 
 // One RigidBodyMpc collection on the device (Mpc.h:49-60).
 struct RigidBodyDev {

@@ -76,7 +76,9 @@ std::string FEStorage::initDofs() {
     elemDofMask[e] |= elMask;
   }
   // MpcCollection::pre + registerMpcsInStorage (FEStorage.cpp:351-354; RigidBodyMpc::pre, Mpc.cpp:52-68)
-  mpcs.clear();
+  // the MPCs added directly come first (main.cpp:207-212 adds them before the collections
+  // register theirs), then each rigid body's rows
+  mpcs = ceMpcs;
   for (auto& rb : rigidBodies) {
     if (rb.masterNode < 1 || rb.masterNode > nN) return "rigid body master node out of range";
     rb.firstMpc = (uint32_t)mpcs.size();

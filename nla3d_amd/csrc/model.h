@@ -75,6 +75,7 @@ class FEStorage {
   // ---- BCs and MPCs ----
   std::vector<BC> fixs, loads;
   std::vector<Mpc> mpcs;
+  std::vector<Mpc> ceMpcs;  // constant linear MPCs (FEStorage::addMpc; CE records): first in mpcs
   std::vector<RigidBodySpec> rigidBodies;
 
   // ---- DoF tables (DofCollection) ----

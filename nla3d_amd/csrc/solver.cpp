@@ -239,6 +239,7 @@ void Ctx::setupKrylov() {
   d_loadEq.upload(le);
   d_loadVal.upload(lv);
   d_R.alloc((size_t)m.nAll() * 8);
+  d_Rsave.alloc((size_t)std::max<uint32_t>(nC, 1) * 8);
   hipCheck(hipMemset(d_R.p, 0, d_R.bytes), "memset R");
   d_Ucprev.alloc((size_t)std::max<uint32_t>(nC, 1) * 8);
   hipCheck(hipMemset(d_Ucprev.p, 0, d_Ucprev.bytes), "memset Ucprev");
@@ -252,6 +253,12 @@ void Ctx::setupKrylov() {
   if (!h_mstate) hipCheck(hipHostMalloc((void**)&h_mstate, sizeof(MinresState), hipHostMallocDefault), "pinned");
   // block(2) positions of the MPC entries, mirrored after the MPC rows are set
   std::vector<int> mpcIdx;
+  if (nCeRows > 0) {
+    std::vector<int> tg(ceTgt.bytes / 4);
+    hipCheck(hipMemcpy(tg.data(), ceTgt.p, ceTgt.bytes, hipMemcpyDeviceToHost), "ce tgt");
+    for (int t : tg)
+      if ((uint64_t)t >= m.offB2() && (uint64_t)t < m.nnzAll()) mpcIdx.push_back(t - (int)m.offB2());
+  }
   for (auto& rb : rbs) {
     std::vector<int> tg((size_t)rb.dev.nRows * 5);
     hipCheck(hipMemcpy(tg.data(), rb.tgt.p, tg.size() * 4, hipMemcpyDeviceToHost), "tgt");
@@ -396,6 +403,10 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
   const double* K12 = c.d_vals.as<double>() + m.offB12();
   double* R = c.d_R.as<double>();
   double t0 = nowMs();
+  // criterion 1 may end the load step without a solve in this iteration: keep the reactions
+  // of the last solve to put back then
+  const bool mayStop = o.criterion == 1 && it > 1 && nC > 0;
+  if (mayStop) hipCheck(hipMemcpyAsync(c.d_Rsave.p, R, (size_t)nC * 8, hipMemcpyDeviceToDevice, c.stream), "save R");
   hipCheck(hipMemsetAsync(R, 0, c.d_R.bytes, c.stream), "zero R");  // vecR.zero() (FESolver.cpp:445)
   c.assemble();
   newton_apply_bc(c.stream, (int)m.fixs.size(), c.d_fixEq.as<int>(), c.d_fixVal.as<double>(), (int)m.loads.size(),
@@ -423,7 +434,10 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
   }
   if (o.criterion == 1) {
     st.last_criterion = rhs1 > 0 ? rn / rhs1 : 0.0;
-    if ((it > 1 && rn <= o.convergence * rhs1) || rhs1 == 0.0) return 1;
+    if ((it > 1 && rn <= o.convergence * rhs1) || rhs1 == 0.0) {
+      if (mayStop) hipCheck(hipMemcpyAsync(R, c.d_Rsave.p, (size_t)nC * 8, hipMemcpyDeviceToDevice, c.stream), "R");
+      return 1;
+    }
   }
   st.newton_iterations++;
   // solve K.block(2) dU_sl = rhs (FESolver.cpp:457)

@@ -261,9 +261,10 @@ class SparsityInfo:
 class OracleModel:
     """FEStorage + NonlinearFESolver restated for small models."""
 
-    def __init__(self, xyz, elements, material=None, fixes=(), loads=(), rigid_bodies=(), elem_params=None):
+    def __init__(self, xyz, elements, material=None, fixes=(), loads=(), rigid_bodies=(), elem_params=None, mpcs=()):
         """elements: list of (kind, [1-based nodes]); fixes/loads: (node, dof, value);
-        rigid_bodies: (master, [slaves], [dofs]); material: (type, MC)."""
+        rigid_bodies: (master, [slaves], [dofs]); material: (type, MC); mpcs: constant
+        linear MPCs ([(node, dof, coef), ...], b) as FEStorage::addMpc (FEStorage.cpp:175-179)."""
         self.xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
         self.elements = [(k, list(map(int, n))) for k, n in elements]
         self.material = material
@@ -271,6 +272,7 @@ class OracleModel:
         self.loads = list(loads)
         self.rigid_bodies = [(int(m), list(map(int, s)), list(d)) for m, s, d in rigid_bodies]
         self.elem_params = elem_params
+        self.ce_mpcs = [([(int(n), int(d), float(c)) for n, d, c in terms], float(b)) for terms, b in mpcs]
 
     # FEStorage::initDofs (FEStorage.cpp:338-379)
     def initDofs(self):
@@ -285,7 +287,9 @@ class OracleModel:
             else:
                 for n in nodes:
                     self.nodeMask[n - 1] |= {TEMP}
-        self.mpcs = []  # [terms list of (node, dof, coef), b, eqNum]
+        # [terms list of (node, dof, coef), b, eqNum]; the directly added MPCs come first
+        # (main.cpp:207-212), then the rigid bodies' rows (MpcCollection::registerMpcsInStorage)
+        self.mpcs = [[list(terms), b, 0] for terms, b in self.ce_mpcs]
         self.rb_first = []
         for master, slaves, dofs in self.rigid_bodies:
             self.rb_first.append(len(self.mpcs))
