@@ -464,6 +464,119 @@ void orc_quadth_element(const double* X, double k, double rho, double c, double 
   }
 }
 
+/* ------------------------------------------------------------------------ */
+/* PLANE41 (plane-strain u/p quadrilateral), elements/PLANE41.cpp             */
+/* ------------------------------------------------------------------------ */
+
+/* r = a (ma x na) * b (na x nb), the loop order of Mat operator* (Mat.h:198-213) */
+static void mat_mul(int ma, int na, int nb, const double* a, const double* b, double* r) {
+  for (int i = 0; i < ma; i++)
+    for (int j = 0; j < nb; j++) {
+      double e = 0.0;
+      for (int l = 0; l < na; l++) e += a[i * na + l] * b[l * nb + j];
+      r[i * nb + j] = e;
+    }
+}
+
+static void transpose(int m, int n, const double* a, double* t) {
+  for (int i = 0; i < m; i++)
+    for (int j = 0; j < n; j++) t[j * m + i] = a[i * n + j];
+}
+
+/* update() at U (:120-150) then buildK() (:31-104) of one element: out[0..35] the node-node
+ * entries in assembleK's order (PLANE41.h:62-70: i <= j nodes, di, dj, the diagonal node
+ * block's lower entries), out[36..43] node-pressure, out[44] pressure-pressure, out[45..53]
+ * Fe = [-Qe (8), -Fp].  X: 4 nodes x 3 coordinates; U: [UX1 UY1 ... UX4 UY4]. */
+void orc_plane41_element(const double* X, const double* U, double p_e, int type, const double* MC, double* out) {
+  double NiXj[4 * 4 * 2], det[4];
+  orc_quad4_jacob(X, NiXj, det);
+  const int comp[3] = {0, 3, 1}; /* M_XX, M_YY, M_XY (PLANE41.cpp:11) */
+  double k = orc_mat_K(type, MC);
+  double Kuu[64] = {0}, Kup[8] = {0}, Qe[8] = {0}, Kpp = 0.0, Fp = 0.0;
+  for (int np = 0; np < 4; np++) {
+    const double* N = NiXj + np * 8; /* N[nod*2 + 0] = dN/dx, [1] = dN/dy */
+    double Bom[32] = {0};            /* make_Bomega :113-118, 4 x 8 */
+    for (int nod = 0; nod < 4; nod++) {
+      Bom[0 * 8 + 2 * nod] = N[nod * 2 + 0];
+      Bom[1 * 8 + 2 * nod] = N[nod * 2 + 1];
+      Bom[2 * 8 + 2 * nod + 1] = N[nod * 2 + 0];
+      Bom[3 * 8 + 2 * nod + 1] = N[nod * 2 + 1];
+    }
+    /* update (:136-149) */
+    double O[4];
+    for (int i = 0; i < 4; i++) {
+      double e = 0.0;
+      for (int l = 0; l < 8; l++) e += Bom[i * 8 + l] * U[l];
+      O[i] = e;
+    }
+    double Cp[3];
+    Cp[0] = 1.0 + 2 * O[0] + 1.0 * (O[0] * O[0] + O[2] * O[2]);
+    Cp[1] = 1.0 + 2 * O[3] + 1.0 * (O[3] * O[3] + O[1] * O[1]);
+    Cp[2] = O[1] + O[2] + O[0] * O[1] + O[2] * O[3];
+    double CV[6] = {Cp[0], Cp[2], 0.0, Cp[1], 0.0, 1.0};
+    double S6[6];
+    orc_getS_UP(type, MC, CV, p_e, S6);
+    double S[3] = {S6[comp[0]], S6[comp[1]], S6[comp[2]]};
+    /* buildK (:50-87) */
+    double dWt = 1.0 * det[np];
+    double Dd6[21], Dp6[6];
+    orc_getDdDp_UP(type, MC, CV, p_e, Dd6, Dp6);
+    double E[9];
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        int i = comp[a] < comp[b] ? comp[a] : comp[b], j = comp[a] < comp[b] ? comp[b] : comp[a];
+        E[a * 3 + b] = Dd6[i * 6 - i * (i - 1) / 2 + (j - i)]; /* packed upper, row i */
+      }
+    double Ep[3] = {Dp6[comp[0]], Dp6[comp[1]], Dp6[comp[2]]};
+    double J = J_C(CV);
+    double B[24] = {0}; /* make_B :106-111, 3 x 8 */
+    for (int nod = 0; nod < 4; nod++) {
+      B[0 * 8 + 2 * nod] = N[nod * 2 + 0];
+      B[1 * 8 + 2 * nod + 1] = N[nod * 2 + 1];
+      B[2 * 8 + 2 * nod] = N[nod * 2 + 1];
+      B[2 * 8 + 2 * nod + 1] = N[nod * 2 + 0];
+    }
+    double Sm[16] = {S[0], S[2], 0.0, 0.0, S[2], S[1], 0.0, 0.0, 0.0, 0.0, S[0], S[2], 0.0, 0.0, S[2], S[1]};
+    double Om[12] = {O[0], 0.0, O[2], 0.0, 0.0, O[1], 0.0, O[3], O[1], O[0], O[3], O[2]};
+    double Bl[24];
+    mat_mul(3, 4, 8, Om, Bom, Bl);
+    for (int i = 0; i < 24; i++) B[i] += Bl[i];
+    double Bt[24], BtE[24], BtEB[64], Bomt[32], BomtS[32], BomtSBom[64];
+    transpose(3, 8, B, Bt);
+    mat_mul(8, 3, 3, Bt, E, BtE);
+    mat_mul(8, 3, 8, BtE, B, BtEB);
+    transpose(4, 8, Bom, Bomt);
+    mat_mul(8, 4, 4, Bomt, Sm, BomtS);
+    mat_mul(8, 4, 8, BomtS, Bom, BomtSBom);
+    for (int i = 0; i < 64; i++) Kuu[i] += (BtEB[i] * 2.0 + BomtSBom[i]) * dWt;
+    Fp += (J - 1 - p_e / k) * dWt;
+    for (int i = 0; i < 8; i++) {
+      double e = 0.0;
+      for (int l = 0; l < 3; l++) e += Bt[i * 3 + l] * S[l];
+      Qe[i] += e * dWt;
+    }
+    for (int i = 0; i < 8; i++) {
+      double e = 0.0;
+      for (int l = 0; l < 3; l++) e += Bt[i * 3 + l] * Ep[l];
+      Kup[i] += e * dWt;
+    }
+    Kpp -= 1.0 / k * dWt;
+  }
+  /* assembleK order (PLANE41.h:55-89) */
+  int s = 0;
+  for (int i = 0; i < 4; i++)
+    for (int j = i; j < 4; j++)
+      for (int di = 0; di < 2; di++)
+        for (int dj = 0; dj < 2; dj++) {
+          if (i == j && dj > di) continue;
+          out[s++] = Kuu[(i * 2 + di) * 8 + j * 2 + dj];
+        }
+  for (int i = 0; i < 8; i++) out[s++] = Kup[i];
+  out[s++] = Kpp;
+  for (int i = 0; i < 8; i++) out[s++] = -Qe[i];
+  out[s++] = -Fp;
+}
+
 /* SurfaceLINETH::pre :79-91 and buildK :94-129; line makeJacob isoparametric.cpp:13-27 */
 void orc_lineth_element(const double* X, double flux, double htc, double etemp0, double etemp1,
                         double* Ke, double* Fe) {

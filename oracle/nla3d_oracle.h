@@ -82,6 +82,12 @@ void orc_quadth_element(const double* X, double k, double rho, double c, double 
 void orc_lineth_element(const double* X, double flux, double htc, double etemp0, double etemp1,
                         double* Ke, double* Fe);
 
+/* --- PLANE41 (plane-strain u/p quadrilateral), elements/PLANE41.cpp: update() at U then
+ *     buildK(); out[54]: Ke entries in assembleK order (36 node-node, 8 node-pressure,
+ *     1 pressure-pressure) then Fe (8 node, 1 pressure).  Pinned by central differences
+ *     (tangent = -d(F)/d(u, p)), zero residual under rigid motion, sum Fe = 0. --- */
+void orc_plane41_element(const double* X, const double* U, double p_e, int type, const double* MC, double* out);
+
 /* --- TimeControl, FESolver.cpp:34-56: nextStep(delta) until it returns false;
  *     returns the number of steps, times[] the step end times --- */
 int orc_timecontrol(double start, double end, double delta, double* times, int max_steps);

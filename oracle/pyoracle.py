@@ -30,7 +30,7 @@ _lib = None
 
 UX, UY, UZ, ROTX, ROTY, ROTZ, HYDRO_PRESSURE, TEMP = range(8)
 NUM_DOF_TYPES = 8
-SOLID81, QUADTH, SURFACE_LINETH = 1, 2, 3
+SOLID81, QUADTH, SURFACE_LINETH, PLANE41 = 1, 2, 3, 4
 
 _D = ctypes.POINTER(ctypes.c_double)
 
@@ -57,6 +57,7 @@ def lib():
             "orc_quadth_element": [_D, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D, _D],
             "orc_lineth_element": [_D, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D],
             "orc_timecontrol": [ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, ctypes.c_int],
+            "orc_plane41_element": [_D, _D, ctypes.c_double, ctypes.c_int, _D, _D],
         }.items():
             f = getattr(L, name)
             f.argtypes = args
@@ -181,6 +182,28 @@ def quadth_element(X, k, rho, c, volFlux=0.0):
     return Ke, Fe, Ce
 
 
+def plane41_element(X, U, p, mtype, MC):
+    """update() at (U, p) then buildK() of one PLANE41 element: 54 values (assembleK order:
+    36 node-node, 8 node-pressure, 1 pressure-pressure, then Fe: 8 node, 1 pressure)"""
+    X = _f64(X, (4, 3)); U = _f64(U, (8,)); out = np.zeros(54)
+    lib().orc_plane41_element(_p(X), _p(U), float(p), int(mtype), _p(_f64(MC)), _p(out))
+    return out
+
+
+# local DoF pairs of the 45 matrix slots of plane41_element (0..7 node DoFs 2i+d, 8 pressure)
+def plane41_slot_pairs():
+    pairs = []
+    for i in range(4):
+        for j in range(i, 4):
+            for di in range(2):
+                for dj in range(2):
+                    if i == j and dj > di:
+                        continue
+                    pairs.append((2 * i + di, 2 * j + dj))
+    pairs += [(a, 8) for a in range(8)] + [(8, 8)]
+    return pairs
+
+
 def lineth_element(X, flux, htc, e0, e1):
     X = _f64(X, (2, 3)); Ke = np.zeros(3); Fe = np.zeros(2)
     lib().orc_lineth_element(_p(X), flux, htc, e0, e1, _p(Ke), _p(Fe))
@@ -283,6 +306,10 @@ class OracleModel:
             if kind == SOLID81:
                 for n in nodes:
                     self.nodeMask[n - 1] |= {UX, UY, UZ}
+                self.elemMask[e] |= {HYDRO_PRESSURE}
+            elif kind == PLANE41:  # PLANE41.cpp:15-29
+                for n in nodes:
+                    self.nodeMask[n - 1] |= {UX, UY}
                 self.elemMask[e] |= {HYDRO_PRESSURE}
             else:
                 for n in nodes:
@@ -403,6 +430,8 @@ class OracleModel:
             Xe.append(self.xyz[np.array(nodes) - 1])
             Ue.append([U[self.nodeEq[n - 1][d] - 1] for n in nodes for d in range(3)])
             pe.append(U[self.elemEq[e][HYDRO_PRESSURE] - 1])
+        if not Xe:
+            return np.zeros((0, 350))
         return solid81_elements(np.array(Xe), np.array(Ue), np.array(pe), *self.material)
 
     # FEStorage::assembleGlobalEqMatrices (FEStorage.cpp:26-78) for SOLID81 + RigidBodyMpc
@@ -413,6 +442,15 @@ class OracleModel:
         ke = self.solid81_elements_at(U)
         s = 0
         for e, (kind, nodes) in enumerate(self.elements):
+            if kind == PLANE41:  # PLANE41::assembleK (PLANE41.h:55-89)
+                eqn = [self.nodeEq[n - 1][d] for n in nodes for d in (UX, UY)] + [self.elemEq[e][HYDRO_PRESSURE]]
+                out = plane41_element(self.xyz[np.array(nodes) - 1], [U[q - 1] for q in eqn[:8]], U[eqn[8] - 1],
+                                      self.material[0], self.material[1])
+                for k, (a, b) in enumerate(plane41_slot_pairs()):
+                    self.addValueK(eqn[a], eqn[b], out[k])
+                for a in range(9):
+                    F[eqn[a] - 1] += out[45 + a]
+                continue
             if kind != SOLID81:
                 continue
             out = ke[s]
