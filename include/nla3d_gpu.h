@@ -41,6 +41,7 @@ extern "C" {
 #define NLA3D_GPU_SOLID81 1
 #define NLA3D_GPU_QUADTH 2
 #define NLA3D_GPU_SURFACE_LINETH 3
+#define NLA3D_GPU_PLANE41 4 /* plane-strain u/p quadrilateral (PLANE41.h:21-47) */
 
 typedef struct nla3d_gpu_ctx* nla3d_gpu_t;
 
@@ -53,7 +54,8 @@ const char* nla3d_gpu_last_error(nla3d_gpu_t h);
 /* FEStorage::createNodes + Node::pos (FEStorage.cpp:199-214): xyz[3*n] */
 int nla3d_gpu_set_nodes(nla3d_gpu_t h, uint32_t n_nodes, const double* xyz);
 /* FEStorage::createElements (FEStorage.h:675-696).  conn: nodes per element
- * (8 SOLID81, 4 QUADTH, 2 SurfaceLINETH), 1-based.  params: 4 doubles per
+ * (8 SOLID81, 4 QUADTH, 2 SurfaceLINETH, 4 PLANE41), 1-based.  The u/p solids
+ * (SOLID81 or PLANE41, not both) are added as one contiguous range.  params: 4 doubles per
  * element or NULL: QUADTH {k, rho, c, volFlux} (QUADTH.h:27-34),
  * SurfaceLINETH {flux, htc, etemp0, etemp1} (QUADTH.h:49-51). */
 int nla3d_gpu_add_elements(nla3d_gpu_t h, int kind, uint32_t n, const uint32_t* conn, const double* params);
@@ -97,8 +99,9 @@ int nla3d_gpu_get_pattern(nla3d_gpu_t h, int block, uint32_t* iofeir, uint32_t* 
 int nla3d_gpu_set_solution(nla3d_gpu_t h, const double* U);
 int nla3d_gpu_get_solution(nla3d_gpu_t h, double* U);
 /* FEStorage::assembleGlobalEqMatrices (FEStorage.cpp:26-78) on the device for
- * SOLID81 at the current solution: per element update() + buildK()
- * (SOLID81.cpp:29-115), deterministic scatter into K and F, then the MPC rows. */
+ * SOLID81 or PLANE41 at the current solution: per element update() + buildK()
+ * (SOLID81.cpp:29-115, PLANE41.cpp:31-149), deterministic scatter into K and F,
+ * then the MPC rows. */
 int nla3d_gpu_assemble(nla3d_gpu_t h);
 /* copy K block values (block = 1, 12, 2) or F (length nDofs + nMpc) to the host */
 int nla3d_gpu_get_K(nla3d_gpu_t h, int block, double* values);
@@ -110,6 +113,11 @@ int nla3d_gpu_solid81_element_matrices(nla3d_gpu_t h, double* out);
 /* Gauss-point state after update() (SOLID81.h:38-45): per element and Gauss
  * point O[9], C[6], S[6] -> out[nSolid81 * 8 * 21]. */
 int nla3d_gpu_solid81_gp_state(nla3d_gpu_t h, double* out);
+/* Per-element PLANE41 buildK output at the current solution, in the reference's
+ * assembleK order (PLANE41.h:55-89): 36 Kuu entries (node pairs i <= j, DoF
+ * pairs UX UY, the lower one only on a diagonal node block), Kup (8), Kpp,
+ * Fe = [-Qe (8), -Fp] -> out[nPlane41 * 54]. */
+int nla3d_gpu_plane41_element_matrices(nla3d_gpu_t h, double* out);
 
 /* ---- solution procedures (FESolver, FESolver.h:60-242) ------------------- */
 /* Options of the Newton-Raphson loop (NonlinearFESolver, FESolver.h:205-218)

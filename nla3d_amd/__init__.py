@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libnla3d_gpu.so")
 # Dof::dofType (Dof.h:15-25)
 UX, UY, UZ, ROTX, ROTY, ROTZ, HYDRO_PRESSURE, TEMP = range(8)
 # element kinds
-SOLID81, QUADTH, SURFACE_LINETH = 1, 2, 3
+SOLID81, QUADTH, SURFACE_LINETH, PLANE41 = 1, 2, 3, 4
 # MaterialFactory ids (MaterialFactory.cpp:9-13)
 NEO_HOOKEAN, BIDERMAN, MOONEY_RIVLIN = 1, 2, 3
 
@@ -94,6 +94,7 @@ ABI = {
     "nla3d_gpu_get_K": [_P, ctypes.c_int, _DP],
     "nla3d_gpu_get_F": [_P, _DP],
     "nla3d_gpu_solid81_element_matrices": [_P, _DP],
+    "nla3d_gpu_plane41_element_matrices": [_P, _DP],
     "nla3d_gpu_solid81_gp_state": [_P, _DP],
     "nla3d_gpu_time_assembly": [_P, ctypes.c_int, _DP, _DP],
     "nla3d_gpu_time_spmv": [_P, ctypes.c_int, _DP],
@@ -320,6 +321,12 @@ class GpuModel:
     def solid81_element_matrices(self, n_solid):
         out = np.zeros((n_solid, 350))
         self._call("nla3d_gpu_solid81_element_matrices", _ptr(out, ctypes.c_double))
+        return out
+
+    def plane41_element_matrices(self, n_plane):
+        """per-element PLANE41 outputs [n, 54] in assembleK order (PLANE41.h:55-89)"""
+        out = np.zeros((n_plane, 54))
+        self._call("nla3d_gpu_plane41_element_matrices", _ptr(out, ctypes.c_double))
         return out
 
     def solid81_gp_state(self, n_solid):

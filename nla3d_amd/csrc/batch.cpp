@@ -534,7 +534,7 @@ int nla3d_gpu_batch_create(nla3d_gpu_batch_t* out, const nla3d_gpu_t* models, in
     for (int k = 0; k < n; k++) {
       if (!models[k]) throw ApiError(-1, "null model handle");
       Ctx* c = reinterpret_cast<Ctx*>(models[k]);
-      c->requireSolid();
+      c->requireSolid81();
       if (b->device == -2) b->device = c->device;
       if (c->device != b->device) throw ApiError(-1, "all models of a batch must be on one device");
       if (!c->krylovReady) throw ApiError(-3, "model has no Krylov data");

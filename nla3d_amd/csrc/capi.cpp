@@ -53,23 +53,29 @@ static void parallelFor(size_t n, const std::function<void(size_t, size_t)>& fn)
   for (auto& x : th) x.join();
 }
 
-// Build the SOLID81 (element, slot) -> target map and sort it by target
-// (stable in element order): the precomputed element-to-CSR index map of
-// the deterministic scatter.
 void setupThermal(Ctx& c);  // thermal.cpp
 
-void buildSolid81ScatterMap(Ctx& c) {
+// Build the u/p element (element, slot) -> target map and sort it by target (stable in
+// element order): the precomputed element-to-CSR index map of the deterministic scatter.
+// SOLID81 outputs are element-major ([e][350], library slot order), PLANE41 outputs
+// slot-major ([54][e], assembleK order); src holds the position in that layout.
+void buildUpScatterMap(Ctx& c) {
   FEStorage& m = c.model;
   const uint32_t nS = m.solidCount;
+  const bool plane = m.upKind == ELEM_PLANE41;
+  const int slots = plane ? kPlane41Slots : kSolid81Slots;
+  const int nn = plane ? 4 : 8, nd = plane ? 2 : 3, nLoc = nn * nd;  // node DoFs; nLoc = the pressure
   const uint64_t nK = m.nnzAll();
   const uint64_t nT = nK + m.nAll();
-  if (nT >= (1ull << 31) || (uint64_t)nS * kSolid81Slots >= (1ull << 31))
+  if (nT >= (1ull << 31) || (uint64_t)nS * slots >= (1ull << 31))
     throw ApiError(NLA3D_GPU_EINVAL, "model too large for 32-bit scatter indices");
-  int slot2a3[kSolid81Slots];
-  solid81_slot_to_assemble3(slot2a3);
-  // local DoF pair of each slot: 0..23 node DoFs (3i+d), 24 pressure; F slots have b = -1
+  // local DoF pair of each slot: node DoFs nd*i+d, then the pressure; F slots have b = -1
   int la[kSolid81Slots], lb[kSolid81Slots];
-  {
+  if (plane) {
+    plane41_slot_pairs(la, lb);
+  } else {
+    int slot2a3[kSolid81Slots];
+    solid81_slot_to_assemble3(slot2a3);
     int k = 0;
     // recover (a,b) of every packed Kuu index
     int pa[300], pb[300];
@@ -88,19 +94,19 @@ void buildSolid81ScatterMap(Ctx& c) {
     for (int s = 325; s < 349; s++) { la[s] = s - 325; lb[s] = -1; }
     la[349] = 24; lb[349] = -1;
   }
-  std::vector<int> tgt((size_t)nS * kSolid81Slots);
+  std::vector<int> tgt((size_t)nS * slots);
   std::string err;
   std::mutex mu;
   parallelFor(nS, [&](size_t b, size_t e) {
     uint32_t eq[25];
     for (size_t s = b; s < e; s++) {
       const uint32_t el = m.solidFirst + (uint32_t)s;  // 0-based global element
-      for (int i = 0; i < 8; i++) {
+      for (int i = 0; i < nn; i++) {
         uint32_t node = m.elemNodes[m.elemNodePtr[el] + i];
-        for (int d = 0; d < 3; d++) eq[3 * i + d] = m.nodeDofEq(node, d);
+        for (int d = 0; d < nd; d++) eq[nd * i + d] = m.nodeDofEq(node, d);
       }
-      eq[24] = m.elemDofEq(el + 1, HYDRO_PRESSURE);
-      for (int k = 0; k < kSolid81Slots; k++) {
+      eq[nLoc] = m.elemDofEq(el + 1, HYDRO_PRESSURE);
+      for (int k = 0; k < slots; k++) {
         int64_t t;
         if (lb[k] < 0) {
           t = (int64_t)nK + eq[la[k]] - 1;
@@ -113,7 +119,7 @@ void buildSolid81ScatterMap(Ctx& c) {
             return;
           }
         }
-        tgt[s * kSolid81Slots + k] = (int)t;
+        tgt[s * slots + k] = (int)t;
       }
     }
   });
@@ -125,7 +131,8 @@ void buildSolid81ScatterMap(Ctx& c) {
   std::vector<int> src(tgt.size());
   {
     std::vector<int> pos(seg.begin(), seg.end() - 1);
-    for (size_t i = 0; i < tgt.size(); i++) src[pos[tgt[i]]++] = (int)i;
+    for (size_t i = 0; i < tgt.size(); i++)
+      src[pos[tgt[i]]++] = plane ? (int)((i % slots) * nS + i / slots) : (int)i;
   }
   c.nTargets = (int)nT;
   c.nContrib = (int64_t)src.size();
@@ -392,9 +399,9 @@ int nla3d_gpu_get_F(nla3d_gpu_t h, double* F) {
 
 int nla3d_gpu_solid81_element_matrices(nla3d_gpu_t h, double* out) {
   return guard(h, [&](Ctx& c) {
-    c.requireSolid();
+    c.requireSolid81();
     if (!out) throw ApiError(NLA3D_GPU_EINVAL, "NULL");
-    c.runSolid81(nullptr);
+    c.runElements(nullptr);
     const uint32_t nS = c.model.solidCount;
     std::vector<double> ke((size_t)nS * kSolid81Slots);
     hipCheck(hipMemcpyAsync(ke.data(), c.d_Ke.p, ke.size() * 8, hipMemcpyDeviceToHost, c.stream), "copy Ke");
@@ -408,13 +415,28 @@ int nla3d_gpu_solid81_element_matrices(nla3d_gpu_t h, double* out) {
 
 int nla3d_gpu_solid81_gp_state(nla3d_gpu_t h, double* out) {
   return guard(h, [&](Ctx& c) {
-    c.requireSolid();
+    c.requireSolid81();
     if (!out) throw ApiError(NLA3D_GPU_EINVAL, "NULL");
     DevBuf st;
     st.alloc((size_t)c.model.solidCount * 8 * kSolid81GpState * 8);
-    c.runSolid81(st.as<double>());
+    c.runElements(st.as<double>());
     hipCheck(hipMemcpyAsync(out, st.p, st.bytes, hipMemcpyDeviceToHost, c.stream), "copy state");
     hipCheck(hipStreamSynchronize(c.stream), "sync");
+  });
+}
+
+int nla3d_gpu_plane41_element_matrices(nla3d_gpu_t h, double* out) {
+  return guard(h, [&](Ctx& c) {
+    c.requireSolid();
+    if (c.model.upKind != ELEM_PLANE41) throw ApiError(NLA3D_GPU_ESTATE, "this entry point needs PLANE41 elements");
+    if (!out) throw ApiError(NLA3D_GPU_EINVAL, "NULL");
+    c.runElements(nullptr);
+    const size_t nS = c.model.solidCount;
+    std::vector<double> ke(nS * kPlane41Slots);
+    hipCheck(hipMemcpyAsync(ke.data(), c.d_Ke.p, ke.size() * 8, hipMemcpyDeviceToHost, c.stream), "copy Ke");
+    hipCheck(hipStreamSynchronize(c.stream), "sync");
+    for (size_t e = 0; e < nS; e++)
+      for (int k = 0; k < kPlane41Slots; k++) out[e * kPlane41Slots + k] = ke[(size_t)k * nS + e];
   });
 }
 
@@ -427,7 +449,7 @@ int nla3d_gpu_time_assembly(nla3d_gpu_t h, int iters, double* ms_element, double
     double te = 0, ts = 0;
     for (int it = 0; it < iters; it++) {
       hipCheck(hipEventRecord(ev[0], c.stream), "rec");
-      c.runSolid81(nullptr);
+      c.runElements(nullptr);
       hipCheck(hipEventRecord(ev[1], c.stream), "rec");
       c.scatter();
       hipCheck(hipEventRecord(ev[2], c.stream), "rec");

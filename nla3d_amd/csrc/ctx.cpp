@@ -6,7 +6,7 @@
 namespace nla3d_amd {
 
 // defined in capi.cpp
-void buildSolid81ScatterMap(struct Ctx& c);
+void buildUpScatterMap(struct Ctx& c);
 
 void Ctx::setupDevice() {
   FEStorage& m = model;
@@ -17,16 +17,18 @@ void Ctx::setupDevice() {
     for (int d = 0; d < 3; d++) eq3[(size_t)n * 3 + d] = (int)m.nodeEq[(size_t)n * 8 + d] - 1;
   d_nodeEq3.upload(eq3);
   if (m.solidCount) {
-    std::vector<int> conn((size_t)m.solidCount * 8), peq(m.solidCount);
+    const int nn = m.upKind == ELEM_PLANE41 ? 4 : 8;
+    const int slots = m.upKind == ELEM_PLANE41 ? kPlane41Slots : kSolid81Slots;
+    std::vector<int> conn((size_t)m.solidCount * nn), peq(m.solidCount);
     for (uint32_t s = 0; s < m.solidCount; s++) {
       const uint32_t el = m.solidFirst + s;
-      for (int i = 0; i < 8; i++) conn[(size_t)s * 8 + i] = (int)m.elemNodes[m.elemNodePtr[el] + i] - 1;
+      for (int i = 0; i < nn; i++) conn[(size_t)s * nn + i] = (int)m.elemNodes[m.elemNodePtr[el] + i] - 1;
       peq[s] = (int)m.elemEq[(size_t)el * 8 + HYDRO_PRESSURE] - 1;
     }
     d_conn.upload(conn);
     d_elemEqP.upload(peq);
-    d_Ke.alloc((size_t)m.solidCount * kSolid81Slots * 8);
-    buildSolid81ScatterMap(*this);
+    d_Ke.alloc((size_t)m.solidCount * slots * 8);
+    buildUpScatterMap(*this);
   }
   d_U.alloc((size_t)m.nAll() * 8);
   hipCheck(hipMemset(d_U.p, 0, d_U.bytes), "memset U");
@@ -86,7 +88,14 @@ void Ctx::setupDevice() {
   }
 }
 
-void Ctx::runSolid81(double* gp_state) {
+void Ctx::runElements(double* gp_state) {
+  if (model.upKind == ELEM_PLANE41) {
+    if (gp_state) throw ApiError(-3, "Gauss-point state is available for SOLID81 only");
+    launch_plane41(stream, (int)model.solidCount, d_xyz.as<double>(), d_conn.as<int>(), d_nodeEq3.as<int>(),
+                   d_elemEqP.as<int>(), d_U.as<double>(), mat, d_Ke.as<double>());
+    hipCheck(hipGetLastError(), "plane41 kernel launch");
+    return;
+  }
   launch_solid81(stream, (int)model.solidCount, d_xyz.as<double>(), d_conn.as<int>(), d_nodeEq3.as<int>(),
                  d_elemEqP.as<int>(), d_U.as<double>(), mat, d_Ke.as<double>(), gp_state);
   hipCheck(hipGetLastError(), "solid81 kernel launch");
@@ -119,7 +128,7 @@ void Ctx::applyMpc() {
 // every K and F entry (entries without element contributions become +0.0,
 // which is zeroK/zeroF), then the MPC rows are set.
 void Ctx::assemble() {
-  runSolid81(nullptr);
+  runElements(nullptr);
   scatter();
   applyMpc();
 }

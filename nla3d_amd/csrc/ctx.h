@@ -56,13 +56,13 @@ struct Ctx {
   bool hasMaterial = false;
   uint32_t maxInRow = 100;  // BlockSparseSymMatrix<2> default (BlockSparseMatrix.h:42)
 
-  // device copies of the mesh and DoF maps (SOLID81 range)
+  // device copies of the mesh and DoF maps (u/p element range: SOLID81 or PLANE41)
   DevBuf d_xyz;        // double [nN*3]
-  DevBuf d_conn;       // int [nSolid*8], 0-based node index
+  DevBuf d_conn;       // int [nSolid*nn] (nn = 8 SOLID81, 4 PLANE41), 0-based node index
   DevBuf d_nodeEq3;    // int [nN*3], 0-based equation index of UX UY UZ or -1
   DevBuf d_elemEqP;    // int [nSolid], 0-based equation index of HYDRO_PRESSURE or -1
   DevBuf d_U;          // double [nAll]
-  DevBuf d_Ke;         // double [nSolid*350], element outputs in slot order
+  DevBuf d_Ke;         // element outputs: SOLID81 [nSolid][350] slot order, PLANE41 [54][nSolid]
   DevBuf d_vals;       // double [nnz1 + nnz12 + nnz2 + nAll]: K blocks then F
   DevBuf d_seg;        // int [nTargets+1]
   DevBuf d_src;        // int [nContrib]
@@ -122,11 +122,16 @@ struct Ctx {
   void requireSolid() const {
     requireDevice();
     if (!initialized) throw ApiError(-3, "call nla3d_gpu_init_solution first");
-    if (model.solidCount == 0) throw ApiError(-3, "model has no SOLID81 elements");
+    if (model.solidCount == 0) throw ApiError(-3, "model has no SOLID81 or PLANE41 elements");
     if (!hasMaterial) throw ApiError(-3, "material is not defined");
   }
+  void requireSolid81() const {
+    requireSolid();
+    if (model.upKind != ELEM_SOLID81) throw ApiError(-3, "this entry point needs SOLID81 elements");
+  }
   void setupDevice();
-  void runSolid81(double* gp_state);
+  // the u/p element kernel (SOLID81 or PLANE41) into d_Ke; gp_state (SOLID81 only) may be null
+  void runElements(double* gp_state);
   void scatter();
   void applyMpc();
   void assemble();

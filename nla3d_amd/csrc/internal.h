@@ -30,6 +30,15 @@ void launch_solid81(hipStream_t stream, int nE, const double* xyz, const int* co
                     const int* elem_eq, const double* U, const MaterialParams& mat, double* Ke,
                     double* gp_state);
 
+// PLANE41 (plane-strain u/p quadrilateral): 54 values per element, slot-major
+// (Ke[s * nE + e]) in the reference's assembleK order; conn[nE*4], node_eq as above (the UX
+// and UY entries are used).  plane41_slot_pairs gives each slot's local DoF pair (0..7 node
+// DoFs 2a+d, 8 the pressure; lb = -1 for the 9 force slots).
+constexpr int kPlane41Slots = 54;
+void launch_plane41(hipStream_t stream, int nE, const double* xyz, const int* conn, const int* node_eq,
+                    const int* elem_eq, const double* U, const MaterialParams& mat, double* Ke);
+void plane41_slot_pairs(int* la, int* lb);
+
 // One model of a batched assembly (device pointers of that model's buffers).
 struct Solid81Batch {
   int nE;

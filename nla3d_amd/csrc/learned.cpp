@@ -76,7 +76,7 @@ static LearnedWork& work(Ctx& c) {
 }
 
 static void requireLearnable(Ctx& c) {
-  c.requireSolid();
+  c.requireSolid81();
   if (!c.krylovReady) throw ApiError(-3, "Krylov data not set up");
 }
 

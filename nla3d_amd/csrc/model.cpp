@@ -33,13 +33,17 @@ static void parallelFor(size_t n, const std::function<void(size_t, size_t)>& fn)
 }
 
 std::string FEStorage::addElements(int kind, uint32_t n, const uint32_t* conn1, const double* params) {
-  int nn = (kind == ELEM_SOLID81) ? 8 : (kind == ELEM_QUADTH ? 4 : (kind == ELEM_SURFACE_LINETH ? 2 : 0));
+  int nn = (kind == ELEM_SOLID81) ? 8
+           : (kind == ELEM_QUADTH || kind == ELEM_PLANE41) ? 4
+           : (kind == ELEM_SURFACE_LINETH ? 2 : 0);
   if (nn == 0) return "unknown element kind " + std::to_string(kind);
-  if (kind == ELEM_SOLID81) {
+  if (kind == ELEM_SOLID81 || kind == ELEM_PLANE41) {
+    if (solidCount > 0 && upKind != kind) return "SOLID81 and PLANE41 elements cannot be mixed in one model";
     if (solidCount > 0 && solidFirst + solidCount != nElements())
-      return "SOLID81 elements must be added as one contiguous range";
+      return "u/p solid elements must be added as one contiguous range";
     if (solidCount == 0) solidFirst = nElements();
     solidCount += n;
+    upKind = kind;
   }
   for (uint32_t e = 0; e < n; e++) {
     elemKind.push_back((uint8_t)kind);
@@ -65,6 +69,10 @@ std::string FEStorage::initDofs() {
     switch (elemKind[e]) {
       case ELEM_SOLID81:  // SOLID81.cpp:21-25
         nodeMask = (1u << UX) | (1u << UY) | (1u << UZ);
+        elMask = 1u << HYDRO_PRESSURE;
+        break;
+      case ELEM_PLANE41:  // PLANE41.cpp:24-28
+        nodeMask = (1u << UX) | (1u << UY);
         elMask = 1u << HYDRO_PRESSURE;
         break;
       case ELEM_QUADTH:  // QUADTH.cpp:16-19

@@ -20,7 +20,7 @@ namespace nla3d_amd {
 enum DofType { UX = 0, UY, UZ, ROTX, ROTY, ROTZ, HYDRO_PRESSURE, TEMP, NUM_DOF_TYPES };
 
 // element types handled by the library (ElementType, sys.h:151-163)
-enum ElemKind { ELEM_SOLID81 = 1, ELEM_QUADTH = 2, ELEM_SURFACE_LINETH = 3 };
+enum ElemKind { ELEM_SOLID81 = 1, ELEM_QUADTH = 2, ELEM_SURFACE_LINETH = 3, ELEM_PLANE41 = 4 };
 
 struct MpcTerm {  // Mpc.h:15-27
   uint32_t node;
@@ -67,8 +67,10 @@ class FEStorage {
   std::vector<uint64_t> elemNodePtr{0};  // CSR offsets into elemNodes
   std::vector<uint32_t> elemNodes;       // 1-based node numbers
   uint32_t nElements() const { return (uint32_t)elemKind.size(); }
-  // contiguous range of SOLID81 elements (the GPU kernel needs them dense)
+  // contiguous range of the u/p solid elements (the GPU kernel needs them dense): all
+  // SOLID81 or all PLANE41 (upKind)
   uint32_t solidFirst = 0, solidCount = 0;
+  int upKind = 0;
   // thermal element data (QUADTH k, rho, c, volFlux ; SurfaceLINETH flux, htc, etemp0, etemp1)
   std::vector<double> elemParams;        // 4 per element
 

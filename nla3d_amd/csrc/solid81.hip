@@ -34,6 +34,7 @@
 // solid81_slot_to_assemble3() gives the permutation to the reference's
 // assemble3 order (SOLID81.h:54-94).
 
+#include "hyper.h"
 #include "internal.h"
 
 namespace nla3d_amd {
@@ -70,27 +71,6 @@ __device__ constexpr int kSz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
 // Voigt component -> tensor indices
 __device__ constexpr int kVi[6] = {0, 0, 0, 1, 1, 2};
 __device__ constexpr int kVj[6] = {0, 1, 2, 1, 2, 2};
-
-// W_first/second_derivatives of the three materials (materials_hyperelastic.cpp:135-198).
-// alpha = {a1, a2, a11, a12, a22}
-__device__ __forceinline__ void material_alpha(const MaterialParams& m, double I1bar, double* al) {
-  if (m.type == 1) {  // neo-Hookean: G, K
-    al[0] = 0.5 * m.mc[0];
-    al[1] = 0.0;
-    al[2] = 0.0;
-  } else if (m.type == 2) {  // Biderman: C10 C20 C30 C01 K
-    double x = I1bar - 3.0;
-    al[0] = m.mc[0] + 2.0 * m.mc[1] * x + 3.0 * m.mc[2] * x * x;
-    al[1] = m.mc[3];
-    al[2] = 2.0 * m.mc[1] + 6.0 * m.mc[2] * x;
-  } else {  // Mooney-Rivlin: C10 C01 K
-    al[0] = m.mc[0];
-    al[1] = m.mc[1];
-    al[2] = 0.0;
-  }
-  al[3] = 0.0;
-  al[4] = 0.0;
-}
 
 // Phase A: one Gauss point of one element.
 __device__ __forceinline__ void gauss_point(const double (*X)[3], const double (*U)[3], double p_e, int g,

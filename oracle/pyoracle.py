@@ -23,6 +23,8 @@ import math
 import os
 
 import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
@@ -31,6 +33,9 @@ _lib = None
 UX, UY, UZ, ROTX, ROTY, ROTZ, HYDRO_PRESSURE, TEMP = range(8)
 NUM_DOF_TYPES = 8
 SOLID81, QUADTH, SURFACE_LINETH, PLANE41 = 1, 2, 3, 4
+# block(2) size above which the Newton step uses a sparse direct solve (SuperLU) instead of a
+# dense one; both are FP64 direct solves standing in for PARDISO
+SPARSE_ABOVE = 4000
 
 _D = ctypes.POINTER(ctypes.c_double)
 
@@ -441,7 +446,12 @@ class OracleModel:
         F = np.zeros(self.nAll)
         ke = self.solid81_elements_at(U)
         s = 0
+        plane_only = all(kind != SOLID81 for kind, _ in self.elements)
+        if plane_only:
+            self._assemble_plane41_batched(U, F)
         for e, (kind, nodes) in enumerate(self.elements):
+            if kind == PLANE41 and plane_only:
+                continue
             if kind == PLANE41:  # PLANE41::assembleK (PLANE41.h:55-89)
                 eqn = [self.nodeEq[n - 1][d] for n in nodes for d in (UX, UY)] + [self.elemEq[e][HYDRO_PRESSURE]]
                 out = plane41_element(self.xyz[np.array(nodes) - 1], [U[q - 1] for q in eqn[:8]], U[eqn[8] - 1],
@@ -478,6 +488,43 @@ class OracleModel:
             for node, dof, coef in terms:
                 self.addValueK(eq, self.nodeEq[node - 1][dof], coef)
         return {b: blk.values.copy() for b, blk in self.blocks.items()}, F
+
+    def _assemble_plane41_batched(self, U, F):
+        """PLANE41::assembleK for every PLANE41 element of a model without SOLID81: the same
+        addValue calls as the per-element loop, with the positions looked up once
+        (SparsityInfo::getIndex) and the values added by np.add.at, which adds repeated
+        positions one after another in (element, slot) order, the loop's order."""
+        if getattr(self, "_p41", None) is None:
+            pairs = plane41_slot_pairs()
+            els, eqs, blk, idx = [], [], {1: [], 12: [], 2: []}, {1: [], 12: [], 2: []}
+            for e, (kind, nodes) in enumerate(self.elements):
+                if kind != PLANE41:
+                    continue
+                eqn = [self.nodeEq[n - 1][d] for n in nodes for d in (UX, UY)] + [self.elemEq[e][HYDRO_PRESSURE]]
+                els.append(e)
+                eqs.append(eqn)
+                for k, (a, b) in enumerate(pairs):
+                    i, j = min(eqn[a], eqn[b]), max(eqn[a], eqn[b])
+                    bi, pi = self._block(i)
+                    bj, pj = self._block(j)
+                    bb = bi if bi == bj else 12
+                    q = self.blocks[bb].getIndex(pi, pj)
+                    if q < 0:
+                        raise RuntimeError(f"The position({i}, {j}) is absent in the matrix")
+                    blk[bb].append(len(els) * 45 - 45 + k)
+                    idx[bb].append(q)
+            self._p41 = (np.array(els, dtype=np.int64), np.array(eqs, dtype=np.int64),
+                         {b: (np.array(blk[b], dtype=np.int64), np.array(idx[b], dtype=np.int64)) for b in blk})
+        els, eqs, where = self._p41
+        out = np.empty((len(els), 54))
+        for r, e in enumerate(els):
+            nodes = self.elements[e][1]
+            out[r] = plane41_element(self.xyz[np.array(nodes) - 1], U[eqs[r, :8] - 1], U[eqs[r, 8] - 1],
+                                     self.material[0], self.material[1])
+        kv = out[:, :45].ravel()
+        for b, (src, q) in where.items():
+            np.add.at(self.blocks[b].values, q, kv[src])
+        np.add.at(F, (eqs - 1).ravel(), out[:, 45:].ravel())
 
     # RigidBodyMpc::update (Mpc.cpp:70-164)
     def rigid_body_update(self, U):
@@ -531,6 +578,22 @@ class OracleModel:
                 K2[i, j] = K2[j, i] = K[2][k]
         return K12, K2
 
+    @staticmethod
+    def sparse_sym(b, vals, n):
+        """symmetric scipy CSR from an upper 1-based CSR block"""
+        rows = np.repeat(np.arange(n), np.diff(b.iofeir.astype(np.int64)))
+        cols = b.columns.astype(np.int64) - 1
+        up = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+        return (up + sp.triu(up, 1).T).tocsr()
+
+    def sparse_blocks(self, K):
+        """K values per block -> sparse K12 (nC x n2) and symmetric K2 (n2 x n2); large models"""
+        nC, n2 = self.nC, self.nU + self.nMpc
+        b = self.blocks[12]
+        rows = np.repeat(np.arange(nC), np.diff(b.iofeir.astype(np.int64)))
+        K12 = sp.csr_matrix((K[12], (rows, b.columns.astype(np.int64) - 1)), shape=(nC, n2))
+        return K12, self.sparse_sym(self.blocks[2], K[2], n2)
+
     def dense_k11(self, K):
         """symmetric K11 (nC x nC) from the upper CSR of block 1"""
         nC = self.nC
@@ -553,7 +616,8 @@ class OracleModel:
         self.Ucprev = np.zeros(self.nC)
 
     def solve_load_step(self, t, criterion=0, convergence=1e-3, max_iterations=20):
-        """one load step at normalised time t; dense FP64 solve stands in for PARDISO"""
+        """one load step at normalised time t; a direct FP64 solve (dense, or SuperLU above
+        SPARSE_ABOVE unknowns) stands in for PARDISO"""
         nC, nU = self.nC, self.nU
         U = self.U
         rhs1 = None
@@ -565,7 +629,7 @@ class OracleModel:
             for node, dof, val in self.fixes:
                 U[self.nodeEq[node - 1][dof] - 1] = val * t
             dUc = U[:nC] - self.Ucprev
-            K12, K2 = self.dense_blocks(K)
+            K12, K2 = self.sparse_blocks(K) if nU + self.nMpc > SPARSE_ABOVE else self.dense_blocks(K)
             rhs = F[nC:] + R[nC:] - K12.T @ dUc
             # equilibrium residual: lambda is a total, not an increment (FESolver.cpp:461), so the
             # multiplier forces K2 [0; lambda] are still in rhs and are taken out of the measure
@@ -576,12 +640,13 @@ class OracleModel:
                 rhs1 = rn
             if criterion == 1 and it > 1 and rn <= convergence * rhs1:
                 return it - 1
-            dU = np.linalg.solve(K2, rhs)
+            dU = spla.spsolve(K2.tocsc(), rhs) if sp.issparse(K2) else np.linalg.solve(K2, rhs)
             U[nC:nC + nU] += dU[:nU]
             U[nC + nU:] = dU[nU:]
             # constrained-DoF reactions: vecRc = K11 dUc + K12 dUsl - Fc (FESolver.cpp:464-467)
             self.R = R
-            self.R[:nC] = self.dense_k11(K) @ dUc + K12 @ dU - F[:nC]
+            K11 = self.sparse_sym(self.blocks[1], K[1], nC) if sp.issparse(K2) else self.dense_k11(K)
+            self.R[:nC] = K11 @ dUc + K12 @ dU - F[:nC]
             self.Ucprev = U[:nC].copy()
             if criterion == 0:
                 crit = np.abs(dU[:nU]).sum() / nU / convergence

@@ -122,3 +122,26 @@ def test_max_in_row_overflow_is_reported_like_the_reference(gpu_lib):
     s = m2.init_solution()
     io, cols = m2.pattern(12)
     assert np.diff(io.astype(np.int64)).max() >= 147
+
+
+def test_plane41_numbering_and_pattern(gpu_lib):
+    """PLANE41 registers UX UY per node and HYDRO_PRESSURE per element (PLANE41.cpp:24-28)"""
+    xyz, conn, (I, J) = meshgen.quad_grid(5, 3, 1.0, 0.6)
+    fixes = [(int(n), d, 0.0) for n in np.nonzero(J == 0)[0] + 1 for d in (0, 1)]
+    fixes += [(int(n), 1, -0.1) for n in np.nonzero(J == 3)[0] + 1]
+    m = nla3d_amd.GpuModel(device=-1)
+    m.set_nodes(xyz)
+    m.add_elements(nla3d_amd.PLANE41, conn)
+    for n, d, v in fixes:
+        m.add_fix(n, d, v)
+    m.init_solution()
+    compare(m, OracleModel(xyz, [(4, list(map(int, c))) for c in conn], fixes=fixes).setup())
+
+
+def test_solid81_and_plane41_cannot_be_mixed(gpu_lib):
+    xyz, conn, _ = meshgen.hex_box(1, 1, 1)
+    m = nla3d_amd.GpuModel(device=-1)
+    m.set_nodes(xyz)
+    m.add_elements(nla3d_amd.SOLID81, conn)
+    with pytest.raises(nla3d_amd.Nla3dGpuError):
+        m.add_elements(nla3d_amd.PLANE41, [[1, 2, 4, 3]])

@@ -56,3 +56,40 @@ def test_newton_converges_quadratically():
     om.newton_init()
     for t in (0.5, 1.0):
         assert om.solve_load_step(t, criterion=1, convergence=1e-10, max_iterations=20) <= 5
+
+
+def test_a2000_damper_reaction_curve_matches_ansys():
+    """the reference's a2000_damper benchmark (test/CMakeLists.txt:142-146) on the oracle:
+    3,101 PLANE41 elements, Neo-Hookean G = 1, K = 500, 20 load steps; the DISP_BC UY
+    reaction curve against ANSYS's (readRefCurveData / compareCurves, main.cpp:326-361:
+    mean absolute difference, the reference's threshold 0.2)"""
+    import os
+
+    from nla3d_amd.cdb import read_cdb
+
+    gold = os.path.join(os.path.dirname(__file__), "golden", "a2000_damper")
+    d = read_cdb(os.path.join(gold, "model.cdb.gz"))
+    om = po.OracleModel(np.array(d["nodes"]), [(po.PLANE41, list(n)) for t, n in d["cells"] if t == 1],
+                        material=(1, [1.0, 500.0]), fixes=d["fixes"]).setup()
+    om.newton_init()
+    eqs = [om.nodeEq[n - 1][po.UY] - 1 for n in d["comps"]["DISP_BC"]]
+    curve = [0.0]
+    for t in po.load_steps(20):
+        om.solve_load_step(t, criterion=0, convergence=1e-3, max_iterations=15)
+        curve.append(float(om.R[eqs].sum()))
+    ref = a2000_curve(gold)
+    assert len(curve) == len(ref)
+    assert np.abs(np.array(curve) - ref).mean() < 0.2
+
+
+def a2000_curve(gold):
+    """readRefCurveData (main.cpp:342-361): third column, consecutive repeats dropped"""
+    out = []
+    with open(gold + "/loading_curve_ansys.txt") as f:
+        next(f)
+        for line in f:
+            if line.split():
+                v = float(line.split()[2])
+                if not out or v != out[-1]:
+                    out.append(v)
+    return np.array(out)
