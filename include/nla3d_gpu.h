@@ -119,6 +119,16 @@ int nla3d_gpu_solid81_gp_state(nla3d_gpu_t h, double* out);
  * Fe = [-Qe (8), -Fp] -> out[nPlane41 * 54]. */
 int nla3d_gpu_plane41_element_matrices(nla3d_gpu_t h, double* out);
 
+/* Element results at the current solution for every u/p element: Element::getScalar /
+ * getVector / getTensor (element.h:122-131; SOLID81.cpp:191-345, PLANE41.cpp:153-300) with
+ * the reference's query numbers (query.h): kind 0 scalarQuery (1 SP, 3 WU, 4 WP, 5 VOL;
+ * PLANE41: SP only) -> out[nE]; kind 1 vectorQuery (1 IC; SOLID81 only) -> out[nE * 3];
+ * kind 2 tensorQuery (1 COUCHY, 2 PK2, 3 E, 4 C) -> out[nE * 6] in the order XX XY XZ YY YZ
+ * ZZ.  gp: Gauss point (0..7 SOLID81, 0..3 PLANE41) or -1 for GP_MEAN.  The reference's
+ * arithmetic is kept, quirks included (SOLID81 COUCHY/PK2 count p J C^-1 twice; PLANE41
+ * COUCHY is F^T S F / J without the Gauss weight). */
+int nla3d_gpu_element_query(nla3d_gpu_t h, int kind, int query, int gp, double* out);
+
 /* ---- solution procedures (FESolver, FESolver.h:60-242) ------------------- */
 /* Options of the Newton-Raphson loop (NonlinearFESolver, FESolver.h:205-218)
  * and of the Krylov solve that replaces EquationSolver::solveEquations. */

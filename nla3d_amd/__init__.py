@@ -19,6 +19,12 @@ UX, UY, UZ, ROTX, ROTY, ROTZ, HYDRO_PRESSURE, TEMP = range(8)
 SOLID81, QUADTH, SURFACE_LINETH, PLANE41 = 1, 2, 3, 4
 # MaterialFactory ids (MaterialFactory.cpp:9-13)
 NEO_HOOKEAN, BIDERMAN, MOONEY_RIVLIN = 1, 2, 3
+# element result queries (query.h): kinds, and scalarQuery / vectorQuery / tensorQuery numbers
+QUERY_SCALAR, QUERY_VECTOR, QUERY_TENSOR = 0, 1, 2
+SP, WU, WP, VOL = 1, 3, 4, 5
+IC = 1
+COUCHY, PK2, E, C = 1, 2, 3, 4
+GP_MEAN = -1
 
 _lib = None
 
@@ -95,6 +101,7 @@ ABI = {
     "nla3d_gpu_get_F": [_P, _DP],
     "nla3d_gpu_solid81_element_matrices": [_P, _DP],
     "nla3d_gpu_plane41_element_matrices": [_P, _DP],
+    "nla3d_gpu_element_query": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _DP],
     "nla3d_gpu_solid81_gp_state": [_P, _DP],
     "nla3d_gpu_time_assembly": [_P, ctypes.c_int, _DP, _DP],
     "nla3d_gpu_time_spmv": [_P, ctypes.c_int, _DP],
@@ -327,6 +334,14 @@ class GpuModel:
         """per-element PLANE41 outputs [n, 54] in assembleK order (PLANE41.h:55-89)"""
         out = np.zeros((n_plane, 54))
         self._call("nla3d_gpu_plane41_element_matrices", _ptr(out, ctypes.c_double))
+        return out
+
+    def element_query(self, n_elements, kind, query, gp=-1):
+        """Element::getScalar / getVector / getTensor for every u/p element (kind 0 / 1 / 2 with
+        the reference's query numbers, gp = Gauss point or -1 for GP_MEAN) -> [n, 1 | 3 | 6]"""
+        width = (1, 3, 6)[kind] if kind in (0, 1, 2) else 1
+        out = np.zeros((n_elements, width))
+        self._call("nla3d_gpu_element_query", int(kind), int(query), int(gp), _ptr(out, ctypes.c_double))
         return out
 
     def solid81_gp_state(self, n_solid):

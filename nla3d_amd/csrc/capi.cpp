@@ -440,6 +440,33 @@ int nla3d_gpu_plane41_element_matrices(nla3d_gpu_t h, double* out) {
   });
 }
 
+int nla3d_gpu_element_query(nla3d_gpu_t h, int kind, int query, int gp, double* out) {
+  return guard(h, [&](Ctx& c) {
+    c.requireSolid();
+    if (!out) throw ApiError(NLA3D_GPU_EINVAL, "NULL");
+    const bool hex = c.model.upKind == ELEM_SOLID81;
+    const int ngp = hex ? 8 : 4;
+    if (gp < -1 || gp >= ngp) throw ApiError(NLA3D_GPU_EINVAL, "Gauss point out of range");
+    // the queries each element answers (SOLID81.cpp:191-345, PLANE41.cpp:153-300); PLANE41's
+    // IC indexes past its argument (PLANE41.cpp:205-207) and is not offered
+    bool ok;
+    if (kind == 0) ok = hex ? (query == 1 || query == 3 || query == 4 || query == 5) : query == 1;
+    else if (kind == 1) ok = hex && query == 1;
+    else if (kind == 2) ok = query >= 1 && query <= 4;
+    else ok = false;
+    if (!ok) throw ApiError(NLA3D_GPU_EINVAL, "query not answered by this element type");
+    const int width = kind == 0 ? 1 : (kind == 1 ? 3 : 6);
+    const size_t nS = c.model.solidCount;
+    DevBuf buf;
+    buf.alloc(nS * width * 8);
+    launch_element_query(c.stream, hex, (int)nS, c.d_xyz.as<double>(), c.d_conn.as<int>(), c.d_nodeEq3.as<int>(),
+                         c.d_elemEqP.as<int>(), c.d_U.as<double>(), c.mat, kind, query, gp, buf.as<double>());
+    hipCheck(hipGetLastError(), "query kernel launch");
+    hipCheck(hipMemcpyAsync(out, buf.p, buf.bytes, hipMemcpyDeviceToHost, c.stream), "copy query");
+    hipCheck(hipStreamSynchronize(c.stream), "sync");
+  });
+}
+
 int nla3d_gpu_time_assembly(nla3d_gpu_t h, int iters, double* ms_element, double* ms_scatter) {
   return guard(h, [&](Ctx& c) {
     c.requireSolid();

@@ -39,6 +39,13 @@ void launch_plane41(hipStream_t stream, int nE, const double* xyz, const int* co
                     const int* elem_eq, const double* U, const MaterialParams& mat, double* Ke);
 void plane41_slot_pairs(int* la, int* lb);
 
+// Element result queries (query.hip): per u/p element, kind 0 scalar / 1 vector / 2 tensor with
+// the reference's query numbers; gp = Gauss point or -1 (GP_MEAN); out[nE * width],
+// width 1 / 3 / 6.  conn holds 8 (hex) or 4 nodes per element.
+void launch_element_query(hipStream_t stream, bool hex, int nE, const double* xyz, const int* conn, const int* node_eq,
+                          const int* elem_eq, const double* U, const MaterialParams& mat, int kind, int query, int gp,
+                          double* out);
+
 // One model of a batched assembly (device pointers of that model's buffers).
 struct Solid81Batch {
   int nE;

@@ -209,6 +209,142 @@ def plane41_slot_pairs():
     return pairs
 
 
+# ---------------- element result queries (Element::getScalar/getVector/getTensor) ----------------
+# query numbers as query.h: scalar SP 1, WU 3, WP 4, VOL 5; vector IC 1; tensor COUCHY 1, PK2 2, E 3, C 4
+def _ic_c(C):
+    """solidmech::IC_C (solidmech.cpp:33-38): I1, I2 and I3 = J_C = sqrt(det C)"""
+    I1 = C[0] + C[3] + C[5]
+    I2 = C[0] * C[3] + C[3] * C[5] + C[0] * C[5] - C[1] * C[1] - C[4] * C[4] - C[2] * C[2]
+    J = math.sqrt(C[0] * (C[3] * C[5] - C[4] * C[4]) - C[1] * (C[1] * C[5] - C[4] * C[2])
+                  + C[2] * (C[1] * C[4] - C[3] * C[2]))
+    return I1, I2, J
+
+
+def _inv_c(C, J):
+    """solidmech::invC_C (solidmech.cpp:21-30)"""
+    oo = 1.0 / (J * J)
+    return [oo * (C[3] * C[5] - C[4] * C[4]), oo * (C[2] * C[4] - C[1] * C[5]), oo * (C[1] * C[4] - C[2] * C[3]),
+            oo * (C[0] * C[5] - C[2] * C[2]), oo * (C[1] * C[2] - C[0] * C[4]), oo * (C[0] * C[3] - C[1] * C[1])]
+
+
+def _W(mtype, MC, I1, I2):
+    """Mat_Comp_*::W (materials_hyperelastic.cpp:150-200)"""
+    if mtype == 1:
+        return 0.5 * MC[0] * (I1 - 3.0)
+    if mtype == 2:
+        return (MC[0] * (I1 - 3.0) + MC[1] * (I1 - 3.0) * (I1 - 3.0) + MC[2] * (I1 - 3.0) * (I1 - 3.0) * (I1 - 3.0)
+                + MC[3] * (I2 - 3.0))
+    return MC[0] * (I1 - 3.0) + MC[1] * (I2 - 3.0)
+
+
+def _btdb(B, D, coef, R):
+    """matBTDBprod (math/Mat.h:687-741): R (packed upper 3x3) += coef B^T D B, D packed upper"""
+    A = [[0.0] * 3 for _ in range(3)]
+    Dp = 0
+    for k in range(3):  # B^T times the upper part of D, diagonal included
+        for j in range(k, 3):
+            for i in range(3):
+                A[i][j] += B[k][i] * D[Dp]
+            Dp += 1
+    Dp = 0
+    for j in range(3):  # B^T times the strict upper part of D transposed
+        Dp += 1
+        for k in range(j + 1, 3):
+            for i in range(3):
+                A[i][j] += B[k][i] * D[Dp]
+            Dp += 1
+    r = 0
+    for i in range(3):
+        for j in range(i, 3):
+            for k in range(3):
+                R[r] += A[i][k] * B[k][j] * coef
+            r += 1
+
+
+def element_query(etype, X, U, p, mtype, MC, kind, query, gp=-1):
+    """getScalar / getVector / getTensor of one SOLID81 (SOLID81.cpp:191-345) or PLANE41
+    (PLANE41.cpp:153-300) element at update() state (U node-major, p the element pressure);
+    gp = Gauss point or -1 (GP_MEAN: each point weighted by intWeight / volume).  None where the
+    reference element returns false (and for PLANE41 IC, which indexes past its argument)."""
+    hexa = etype == SOLID81
+    if hexa:
+        NiXj, det = hex8_jacob(X)
+        O, Cs, Ss = solid81_update(NiXj, U, p, mtype, MC)
+    else:
+        NiXj, det = quad4_jacob(X)
+        U = np.asarray(U, dtype=np.float64)
+        O = np.zeros((4, 4))
+        Cs = np.zeros((4, 6))
+        for np_ in range(4):  # update (PLANE41.cpp:136-149): O = Bomega U, C from O
+            N = NiXj[np_]
+            o = [sum(N[a][0] * U[2 * a] for a in range(4)), sum(N[a][1] * U[2 * a] for a in range(4)),
+                 sum(N[a][0] * U[2 * a + 1] for a in range(4)), sum(N[a][1] * U[2 * a + 1] for a in range(4))]
+            O[np_] = o
+            cxx = 1.0 + 2 * o[0] + 1.0 * (o[0] * o[0] + o[2] * o[2])
+            cyy = 1.0 + 2 * o[3] + 1.0 * (o[3] * o[3] + o[1] * o[1])
+            cxy = o[1] + o[2] + o[0] * o[1] + o[2] * o[3]
+            Cs[np_] = [cxx, cxy, 0.0, cyy, 0.0, 1.0]  # CVec
+    ngp = 8 if hexa else 4
+    vol = float(sum(det))  # volume(): Gauss weights 1
+    width = (1, 3, 6)[kind]
+    if hexa:
+        ok = {0: (1, 3, 4, 5), 1: (1,), 2: (1, 2, 3, 4)}[kind]
+    else:
+        ok = {0: (1,), 1: (), 2: (1, 2, 3, 4)}[kind]
+    if query not in ok:
+        return None
+    R = [0.0] * width
+    pts = range(ngp) if gp < 0 else [gp]
+    for g in pts:
+        scale = det[g] / vol if gp < 0 else 1.0
+        C = list(Cs[g])
+        if kind == 0:
+            if query == 1:
+                R[0] += p * scale
+            elif query == 3:
+                I1, I2, J = _ic_c(C)
+                R[0] += _W(mtype, MC, I1, I2) * scale
+            elif query == 4:
+                J = _ic_c(C)[2]
+                R[0] += 0.5 * mat_K(mtype, MC) * (J - 1.0) * (J - 1.0) * scale
+            else:
+                R[0] += vol * scale
+        elif kind == 1:
+            I1, I2, J = _ic_c(C)
+            R[0] += I1 * scale
+            R[1] += I2 * scale
+            R[2] += J * scale
+        elif query == 4:
+            for v in range(6):
+                R[v] += C[v] * scale
+        elif query == 3:
+            for v in range(6):
+                R[v] += (C[v] - (1.0 if v in (0, 3, 5) else 0.0)) * 0.5 * scale
+        elif hexa:
+            J = _ic_c(C)[2]
+            ci = _inv_c(C, J)
+            S = [Ss[g][v] + p * J * ci[v] for v in range(6)]
+            if query == 2:
+                for v in range(6):
+                    R[v] += S[v] * scale
+            else:  # matF holds F^T (SOLID81.cpp:284-293): R += scale/J F S F^T
+                o = O[g]
+                Ft = [[1 + o[0], o[3], o[6]], [o[1], 1 + o[4], o[7]], [o[2], o[5], 1 + o[8]]]
+                _btdb(Ft, S, 1.0 / J * scale, R)
+        else:
+            S = list(getS_UP(mtype, MC, C, p))
+            if query == 2:
+                for v in range(6):
+                    R[v] += S[v] * scale
+            else:  # matF = F (PLANE41.cpp:247-252), no weight
+                o = O[g]
+                F = [[1 + o[0], o[1], 0.0], [o[2], 1 + o[3], 0.0], [0.0, 0.0, 1.0]]
+                J = (F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) - F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0])
+                     + F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]))
+                _btdb(F, S, 1.0 / J, R)
+    return np.array(R)
+
+
 def lineth_element(X, flux, htc, e0, e1):
     X = _f64(X, (2, 3)); Ke = np.zeros(3); Fe = np.zeros(2)
     lib().orc_lineth_element(_p(X), flux, htc, e0, e1, _p(Ke), _p(Fe))
