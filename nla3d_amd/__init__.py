@@ -238,6 +238,7 @@ class GpuModel:
     def set_nodes(self, xyz):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         self._call("nla3d_gpu_set_nodes", len(xyz), _ptr(xyz, ctypes.c_double))
+        self.xyz = xyz.copy()
 
     def add_elements(self, kind, conn, params=None):
         conn = np.ascontiguousarray(conn, dtype=np.uint32)
@@ -247,6 +248,9 @@ class GpuModel:
             params = np.ascontiguousarray(params, dtype=np.float64).reshape(n, 4)
             pp = _ptr(params, ctypes.c_double)
         self._call("nla3d_gpu_add_elements", kind, n, _ptr(conn, ctypes.c_uint32), pp)
+        if not hasattr(self, "cells"):
+            self.cells = []  # (kind, 1-based connectivity [n, nodes]) in the order added
+        self.cells.append((int(kind), conn.reshape(n, -1).copy()))
 
     def set_material(self, mtype, consts):
         mc = np.ascontiguousarray(consts, dtype=np.float64)

@@ -70,3 +70,27 @@ def test_plane41_queries(gpu_lib, gp):
             m.element_query(len(conn), kind, query, gp)
     with pytest.raises(nla3d_amd.Nla3dGpuError):
         m.element_query(len(conn), 2, 1, 4)  # Gauss point out of range
+
+
+def test_vtk_file_of_a_solved_plane_model(gpu_lib, tmp_path):
+    """write_vtk on a device model after a Newton load step: displacements, reactions and the
+    device queries appear under their labels (nla3d_amd.vtk, VtkProcessor.cpp:110-297)"""
+    from nla3d_amd.vtk import _num, write_vtk
+    print("[gpu] vtk output of a PLANE41 model", flush=True)
+    m, o, xyz, conn = plane_models((1, [1.0, 500.0]), top_uy=-0.1)
+    m.solve_load_step(1.0, m.newton_defaults(criterion=1, convergence=1e-10, krylov_tol=1e-13,
+                                             krylov_max_iter=20000, max_iterations=20))
+    path = tmp_path / "plate.vtk"
+    write_vtk(str(path), m)
+    lines = path.read_text().split("\n")
+    U, R = m.get_solution(), m.get_R()
+    ne, ee = m.equation_numbers()
+    d = lines.index("VECTORS disp float")
+    r = lines.index("VECTORS reaction float")
+    for n in range(len(xyz)):
+        assert lines[d + 1 + n] == " ".join(_num(U[ne[n, k] - 1]) if ne[n, k] else "0" for k in range(3))
+        assert lines[r + 1 + n] == " ".join(_num(R[ne[n, k] - 1]) if ne[n, k] else "0" for k in range(3))
+    c = lines.index("SCALARS COUCHY_XY float 1")
+    want = m.element_query(len(conn), 2, 1, -1)[:, 1]
+    assert lines[c + 2:c + 2 + len(conn)] == [_num(v) for v in want]
+    assert "SCALARS IC float 1" not in lines and "VECTORS IC float" not in lines
