@@ -548,6 +548,7 @@ class OracleModel:
         self.initDofs()
         self.assignEquationNumbers()
         self.initSolutionData()
+        self._p41 = None  # positions of the batched PLANE41 assembly, rebuilt on first use
         return self
 
     # BlockSparseSymMatrix::addValue (BlockSparseMatrix.h:155-168)
