@@ -397,16 +397,23 @@ __global__ __launch_bounds__(256) void cg_sell_spmv_kernel(CgSell A, const doubl
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// x += alpha p ; r -= alpha q ; z = D^-1 r ; partial r . z
+// alpha = rz / (p . q) from the SpMV's partials (every workgroup sums them in the same
+// order, so all hold the same alpha); x += alpha p ; r -= alpha q ; z = D^-1 r ; partial r . z
+// into a second partial buffer (the first is still being read by other workgroups)
 __global__ __launch_bounds__(256) void cg_jacobi_update_kernel(int n, const double* __restrict__ dinv,
                                                               double* __restrict__ x, double* __restrict__ r,
                                                               const double* __restrict__ p,
                                                               const double* __restrict__ q, double* __restrict__ z,
                                                               const CgState* __restrict__ st,
+                                                              const double* __restrict__ pqPartial, int nPart,
                                                               double* __restrict__ partial) {
   __shared__ double sh[4];
+  __shared__ double alpha;
   if (st->converged) return;
-  const double a = st->alpha;
+  const double pq = finish(pqPartial, nPart, sh);
+  if (threadIdx.x == 0) alpha = pq != 0.0 ? st->rz / pq : 0.0;
+  __syncthreads();
+  const double a = alpha;
   double contrib = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     x[i] += a * p[i];
@@ -473,8 +480,9 @@ static int cg_solve_t(hipStream_t s, const KrylovMatrix& M, const Precond& P, Cg
   return hstate->itn;
 }
 
-// scalar rows: two kernels per iteration besides the two one-block scalar steps (the
-// search direction is formed inside the SpMV, the Jacobi pass reads one inverse per row)
+// scalar rows: two kernels per iteration besides one one-block scalar step (the search
+// direction is formed inside the SpMV, alpha inside the Jacobi pass, which reads one inverse
+// per row)
 static int cg_solve_sell(hipStream_t s, const KrylovMatrix& M, const Precond& P, CgWork& w, const double* b, double* x,
                          double tol, int maxit, CgState* hstate, const CgSell& A) {
   const int nb = kKrylovBlocks;
@@ -497,10 +505,9 @@ static int cg_solve_sell(hipStream_t s, const KrylovMatrix& M, const Precond& P,
       else
         hipLaunchKernelGGL(cg_sell_spmv_kernel<false>, dim3(nb), dim3(256), 0, s, A, w.z, pa, pb, w.q, w.st,
                            w.partial);
-      hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(256), 0, s, w.st, w.partial, nb);
       hipLaunchKernelGGL(cg_jacobi_update_kernel, dim3(nb), dim3(256), 0, s, A.n, A.dinv, x, w.r, pb, w.q, w.z, w.st,
-                         w.partial);
-      hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(256), 0, s, w.st, w.partial, nb, tol);
+                         w.partial, nb, w.partial2);
+      hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(256), 0, s, w.st, w.partial2, nb, tol);
       std::swap(pa, pb);
     }
     if (hipMemcpyAsync(hstate, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
