@@ -311,8 +311,8 @@ def thermal_mode(args):
                                f"nnz(block2 upper) {nnz}, dt 1e-3", "mode": "thermal",
                    "cg_iterations_per_step": its / args.steps, "ms_per_cg_iteration": ms_it},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
-                     "traffic": None, "kernel": "one Jacobi-CG iteration (4 launches: sliced-ELLPACK SpMV that also forms "
-                                                          "the search direction, a one-block scalar step, the Jacobi update, "
+                     "traffic": None, "kernel": "one Jacobi-CG iteration (3 launches: sliced-ELLPACK SpMV that also forms "
+                                                          "the search direction, the Jacobi update that also sums alpha, "
                                                           "a one-block scalar step)", "algorithmic_bytes": it_bytes},
         "cpu_baseline": None,
     }
