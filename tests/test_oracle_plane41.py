@@ -93,3 +93,24 @@ def a2000_curve(gold):
                 if not out or v != out[-1]:
                     out.append(v)
     return np.array(out)
+
+
+def test_a2000_cdb_records():
+    """readCdbFile on the a2000_damper model (FEReaders.cpp:321-729): the mesh, the three node
+    components and the D records the benchmark runs on"""
+    import os
+
+    from nla3d_amd.cdb import read_cdb
+
+    d = read_cdb(os.path.join(os.path.dirname(__file__), "golden", "a2000_damper", "model.cdb.gz"))
+    assert len(d["nodes"]) == 3225 and len(d["cells"]) == 3101
+    assert all(t == 1 and len(n) == 4 for t, n in d["cells"])
+    assert np.abs(np.array(d["nodes"])[:, 2]).max() == 0.0
+    comps = {k: set(v) for k, v in d["comps"].items()}
+    assert {k: len(v) for k, v in comps.items()} == {"DISP_BC": 79, "FIX_BC": 73, "SYM_BC": 7}
+    fixes = {(n, dof): v for n, dof, v in d["fixes"]}
+    assert len(fixes) == len(d["fixes"]) == 308
+    assert all(fixes.get((n, 1)) == -9.0 for n in comps["DISP_BC"])
+    assert all(fixes.get((n, 0)) == 0.0 and fixes.get((n, 1)) == 0.0 for n in comps["FIX_BC"])
+    assert all(fixes.get((n, 0)) == 0.0 for n in comps["SYM_BC"])
+    assert not d["mpcs"]
