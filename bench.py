@@ -8,33 +8,42 @@ Workload (synthetic, seeded): [0,2]x[0,2]x[0,1] box, 120x120x60 hexahedra
 face fixed, top face tied by a RigidBodyMpc to a master node carrying +15% UZ
 and a 20 degree ROTZ, ramped over 20 load steps (TimeControl).
 
-Preconditioner: a per-node MLP correction of block Jacobi is trained first
-(2,000 tangent/rhs pairs from Newton iterations of seeded 8..16-element-wide
-training boxes, checkpointed on a 24x24x12 validation system), then block
-Jacobi judges it on the system of the first (untimed) Newton iteration; the
-timed iterations use the learned corrections only if they saved more than
---judge-margin of the Krylov iterations there, plain block Jacobi otherwise.
-Both counts are reported ("judge").
+Preconditioner: block Jacobi on the node DoFs and one V-cycle of the Schur-row
+multigrid on the pressure and multiplier rows (default --preconditioner
+schur-multigrid).  A per-node MLP correction of the block-Jacobi node blocks is
+trained first (2,000 tangent/rhs pairs from Newton iterations of seeded
+8..16-element-wide training boxes, checkpointed on a 24x24x12 validation
+system), then judged on the system of the first (untimed) Newton iteration
+against the same method without the network and against plain block Jacobi;
+the timed iterations use the network only if it saved more than
+--judge-margin of the Krylov iterations there.  All counts are reported
+("judge").
 
 A step (mode "newton", the default) is one equilibrium iteration of
 NonlinearFESolver::solve (FESolver.cpp:443-490), all on the device with the
 model resident in HBM: update + buildK of every element, deterministic
 scatter into the three CSR blocks and F, rigid-body MPC rows, boundary
 conditions, rhs with the eliminated constrained DoFs, the block(2) solve
-(MINRES + block Jacobi to a 1e-9 relative preconditioned residual) and the
-increment update.  Newton runs to a 1e-8 relative residual; iterations that
-only confirm convergence (an assembly without a solve) are timed but not
-counted as steps.  value = elements / s = 864,000 / (Newton-iteration wall).
-Mode "assemble" times only the assembly (update + buildK + scatter + MPC).
+(MINRES to a 1e-9 relative preconditioned residual) and the increment update.
+Newton runs to a 1e-8 relative residual; iterations that only confirm
+convergence (an assembly without a solve) are timed but not counted as steps.
+value = elements / s = 864,000 / (Newton-iteration wall).
 
-roofline: the dominant kernel of a Newton iteration, the Krylov SpMV over the
-full-symmetric block(2) CSR (HBM-bound), timed with HIP events on the
-library's stream; algorithmic bytes are those of a symmetric (upper) CSR
-SpMV.  The element kernel is reported beside it (FP64-bound).
-cpu_baseline: the reference assembly loop (oracle/nla3d_oracle.c: update +
-buildK + binary-search addValue, one core) on a bounded sample of the same
-model's elements, in elements/s; it excludes the reference's PARDISO solve
-(unavailable), so it is an upper bound on the CPU path's Newton-iteration rate.
+roofline: the dominant kernel of a Newton iteration, timed with HIP events on
+the library's stream: the finest-level Chebyshev smoother step of the Schur
+multigrid (HBM-bound; traffic from the committed PMC summary), or the block(2)
+SpMV without the multigrid.  The SpMV and the element kernel (FP64-bound) are
+reported beside it.
+cpu_baseline: one reference Newton iteration on one core = the reference
+assembly loop (oracle/nla3d_oracle.c: update + buildK + binary-search
+addValue) timed on a bounded sample of the same model's elements, plus the
+direct solve of K.block(2) (SuperLU, one thread, in place of the absent
+PARDISO) timed on C3-like boxes and extrapolated to this size (labelled).
+
+Other modes: "assemble" (the assembly alone), "probe" (short kernel runs for
+hardware-counter passes), "thermal" (QUADTH transient, 1024x1024), "batch"
+(256 independent models), "damper" (the reference's a2000_damper and
+3d_damper BENCH tests end to end, with the oracle's a2000_damper run beside).
 """
 import argparse
 import ctypes
@@ -264,6 +273,105 @@ def batch_mode(args):
     print(json.dumps(out))
 
 
+def _ansys_curve(path):
+    """readRefCurveData (main.cpp:342-361): third column, consecutive repeats dropped"""
+    out = []
+    with open(path) as f:
+        next(f)
+        for line in f:
+            if line.split():
+                v = float(line.split()[2])
+                if not out or v != out[-1]:
+                    out.append(v)
+    return np.array(out)
+
+
+def damper_mode(args):
+    """The reference's two BENCH tests (test/CMakeLists.txt:142-153) end to end on the device:
+    a2000_damper (3,101 PLANE41, Neo-Hookean G = 1, K = 500, DISP_BC UY reaction) and
+    3d_damper (7,016 SOLID81 + 497 CE constraints, G = 10, K = 5000, TOP_SIDE UY reaction),
+    20 load steps each with the reference's convergence rule (criterion 0, 1e-3), MINRES to
+    1e-10 with block Jacobi + the Schur multigrid.  A step is one run of both; the reaction
+    curves are compared with ANSYS's (mean absolute difference, main.cpp:326-338).  The CPU
+    baseline is the oracle's Newton loop on a2000_damper (SuperLU direct solves, one thread)."""
+    import nla3d_amd
+    from nla3d_amd.cdb import read_cdb
+    from oracle.pyoracle import load_steps
+    gold = os.path.join(ROOT, "tests", "golden")
+    cases = [("a2000_damper", nla3d_amd.PLANE41, (1.0, 500.0), "DISP_BC", 0.2),
+             ("3d_damper", nla3d_amd.SOLID81, (10.0, 5000.0), "TOP_SIDE", 3.0)]
+    data = {name: read_cdb(os.path.join(gold, name, "model.cdb.gz")) for name, *_ in cases}
+
+    def run(name, kind, mc, comp):
+        d = data[name]
+        t0 = time.perf_counter()
+        m = nla3d_amd.GpuModel(0)
+        m.set_nodes(d["nodes"])
+        m.add_elements(kind, [n for t, n in d["cells"] if t == 1])
+        m.set_material(nla3d_amd.NEO_HOOKEAN, list(mc))
+        for n, dof, v in d["fixes"]:
+            m.add_fix(n, dof, v)
+        for terms, b in d["mpcs"]:
+            m.add_mpc(terms, b)
+        m.init_solution()
+        opts = m.newton_defaults(criterion=0, convergence=1e-3, max_iterations=15, krylov_tol=1e-10,
+                                 krylov_max_iter=20000, preconditioner=2)
+        ne, _ = m.equation_numbers()
+        eqs = np.array([int(ne[n - 1, 1]) for n in d["comps"][comp]]) - 1
+        curve, newton, krylov = [0.0], 0, 0
+        for t in load_steps(20):
+            st = m.solve_load_step(t, opts)
+            curve.append(float(m.get_R()[eqs].sum()))
+            newton += st.last_step_iterations
+            krylov += st.krylov_iterations
+        wall = time.perf_counter() - t0
+        err = float(np.abs(np.array(curve) - _ansys_curve(os.path.join(gold, name, "loading_curve_ansys.txt"))).mean())
+        m.close()
+        return wall, err, newton, krylov
+
+    for _ in range(args.warmup):
+        run(*cases[0][:4])
+    res = {name: [] for name, *_ in cases}
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        for name, kind, mc, comp, bar in cases:
+            w, err, nw, kr = run(name, kind, mc, comp)
+            res[name].append((w, err, nw, kr))
+            print(f"[bench] {name}: {w:.2f}s, reaction curve mean abs {err:.4f} (bar {bar}), "
+                  f"{nw} Newton iterations", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / args.steps
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle as po
+        d = data["a2000_damper"]
+        t1 = time.perf_counter()
+        om = po.OracleModel(np.array(d["nodes"]), [(po.PLANE41, list(n)) for t, n in d["cells"] if t == 1],
+                            material=(1, [1.0, 500.0]), fixes=d["fixes"]).setup()
+        om.newton_init()
+        for t in po.load_steps(20):
+            om.solve_load_step(t, criterion=0, convergence=1e-3, max_iterations=15)
+        cpu_s = time.perf_counter() - t1
+        cpu = {"value": cpu_s, "unit": "s per a2000_damper run", "cores": 1, "kind": "port",
+               "sample": "the whole a2000_damper run (20 load steps) through the oracle's Newton loop: "
+                         "vectorised PLANE41 assembly over the C element routine, SuperLU direct solves"}
+    a, b = res["a2000_damper"], res["3d_damper"]
+    out = {
+        "metric": "reference BENCH tests a2000_damper + 3d_damper end to end (20 load steps each); wall s",
+        "value": ms_step / 1e3, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64", "data": "the reference's own models (tests/golden)",
+        "config": {"workload": "a2000_damper (3,101 PLANE41) then 3d_damper (7,016 SOLID81, 497 CE rows)",
+                   "mode": "damper",
+                   "a2000_damper": {"seconds": [r[0] for r in a], "curve_mean_abs": a[-1][1], "bar": 0.2,
+                                    "newton_iterations": a[-1][2], "krylov_iterations": a[-1][3]},
+                   "3d_damper": {"seconds": [r[0] for r in b], "curve_mean_abs": b[-1][1], "bar": 3.0,
+                                 "newton_iterations": b[-1][2], "krylov_iterations": b[-1][3]}},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def thermal_mode(args):
     """QUADTH transient: 1024 x 1024 quads on the unit square (1,050,625 nodes),
     conductivity exp(N(0, 0.5^2)) per element, rho c = 1, T = 1 on the left edge,
@@ -324,7 +432,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["newton", "assemble", "probe", "batch", "thermal"], default="newton")
+    ap.add_argument("--mode", choices=["newton", "assemble", "probe", "batch", "thermal", "damper"],
+                    default="newton")
     ap.add_argument("--batch-models", type=int, default=256)
     ap.add_argument("--shape", type=str, default="120,120,60")
     ap.add_argument("--load-steps", type=int, default=20)
@@ -354,6 +463,8 @@ def main():
         return batch_mode(args)
     if args.mode == "thermal":
         return thermal_mode(args)
+    if args.mode == "damper":
+        return damper_mode(args)
     nx, ny, nz = (int(v) for v in args.shape.split(","))
     t_setup = time.perf_counter()
     m = nla3d_amd.GpuModel(0)
