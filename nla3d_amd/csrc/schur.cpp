@@ -22,13 +22,22 @@ constexpr int kMaxQPerNode = 64;
 // Jacobi-Chebyshev smoothing on [lmax / div, 1.05 lmax] with `steps` terms.  The finest level
 // gets a long polynomial over a wide interval: S's spectrum holds many small eigenvalues
 // (checkerboard pressure modes, whose smooth envelopes no piecewise-constant aggregate
-// represents), so a short smoother leaves them all to the coarse grid.  Coarser levels
-// keep a short one.
-constexpr int kSteps0 = 16;
-constexpr double kLoDiv0 = 400.0;
+// represents), so a short smoother leaves them all to the coarse grid.  How long pays off
+// grows with the matrix: finestSteps() gives 8 terms on [lmax / 100, .] up to 10^4 rows,
+// rising with log10(rows) to kSteps0 = 16 terms on [lmax / 400, .] at 10^6 rows (the interval
+// widening with the square of the length).  Coarser levels keep a short one.
+constexpr int kSteps0 = 16;  // longest finest-level smoother
+constexpr int kSteps0Min = 8;
 constexpr int kStepsC = 3;
 constexpr double kLoDivC = 10.0;
 constexpr int kPowerSteps = 40;     // power iteration for lmax of D^-1 A
+
+// finest-level smoother length and interval for a Schur matrix of n rows
+static int finestSteps(int n) {
+  const double s = kSteps0Min + 4.0 * std::log10(std::max(n, 1) / 1.0e4);
+  return std::max(kSteps0Min, std::min(kSteps0, (int)std::lround(s)));
+}
+static double finestDiv(int steps) { return 100.0 * (steps / 8.0) * (steps / 8.0); }
 constexpr double kLmaxSafety = 1.05;  // the estimate approaches lmax from below
 
 struct AmgLevel {
@@ -465,8 +474,8 @@ static void vcycle(SchurAmg& S, hipStream_t s, size_t l) {
     launch_dense_matvec(s, A.n, S.cinv.as<double>(), A.r.as<double>(), A.x.as<double>());
     return;
   }
-  const int steps = l == 0 ? kSteps0 : kStepsC;
-  const ChebCoef k = chebCoef(A.lmax, steps, l == 0 ? kLoDiv0 : kLoDivC);
+  const int steps = l == 0 ? finestSteps(A.n) : kStepsC;
+  const ChebCoef k = chebCoef(A.lmax, steps, l == 0 ? finestDiv(steps) : kLoDivC);
   const AmgSell M = A.sell();
   double* dinv = A.dinv.as<double>();
   double* din = A.da.as<double>();
@@ -586,7 +595,8 @@ int nla3d_gpu_time_smoother(nla3d_gpu_t h, int iters, double* ms, long long* byt
     SchurAmg& S = *c.schur;
     if (S.L.size() < 2) throw ApiError(-3, "the hierarchy has no smoothed level");
     AmgLevel& A = *S.L[0];
-    const ChebCoef k = chebCoef(A.lmax, kSteps0, kLoDiv0);
+    const int steps0 = finestSteps(A.n);
+    const ChebCoef k = chebCoef(A.lmax, steps0, finestDiv(steps0));
     const AmgSell M = A.sell();
     hipStream_t s = c.stream;
     double* din = A.da.as<double>();
@@ -603,7 +613,7 @@ int nla3d_gpu_time_smoother(nla3d_gpu_t h, int iters, double* ms, long long* byt
     hipCheck(hipEventCreate(&e1), "event");
     hipCheck(hipEventRecord(e0, s), "rec");
     for (int i = 0; i < iters; i++) {
-      const int j = 1 + i % (kSteps0 - 1);
+      const int j = 1 + i % (steps0 - 1);
       launch_cheb_step(s, M, din, A.dinv.as<double>(), k.c1[j], k.c2[j], A.res.as<double>(), dout,
                        A.x.as<double>());
       std::swap(din, dout);

@@ -17,6 +17,8 @@ Pinned by: a one-level hierarchy gives S^-1 exactly; the cycle is a symmetric
 positive-definite operator; the smoother alone reduces the energy-norm error; S is
 symmetric positive definite on the models of the tests.
 """
+import math
+
 import numpy as np
 import scipy.sparse as sp
 
@@ -37,8 +39,20 @@ def schur_matrix(K2, groups, bj):
     return q, (Bq.T @ Dinv @ Bq - C).toarray()
 
 
-STEPS0, DIV0 = 16, 400.0   # finest level
+STEPS0, DIV0 = 16, 400.0   # finest level of a 10^6-row matrix (the longest smoother)
 STEPSC, DIVC = 3, 10.0     # coarser levels
+
+
+def finest_steps(n):
+    """finest-level smoother length for n rows: 8 terms up to 10^4 rows, + 4 per decade, at
+    most STEPS0 (the library's finestSteps)"""
+    s = 8 + 4.0 * math.log10(max(n, 1) / 1.0e4)
+    return max(8, min(STEPS0, int(math.floor(s + 0.5))))
+
+
+def finest_div(steps):
+    """the interval [lmax / div, 1.05 lmax] widens with the square of the length"""
+    return 100.0 * (steps / 8.0) ** 2
 
 
 def cheb_coef(lmax, steps, div=10.0):
@@ -89,7 +103,8 @@ def vcycle(mats, aggs, lmaxs, r, level=0):
     A = mats[level]
     if level == len(mats) - 1:
         return np.linalg.solve(A.toarray(), r)
-    steps, div = (STEPS0, DIV0) if level == 0 else (STEPSC, DIVC)
+    steps = finest_steps(A.shape[0]) if level == 0 else STEPSC
+    div = finest_div(steps) if level == 0 else DIVC
     dinv = 1.0 / A.diagonal()
     As = rounded_f32(A)
     x = smooth(As, dinv, r, np.zeros_like(r), lmaxs[level], steps, div)

@@ -48,7 +48,7 @@ def test_chebyshev_error_polynomial_is_the_minimax_one():
     (for both the finest-level and the coarse-level settings)"""
     lmax = 2.0
     mu = np.linspace(1e-6, 1.05 * lmax, 4001)
-    for steps, div in ((ps.STEPS0, ps.DIV0), (ps.STEPSC, ps.DIVC)):
+    for steps, div in ((ps.STEPS0, ps.DIV0), (8, ps.finest_div(8)), (ps.STEPSC, ps.DIVC)):
         e = ps.smooth(np.diag(mu), np.ones_like(mu), np.zeros_like(mu), np.ones_like(mu), lmax, steps, div)
         hi, lo = 1.05 * lmax, lmax / div
         theta, delta = 0.5 * (hi + lo), 0.5 * (hi - lo)
@@ -60,3 +60,11 @@ def test_chebyshev_error_polynomial_is_the_minimax_one():
         inside = mu >= lo
         assert np.abs(e[inside]).max() <= 1.0 / T(theta / delta) + 1e-12
         assert np.abs(e[~inside]).max() < 1.0
+
+
+def test_finest_smoother_length_follows_the_size():
+    """8 terms on [lmax/100, .] up to 10^4 rows, 4 more per decade, 16 on [lmax/400, .] from
+    10^6 rows (the C3 Schur matrix has 907,923 rows)"""
+    assert [ps.finest_steps(n) for n in (1, 3101, 7513, 10000, 32768, 100000, 907923, 10 ** 7)] == \
+        [8, 8, 8, 8, 10, 12, 16, 16]
+    assert ps.finest_div(8) == 100.0 and ps.finest_div(16) == ps.DIV0 == 400.0
