@@ -19,6 +19,10 @@ namespace nla3d_amd {
 constexpr int kCoarsest = 256;   // at most this many rows are inverted densely
 constexpr int kMaxCoarse = 2048; // coarsening that stalls above this disables the preconditioner
 constexpr int kMaxQPerNode = 64;
+// coarse levels whose smoother copy holds at most this many slots (and coarsest levels) run
+// inside one workgroup: below it a pass is cheaper than a launch, above it (C3's 1,372-row
+// level with ~80 entries per row) one compute unit streams the level slower than the device
+constexpr int kFuseSlots = 32768;
 // Jacobi-Chebyshev smoothing on [lmax / div, 1.05 lmax] with `steps` terms.  The finest level
 // gets a long polynomial over a wide interval: S's spectrum holds many small eigenvalues
 // (checkerboard pressure modes, whose smooth envelopes no piecewise-constant aggregate
@@ -68,6 +72,9 @@ struct SchurAmg {
   std::vector<std::unique_ptr<AmgLevel>> L;
   DevBuf cinv;
   bool fresh = false;
+  // levels fuseFrom.. (each at most kFuseSlots smoother slots) run as one workgroup launch; -1: none
+  int fuseFrom = -1;
+  DevBuf fused;  // FusedLevel[L.size() - fuseFrom], refreshed with the values
 };
 
 // stable counting sort of (target, source) pairs by target: seg[t]..seg[t+1] lists the sources
@@ -340,6 +347,12 @@ void setupSchur(Ctx& c, const std::vector<int64_t>& rp, const std::vector<int>& 
   for (size_t l = 0; l < S->L.size(); l++) allocLevel(*S->L[l], l + 1 == S->L.size());
   const int ncst = S->L.back()->n;
   S->cinv.alloc((size_t)ncst * ncst * 8);
+  for (size_t l = 1; l < S->L.size(); l++)
+    if (S->L[l]->nSlots <= kFuseSlots) {  // the coarsest level has no smoother copy (0 slots)
+      S->fuseFrom = (int)l;
+      S->fused.alloc((S->L.size() - l) * sizeof(FusedLevel));
+      break;
+    }
   S->qrow.upload(S->h_qrow);
   S->nNodes = nNodes;
   S->nPairs = (int)qlist.size();
@@ -364,6 +377,27 @@ static double hostDot(Ctx& c, int n, const double* a, const double* b) {
   hipCheck(hipMemcpyAsync(&v, c.d_scal.as<double>() + 9, 8, hipMemcpyDeviceToHost, c.stream), "dot");
   hipCheck(hipStreamSynchronize(c.stream), "dot sync");
   return v;
+}
+
+// Chebyshev coefficients on [lmax/div, 1.05 lmax] (Jacobi-preconditioned, steps terms)
+struct ChebCoef {
+  double c0;
+  double c1[kSteps0], c2[kSteps0];
+};
+static_assert(kStepsC <= kSteps0, "coefficient arrays sized by the longest smoother");
+static ChebCoef chebCoef(double lmax, int steps, double div) {
+  const double hi = 1.05 * lmax, lo = lmax / div;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+  ChebCoef k{};
+  k.c0 = 1.0 / theta;
+  double rho = 1.0 / sigma;
+  for (int i = 1; i < steps && i < kSteps0; i++) {
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    k.c1[i] = rn * rho;
+    k.c2[i] = 2.0 * rn / delta;
+    rho = rn;
+  }
+  return k;
 }
 
 // values of every level for the current tangent (block-Jacobi blocks must be current);
@@ -442,33 +476,50 @@ bool schurRefresh(Ctx& c, std::string* why) {
     for (int i = 0; i < n; i++) inv[(size_t)i * n + j] = e[i];
   }
   hipCheck(hipMemcpyAsync(S.cinv.p, inv.data(), inv.size() * 8, hipMemcpyHostToDevice, s), "cinv");
+  if (S.fuseFrom > 0) {  // descriptors of the fused coarse levels with this tangent's coefficients
+    std::vector<FusedLevel> fl(S.L.size() - S.fuseFrom);
+    for (size_t l = S.fuseFrom; l < S.L.size(); l++) {
+      AmgLevel& A = *S.L[l];
+      FusedLevel& f = fl[l - S.fuseFrom];
+      std::memset(&f, 0, sizeof f);
+      f.n = A.n;
+      f.nc = A.nc;
+      f.r = A.r.as<double>();
+      f.x = A.x.as<double>();
+      if (l + 1 == S.L.size()) {
+        f.cinv = S.cinv.as<double>();
+        continue;
+      }
+      const ChebCoef k = chebCoef(A.lmax, kStepsC, kLoDivC);
+      f.steps = kStepsC;
+      f.A = A.sell();
+      f.dinv = A.dinv.as<double>();
+      f.c0 = k.c0;
+      for (int i = 1; i < kStepsC; i++) {
+        f.c1[i] = k.c1[i];
+        f.c2[i] = k.c2[i];
+      }
+      f.res = A.res.as<double>();
+      f.da = A.da.as<double>();
+      f.db = A.db.as<double>();
+      f.t = A.t.as<double>();
+      f.memPtr = A.memPtr.as<int>();
+      f.mem = A.mem.as<int>();
+      f.agg = A.agg.as<int>();
+    }
+    hipCheck(hipMemcpyAsync(S.fused.p, fl.data(), fl.size() * sizeof(FusedLevel), hipMemcpyHostToDevice, s),
+             "fused levels");
+  }
   hipCheck(hipStreamSynchronize(s), "sync");
   S.fresh = true;
   return true;
 }
 
-// Chebyshev coefficients on [lmax/div, 1.05 lmax] (Jacobi-preconditioned, steps terms)
-struct ChebCoef {
-  double c0;
-  double c1[kSteps0], c2[kSteps0];
-};
-static_assert(kStepsC <= kSteps0, "coefficient arrays sized by the longest smoother");
-static ChebCoef chebCoef(double lmax, int steps, double div) {
-  const double hi = 1.05 * lmax, lo = lmax / div;
-  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
-  ChebCoef k{};
-  k.c0 = 1.0 / theta;
-  double rho = 1.0 / sigma;
-  for (int i = 1; i < steps && i < kSteps0; i++) {
-    const double rn = 1.0 / (2.0 * sigma - rho);
-    k.c1[i] = rn * rho;
-    k.c2[i] = 2.0 * rn / delta;
-    rho = rn;
-  }
-  return k;
-}
-
 static void vcycle(SchurAmg& S, hipStream_t s, size_t l) {
+  if (l > 0 && (int)l == S.fuseFrom) {
+    launch_coarse_cycle(s, S.fused.as<FusedLevel>(), (int)(S.L.size() - l));
+    return;
+  }
   AmgLevel& A = *S.L[l];
   if (l + 1 == S.L.size()) {
     launch_dense_matvec(s, A.n, S.cinv.as<double>(), A.r.as<double>(), A.x.as<double>());

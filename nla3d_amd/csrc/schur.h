@@ -43,6 +43,23 @@ struct AmgSell {
   const float* val;
 };
 
+// One level of the fused coarse cycle (schur.hip coarse_cycle_kernel): the level's smoother
+// matrix, diagonal, Chebyshev coefficients (c1[i], c2[i] for i = 1 .. steps-1) and buffers; the
+// coarsest level (steps = 0) holds its dense inverse instead.
+constexpr int kFusedMaxSteps = 4;
+struct FusedLevel {
+  int n, nc, steps;
+  AmgSell A;
+  const double* dinv;
+  double c0, c1[kFusedMaxSteps], c2[kFusedMaxSteps];
+  double *r, *x, *res, *da, *db, *t;
+  const int *memPtr, *mem, *agg;
+  const double* cinv;  // coarsest level only
+};
+// the V-cycle below and including level 0 of `lv` (nl levels, the last the coarsest) in one
+// workgroup: every level must be small (one workgroup walks all its rows per pass)
+void launch_coarse_cycle(hipStream_t s, const FusedLevel* lv, int nl);
+
 // level-0 assembly of S: one thread per (node, Schur column a) writes B(:,a)^T D^-1 B(:,b)
 void launch_schur_contrib(hipStream_t s, int nPairs, const int* pairNode, const int* nodeGroup, const int* nodeSize,
                           const int* qPtr, const int* cPtr, const int* posB, const double* kval, const double* binv,

@@ -233,6 +233,88 @@ __global__ __launch_bounds__(256) void dinv_spmv_kernel(AmgCsr A, const double* 
 inline int rowGrid(int n) { return std::min(kGrid, std::max(1, cdiv(n, 256 / kLanes))); }
 inline int elemGrid(int n) { return std::min(kGrid, std::max(1, cdiv(n, 256))); }
 
+// ---- the coarse levels of the V-cycle in one workgroup --------------------------------
+// The same per-row arithmetic as the one-level kernels above (cheb_start, cheb_step,
+// residual, restrict, dense_matvec, prolong, cheb_start_x, the final x += d), phase by phase
+// with a workgroup barrier between phases: for small levels the cycle is bound by launch
+// latency, not by the rows.
+constexpr int kCoarseThreads = 1024;
+
+__device__ __forceinline__ double sell_dot_any(const AmgSell& A, int i, const double* __restrict__ x) {
+  return A.col16 ? sell_dot<true>(A, i, x) : sell_dot<false>(A, i, x);
+}
+
+__device__ void coarse_smooth(const FusedLevel& L, bool fromZero) {
+  double* din = L.da;
+  double* dout = L.db;
+  for (int i = threadIdx.x; i < L.n; i += blockDim.x) {
+    const double ri = fromZero ? L.r[i] : L.r[i] - sell_dot_any(L.A, i, L.x);
+    L.res[i] = ri;
+    din[i] = L.c0 * L.dinv[i] * ri;
+    if (fromZero) L.x[i] = 0.0;
+  }
+  __syncthreads();
+  for (int k = 1; k < L.steps; k++) {
+    for (int i = threadIdx.x; i < L.n; i += blockDim.x) {
+      const double ri = L.res[i] - sell_dot_any(L.A, i, din);
+      L.res[i] = ri;
+      const double di = din[i];
+      dout[i] = L.c1[k] * di + L.c2[k] * L.dinv[i] * ri;
+      L.x[i] += di;
+    }
+    __syncthreads();
+    double* tmp = din;
+    din = dout;
+    dout = tmp;
+  }
+}
+
+// the buffer holding the last smoothing increment after coarse_smooth (da or db by parity)
+__device__ __forceinline__ double* last_increment(const FusedLevel& L) { return ((L.steps - 1) & 1) ? L.db : L.da; }
+
+__global__ __launch_bounds__(kCoarseThreads) void coarse_cycle_kernel(const FusedLevel* __restrict__ lv, int nl) {
+  // descend
+  for (int l = 0; l + 1 < nl; l++) {
+    const FusedLevel& L = lv[l];
+    coarse_smooth(L, true);
+    const double* dlast = last_increment(L);
+    for (int i = threadIdx.x; i < L.n; i += blockDim.x) {
+      L.t[i] = L.res[i] - sell_dot_any(L.A, i, dlast);
+      L.x[i] += dlast[i];
+    }
+    __syncthreads();
+    const FusedLevel& C = lv[l + 1];
+    for (int I = threadIdx.x; I < L.nc; I += blockDim.x) {
+      double sum = 0.0;
+      for (int k = L.memPtr[I]; k < L.memPtr[I + 1]; k++) sum += L.t[L.mem[k]];
+      C.r[I] = sum;
+    }
+    __syncthreads();
+  }
+  {  // coarsest: x = Ainv r, one wave per row
+    const FusedLevel& Z = lv[nl - 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int i = wave; i < Z.n; i += nw) {
+      double sum = 0.0;
+      for (int j = lane; j < Z.n; j += 64) sum += Z.cinv[(size_t)i * Z.n + j] * Z.r[j];
+      for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+      if (lane == 0) Z.x[i] = sum;
+    }
+    __syncthreads();
+  }
+  // ascend
+  for (int l = nl - 2; l >= 0; l--) {
+    const FusedLevel& L = lv[l];
+    const FusedLevel& C = lv[l + 1];
+    for (int i = threadIdx.x; i < L.n; i += blockDim.x) L.x[i] += C.x[L.agg[i]];
+    __syncthreads();
+    coarse_smooth(L, false);
+    const double* dlast = last_increment(L);
+    for (int i = threadIdx.x; i < L.n; i += blockDim.x) L.x[i] = 1.0 * dlast[i] + 1.0 * L.x[i];
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 void launch_schur_contrib(hipStream_t s, int nPairs, const int* pairNode, const int* nodeGroup, const int* nodeSize,
@@ -297,6 +379,10 @@ void launch_residual_restrict(hipStream_t s, const AmgSell& A, const double* res
 
 void launch_prolong(hipStream_t s, int n, const int* agg, const double* xc, double* x) {
   hipLaunchKernelGGL(prolong_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, agg, xc, x);
+}
+
+void launch_coarse_cycle(hipStream_t s, const FusedLevel* lv, int nl) {
+  hipLaunchKernelGGL(coarse_cycle_kernel, dim3(1), dim3(kCoarseThreads), 0, s, lv, nl);
 }
 
 void launch_dense_matvec(hipStream_t s, int n, const double* Ainv, const double* r, double* x) {
