@@ -347,8 +347,9 @@ void setupSchur(Ctx& c, const std::vector<int64_t>& rp, const std::vector<int>& 
   for (size_t l = 0; l < S->L.size(); l++) allocLevel(*S->L[l], l + 1 == S->L.size());
   const int ncst = S->L.back()->n;
   S->cinv.alloc((size_t)ncst * ncst * 8);
-  for (size_t l = 1; l < S->L.size(); l++)
-    if (S->L[l]->nSlots <= kFuseSlots) {  // the coarsest level has no smoother copy (0 slots)
+  // (a range holding only the coarsest level stays a multi-workgroup dense product)
+  for (size_t l = 1; l + 1 < S->L.size(); l++)
+    if (S->L[l]->nSlots <= kFuseSlots) {
       S->fuseFrom = (int)l;
       S->fused.alloc((S->L.size() - l) * sizeof(FusedLevel));
       break;
