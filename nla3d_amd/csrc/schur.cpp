@@ -34,6 +34,7 @@ constexpr int kSteps0 = 16;  // longest finest-level smoother
 constexpr int kSteps0Min = 8;
 constexpr int kStepsC = 3;
 constexpr double kLoDivC = 10.0;
+constexpr double kSchurAlpha = 0.5;  // S = alpha B^T D^-1 B - C (see schur.h)
 constexpr int kPowerSteps = 40;     // power iteration for lmax of D^-1 A
 
 // finest-level smoother length and interval for a Schur matrix of n rows
@@ -410,7 +411,7 @@ bool schurRefresh(Ctx& c, std::string* why) {
   AmgLevel& L0 = *S.L[0];
   launch_schur_contrib(s, S.nPairs, S.pairNode.as<int>(), S.nodeGroup.as<int>(), S.nodeSize.as<int>(),
                        S.qPtr.as<int>(), S.cPtr.as<int>(), S.posB.as<int>(), kval, c.d_binv.as<double>(),
-                       S.contrib.as<double>());
+                       kSchurAlpha, S.contrib.as<double>());
   launch_gather_sum(s, L0.nnz, S.sSeg.as<int>(), S.sSrc.as<int>(), S.contrib.as<double>(), L0.val.as<double>());
   launch_schur_sub_c(s, S.nq, S.sDiag.as<int>(), S.cPos.as<int>(), kval, L0.val.as<double>());
   for (size_t l = 0; l + 1 < S.L.size(); l++) {

@@ -3,10 +3,12 @@
 // The rows of block(2) that block Jacobi treats by a diagonal Schur estimate (element
 // pressures, MPC multipliers) are preconditioned instead by one V-cycle of
 // aggregation multigrid on the sparse Schur approximation
-//     S = B^T D^-1 B - C,
+//     S = alpha B^T D^-1 B - C,   alpha = 1/2,
 // B the couplings of those rows to the node DoFs, D^-1 the block-Jacobi inverse node
 // blocks, C their own block (the pressure rows' -V/k).  S is SPD when the node blocks
-// are.  Levels: piecewise-constant aggregation of S's graph (greedy, distance one),
+// are.  alpha is empirical: the block-Jacobi blocks misjudge the displacement block's
+// smooth modes, and halving their Schur term cut MINRES iterations by 6 % at 3.5M
+// unknowns and 9 % at 60k (1/4 was best for the small, 1/2 .. 1/3 for the large case).  Levels: piecewise-constant aggregation of S's graph (greedy, distance one),
 // Galerkin coarse matrices P^T A P (FP64), Jacobi-Chebyshev smoothing (the same
 // polynomial before and after, so the cycle is symmetric; a long one on the finest level)
 // whose products use an FP32 sliced-ELLPACK copy of each level, an explicit dense FP64
@@ -60,10 +62,10 @@ struct FusedLevel {
 // workgroup: every level must be small (one workgroup walks all its rows per pass)
 void launch_coarse_cycle(hipStream_t s, const FusedLevel* lv, int nl);
 
-// level-0 assembly of S: one thread per (node, Schur column a) writes B(:,a)^T D^-1 B(:,b)
+// level-0 assembly of S: one thread per (node, Schur column a) writes alpha B(:,a)^T D^-1 B(:,b)
 void launch_schur_contrib(hipStream_t s, int nPairs, const int* pairNode, const int* nodeGroup, const int* nodeSize,
                           const int* qPtr, const int* cPtr, const int* posB, const double* kval, const double* binv,
-                          double* contrib);
+                          double alpha, double* contrib);
 void launch_schur_sub_c(hipStream_t s, int nq, const int* sDiag, const int* cPos, const double* kval, double* sval);
 void launch_gather_sum(hipStream_t s, int n, const int* seg, const int* src, const double* in, double* out);
 void launch_inv_diag(hipStream_t s, const AmgCsr& A, double* dinv);

@@ -55,7 +55,8 @@ __global__ __launch_bounds__(256) void schur_contrib_kernel(int nPairs, const in
                                                           const int* __restrict__ nodeSize,
                                                           const int* __restrict__ qPtr, const int* __restrict__ cPtr,
                                                           const int* __restrict__ posB, const double* __restrict__ kval,
-                                                          const double* __restrict__ binv, double* __restrict__ contrib) {
+                                                          const double* __restrict__ binv, double alpha,
+                                                          double* __restrict__ contrib) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nPairs) return;
   const int n = pairNode[k];
@@ -76,7 +77,7 @@ __global__ __launch_bounds__(256) void schur_contrib_kernel(int nPairs, const in
       const int p = posB[(size_t)(q0 + b) * 3 + r];
       if (p >= 0) s += wa[r] * kval[p];
     }
-    out[b] = s;
+    out[b] = alpha * s;
   }
 }
 
@@ -319,10 +320,10 @@ __global__ __launch_bounds__(kCoarseThreads) void coarse_cycle_kernel(const Fuse
 
 void launch_schur_contrib(hipStream_t s, int nPairs, const int* pairNode, const int* nodeGroup, const int* nodeSize,
                           const int* qPtr, const int* cPtr, const int* posB, const double* kval, const double* binv,
-                          double* contrib) {
+                          double alpha, double* contrib) {
   if (nPairs <= 0) return;
   hipLaunchKernelGGL(schur_contrib_kernel, dim3(cdiv(nPairs, 256)), dim3(256), 0, s, nPairs, pairNode, nodeGroup,
-                     nodeSize, qPtr, cPtr, posB, kval, binv, contrib);
+                     nodeSize, qPtr, cPtr, posB, kval, binv, alpha, contrib);
 }
 
 void launch_schur_sub_c(hipStream_t s, int nq, const int* sDiag, const int* cPos, const double* kval, double* sval) {

@@ -36,9 +36,10 @@ def schur_matrix(K2, groups, bj):
     K = K2.tocsr()
     Bq = K[urows][:, q]
     C = K[q][:, q]
-    return q, (Bq.T @ Dinv @ Bq - C).toarray()
+    return q, (SCHUR_ALPHA * (Bq.T @ Dinv @ Bq) - C).toarray()
 
 
+SCHUR_ALPHA = 0.5  # S = alpha B^T D^-1 B - C: the weight of the block-Jacobi Schur term
 STEPS0, DIV0 = 16, 400.0   # finest level of a 10^6-row matrix (the longest smoother)
 STEPSC, DIVC = 3, 10.0     # coarser levels
 
