@@ -106,3 +106,23 @@ def test_solve_nonlinear_full_schedule(gpu_lib):
     for t in load_steps(10):
         o.solve_load_step(t)
     assert np.abs(m.get_solution() - o.U).max() <= 1e-8 * np.abs(o.U).max()
+
+
+def test_reference_cpu_workload_8x8x8(gpu_lib):
+    """the reference's CPU-runnable case: 8^3 jittered unit cube, Mooney-Rivlin C10 = 0.5,
+    C01 = 0.1, K = 100, bottom fixed, top UZ +20 % over 10 load steps (11 with TimeControl's
+    rounding), Newton to a 1e-6 relative residual: the same equilibrium iterations as the
+    oracle in every load step and its converged state to 1e-8"""
+    print("[gpu] 8x8x8 reference workload, 11 load steps", flush=True)
+    m, o = compression(8)
+    o.newton_init()
+    opts = m.newton_defaults(criterion=1, convergence=1e-6, krylov_tol=1e-13, krylov_max_iter=50000,
+                             max_iterations=20, preconditioner=2)
+    steps = load_steps(10)
+    assert len(steps) == 11
+    for t in steps:
+        st = m.solve_load_step(t, opts)
+        it = o.solve_load_step(t, criterion=1, convergence=1e-6, max_iterations=20)
+        assert st.last_step_iterations == it, t
+        U = m.get_solution()
+        assert np.abs(U - o.U).max() <= 1e-8 * np.abs(o.U).max(), t
