@@ -517,14 +517,20 @@ bool schurRefresh(Ctx& c, std::string* why) {
   return true;
 }
 
-static void vcycle(SchurAmg& S, hipStream_t s, size_t l) {
+// one V-cycle from level l.  At level 0 of the MINRES stage (rext != null) the right-hand side
+// is gathered from the Krylov vector rext and the result scattered into zext (the Schur rows,
+// S.qrow; st gates the write) inside the first and last passes; otherwise it is A.r -> A.x.
+static void vcycle(SchurAmg& S, hipStream_t s, size_t l, const double* rext = nullptr, double* zext = nullptr,
+                   const MinresState* st = nullptr) {
   if (l > 0 && (int)l == S.fuseFrom) {
     launch_coarse_cycle(s, S.fused.as<FusedLevel>(), (int)(S.L.size() - l));
     return;
   }
   AmgLevel& A = *S.L[l];
   if (l + 1 == S.L.size()) {
+    if (rext) launch_q_gather(s, S.nq, S.qrow.as<int>(), rext, st, A.r.as<double>());
     launch_dense_matvec(s, A.n, S.cinv.as<double>(), A.r.as<double>(), A.x.as<double>());
+    if (zext) launch_q_scatter(s, S.nq, S.qrow.as<int>(), A.x.as<double>(), st, zext);
     return;
   }
   const int steps = l == 0 ? finestSteps(A.n) : kStepsC;
@@ -533,7 +539,11 @@ static void vcycle(SchurAmg& S, hipStream_t s, size_t l) {
   double* dinv = A.dinv.as<double>();
   double* din = A.da.as<double>();
   double* dout = A.db.as<double>();
-  launch_cheb_start(s, A.n, A.r.as<double>(), dinv, k.c0, A.res.as<double>(), din, A.x.as<double>());
+  if (rext)
+    launch_cheb_start_gather(s, A.n, S.qrow.as<int>(), rext, dinv, k.c0, A.r.as<double>(), A.res.as<double>(), din,
+                             A.x.as<double>());
+  else
+    launch_cheb_start(s, A.n, A.r.as<double>(), dinv, k.c0, A.res.as<double>(), din, A.x.as<double>());
   for (int i = 1; i < steps; i++) {
     launch_cheb_step(s, M, din, dinv, k.c1[i], k.c2[i], A.res.as<double>(), dout, A.x.as<double>());
     std::swap(din, dout);
@@ -550,7 +560,10 @@ static void vcycle(SchurAmg& S, hipStream_t s, size_t l) {
     launch_cheb_step(s, M, din, dinv, k.c1[i], k.c2[i], A.res.as<double>(), dout, A.x.as<double>());
     std::swap(din, dout);
   }
-  launch_axpby(s, A.n, 1.0, din, 1.0, A.x.as<double>(), A.x.as<double>());
+  if (zext)
+    launch_finish_scatter(s, A.n, S.qrow.as<int>(), din, A.x.as<double>(), st, zext);
+  else
+    launch_axpby(s, A.n, 1.0, din, 1.0, A.x.as<double>(), A.x.as<double>());
 }
 
 // the MINRES stage: z[Schur rows] = V-cycle(r[Schur rows]); partials of r . z
@@ -558,10 +571,7 @@ PrecondHook schurHook(Ctx& c) {
   std::shared_ptr<SchurAmg> S = c.schur;
   const int n2 = c.n2;
   return [S, n2](hipStream_t s, const double* r, double* z, double* partial, const MinresState* st) {
-    AmgLevel& L0 = *S->L[0];
-    launch_q_gather(s, S->nq, S->qrow.as<int>(), r, st, L0.r.as<double>());
-    vcycle(*S, s, 0);
-    launch_q_scatter(s, S->nq, S->qrow.as<int>(), L0.x.as<double>(), st, z);
+    vcycle(*S, s, 0, r, z, st);
     launch_dot_partials(s, n2, r, z, st, partial);
   };
 }

@@ -78,6 +78,12 @@ void launch_cheb_start(hipStream_t s, int n, const double* r, const double* dinv
                        double* x);
 void launch_cheb_start_x(hipStream_t s, const AmgSell& A, const double* r, const double* dinv, double c0, double* res,
                          double* d, const double* x);
+// the MINRES stage's ends: the first pass with the Schur rows gathered from the Krylov vector
+// (r = rext[qrow], kept in r), and the last with z[qrow] = d + x (skipped once converged)
+void launch_cheb_start_gather(hipStream_t s, int n, const int* qrow, const double* rext, const double* dinv, double c0,
+                              double* r, double* res, double* d, double* x);
+void launch_finish_scatter(hipStream_t s, int n, const int* qrow, const double* d, const double* x,
+                           const MinresState* st, double* z);
 void launch_cheb_step(hipStream_t s, const AmgSell& A, const double* din, const double* dinv, double c1, double c2,
                       double* res, double* dout, double* x);
 void launch_residual_restrict(hipStream_t s, const AmgSell& A, const double* res, const double* dlast, double* t,

@@ -121,6 +121,34 @@ __global__ __launch_bounds__(256) void cheb_start_kernel(int n, const double* __
   }
 }
 
+// the MINRES stage's entry: r = rext[qrow] (the Schur rows of the Krylov vector, kept for the
+// post-smoother), then cheb_start: res = r ; d = c0 D^-1 r ; x = 0
+__global__ __launch_bounds__(256) void cheb_start_gather_kernel(int n, const int* __restrict__ qrow,
+                                                              const double* __restrict__ rext,
+                                                              const double* __restrict__ dinv, double c0,
+                                                              double* __restrict__ r, double* __restrict__ res,
+                                                              double* __restrict__ d, double* __restrict__ x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double ri = rext[qrow[i]];
+    r[i] = ri;
+    res[i] = ri;
+    d[i] = c0 * dinv[i] * ri;
+    x[i] = 0.0;
+  }
+}
+
+// the MINRES stage's exit: z[qrow] = d + x (the last smoothing increment added on the way out),
+// nothing once the solve has converged
+__global__ __launch_bounds__(256) void finish_scatter_kernel(int n, const int* __restrict__ qrow,
+                                                           const double* __restrict__ d,
+                                                           const double* __restrict__ x,
+                                                           const MinresState* __restrict__ st,
+                                                           double* __restrict__ z) {
+  if (st && st->converged) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    z[qrow[i]] = 1.0 * d[i] + 1.0 * x[i];
+}
+
 // res = r - A x ; d = c0 D^-1 res   (x is read, not written)
 template <bool kNarrow>
 __global__ __launch_bounds__(256) void cheb_start_x_kernel(AmgSell A, const double* __restrict__ r,
@@ -344,6 +372,17 @@ void launch_inv_diag(hipStream_t s, const AmgCsr& A, double* dinv) {
 void launch_cheb_start(hipStream_t s, int n, const double* r, const double* dinv, double c0, double* res, double* d,
                        double* x) {
   hipLaunchKernelGGL(cheb_start_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, r, dinv, c0, res, d, x);
+}
+
+void launch_cheb_start_gather(hipStream_t s, int n, const int* qrow, const double* rext, const double* dinv, double c0,
+                              double* r, double* res, double* d, double* x) {
+  hipLaunchKernelGGL(cheb_start_gather_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, qrow, rext, dinv, c0, r, res, d,
+                     x);
+}
+
+void launch_finish_scatter(hipStream_t s, int n, const int* qrow, const double* d, const double* x,
+                           const MinresState* st, double* z) {
+  hipLaunchKernelGGL(finish_scatter_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, qrow, d, x, st, z);
 }
 
 void launch_sell_fill(hipStream_t s, int nSlots, const int* pos, const double* csrVal, float* sellVal) {
