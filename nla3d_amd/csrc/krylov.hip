@@ -272,23 +272,31 @@ __global__ __launch_bounds__(256) void precond_kernel(int nGroups, const int* __
                                                      double* __restrict__ Z, MinresState* __restrict__ st,
                                                      double* __restrict__ partial) {
   __shared__ double sh[4];
+  __shared__ double sB[256 * 9];  // this pass's 256 inverse blocks, loaded with coalesced reads
   if (st->converged) return;
   const double c2 = st->c_r2;
   double contrib = 0.0;
-  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nGroups; g += gridDim.x * blockDim.x) {
-    const int r0 = gStart[g], size = gInfo[g] & 3;
-    double t[3] = {0, 0, 0};
-    for (int i = 0; i < size; i++) {
-      t[i] = T[r0 + i] - c2 * R2[r0 + i];
-      T[r0 + i] = t[i];
+  for (int g0 = blockIdx.x * 256; g0 < nGroups; g0 += gridDim.x * 256) {
+    const int ng = min(256, nGroups - g0);
+    for (int k = threadIdx.x; k < ng * 9; k += 256) sB[k] = binv[(size_t)g0 * 9 + k];
+    __syncthreads();
+    const int g = g0 + threadIdx.x;
+    if (g < nGroups) {
+      const int r0 = gStart[g], size = gInfo[g] & 3;
+      double t[3] = {0, 0, 0};
+      for (int i = 0; i < size; i++) {
+        t[i] = T[r0 + i] - c2 * R2[r0 + i];
+        T[r0 + i] = t[i];
+      }
+      const double* B = sB + threadIdx.x * 9;
+      for (int i = 0; i < size; i++) {
+        double z = 0.0;
+        for (int j = 0; j < size; j++) z += B[i * 3 + j] * t[j];
+        Z[r0 + i] = z;
+        contrib += t[i] * z;
+      }
     }
-    const double* B = binv + (size_t)g * 9;
-    for (int i = 0; i < size; i++) {
-      double z = 0.0;
-      for (int j = 0; j < size; j++) z += B[i * 3 + j] * t[j];
-      Z[r0 + i] = z;
-      contrib += t[i] * z;
-    }
+    __syncthreads();
   }
   const double b = block_sum<256>(contrib, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = b;
