@@ -188,15 +188,20 @@ __global__ __launch_bounds__(256) void residual_kernel(AmgSell A, const double* 
   x[i] += dlast[i];
 }
 
-// rc[I] = sum of t over the members of aggregate I, in member order
+// rc[I] = sum of t over the members of aggregate I: 16 lanes per aggregate, each summing every
+// 16th member in member order, then a fixed butterfly over the lanes (an aggregate holds ~25
+// rows, so one thread per aggregate left the launch latency-bound with few threads)
+constexpr int kRestrictLanes = 16;
 __global__ __launch_bounds__(256) void restrict_kernel(int nc, const int* __restrict__ memPtr,
                                                      const int* __restrict__ mem, const double* __restrict__ t,
                                                      double* __restrict__ rc) {
-  const int I = blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= nc) return;
+  const int lane = threadIdx.x & (kRestrictLanes - 1);
+  const int I = (blockIdx.x * blockDim.x + threadIdx.x) / kRestrictLanes;
+  if (I >= nc) return;  // the 16 lanes of an aggregate leave together
   double s = 0.0;
-  for (int k = memPtr[I]; k < memPtr[I + 1]; k++) s += t[mem[k]];
-  rc[I] = s;
+  for (int k = memPtr[I] + lane; k < memPtr[I + 1]; k += kRestrictLanes) s += t[mem[k]];
+  for (int o = kRestrictLanes / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kRestrictLanes);
+  if (lane == 0) rc[I] = s;
 }
 
 __global__ __launch_bounds__(256) void prolong_kernel(int n, const int* __restrict__ agg, const double* __restrict__ xc,
@@ -414,7 +419,8 @@ void launch_residual_restrict(hipStream_t s, const AmgSell& A, const double* res
     hipLaunchKernelGGL(residual_kernel<true>, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, res, dlast, t, x);
   else
     hipLaunchKernelGGL(residual_kernel<false>, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, res, dlast, t, x);
-  hipLaunchKernelGGL(restrict_kernel, dim3(cdiv(nc, 256)), dim3(256), 0, s, nc, memPtr, mem, t, rc);
+  hipLaunchKernelGGL(restrict_kernel, dim3(cdiv((long long)nc * kRestrictLanes, 256)), dim3(256), 0, s, nc, memPtr,
+                     mem, t, rc);
 }
 
 void launch_prolong(hipStream_t s, int n, const int* agg, const double* xc, double* x) {
