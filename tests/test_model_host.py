@@ -71,6 +71,31 @@ def test_unconstrained_no_fix_reports_sizes(gpu_lib):
     assert s["nConstrained"] == 0 and s["nnz1"] == 0 and s["nDofs"] == 12 * 3 + 2
 
 
+def test_empty_model_is_an_error_like_the_reference(gpu_lib):
+    """FEStorage::initDofs stops on a model without nodes and elements (FEStorage.cpp:340-341)"""
+    m = nla3d_amd.GpuModel(device=-1)
+    m.set_nodes(np.zeros((0, 3)))
+    with pytest.raises(nla3d_amd.Nla3dGpuError, match="No any nodes or elements"):
+        m.init_solution()
+
+
+def test_all_node_dofs_fixed_leaves_pressure_rows(gpu_lib):
+    """every displacement constrained: block(2) holds only the elements' HYDRO_PRESSURE rows,
+    numbered and patterned as the oracle does"""
+    m = nla3d_amd.GpuModel(device=-1)
+    xyz, conn, _ = meshgen.hex_box(2, 2, 1, jitter=0.1, seed=3)
+    m.set_nodes(xyz)
+    m.add_elements(nla3d_amd.SOLID81, conn)
+    m.set_material(1, [1.0, 100.0])
+    fixes = [(n, d, 0.01 * d) for n in range(1, len(xyz) + 1) for d in range(3)]
+    for n, d, v in fixes:
+        m.add_fix(n, d, v)
+    s = m.init_solution()
+    o = oracle_for(xyz, conn, fixes, material=(1, [1.0, 100.0]))
+    compare(m, o)
+    assert s["nConstrained"] == 3 * len(xyz) and s["nDofs"] == 3 * len(xyz) + len(conn)
+
+
 def test_bad_node_number_is_an_error(gpu_lib):
     m = nla3d_amd.GpuModel(device=-1)
     xyz, conn, _ = meshgen.hex_box(1, 1, 1)
