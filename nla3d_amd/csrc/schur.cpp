@@ -545,7 +545,8 @@ static void vcycle(SchurAmg& S, hipStream_t s, size_t l, const double* rext = nu
   else
     launch_cheb_start(s, A.n, A.r.as<double>(), dinv, k.c0, A.res.as<double>(), din, A.x.as<double>());
   for (int i = 1; i < steps; i++) {
-    launch_cheb_step(s, M, din, dinv, k.c1[i], k.c2[i], A.res.as<double>(), dout, A.x.as<double>());
+    launch_cheb_step(s, M, din, dinv, k.c1[i], k.c2[i], A.res.as<double>(), dout, A.x.as<double>(),
+                     chebXMode(i, steps));
     std::swap(din, dout);
   }
   AmgLevel& C = *S.L[l + 1];
@@ -557,7 +558,8 @@ static void vcycle(SchurAmg& S, hipStream_t s, size_t l, const double* rext = nu
   dout = A.db.as<double>();
   launch_cheb_start_x(s, M, A.r.as<double>(), dinv, k.c0, A.res.as<double>(), din, A.x.as<double>());
   for (int i = 1; i < steps; i++) {
-    launch_cheb_step(s, M, din, dinv, k.c1[i], k.c2[i], A.res.as<double>(), dout, A.x.as<double>());
+    launch_cheb_step(s, M, din, dinv, k.c1[i], k.c2[i], A.res.as<double>(), dout, A.x.as<double>(),
+                     chebXMode(i, steps));
     std::swap(din, dout);
   }
   if (zext)
@@ -669,16 +671,19 @@ int nla3d_gpu_time_smoother(nla3d_gpu_t h, int iters, double* ms, long long* byt
     launch_cheb_start(s, A.n, A.r.as<double>(), A.dinv.as<double>(), k.c0, A.res.as<double>(), din,
                       A.x.as<double>());
     launch_cheb_step(s, M, din, A.dinv.as<double>(), k.c1[1], k.c2[1], A.res.as<double>(), dout,
-                     A.x.as<double>());  // warm-up
+                     A.x.as<double>(), kChebPlain);  // warm-up
     std::swap(din, dout);
     hipEvent_t e0, e1;
     hipCheck(hipEventCreate(&e0), "event");
     hipCheck(hipEventCreate(&e1), "event");
     hipCheck(hipEventRecord(e0, s), "rec");
+    long long xPasses = 0;  // vector passes of x over the timed steps (2 when added to, else 0)
     for (int i = 0; i < iters; i++) {
       const int j = 1 + i % (steps0 - 1);
+      const int mode = chebXMode(j, steps0);
       launch_cheb_step(s, M, din, A.dinv.as<double>(), k.c1[j], k.c2[j], A.res.as<double>(), dout,
-                       A.x.as<double>());
+                       A.x.as<double>(), mode);
+      xPasses += mode == kChebSkip ? 0 : 2;
       std::swap(din, dout);
     }
     hipCheck(hipEventRecord(e1, s), "rec");
@@ -689,11 +694,13 @@ int nla3d_gpu_time_smoother(nla3d_gpu_t h, int iters, double* ms, long long* byt
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (bytes) {
-      // stored matrix (FP32 values, column offsets or indices, slice offsets and bases) and
-      // seven vectors of the step: din, res, D^-1, x read; res, dout, x written
+      // stored matrix (FP32 values, column offsets or indices, slice offsets and bases), the
+      // five vectors of every step (din, res, D^-1 read; res, dout written) and x, read and
+      // written by the steps that add to it: the mean over the timed steps, which follow the
+      // sweep's order of modes
       const long long nSl = (A.n + 63) / 64;
       *bytes = (long long)A.nSlots * (4 + (A.narrow ? 2 : 4)) + (nSl + 1) * 4 + (A.narrow ? nSl * 4 : 0) +
-               7ll * A.n * 8;
+               5ll * A.n * 8 + (xPasses * A.n * 8 + iters / 2) / iters;
     }
   });
 }

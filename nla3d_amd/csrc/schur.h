@@ -84,8 +84,15 @@ void launch_cheb_start_gather(hipStream_t s, int n, const int* qrow, const doubl
                               double* r, double* res, double* d, double* x);
 void launch_finish_scatter(hipStream_t s, int n, const int* qrow, const double* d, const double* x,
                            const MinresState* st, double* z);
+// how Chebyshev step i of a sweep of `steps` terms (i = 1 .. steps-1) adds the increments to x:
+// odd steps with a step after them add their input and output increments (kChebPair), the even
+// steps after them add nothing (kChebSkip), and a last odd step adds its input (kChebPlain), so
+// the increment a sweep ends with is always still to be added, as with one addition per step
+// (x then takes 5 of the 7 vector passes of a step every other step)
+enum { kChebPlain = 0, kChebPair = 1, kChebSkip = 2 };
+inline int chebXMode(int i, int steps) { return (i & 1) ? (i + 1 < steps ? kChebPair : kChebPlain) : kChebSkip; }
 void launch_cheb_step(hipStream_t s, const AmgSell& A, const double* din, const double* dinv, double c1, double c2,
-                      double* res, double* dout, double* x);
+                      double* res, double* dout, double* x, int xmode);
 void launch_residual_restrict(hipStream_t s, const AmgSell& A, const double* res, const double* dlast, double* t,
                               int nc, const int* memPtr, const int* mem, double* rc, double* x);
 void launch_prolong(hipStream_t s, int n, const int* agg, const double* xc, double* x);

@@ -162,19 +162,24 @@ __global__ __launch_bounds__(256) void cheb_start_x_kernel(AmgSell A, const doub
   d[i] = c0 * dinv[i] * ri;
 }
 
-// res -= A din ; dout = c1 din + c2 D^-1 res ; x += din
+// res -= A din ; dout = c1 din + c2 D^-1 res ; then by xmode: x += din (kChebPlain),
+// x += din + dout (kChebPair: the next step, kChebSkip, leaves x alone), or nothing
 template <bool kNarrow>
 __global__ __launch_bounds__(256) void cheb_step_kernel(AmgSell A, const double* __restrict__ din,
                                                       const double* __restrict__ dinv, double c1, double c2,
                                                       double* __restrict__ res, double* __restrict__ dout,
-                                                      double* __restrict__ x) {
+                                                      double* __restrict__ x, int xmode) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
   const double ri = res[i] - sell_dot<kNarrow>(A, i, din);
   res[i] = ri;
   const double di = din[i];
-  dout[i] = c1 * di + c2 * dinv[i] * ri;
-  x[i] += di;
+  const double dn = c1 * di + c2 * dinv[i] * ri;
+  dout[i] = dn;
+  if (xmode == kChebPlain)
+    x[i] += di;
+  else if (xmode == kChebPair)
+    x[i] += di + dn;
 }
 
 // t = res - A dlast (the residual of x + dlast) ; x += dlast
@@ -404,13 +409,13 @@ void launch_cheb_start_x(hipStream_t s, const AmgSell& A, const double* r, const
 }
 
 void launch_cheb_step(hipStream_t s, const AmgSell& A, const double* din, const double* dinv, double c1, double c2,
-                      double* res, double* dout, double* x) {
+                      double* res, double* dout, double* x, int xmode) {
   if (A.col16)
     hipLaunchKernelGGL(cheb_step_kernel<true>, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, din, dinv, c1, c2, res, dout,
-                       x);
+                       x, xmode);
   else
     hipLaunchKernelGGL(cheb_step_kernel<false>, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, din, dinv, c1, c2, res, dout,
-                       x);
+                       x, xmode);
 }
 
 void launch_residual_restrict(hipStream_t s, const AmgSell& A, const double* res, const double* dlast, double* t,
