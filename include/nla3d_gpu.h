@@ -358,9 +358,11 @@ int nla3d_gpu_time_assembly(nla3d_gpu_t h, int iters, double* ms_element, double
  * CSR, y = K2 x) over `iters` launches, HIP events on the handle's stream. */
 int nla3d_gpu_time_spmv(nla3d_gpu_t h, int iters, double* ms);
 /* Mean duration in ms of one finest-level step of the Schur multigrid's Chebyshev
- * smoother (cheb_step_kernel: res -= A d, d' = c1 d + c2 D^-1 res, x += d over the FP32
- * sliced-ELLPACK copy) over `iters` launches, HIP events on the handle's stream; bytes
- * (may be NULL) receives the bytes one launch must move (stored matrix + 7 vectors). */
+ * smoother (cheb_step_kernel: res -= A d, d' = c1 d + c2 D^-1 res over the FP32
+ * sliced-ELLPACK copy; x += d + d' on odd steps and nothing on the even steps after them,
+ * in a sweep's order) over `iters` launches, HIP events on the handle's stream; bytes (may
+ * be NULL) receives the mean bytes one launch must move (stored matrix + 5 vectors + x
+ * read and written by the steps that add to it). */
 int nla3d_gpu_time_smoother(nla3d_gpu_t h, int iters, double* ms, long long* bytes);
 /* One launch of a plain device copy of `bytes` bytes (read + write), timed with
  * HIP events: the reference point for scaling hardware byte counters. */
