@@ -17,8 +17,9 @@ so this restates the definitions the GPU path implements, in FP64 numpy:
 Pinning: mlp_backward against central differences of mlp_forward; the MINRES
 recurrence against a dense solve (its residual estimate equals the true
 residual of its iterate); minres_log_rate's gradient on the GPU is checked
-against central differences of this function.  node_features has no
-independent check beyond its definition: parity unpinned for that function.
+against central differences of this function; node_features by its invariances
+(unchanged under K -> sK, carried along by a renumbering of the nodes) and by
+closed-form values on one undeformed unit cube (tests/test_oracle_pins.py).
 """
 import math
 

@@ -382,6 +382,10 @@ class SparsityInfo:
             raise RuntimeError("maxInRow overflow!")
         r.append(j)
 
+    def addEntrySym(self, i, j):
+        """SparseSymMatrix::addEntry (SparseMatrix.h:259-265): the upper triangle only"""
+        self.addEntry(min(i, j), max(i, j))
+
     def compress(self):
         self.iofeir = np.zeros(self.nRows + 1, dtype=np.uint32)
         cols = []
@@ -420,6 +424,48 @@ class SparsityInfo:
                 en = ind
             else:
                 st = ind
+
+
+def dense_rect(si, vals, ncols):
+    """SparseMatrix (SparseMatrix.h:131-159) as a dense array: values[k] at (i, columns[k]) for
+    k in [iofeir[i] - 1, iofeir[i + 1] - 1), 1-based arrays; A @ v is matBVprod and A.T @ v
+    matBTVprod (SparseMatrix.cpp:522-561)"""
+    n = si.nRows
+    A = np.zeros((n, ncols))
+    for i in range(n):
+        for k in range(int(si.iofeir[i]) - 1, int(si.iofeir[i + 1]) - 1):
+            A[i, int(si.columns[k]) - 1] = vals[k]
+    return A
+
+
+def dense_sym(si, vals):
+    """SparseSymMatrix (SparseMatrix.h:162-186) as a dense symmetric array: the upper entries
+    mirrored (value(i, j) swaps to i <= j, SparseMatrix.cpp:467-479); A @ v is its matBVprod
+    (SparseMatrix.cpp:497-519) without the |v| > 1e-20 skip"""
+    A = dense_rect(si, vals, si.nRows)
+    return A + np.triu(A, 1).T
+
+
+def sparse_rect(si, vals, ncols):
+    """dense_rect as a scipy CSR (models too large for dense arrays)"""
+    n = si.nRows
+    rows = np.repeat(np.arange(n), np.diff(si.iofeir.astype(np.int64)))
+    return sp.csr_matrix((vals, (rows, si.columns.astype(np.int64) - 1)), shape=(n, ncols))
+
+
+def sparse_sym(si, vals):
+    """dense_sym as a scipy CSR"""
+    up = sparse_rect(si, vals, si.nRows)
+    return (up + sp.triu(up, 1).T).tocsr()
+
+
+def sparse_sym_pattern(n, max_in_row=100):
+    """the SparsityInfo of SparseSymMatrix(n) (SparseMatrix.cpp:423-431): every diagonal entry
+    added by the constructor (BlockSparseSymMatrix's reinit does not, SparseMatrix.cpp:442-448)"""
+    si = SparsityInfo(n, n, max_in_row)
+    for i in range(1, n + 1):
+        si.addEntry(i, i)
+    return si
 
 
 class OracleModel:
@@ -701,46 +747,22 @@ class OracleModel:
     # ---------------- Newton-Raphson (NonlinearFESolver::solve, FESolver.cpp:407-521) ----------------
     def dense_blocks(self, K):
         """K values per block -> dense K12 (nC x n2) and symmetric K2 (n2 x n2)"""
-        nC, n2 = self.nC, self.nU + self.nMpc
-        K12 = np.zeros((nC, n2))
-        b = self.blocks[12]
-        for i in range(nC):
-            for k in range(int(b.iofeir[i]) - 1, int(b.iofeir[i + 1]) - 1):
-                K12[i, int(b.columns[k]) - 1] = K[12][k]
-        K2 = np.zeros((n2, n2))
-        b = self.blocks[2]
-        for i in range(n2):
-            for k in range(int(b.iofeir[i]) - 1, int(b.iofeir[i + 1]) - 1):
-                j = int(b.columns[k]) - 1
-                K2[i, j] = K2[j, i] = K[2][k]
-        return K12, K2
+        n2 = self.nU + self.nMpc
+        return dense_rect(self.blocks[12], K[12], n2), dense_sym(self.blocks[2], K[2])
 
     @staticmethod
     def sparse_sym(b, vals, n):
         """symmetric scipy CSR from an upper 1-based CSR block"""
-        rows = np.repeat(np.arange(n), np.diff(b.iofeir.astype(np.int64)))
-        cols = b.columns.astype(np.int64) - 1
-        up = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
-        return (up + sp.triu(up, 1).T).tocsr()
+        return sparse_sym(b, vals)
 
     def sparse_blocks(self, K):
         """K values per block -> sparse K12 (nC x n2) and symmetric K2 (n2 x n2); large models"""
-        nC, n2 = self.nC, self.nU + self.nMpc
-        b = self.blocks[12]
-        rows = np.repeat(np.arange(nC), np.diff(b.iofeir.astype(np.int64)))
-        K12 = sp.csr_matrix((K[12], (rows, b.columns.astype(np.int64) - 1)), shape=(nC, n2))
-        return K12, self.sparse_sym(self.blocks[2], K[2], n2)
+        n2 = self.nU + self.nMpc
+        return sparse_rect(self.blocks[12], K[12], n2), sparse_sym(self.blocks[2], K[2])
 
     def dense_k11(self, K):
         """symmetric K11 (nC x nC) from the upper CSR of block 1"""
-        nC = self.nC
-        K11 = np.zeros((nC, nC))
-        b = self.blocks[1]
-        for i in range(nC):
-            for k in range(int(b.iofeir[i]) - 1, int(b.iofeir[i + 1]) - 1):
-                j = int(b.columns[k]) - 1
-                K11[i, j] = K11[j, i] = K[1][k]
-        return K11
+        return dense_sym(self.blocks[1], K[1])
 
     def reaction(self, node, dof):
         """FEStorage::getReaction(node, dof) (FEStorage.cpp:105-119): vecR at the DoF's equation"""

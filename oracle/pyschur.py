@@ -13,7 +13,9 @@ the device path implements:
   matrix with its entries rounded to FP32 (products and sums in FP64), the diagonal
   scaling and the coarse matrices the FP64 values.
 
-Pinned by: a one-level hierarchy gives S^-1 exactly; the cycle is a symmetric
+Pinned by: schur_matrix equals minus the exact Schur complement (dense elimination) of
+block(2) with its displacement block replaced by its block diagonal over alpha; a
+one-level hierarchy gives S^-1 exactly; the cycle is a symmetric
 positive-definite operator; the smoother alone reduces the energy-norm error; S is
 symmetric positive definite on the models of the tests.
 """
@@ -23,9 +25,11 @@ import numpy as np
 import scipy.sparse as sp
 
 
-def schur_matrix(K2, groups, bj):
+def schur_matrix(K2, groups, bj, alpha=None):
     """(q rows, S as a dense array) from the full block(2) matrix, the block-Jacobi groups
-    [(r0, size, kind)] and their inverse blocks [(r0, size, B)]"""
+    [(r0, size, kind)] and their inverse blocks [(r0, size, B)]; alpha defaults to the
+    library's SCHUR_ALPHA"""
+    alpha = SCHUR_ALPHA if alpha is None else alpha
     q = np.array([r0 for r0, sz, kind in groups if kind == 1], dtype=np.int64)
     urows, blocks = [], []
     for (r0, sz, kind), (_, _, B) in zip(groups, bj):
@@ -36,7 +40,7 @@ def schur_matrix(K2, groups, bj):
     K = K2.tocsr()
     Bq = K[urows][:, q]
     C = K[q][:, q]
-    return q, (SCHUR_ALPHA * (Bq.T @ Dinv @ Bq) - C).toarray()
+    return q, (alpha * (Bq.T @ Dinv @ Bq) - C).toarray()
 
 
 SCHUR_ALPHA = 0.5  # S = alpha B^T D^-1 B - C: the weight of the block-Jacobi Schur term
