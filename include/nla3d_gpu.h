@@ -267,6 +267,9 @@ int nla3d_gpu_solve_block2_pc(nla3d_gpu_t h, const double* rhs, double* x, doubl
  * recomputes the hierarchy's values from the current tangent first). */
 int nla3d_gpu_schur_info(nla3d_gpu_t h, int* available, int* nq, int* nnz0, int* levels, int* sizes, int cap);
 int nla3d_gpu_schur_matrix(nla3d_gpu_t h, int* qrow, int* rowPtr, int* col, double* val);
+/* the cycle's shape: the finest level's Chebyshev length (terms; 0 with one level) and the first
+ * level of the coarse range run as one workgroup launch (-1: none) */
+int nla3d_gpu_schur_cycle_info(nla3d_gpu_t h, int* finest_steps, int* fuse_from);
 int nla3d_gpu_schur_level(nla3d_gpu_t h, int level, int* agg, double* lmax);
 int nla3d_gpu_schur_apply(nla3d_gpu_t h, const double* r, double* z, int refresh);
 

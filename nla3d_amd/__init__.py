@@ -140,6 +140,7 @@ ABI.update({
     "nla3d_gpu_schur_info": [_P, _IP, _IP, _IP, _IP, _IP, ctypes.c_int],
     "nla3d_gpu_schur_matrix": [_P, _IP, _IP, _IP, _DP],
     "nla3d_gpu_schur_level": [_P, ctypes.c_int, _IP, _DP],
+    "nla3d_gpu_schur_cycle_info": [_P, _IP, _IP],
     "nla3d_gpu_schur_apply": [_P, _DP, _DP, ctypes.c_int],
     "nla3d_gpu_time_smoother": [_P, ctypes.c_int, _DP, ctypes.POINTER(ctypes.c_longlong)],
     "nla3d_gpu_get_R": [_P, _DP],
@@ -418,6 +419,12 @@ class GpuModel:
         self._call("nla3d_gpu_schur_matrix", _ptr(qrow, ctypes.c_int), _ptr(rp, ctypes.c_int), _ptr(col, ctypes.c_int),
                    _ptr(val, ctypes.c_double))
         return qrow, rp, col, val
+
+    def schur_cycle_info(self):
+        """(finest-level Chebyshev terms, first level of the one-workgroup coarse range or -1)"""
+        st, ff = ctypes.c_int(), ctypes.c_int()
+        self._call("nla3d_gpu_schur_cycle_info", ctypes.byref(st), ctypes.byref(ff))
+        return st.value, ff.value
 
     def schur_level(self, level, n):
         agg = np.zeros(max(n, 1), dtype=np.int32)

@@ -53,19 +53,18 @@ struct Batch {
   DevBuf d_team, d_queue, d_counter, d_out, d_relres, d_norms;
   int nBlocks = 0, nTargets = 0;
   nla3d_gpu_batch_config cfg{};
-  hipGraphExec_t pre = nullptr;
+  StreamGraph pre;  // the pre-solve chain at time preT
   double preT = std::nan("");
   std::vector<double> rhs1;
   std::vector<int> done;
   ~Batch() {
-    if (pre) (void)hipGraphExecDestroy(pre);
+    pre.reset();
     if (stream) (void)hipStreamDestroy(stream);
     for (hipStream_t w : workers)
       if (w) (void)hipStreamDestroy(w);
   }
   void dropGraph() {
-    if (pre) (void)hipGraphExecDestroy(pre);
-    pre = nullptr;
+    pre.reset();
     preT = std::nan("");
   }
 };
@@ -100,15 +99,12 @@ static void preSolveChain(Batch& b, double t) {
 static void runPreSolve(Batch& b, double t) {
   if (!(b.pre && b.preT == t)) {
     b.dropGraph();
-    hipGraph_t g = nullptr;
-    hipCheck(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    b.pre.begin(b.stream);
     preSolveChain(b, t);
-    hipCheck(hipStreamEndCapture(b.stream, &g), "end capture");
-    hipCheck(hipGraphInstantiate(&b.pre, g, nullptr, nullptr, 0), "instantiate");
-    (void)hipGraphDestroy(g);
+    b.pre.end();
     b.preT = t;
   }
-  hipCheck(hipGraphLaunch(b.pre, b.stream), "graph launch");
+  b.pre.launch(b.stream);
 }
 
 static std::vector<double> readNorms(Batch& b) {

@@ -40,6 +40,45 @@ struct DevBuf {
   }
 };
 
+// A launch sequence captured from a stream into a graph, and its executable.  Both are
+// destroyed together (the graph outlives the executable instantiated from it); a capture
+// still open when the holder goes away, e.g. after a throw inside the captured calls, is
+// ended first, so the stream leaves capture mode.
+struct StreamGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t capturing = nullptr;
+  StreamGraph() = default;
+  StreamGraph(const StreamGraph&) = delete;
+  StreamGraph& operator=(const StreamGraph&) = delete;
+  ~StreamGraph() { reset(); }
+  void begin(hipStream_t s) {
+    reset();
+    hipCheck(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+    capturing = s;
+  }
+  void end() {
+    hipStream_t s = capturing;
+    capturing = nullptr;
+    hipCheck(hipStreamEndCapture(s, &graph), "end capture");
+    hipCheck(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
+  }
+  void launch(hipStream_t s) const { hipCheck(hipGraphLaunch(exec, s), "graph launch"); }
+  void reset() {
+    if (capturing) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(capturing, &g);
+      if (g) (void)hipGraphDestroy(g);
+      capturing = nullptr;
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  explicit operator bool() const { return exec != nullptr; }
+};
+
 struct ThermalData;  // thermal.cpp
 struct LearnedWork;  // learned.cpp
 struct SchurAmg;     // schur.cpp
@@ -138,6 +177,9 @@ struct Ctx {
 
   // solver.cpp
   void setupKrylov();
+  void setupBlock2(const CsrPattern& U2, const std::vector<int8_t>& kind3, const std::vector<uint8_t>& rowType,
+                   const std::vector<uint32_t>& rowNode);
+  void setupModelRows();
   KrylovMatrix krylovMatrix() const;
   Precond precond();
   // solve block(2) x = b with the current K; b, x device pointers [n2]; returns iterations

@@ -630,6 +630,15 @@ int nla3d_gpu_schur_info(nla3d_gpu_t h, int* available, int* nq, int* nnz0, int*
   });
 }
 
+int nla3d_gpu_schur_cycle_info(nla3d_gpu_t h, int* finest_steps, int* fuse_from) {
+  return schurGuard(h, [&](Ctx& c) {
+    SchurAmg* S = c.schur.get();
+    if (!S || !S->available) throw ApiError(-3, "the Schur multigrid is unavailable for this model");
+    if (finest_steps) *finest_steps = S->L.size() > 1 ? finestSteps(S->L[0]->n) : 0;
+    if (fuse_from) *fuse_from = S->fuseFrom;
+  });
+}
+
 int nla3d_gpu_schur_matrix(nla3d_gpu_t h, int* qrow, int* rowPtr, int* col, double* val) {
   return schurGuard(h, [&](Ctx& c) {
     refreshOrThrow(c);

@@ -17,7 +17,38 @@ void Ctx::setupKrylov() {
   FEStorage& m = model;
   const uint32_t nC = m.nConstrainedDofs;
   n2 = (int)(m.nAll() - nC);
-  const CsrPattern& U2 = m.block2;
+  // what each block(2) row is: element DoF (pressure), node DoF, or MPC multiplier
+  std::vector<int8_t> kind3(n2, -1);  // 0 node DoF, 1 element DoF, 2 multiplier
+  std::vector<uint8_t> rowType(n2, 0);
+  std::vector<uint32_t> rowNode(n2, 0);
+  for (uint32_t e = 0; e < m.nElements(); e++)
+    for (int t = 0; t < NUM_DOF_TYPES; t++) {
+      uint32_t q = m.elemEq[(size_t)e * 8 + t];
+      if (q > nC) kind3[q - nC - 1] = 1;
+    }
+  for (uint32_t n = 0; n < m.nNodes(); n++)
+    for (int t = 0; t < NUM_DOF_TYPES; t++) {
+      uint32_t q = m.nodeEq[(size_t)n * 8 + t];
+      if (q > nC) {
+        kind3[q - nC - 1] = 0;
+        rowType[q - nC - 1] = (uint8_t)t;
+        rowNode[q - nC - 1] = n;
+      }
+    }
+  for (auto& mp : m.mpcs) kind3[mp.eqNum - nC - 1] = 2;
+  setupBlock2(m.block2, kind3, rowType, rowNode);
+  setupModelRows();
+  krylovReady = true;
+}
+
+// The block(2) solve's device data from its upper CSR pattern (1-based, the reference's
+// layout) and a description of each row (kind3: 0 node DoF, 1 element DoF, 2 multiplier;
+// the DoF type and node of a node-DoF row): the grouped full-symmetric copy the SpMV reads
+// and the mirror map the scatter writes it through, the block-Jacobi groups, the Schur-row
+// multigrid's structure, the learned-correction row metadata and the MINRES work space.
+void Ctx::setupBlock2(const CsrPattern& U2, const std::vector<int8_t>& kind3, const std::vector<uint8_t>& rowType,
+                      const std::vector<uint32_t>& rowNode) {
+  n2 = (int)U2.nRows;
   const int64_t nnzU = (int64_t)U2.nnz();
   // full-symmetric rows: row i = (entries (j,i), j<i, from the upper rows j) ++ (upper row i)
   std::vector<int> cntLower(n2, 0);
@@ -50,25 +81,8 @@ void Ctx::setupKrylov() {
     }
   }
 
-  // what each block(2) row is: element DoF (pressure), node DoF, or MPC multiplier
-  std::vector<int8_t> rowKind(n2, -1);  // 0 node, 1 schur (element DoF / multiplier)
-  std::vector<uint8_t> rowType(n2, 0);
-  std::vector<uint32_t> rowNode(n2, 0);
-  for (uint32_t e = 0; e < m.nElements(); e++)
-    for (int t = 0; t < NUM_DOF_TYPES; t++) {
-      uint32_t q = m.elemEq[(size_t)e * 8 + t];
-      if (q > nC) rowKind[q - nC - 1] = 1;
-    }
-  for (uint32_t n = 0; n < m.nNodes(); n++)
-    for (int t = 0; t < NUM_DOF_TYPES; t++) {
-      uint32_t q = m.nodeEq[(size_t)n * 8 + t];
-      if (q > nC) {
-        rowKind[q - nC - 1] = 0;
-        rowType[q - nC - 1] = (uint8_t)t;
-        rowNode[q - nC - 1] = n;
-      }
-    }
-  for (auto& mp : m.mpcs) rowKind[mp.eqNum - nC - 1] = 1;
+  std::vector<int8_t> rowKind(n2);  // 0 node, 1 schur (element DoF / multiplier)
+  for (int i = 0; i < n2; i++) rowKind[i] = kind3[i] == 0 ? 0 : 1;
   auto sameNodeTriple = [&](int a, int b) {
     return rowKind[a] == 0 && rowKind[b] == 0 && rowNode[a] == rowNode[b] && rowType[a] / 3 == rowType[b] / 3;
   };
@@ -145,13 +159,10 @@ void Ctx::setupKrylov() {
   {  // learned-correction node list and row metadata
     std::vector<signed char> kind2(n2, 0);
     std::vector<int> node2(n2, -1), pg(n2, 0), lg, lr;
-    for (int i = 0; i < n2; i++) node2[i] = rowKind[i] == 0 ? (int)rowNode[i] : -1;
-    for (uint32_t e = 0; e < m.nElements(); e++)
-      for (int t = 0; t < NUM_DOF_TYPES; t++) {
-        uint32_t q = m.elemEq[(size_t)e * 8 + t];
-        if (q > nC) kind2[q - nC - 1] = 1;
-      }
-    for (auto& mp : m.mpcs) kind2[mp.eqNum - nC - 1] = 2;
+    for (int i = 0; i < n2; i++) {
+      node2[i] = rowKind[i] == 0 ? (int)rowNode[i] : -1;
+      kind2[i] = (signed char)std::max<int>(kind3[i], 0);
+    }
     for (int g = 0; g < nGroups; g++) {
       const int r0 = gStart[g];
       for (int a = 0; a < (gInfo[g] & 3); a++) pg[r0 + a] = g;
@@ -171,6 +182,21 @@ void Ctx::setupKrylov() {
   d_gInfo.upload(gInfo);
   d_gDiagPos.upload(gDiag);
   d_binv.alloc((size_t)nGroups * 9 * 8);
+  // MINRES work space
+  d_rhs.alloc((size_t)n2 * 8);
+  for (auto& b : d_mw) b.alloc((size_t)n2 * 8);
+  const int nPart = std::max(kKrylovBlocks, (std::max(n2, nGroups) + 255) / 256) + 16;
+  d_partial.alloc((size_t)nPart * 8);
+  d_mstate.alloc(sizeof(MinresState));
+  d_scal.alloc(16 * 8);
+  if (!h_mstate) hipCheck(hipHostMalloc((void**)&h_mstate, sizeof(MinresState), hipHostMallocDefault), "pinned");
+}
+
+// the rows of the model outside block(2): K12^T products for the rhs, reactions, boundary
+// conditions, Newton vectors and the block(2) positions of the MPC entries
+void Ctx::setupModelRows() {
+  FEStorage& m = model;
+  const uint32_t nC = m.nConstrainedDofs;
 
   // block(1,2) by columns, for rhs -= K12^T dUc (FESolver.cpp:454)
   const CsrPattern& B12 = m.block12;
@@ -243,14 +269,7 @@ void Ctx::setupKrylov() {
   hipCheck(hipMemset(d_R.p, 0, d_R.bytes), "memset R");
   d_Ucprev.alloc((size_t)std::max<uint32_t>(nC, 1) * 8);
   hipCheck(hipMemset(d_Ucprev.p, 0, d_Ucprev.bytes), "memset Ucprev");
-  d_rhs.alloc((size_t)n2 * 8);
   d_dU.alloc((size_t)m.nAll() * 8);
-  for (auto& b : d_mw) b.alloc((size_t)n2 * 8);
-  const int nPart = std::max(kKrylovBlocks, (std::max(n2, nGroups) + 255) / 256) + 16;
-  d_partial.alloc((size_t)nPart * 8);
-  d_mstate.alloc(sizeof(MinresState));
-  d_scal.alloc(16 * 8);
-  if (!h_mstate) hipCheck(hipHostMalloc((void**)&h_mstate, sizeof(MinresState), hipHostMallocDefault), "pinned");
   // block(2) positions of the MPC entries, mirrored after the MPC rows are set
   std::vector<int> mpcIdx;
   if (nCeRows > 0) {
@@ -267,7 +286,6 @@ void Ctx::setupKrylov() {
   }
   d_mpcMirIdx.upload(mpcIdx);
   nMpcMir = (int)mpcIdx.size();
-  krylovReady = true;
 }
 
 KrylovMatrix Ctx::krylovMatrix() const {
@@ -322,21 +340,18 @@ int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* 
     return !(e && e[0] == '0');
   }();
   const int period = 12;
-  hipGraphExec_t exec = nullptr;
+  StreamGraph graph;  // destroyed (and an open capture ended) on every exit, throws included
   if (graphs && maxit >= 4 * period) {
-    hipGraph_t g = nullptr;
     MinresWork w0 = w;
-    hipCheck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
+    graph.begin(stream);
     for (int k = 0; k < period; k++) minres_iteration(stream, M, P, w, tol, hook);
-    hipCheck(hipStreamEndCapture(stream, &g), "end capture");
-    hipCheck(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0), "instantiate");
-    (void)hipGraphDestroy(g);
+    graph.end();
     if (w.R1 != w0.R1 || w.Yp != w0.Yp || w.W1 != w0.W1) throw ApiError(-2, "MINRES rotation is not periodic");
   }
   int it = 0;
   for (;;) {
-    if (exec && it + period <= maxit) {
-      hipCheck(hipGraphLaunch(exec, stream), "graph launch");
+    if (graph && it + period <= maxit) {
+      graph.launch(stream);
       it += period;
     } else {
       for (int k = 0; k < period && it < maxit; k++, it++) minres_iteration(stream, M, P, w, tol, hook);
@@ -346,7 +361,6 @@ int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* 
     hipCheck(hipStreamSynchronize(stream), "minres sync");
     if (h_mstate->converged || it >= maxit) break;
   }
-  if (exec) (void)hipGraphExecDestroy(exec);
   if (relres_est) *relres_est = h_mstate->beta1 > 0 ? h_mstate->phibar / h_mstate->beta1 : 0.0;
   if (breakdown) *breakdown = h_mstate->breakdown != 0;
   return h_mstate->itn;

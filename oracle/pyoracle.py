@@ -471,10 +471,17 @@ def sparse_sym_pattern(n, max_in_row=100):
 class OracleModel:
     """FEStorage + NonlinearFESolver restated for small models."""
 
-    def __init__(self, xyz, elements, material=None, fixes=(), loads=(), rigid_bodies=(), elem_params=None, mpcs=()):
+    def __init__(self, xyz, elements, material=None, fixes=(), loads=(), rigid_bodies=(), elem_params=None, mpcs=(),
+                 max_in_row=100):
         """elements: list of (kind, [1-based nodes]); fixes/loads: (node, dof, value);
         rigid_bodies: (master, [slaves], [dofs]); material: (type, MC); mpcs: constant
-        linear MPCs ([(node, dof, coef), ...], b) as FEStorage::addMpc (FEStorage.cpp:175-179)."""
+        linear MPCs ([(node, dof, coef), ...], b) as FEStorage::addMpc (FEStorage.cpp:175-179);
+        max_in_row: the row capacity of BlockSparseSymMatrix<2> (BlockSparseMatrix.h:42, the
+        reference's 100), 0 for none (a rigid-body master with more than 100 slave rows)."""
+        self.max_in_row = max_in_row
+        # SOLID81-only models assemble through the C restatement of the element loop
+        # (identical additions in identical order; tests compare the two bit for bit)
+        self.fast_solid81 = True
         self.xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
         self.elements = [(k, list(map(int, n))) for k, n in elements]
         self.material = material
@@ -560,7 +567,8 @@ class OracleModel:
     # FEStorage::initSolutionData (FEStorage.cpp:434-550)
     def initSolutionData(self):
         nC, nS = self.nC, self.nU + self.nMpc
-        self.blocks = {1: SparsityInfo(nC, nC), 12: SparsityInfo(nC, nS), 2: SparsityInfo(nS, nS)}
+        cap = self.max_in_row if self.max_in_row > 0 else max(nC, nS, 1)
+        self.blocks = {1: SparsityInfo(nC, nC, cap), 12: SparsityInfo(nC, nS, cap), 2: SparsityInfo(nS, nS, cap)}
         topology = [set() for _ in range(len(self.xyz))]
         for e, (_, nodes) in enumerate(self.elements):
             for n in nodes:
@@ -627,12 +635,18 @@ class OracleModel:
         for b in self.blocks.values():
             b.values[:] = 0.0
         F = np.zeros(self.nAll)
-        ke = self.solid81_elements_at(U)
-        s = 0
         plane_only = all(kind != SOLID81 for kind, _ in self.elements)
+        solid_only = all(kind == SOLID81 for kind, _ in self.elements)
+        if solid_only and self.fast_solid81:
+            self._assemble_solid81_c(U, F)
+        else:
+            ke = self.solid81_elements_at(U)
+        s = 0
         if plane_only:
             self._assemble_plane41_batched(U, F)
         for e, (kind, nodes) in enumerate(self.elements):
+            if solid_only and self.fast_solid81:
+                break
             if kind == PLANE41 and plane_only:
                 continue
             if kind == PLANE41:  # PLANE41::assembleK (PLANE41.h:55-89)
@@ -671,6 +685,24 @@ class OracleModel:
             for node, dof, coef in terms:
                 self.addValueK(eq, self.nodeEq[node - 1][dof], coef)
         return {b: blk.values.copy() for b, blk in self.blocks.items()}, F
+
+    def _assemble_solid81_c(self, U, F):
+        """the same element loop as assemble()'s (SOLID81.h:54-94 through the binary-search
+        addValue, element order) run by the C restatement solid81_assemble_ref, for models of
+        SOLID81 elements only"""
+        nE = len(self.elements)
+        eq = np.zeros((nE, 25), dtype=np.uint32)
+        for e, (_, nodes) in enumerate(self.elements):
+            eq[e, :24] = [self.nodeEq[n - 1][d] for n in nodes for d in range(3)]
+            eq[e, 24] = self.elemEq[e][HYDRO_PRESSURE]
+        Xe = self.xyz[np.array([nodes for _, nodes in self.elements]) - 1].reshape(nE, 24)
+        Ue = U[eq[:, :24].astype(np.int64) - 1]
+        pe = U[eq[:, 24].astype(np.int64) - 1]
+        pats = [(self.blocks[b].iofeir, self.blocks[b].columns) for b in (1, 12, 2)]
+        vals, Fe = solid81_assemble_ref(Xe, Ue, pe, self.material[0], self.material[1], eq, self.nC, *pats, self.nAll)
+        for b in (1, 12, 2):
+            self.blocks[b].values[:] = vals[b]
+        F += Fe
 
     def _assemble_plane41_batched(self, U, F):
         """PLANE41::assembleK for every PLANE41 element of a model without SOLID81: the same

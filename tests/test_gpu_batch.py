@@ -7,6 +7,7 @@ import pytest
 
 import nla3d_amd
 from nla3d_amd import meshgen
+from oracle.pyoracle import OracleModel, load_steps
 
 pytestmark = pytest.mark.gpu
 
@@ -109,3 +110,48 @@ def test_autotune_leaves_models_unchanged(gpu_lib):
         mb.solve_nonlinear(opts)
         ua, ub = ma.get_solution(), mb.get_solution()
         assert np.abs(ua - ub).max() <= 1e-8 * np.abs(ub).max(), p
+
+
+def test_batch_c4_distributions_match_oracle(gpu_lib):
+    """Models drawn from the batched workload's distributions (n^3 cubes with n log-uniform,
+    here in [2, 6] so the oracle's dense solves finish in seconds; C10 ~ U[0.2, 1],
+    C01 ~ U[0, 0.3], K/C10 ~ U[50, 2000], top UZ strain ~ U[5 %, 30 %]; 10 load steps, which
+    TimeControl runs as 11), the batch's settings (criterion 1 to 1e-8, MINRES to 1e-9,
+    preconditioner 2) and a split that sends the three largest systems to the whole-device
+    solver and the rest to the team launch: every model's displacements after every load
+    step agree with the oracle's Newton loop (FP64 direct solves) within 1e-8 relative"""
+    print("[gpu] batch with the batched workload's distributions vs the oracle", flush=True)
+    seed = 20261203
+    models = meshgen.batch_models(9, seed=seed, n_range=(2, 6))
+    oracles = []
+    for i, (m, p) in enumerate(models):
+        xyz, conn, (I, J, K) = meshgen.hex_box(p["n"], p["n"], p["n"], jitter=0.1, seed=seed + 7919 * (i + 1))
+        fixes = [(int(nd), d, 0.0) for nd in np.nonzero(K == 0)[0] + 1 for d in range(3)]
+        fixes += [(int(nd), 2, p["strain"]) for nd in np.nonzero(K == p["n"])[0] + 1]
+        o = OracleModel(xyz, [(1, c) for c in conn], material=(3, [p["c10"], p["c01"], p["bulk"]]),
+                        fixes=fixes).setup()
+        o.newton_init()
+        oracles.append(o)
+    sizes = sorted(m.sizes["nUnknown"] for m, _ in models)
+    batch = nla3d_amd.Batch([m for m, _ in models])
+    batch.set_config(big_rows=sizes[-4])
+    whole = [k for k, (m, _) in enumerate(models) if m.sizes["nUnknown"] > sizes[-4]]
+    assert len(whole) >= 2 and len(whole) < len(models)
+    opts = nla3d_amd.GpuModel.newton_defaults(criterion=1, convergence=1e-8, krylov_tol=1e-9, krylov_max_iter=30000,
+                                              n_load_steps=10, max_iterations=15)
+    opts.preconditioner = 2
+    stats = batch.new_stats()
+    steps = load_steps(10)
+    assert len(steps) == 11
+    for t in steps:
+        _, conv = batch.solve_load_step(t, opts, stats)
+        assert np.all(conv == 1), (t, conv)
+        errs = []
+        for k, ((m, p), o) in enumerate(zip(models, oracles)):
+            o.solve_load_step(t, criterion=1, convergence=1e-8, max_iterations=15)
+            U = m.get_solution()
+            errs.append(np.abs(U - o.U).max() / np.abs(o.U).max())
+            assert errs[-1] <= 1e-8, (t, k, p, errs[-1])
+        print(f"t={t:.3f}: worst displacement error {max(errs):.2e}", flush=True)
+    assert all(s.precond_fallbacks == 0 for s in stats)
+    print("whole-device models", whole, "sizes", [m.sizes["nUnknown"] for m, _ in models], flush=True)

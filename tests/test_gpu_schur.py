@@ -121,3 +121,53 @@ def test_smoother_timing_entry(gpu_lib):
     assert ms > 0.0
     # at least the FP32 values of every entry and seven vectors
     assert nbytes >= 4 * info["nnz0"] + 7 * 8 * info["nq"]
+
+
+def a2000_damper_model():
+    import os
+    from nla3d_amd.cdb import read_cdb
+    d = read_cdb(os.path.join(os.path.dirname(__file__), "golden", "a2000_damper", "model.cdb.gz"))
+    m = nla3d_amd.GpuModel(0)
+    m.set_nodes(d["nodes"])
+    m.add_elements(nla3d_amd.PLANE41, [n for t, n in d["cells"] if t == 1])
+    m.set_material(nla3d_amd.NEO_HOOKEAN, [1.0, 500.0])
+    for n, dof, v in d["fixes"]:
+        m.add_fix(n, dof, v)
+    m.init_solution()
+    opts = m.newton_defaults(criterion=1, convergence=1e-8, krylov_tol=1e-12, krylov_max_iter=20000)
+    m.newton_iteration(0.3, 1, ctypes.c_double(0.0), opts, nla3d_amd.NewtonStats())
+    return m
+
+
+@pytest.mark.parametrize("case", ["box_30x30x15", "a2000_damper"])
+def test_vcycle_matches_oracle_on_deep_hierarchies(gpu_lib, case):
+    """The V-cycle against the oracle where the short tests do not reach: a 30x30x15
+    rigid-top box (16,383 Schur rows: an odd, 9-term finest sweep, which ends on a step that
+    leaves x alone) and the reference's a2000_damper model (PLANE41, 3,101 Schur rows: the
+    coarse levels below a smoothed one run as one workgroup launch, fuse_from >= 1)"""
+    print(f"[gpu] schur v-cycle, {case}", flush=True)
+    m = rigid_model(30, 30, 15) if case == "box_30x30x15" else a2000_damper_model()
+    info = m.schur_info()
+    steps, fuse = m.schur_cycle_info()
+    print("hierarchy", info, "finest steps", steps, "fused from", fuse, flush=True)
+    assert info["levels"] >= 3
+    if case == "box_30x30x15":
+        assert steps == ps.finest_steps(info["nq"]) == 9
+    else:
+        assert steps == ps.finest_steps(info["nq"]) and 1 <= fuse < info["levels"] - 1
+    qrow, rp, col, val = m.schur_matrix()
+    S = sp.csr_matrix((val, col, rp), shape=(len(qrow), len(qrow)))
+    aggs, lmaxs = [], []
+    for l in range(info["levels"] - 1):
+        agg, lm = m.schur_level(l, info["sizes"][l])
+        aggs.append(agg.astype(np.int64))
+        lmaxs.append(lm)
+    mats = ps.hierarchy(S, aggs)
+    assert [A.shape[0] for A in mats] == info["sizes"]
+    rng = np.random.default_rng(9)
+    y, z = rng.standard_normal(len(qrow)), rng.standard_normal(len(qrow))
+    by, bz = m.schur_apply(y), m.schur_apply(z)
+    bo = ps.vcycle(mats, aggs, lmaxs, y)
+    assert np.abs(by - bo).max() <= 1e-9 * np.abs(bo).max()
+    assert abs(z @ by - y @ bz) <= 1e-10 * abs(z @ by)
+    assert y @ by > 0 and z @ bz > 0

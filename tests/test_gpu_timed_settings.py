@@ -1,0 +1,57 @@
+"""The metric workload's timed settings against the oracle at a size the oracle's direct
+Newton runs in seconds: the 2 x 2 x 1 box with a rigid-body top (+15 % UZ, 20 degree ROTZ),
+Mooney-Rivlin C10 = 0.5, C01 = 0.1, K = 1000, meshed 12 x 12 x 6 with the same jitter and
+seed, over the full 20-step schedule; Newton to a 1e-8 relative residual (criterion 1),
+MINRES to 1e-9 with block Jacobi + the Schur-row multigrid (preconditioner 2).  After every
+load step (FESolver.cpp:457-483) the displacements must agree with the oracle's (FP64
+direct solves in place of PARDISO) within 1e-8 relative (max-abs over the solution vector),
+with no fallback to plain block Jacobi."""
+import numpy as np
+import pytest
+
+import nla3d_amd
+from nla3d_amd import meshgen
+from oracle.pyoracle import OracleModel, load_steps
+
+pytestmark = pytest.mark.gpu
+
+UZ, ROTZ = 0.15, float(np.deg2rad(20.0))
+MAT = (3, [0.5, 0.1, 1000.0])
+
+
+def c3_like(nx, ny, nz):
+    m = nla3d_amd.GpuModel(0)
+    xyz, conn = meshgen.solid81_rigid_top_case(m, nx, ny, nz, 2.0, 2.0, 1.0, UZ, ROTZ, material=MAT)
+    m.set_max_in_row(0)
+    m.init_solution()
+    master = len(xyz)
+    z = xyz[:-1, 2]
+    fixes = [(int(nd), d, 0.0) for nd in np.nonzero(z == 0.0)[0] + 1 for d in range(3)]
+    fixes += [(master, d, v) for d, v in zip(range(6), [0, 0, UZ, 0, 0, ROTZ])]
+    top = list(np.nonzero(z == 1.0)[0] + 1)
+    o = OracleModel(xyz, [(1, c) for c in conn], material=MAT, fixes=fixes, rigid_bodies=[(master, top, [0, 1, 2])],
+                    max_in_row=0).setup()
+    return m, o
+
+
+@pytest.mark.timeout(600)
+def test_timed_settings_match_oracle_every_load_step(gpu_lib):
+    print("[gpu] timed settings (criterion 1e-8, MINRES 1e-9, preconditioner 2), 12x12x6, 20 load steps",
+          flush=True)
+    m, o = c3_like(12, 12, 6)
+    o.newton_init()
+    opts = m.newton_defaults(criterion=1, convergence=1e-8, krylov_tol=1e-9, krylov_max_iter=100000,
+                             preconditioner=2)
+    st = nla3d_amd.NewtonStats()
+    worst = 0.0
+    for t in load_steps(20):
+        m.solve_load_step(t, opts, st)
+        it = o.solve_load_step(t, criterion=1, convergence=1e-8, max_iterations=20)
+        U = m.get_solution()
+        err = np.abs(U - o.U).max() / np.abs(o.U).max()
+        worst = max(worst, err)
+        print(f"t={t:.3f}: {st.last_step_iterations} solves (oracle {it}), last MINRES "
+              f"{st.last_krylov_iterations}, displacement error {err:.2e}", flush=True)
+        assert err <= 1e-8, (t, err)
+    assert st.precond_fallbacks == 0 and st.load_steps == 20
+    print("worst relative error over the schedule", worst, flush=True)

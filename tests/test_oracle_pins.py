@@ -419,3 +419,15 @@ def test_node_features_closed_form_on_a_unit_cube():
         # Schur estimate of the pressure row: |K_pp| + sum over its 12 free columns (1/16) / s
         d = 1.0 / k + 12.0 * (1.0 / 16.0) / s
         assert abs(f[23] - (1.0 / k) / d) <= 1e-14
+
+
+def test_oracle_assembly_c_loop_is_the_python_loop():
+    """OracleModel.assemble runs SOLID81-only models through the C restatement of the element
+    loop; the Python loop (per-entry binary-search addValue in assemble3 order) must give the
+    same K blocks and F bit for bit, with rigid-body rows"""
+    o, _, _, _ = oracle_box(3, 2, 2, U_scale=0.0)
+    U = 0.02 * np.random.default_rng(1).standard_normal(o.nAll)
+    Kc, Fc = o.assemble(U)
+    o.fast_solid81 = False
+    Kp, Fp = o.assemble(U)
+    assert all(np.array_equal(Kc[b], Kp[b]) for b in (1, 12, 2)) and np.array_equal(Fc, Fp)
