@@ -197,6 +197,62 @@ int nla3d_gpu_get_R(nla3d_gpu_t h, double* R);
  * still assembled it is the system that iteration solved. */
 int nla3d_gpu_get_rhs(nla3d_gpu_t h, double* rhs);
 
+/* ---- equation solver on a caller's matrix ---------------------------------------
+ * EquationSolver (math/EquationSolver.h:19-78) for K.block(2) as nla3d's FESolver hands it over
+ * (FESolver.cpp:377, 457, 601, 613): n rows of a SparseSymMatrix as its raw 1-based upper CSR,
+ * values[nnz], iofeir[n+1], columns[nnz] (getValuesArray / getIofeirArray / getColumnsArray,
+ * SparseMatrix.h:114-116), host arrays read during the call.  factorize = PARDISO's phases 11
+ * (first call, or a changed pattern: the pattern is analysed and device storage allocated) and
+ * 22 (every call: values copied to the device, block-Jacobi blocks inverted, Schur-row multigrid
+ * refreshed); substitute = phase 33 for one right-hand side (MINRES); solve = both
+ * (EquationSolver.cpp:141-251).  Errors are return codes instead of the reference's CHECK:
+ * NLA3D_GPU_EPATTERN for arrays that break SparsityInfo's invariants (1-based, sorted, upper),
+ * NLA3D_GPU_ESTATE for substitute before factorize, NLA3D_GPU_ENOTCONV when MINRES does not
+ * reach tol (the unknowns then hold its last iterate).  One HIP stream per solver handle. */
+typedef struct nla3d_gpu_eqsolver* nla3d_gpu_eqsolver_t;
+typedef struct {
+  double tol;          /* MINRES: preconditioned residual <= tol * initial (default 1e-12) */
+  int max_iter;        /* default 20000 */
+  int preconditioner;  /* 0 (default): block Jacobi (3x3 node blocks, the diagonal Schur estimate on the
+                          other rows); 2: plus one V-cycle of the Schur-row multigrid on the rows that are
+                          not node DoFs (a solve it fails is repeated with block Jacobi) */
+} nla3d_gpu_eqsolver_opts;
+typedef struct {
+  int iterations;         /* MINRES iterations of the last substitute (both runs after a fallback) */
+  double relres;          /* its preconditioned residual estimate relative to the initial one */
+  int precond_fallbacks;  /* 1 or 2 when preconditioner 2 was unavailable or failed */
+  int breakdown;          /* the preconditioner was not positive definite */
+  double ms_factorize, ms_substitute;  /* host wall time of the last calls */
+} nla3d_gpu_eqsolver_stats;
+
+void nla3d_gpu_eqsolver_defaults(nla3d_gpu_eqsolver_opts* opts);
+/* EquationSolver's constructor: a solver on `device` (no host-only form) */
+int nla3d_gpu_eqsolver_create(nla3d_gpu_eqsolver_t* solver, int device);
+int nla3d_gpu_eqsolver_destroy(nla3d_gpu_eqsolver_t solver);
+const char* nla3d_gpu_eqsolver_last_error(nla3d_gpu_eqsolver_t solver);
+/* setSymmetric / setPositive have no counterpart: MINRES takes symmetric indefinite matrices */
+int nla3d_gpu_eqsolver_set_options(nla3d_gpu_eqsolver_t solver, const nla3d_gpu_eqsolver_opts* opts);
+/* Optional description of the rows for the preconditioner, FEStorage's numbering read backwards
+ * (FEStorage::getNodeDofEqNumber / getElementDofEqNumber, FEStorage.h:148-149): row i of block(2)
+ * (equation nC + 1 + i) is the DoF row_dof[i] (Dof::dofType, Dof.h:15-25) of node row_node[i]
+ * (1-based), or with row_node[i] = 0 an element DoF (row_dof[i] = 6, HYDRO_PRESSURE) or an MPC
+ * multiplier (row_dof[i] = -1).  n = 0 clears the map: rows are then classified from the matrix
+ * (up to three consecutive rows with identical column lists are one node's DoFs; a row of its own
+ * with a positive diagonal is a one-DoF node; the others are pressure/multiplier rows).  Takes
+ * effect at the next factorize. */
+int nla3d_gpu_eqsolver_set_dof_map(nla3d_gpu_eqsolver_t solver, uint32_t n, const uint32_t* row_node,
+                                   const int* row_dof);
+/* EquationSolver::factorizeEquations(matrix) (EquationSolver.h:23) */
+int nla3d_gpu_eqsolver_factorize(nla3d_gpu_eqsolver_t solver, uint32_t n, const double* values, const uint32_t* iofeir,
+                                 const uint32_t* columns);
+/* EquationSolver::substituteEquations(matrix, rhs, unknowns) (EquationSolver.h:24) with the matrix of the
+ * last factorize: rhs[n] -> unknowns[n] */
+int nla3d_gpu_eqsolver_substitute(nla3d_gpu_eqsolver_t solver, const double* rhs, double* unknowns);
+/* EquationSolver::solveEquations(matrix, rhs, unknowns) (EquationSolver.h:22): factorize + substitute */
+int nla3d_gpu_eqsolver_solve(nla3d_gpu_eqsolver_t solver, uint32_t n, const double* values, const uint32_t* iofeir,
+                             const uint32_t* columns, const double* rhs, double* unknowns);
+int nla3d_gpu_eqsolver_get_stats(nla3d_gpu_eqsolver_t solver, nla3d_gpu_eqsolver_stats* stats);
+
 /* ---- batched models ---------------------------------------------------------
  * Many independent SOLID81 models (no MPCs) on one device, each handle set up as
  * usual through nla3d_gpu_init_solution; the batch does not own them.  One

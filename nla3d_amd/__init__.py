@@ -72,6 +72,17 @@ class BatchConfig(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class EqSolverOpts(ctypes.Structure):
+    """nla3d_gpu_eqsolver_opts (include/nla3d_gpu.h)"""
+    _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int), ("preconditioner", ctypes.c_int)]
+
+
+class EqSolverStats(ctypes.Structure):
+    """nla3d_gpu_eqsolver_stats (include/nla3d_gpu.h)"""
+    _fields_ = [("iterations", ctypes.c_int), ("relres", ctypes.c_double), ("precond_fallbacks", ctypes.c_int),
+                ("breakdown", ctypes.c_int), ("ms_factorize", ctypes.c_double), ("ms_substitute", ctypes.c_double)]
+
+
 _P = ctypes.c_void_p
 _U32P = ctypes.POINTER(ctypes.c_uint32)
 _U64P = ctypes.POINTER(ctypes.c_uint64)
@@ -161,8 +172,20 @@ ABI.update({
                                         _IP],
     "nla3d_gpu_batch_solve_nonlinear": [_P, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats), _IP],
 })
+# equation solver on a caller's matrix (first argument: solver handle)
+ABI.update({
+    "nla3d_gpu_eqsolver_destroy": [_P],
+    "nla3d_gpu_eqsolver_set_options": [_P, ctypes.POINTER(EqSolverOpts)],
+    "nla3d_gpu_eqsolver_set_dof_map": [_P, ctypes.c_uint32, _U32P, _IP],
+    "nla3d_gpu_eqsolver_factorize": [_P, ctypes.c_uint32, _DP, _U32P, _U32P],
+    "nla3d_gpu_eqsolver_substitute": [_P, _DP, _DP],
+    "nla3d_gpu_eqsolver_solve": [_P, ctypes.c_uint32, _DP, _U32P, _U32P, _DP, _DP],
+    "nla3d_gpu_eqsolver_get_stats": [_P, ctypes.POINTER(EqSolverStats)],
+})
 # functions that take no handle
-ABI_FREE = {"nla3d_gpu_newton_defaults": [ctypes.POINTER(NewtonOpts)],
+ABI_FREE = {"nla3d_gpu_eqsolver_create": [ctypes.POINTER(_P), ctypes.c_int],
+            "nla3d_gpu_eqsolver_last_error": [_P],
+            "nla3d_gpu_newton_defaults": [ctypes.POINTER(NewtonOpts)],
             "nla3d_gpu_transient_defaults": [ctypes.POINTER(TransientOpts)],
             "nla3d_gpu_mlp_num_params": [],
             "nla3d_gpu_mlp_create": [ctypes.POINTER(_P), ctypes.c_int, ctypes.c_ulonglong],
@@ -192,6 +215,9 @@ def lib():
         L.nla3d_gpu_last_error.restype = ctypes.c_char_p
         L.nla3d_gpu_mlp_last_error.restype = ctypes.c_char_p
         L.nla3d_gpu_batch_last_error.restype = ctypes.c_char_p
+        L.nla3d_gpu_eqsolver_last_error.restype = ctypes.c_char_p
+        L.nla3d_gpu_eqsolver_defaults.argtypes = [ctypes.POINTER(EqSolverOpts)]
+        L.nla3d_gpu_eqsolver_defaults.restype = None
         _lib = L
     return _lib
 
@@ -691,3 +717,79 @@ class Batch:
         conv = np.zeros(self.n, dtype=np.int32)
         self._call("nla3d_gpu_batch_solve_nonlinear", ctypes.byref(opts), stats, _ptr(conv, ctypes.c_int))
         return stats, conv
+
+
+class EqSolver:
+    """nla3d_gpu_eqsolver_t: EquationSolver (math/EquationSolver.h:19-78) on a caller's
+    K.block(2) given as the reference's 1-based upper CSR (values, iofeir, columns)."""
+
+    def __init__(self, device=0, tol=1e-12, max_iter=20000, preconditioner=0):
+        self._L = lib()
+        self._h = _P()
+        rc = self._L.nla3d_gpu_eqsolver_create(ctypes.byref(self._h), int(device))
+        if rc != 0:
+            raise Nla3dGpuError(rc, f"cannot create an equation solver on device {device}")
+        self.set_options(tol=tol, max_iter=max_iter, preconditioner=preconditioner)
+
+    def _call(self, name, *args):
+        rc = getattr(self._L, name)(self._h, *args)
+        if rc != 0:
+            raise Nla3dGpuError(rc, self._L.nla3d_gpu_eqsolver_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self._L.nla3d_gpu_eqsolver_destroy(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_options(self, tol=1e-12, max_iter=20000, preconditioner=0):
+        o = EqSolverOpts(float(tol), int(max_iter), int(preconditioner))
+        self._call("nla3d_gpu_eqsolver_set_options", ctypes.byref(o))
+
+    def set_dof_map(self, row_node=None, row_dof=None):
+        """row_node (1-based node, 0 for element DoFs / multipliers) and row_dof (Dof type, 6 for an
+        element pressure, -1 for a multiplier) per block(2) row; None clears the map"""
+        if row_node is None:
+            self._call("nla3d_gpu_eqsolver_set_dof_map", 0, None, None)
+            return
+        rn = np.ascontiguousarray(row_node, dtype=np.uint32)
+        rd = np.ascontiguousarray(row_dof, dtype=np.int32)
+        self._call("nla3d_gpu_eqsolver_set_dof_map", len(rn), _ptr(rn, ctypes.c_uint32), _ptr(rd, ctypes.c_int))
+
+    @staticmethod
+    def _csr(values, iofeir, columns):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        io = np.ascontiguousarray(iofeir, dtype=np.uint32)
+        c = np.ascontiguousarray(columns, dtype=np.uint32)
+        return v, io, c
+
+    def factorize(self, values, iofeir, columns):
+        v, io, c = self._csr(values, iofeir, columns)
+        self._keep = (v, io, c)
+        self._n = len(io) - 1
+        self._call("nla3d_gpu_eqsolver_factorize", self._n, _ptr(v, ctypes.c_double), _ptr(io, ctypes.c_uint32),
+                   _ptr(c, ctypes.c_uint32))
+
+    def substitute(self, rhs):
+        b = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.zeros_like(b)
+        self._call("nla3d_gpu_eqsolver_substitute", _ptr(b, ctypes.c_double), _ptr(x, ctypes.c_double))
+        return x
+
+    def solve(self, values, iofeir, columns, rhs):
+        v, io, c = self._csr(values, iofeir, columns)
+        b = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.zeros_like(b)
+        self._call("nla3d_gpu_eqsolver_solve", len(io) - 1, _ptr(v, ctypes.c_double), _ptr(io, ctypes.c_uint32),
+                   _ptr(c, ctypes.c_uint32), _ptr(b, ctypes.c_double), _ptr(x, ctypes.c_double))
+        return x
+
+    def stats(self):
+        st = EqSolverStats()
+        self._call("nla3d_gpu_eqsolver_get_stats", ctypes.byref(st))
+        return st

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void mirror_kernel(int n, const double* __rest
                                                      double* __restrict__ full) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int k = idx[i];
+  const int k = idx ? idx[i] : i;
   const double v = upper[k];
   full[mir[2 * k]] = v;
   full[mir[2 * k + 1]] = v;

@@ -189,6 +189,9 @@ struct Ctx {
   std::shared_ptr<SchurAmg> schur;
 };
 
+// full-symmetric rows of a 1-based upper CSR (solver.cpp)
+void fullSymmetricPattern(const CsrPattern& U2, std::vector<int64_t>& rp, std::vector<int>& col,
+                          std::vector<int64_t>* mirFull);
 // multigrid on the Schur rows (schur.cpp): structure from the full block(2) pattern and the
 // block-Jacobi groups; refresh for the current tangent (block Jacobi current); the MINRES stage
 void setupSchur(Ctx& c, const std::vector<int64_t>& rp, const std::vector<int>& col,

@@ -84,7 +84,7 @@ void launch_gather_batched(hipStream_t stream, int threads, int nModels, const G
 void launch_segmented_gather(hipStream_t stream, int nTargets, const int* seg, const int* src,
                              const double* src_vals, double* out, int mirBegin, int mirEnd, const int* mir,
                              double* full);
-// full[mir[2k]] = full[mir[2k+1]] = upper2[k] for k = idx[0..n)
+// full[mir[2k]] = full[mir[2k+1]] = upper2[k] for k = idx[0..n) (idx null: k = 0..n)
 void launch_mirror(hipStream_t stream, int n, const double* upper2, const int* idx, const int* mir, double* full);
 // constant MPC rows (FEStorage.cpp:51-61): row r adds coef[k] at vals[tgt[k]] for its terms
 // k in [termPtr[r], termPtr[r+1]) in order, and sets vals[fTgt[r]] = b[r]

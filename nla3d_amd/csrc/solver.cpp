@@ -41,6 +41,42 @@ void Ctx::setupKrylov() {
   krylovReady = true;
 }
 
+// Full-symmetric rows (0-based) of a 1-based upper CSR: row i = the entries (j, i), j < i, of
+// the upper rows j, then upper row i, so every row stays sorted.  mirFull (may be null)
+// receives the two full positions of each upper entry (the same one twice on the diagonal).
+void fullSymmetricPattern(const CsrPattern& U2, std::vector<int64_t>& rp, std::vector<int>& col,
+                          std::vector<int64_t>* mirFull) {
+  const int n = (int)U2.nRows;
+  const int64_t nnzU = (int64_t)U2.nnz();
+  std::vector<int> cntLower(n, 0);
+  for (int i = 0; i < n; i++)
+    for (uint32_t k = U2.iofeir[i] - 1; k < U2.iofeir[i + 1] - 1; k++) {
+      int c = (int)U2.columns[k] - 1;
+      if (c != i) cntLower[c]++;
+    }
+  rp.assign(n + 1, 0);
+  for (int i = 0; i < n; i++) rp[i + 1] = rp[i] + cntLower[i] + (U2.iofeir[i + 1] - U2.iofeir[i]);
+  col.assign(rp[n], 0);
+  if (mirFull) mirFull->assign(2 * nnzU, 0);
+  std::vector<int> fillLower(n, 0);
+  for (int i = 0; i < n; i++) {
+    const int64_t base = rp[i] + cntLower[i];
+    for (uint32_t k = U2.iofeir[i] - 1; k < U2.iofeir[i + 1] - 1; k++) {
+      const int c = (int)U2.columns[k] - 1;
+      const int64_t pu = base + (k - (U2.iofeir[i] - 1));
+      col[pu] = c;
+      if (mirFull) (*mirFull)[2 * k] = pu;
+      if (c == i) {
+        if (mirFull) (*mirFull)[2 * k + 1] = pu;
+      } else {
+        const int64_t pl = rp[c] + fillLower[c]++;
+        col[pl] = i;  // rows are visited in order, so each row's lower part stays sorted
+        if (mirFull) (*mirFull)[2 * k + 1] = pl;
+      }
+    }
+  }
+}
+
 // The block(2) solve's device data from its upper CSR pattern (1-based, the reference's
 // layout) and a description of each row (kind3: 0 node DoF, 1 element DoF, 2 multiplier;
 // the DoF type and node of a node-DoF row): the grouped full-symmetric copy the SpMV reads
@@ -50,36 +86,11 @@ void Ctx::setupBlock2(const CsrPattern& U2, const std::vector<int8_t>& kind3, co
                       const std::vector<uint32_t>& rowNode) {
   n2 = (int)U2.nRows;
   const int64_t nnzU = (int64_t)U2.nnz();
-  // full-symmetric rows: row i = (entries (j,i), j<i, from the upper rows j) ++ (upper row i)
-  std::vector<int> cntLower(n2, 0);
-  for (int i = 0; i < n2; i++)
-    for (uint32_t k = U2.iofeir[i] - 1; k < U2.iofeir[i + 1] - 1; k++) {
-      int c = (int)U2.columns[k] - 1;
-      if (c != i) cntLower[c]++;
-    }
-  std::vector<int64_t> rp(n2 + 1, 0);
-  for (int i = 0; i < n2; i++) rp[i + 1] = rp[i] + cntLower[i] + (U2.iofeir[i + 1] - U2.iofeir[i]);
+  std::vector<int64_t> rp, mirFull;
+  std::vector<int> col;
+  fullSymmetricPattern(U2, rp, col, &mirFull);
   fullNnz2 = rp[n2];
   if (fullNnz2 >= (1ll << 31)) throw ApiError(-1, "block(2) too large for 32-bit offsets");
-  std::vector<int> col(fullNnz2);
-  std::vector<int64_t> mirFull(2 * nnzU);  // full (row-major) positions of each upper entry
-  std::vector<int> fillLower(n2, 0);
-  for (int i = 0; i < n2; i++) {
-    const int64_t base = rp[i] + cntLower[i];
-    for (uint32_t k = U2.iofeir[i] - 1; k < U2.iofeir[i + 1] - 1; k++) {
-      const int c = (int)U2.columns[k] - 1;
-      const int64_t pu = base + (k - (U2.iofeir[i] - 1));
-      col[pu] = c;
-      mirFull[2 * k] = pu;
-      if (c == i) {
-        mirFull[2 * k + 1] = pu;
-      } else {
-        const int64_t pl = rp[c] + fillLower[c]++;
-        col[pl] = i;  // rows are visited in order, so each row's lower part stays sorted
-        mirFull[2 * k + 1] = pl;
-      }
-    }
-  }
 
   std::vector<int8_t> rowKind(n2);  // 0 node, 1 schur (element DoF / multiplier)
   for (int i = 0; i < n2; i++) rowKind[i] = kind3[i] == 0 ? 0 : 1;

@@ -806,9 +806,12 @@ class OracleModel:
         self.U = np.zeros(self.nAll)
         self.Ucprev = np.zeros(self.nC)
 
-    def solve_load_step(self, t, criterion=0, convergence=1e-3, max_iterations=20):
+    def solve_load_step(self, t, criterion=0, convergence=1e-3, max_iterations=20, linear_solve=None):
         """one load step at normalised time t; a direct FP64 solve (dense, or SuperLU above
-        SPARSE_ABOVE unknowns) stands in for PARDISO"""
+        SPARSE_ABOVE unknowns) stands in for PARDISO.  linear_solve(values2, rhs) -> dU, if
+        given, replaces that solve (EquationSolver::solveEquations at FESolver.cpp:457 with
+        K.block(2)'s upper CSR values on self.blocks[2]'s pattern): the hook through which a
+        test runs this loop with another equation solver"""
         nC, nU = self.nC, self.nU
         U = self.U
         rhs1 = None
@@ -831,7 +834,10 @@ class OracleModel:
                 rhs1 = rn
             if criterion == 1 and it > 1 and rn <= convergence * rhs1:
                 return it - 1
-            dU = spla.spsolve(K2.tocsc(), rhs) if sp.issparse(K2) else np.linalg.solve(K2, rhs)
+            if linear_solve is not None:
+                dU = linear_solve(K[2], rhs)
+            else:
+                dU = spla.spsolve(K2.tocsc(), rhs) if sp.issparse(K2) else np.linalg.solve(K2, rhs)
             U[nC:nC + nU] += dU[:nU]
             U[nC + nU:] = dU[nU:]
             # constrained-DoF reactions: vecRc = K11 dUc + K12 dUsl - Fc (FESolver.cpp:464-467)

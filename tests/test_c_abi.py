@@ -82,3 +82,88 @@ def test_newton_from_c_matches_oracle(gpu_lib, tmp_path):
         o.solve_load_step(t, criterion=1, convergence=1e-10, max_iterations=20)
     assert U.shape == o.U.shape
     assert np.abs(U - o.U).max() <= 1e-8 * np.abs(o.U).max()
+
+
+# ---------------------------------------------------------------------------
+# the solver-level entry points from plain C (tests/c_abi/csr_solve.c)
+# ---------------------------------------------------------------------------
+def csr_caller():
+    exe = os.path.join(ROOT, "tests", "c_abi", "csr_solve")
+    if not os.path.exists(exe):
+        pytest.skip("tests/c_abi/csr_solve is not built (run __graft_entry__.build())")
+    return exe
+
+
+def saddle_case(seed=3):
+    """K.block(2) of a jittered 3x3x2 SOLID81 box with a rigid-body top (pressure rows, node
+    rows, multiplier rows without a diagonal), assembled by the oracle at a seeded state, its
+    DoF map read off the oracle's numbering, and two right-hand sides"""
+    xyz, conn, (I, J, K) = meshgen.hex_box(3, 3, 2, 2.0, 2.0, 1.0, jitter=0.1, seed=seed)
+    xyz = np.vstack([xyz, [[1.0, 1.0, 1.0]]])
+    master = len(xyz)
+    fixes = [(int(n), d, 0.0) for n in np.nonzero(K == 0)[0] + 1 for d in range(3)]
+    fixes += [(master, d, v) for d, v in enumerate([0.0, 0.0, 0.15, 0.0, 0.0, 0.349])]
+    o = po.OracleModel(xyz, [(po.SOLID81, c) for c in conn], material=(3, [0.5, 0.1, 1000.0]), fixes=fixes,
+                       rigid_bodies=[(master, list(np.nonzero(K == 2)[0] + 1), [0, 1, 2])]).setup()
+    U = 0.01 * np.random.default_rng(seed).standard_normal(o.nAll)
+    Kv, _ = o.assemble(U)
+    n2 = o.nU + o.nMpc
+    row_node, row_dof = np.zeros(n2, dtype=np.int64), np.full(n2, -1, dtype=np.int64)
+    for n, d in enumerate(o.nodeEq):
+        for t, q in d.items():
+            if q > o.nC:
+                row_node[q - o.nC - 1], row_dof[q - o.nC - 1] = n + 1, t
+    for d in o.elemEq:
+        for t, q in d.items():
+            row_dof[q - o.nC - 1] = t
+    rng = np.random.default_rng(seed + 1)
+    return o, Kv[2], row_node, row_dof, rng.standard_normal(n2), rng.standard_normal(n2)
+
+
+def write_case(path, o, vals, rhs1, rhs2, row_node, row_dof):
+    si = o.blocks[2]
+    n = len(si.iofeir) - 1
+    with open(path, "w") as f:
+        f.write(f"{n} {len(si.columns)}\n")
+        f.write(" ".join(str(int(v)) for v in si.iofeir) + "\n")
+        f.write(" ".join(str(int(v)) for v in si.columns) + "\n")
+        f.write(" ".join(repr(float(v)) for v in vals) + "\n")
+        f.write(" ".join(repr(float(v)) for v in np.concatenate([rhs1, rhs2])) + "\n")
+        f.write(" ".join(f"{int(a)} {int(b)}" for a, b in zip(row_node, row_dof)) + "\n")
+
+
+def test_csr_caller_reports_a_missing_device(tmp_path):
+    """without a usable device the caller stops with the library's status (no crash)"""
+    o, vals, rn, rd, b1, b2 = saddle_case()
+    path = tmp_path / "case.txt"
+    write_case(path, o, vals, b1, b2, rn, rd)
+    r = subprocess.run([csr_caller(), "-1", str(path), "0", "1"], capture_output=True, text=True)
+    assert r.returncode == 3 and "nla3d_gpu_eqsolver_create failed" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pc,use_map", [(0, 0), (0, 1), (2, 0), (2, 1)])
+def test_csr_caller_solves_match_dense_solve(tmp_path, pc, use_map):
+    """factorize once + substitute two right-hand sides, then solve with every value doubled, on
+    the caller's 1-based upper CSR: each solution within 1e-8 (relative, max-abs) of the
+    oracle's dense FP64 solve of the same matrix"""
+    print(f"[gpu] C caller csr_solve, preconditioner {pc}, DoF map {use_map}", flush=True)
+    o, vals, rn, rd, b1, b2 = saddle_case()
+    path = tmp_path / "case.txt"
+    write_case(path, o, vals, b1, b2, rn, rd)
+    r = subprocess.run([csr_caller(), "0", str(path), str(pc), str(use_map)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    xs = {0: {}, 1: {}, 2: {}}
+    for line in r.stdout.splitlines():
+        f = line.split()
+        if f[0].startswith("x"):
+            xs[int(f[0][1:])][int(f[1])] = float(f[2])
+        elif f[0] == "iterations":
+            print(line, flush=True)
+            assert int(f[3]) == 0  # no fallback: the multigrid (pc 2) ran every solve
+    A = po.dense_sym(o.blocks[2], vals)
+    for k, (Ak, b) in enumerate(((A, b1), (A, b2), (2.0 * A, b1))):
+        x = np.array([xs[k][i] for i in range(len(b))])
+        xr = np.linalg.solve(Ak, b)
+        assert np.abs(x - xr).max() <= 1e-8 * np.abs(xr).max(), k
