@@ -187,6 +187,18 @@ int nla3d_gpu_solve_nonlinear(nla3d_gpu_t h, const nla3d_gpu_newton_opts* opts, 
  * length nUnknown + nMpc.  iters / relres (preconditioned estimate) may be NULL. */
 int nla3d_gpu_solve_block2(nla3d_gpu_t h, const double* rhs, double* x, double tol, int max_iter, int* iters,
                            double* relres);
+/* No counterpart in the reference: nla3d_gpu_solve_block2 for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops, harnesses that forbid it).
+ * rhs and x are DEVICE arrays of length nUnknown + nMpc on the handle's device (x is
+ * overwritten); the handle must be initialised (nla3d_gpu_init_solution) and assembled.  The
+ * solve calls no hipMalloc / hipFree and instantiates no graph: its MINRES work space is a
+ * static per-device array of NLA3D_GPU_NOALLOC_MAX_ROWS rows (56 MiB on every GPU the library
+ * runs on, held until the process ends); larger systems return NLA3D_GPU_EINVAL.  Calls on one
+ * GPU share that array: calls that overlap in time on one GPU race.  Synchronous (reads the
+ * state back every 12 iterations). */
+#define NLA3D_GPU_NOALLOC_MAX_ROWS (1 << 20)
+int nla3d_gpu_solve_block2_noalloc(nla3d_gpu_t h, const double* rhs, double* x, double tol, int max_iter,
+                                   int* iters, double* relres);
 /* vecR (length nDofs + nMpc, FEStorage::getR): the constrained-DoF reactions of the last
  * Newton iteration, vecRc = K11 dUc + K12 dUsl - Fc (FESolver.cpp:464-467), and the
  * applied loads at the other equations; FEStorage::getReaction(node, dof) is

@@ -72,6 +72,10 @@ class BatchConfig(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+# rows of the largest block(2) nla3d_gpu_solve_block2_noalloc takes (include/nla3d_gpu.h)
+NOALLOC_MAX_ROWS = 1 << 20
+
+
 class EqSolverOpts(ctypes.Structure):
     """nla3d_gpu_eqsolver_opts (include/nla3d_gpu.h)"""
     _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int), ("preconditioner", ctypes.c_int)]
@@ -122,6 +126,9 @@ ABI = {
     "nla3d_gpu_newton_iteration": [_P, ctypes.c_double, ctypes.c_int, _DP, ctypes.POINTER(NewtonOpts),
                                    ctypes.POINTER(NewtonStats), _IP],
     "nla3d_gpu_solve_block2": [_P, _DP, _DP, ctypes.c_double, ctypes.c_int, _IP, _DP],
+    # device pointers (rhs, x) as plain addresses
+    "nla3d_gpu_solve_block2_noalloc": [_P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _IP,
+                                       _DP],
 }
 ABI.update({
     "nla3d_gpu_assemble_thermal": [_P],
@@ -491,6 +498,13 @@ class GpuModel:
         self._call("nla3d_gpu_solve_block2_pc", _ptr(rhs, ctypes.c_double), _ptr(x, ctypes.c_double), float(tol),
                    int(max_iter), int(preconditioner), float(t), ctypes.byref(it), ctypes.byref(rr))
         return x, it.value, rr.value
+
+    def solve_block2_noalloc(self, rhs_dev, x_dev, tol=1e-12, max_iter=20000):
+        """nla3d_gpu_solve_block2_noalloc on device addresses (ints) of rhs and x"""
+        it, rr = ctypes.c_int(), ctypes.c_double()
+        self._call("nla3d_gpu_solve_block2_noalloc", ctypes.c_void_p(rhs_dev), ctypes.c_void_p(x_dev), float(tol),
+                   int(max_iter), ctypes.byref(it), ctypes.byref(rr))
+        return it.value, rr.value
 
     def solve_block2(self, rhs, tol=1e-12, max_iter=20000):
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)
