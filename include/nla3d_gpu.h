@@ -145,8 +145,17 @@ typedef struct {
                             V-cycle of aggregation multigrid on the sparse Schur approximation
                             S = B^T D^-1 B - C for the pressure and multiplier rows; 3: 1 and 2 together
                             (S is formed from the plain block-Jacobi blocks).  A solve of 1, 2 or 3 that
-                            breaks down or does not reach krylov_tol is repeated with plain block Jacobi
+                            breaks down or does not reach its tolerance is repeated with plain block Jacobi
                             (stats.precond_fallbacks) */
+  /* inexact Newton (criterion 1 only; the single-model solvers): a solve's Krylov tolerance is
+     max(krylov_tol, min(krylov_tol_loose, eta)), eta = infinity for the first solve of a load step
+     and, for a later one at residual ||rhs_k||, eta = krylov_forcing * convergence * ||rhs_1|| /
+     ||rhs_k|| (this factor times the residual reduction the load step still needs).  So the first
+     solve runs to krylov_tol_loose, the solve that only has to finish the load step to at most that,
+     and the others to krylov_tol.  0 (the defaults) leaves every solve at krylov_tol.  The
+     converged state is set by the residual criterion either way. */
+  double krylov_tol_loose;
+  double krylov_forcing;
 } nla3d_gpu_newton_opts;
 
 typedef struct {

@@ -33,7 +33,8 @@ class NewtonOpts(ctypes.Structure):
     """nla3d_gpu_newton_opts (include/nla3d_gpu.h)"""
     _fields_ = [("n_load_steps", ctypes.c_int), ("max_iterations", ctypes.c_int), ("criterion", ctypes.c_int),
                 ("convergence", ctypes.c_double), ("krylov_tol", ctypes.c_double),
-                ("krylov_max_iter", ctypes.c_int), ("preconditioner", ctypes.c_int)]
+                ("krylov_max_iter", ctypes.c_int), ("preconditioner", ctypes.c_int),
+                ("krylov_tol_loose", ctypes.c_double), ("krylov_forcing", ctypes.c_double)]
 
 
 class TransientOpts(ctypes.Structure):

@@ -2,6 +2,7 @@
 // (once per model), the refresh of values with every tangent, the V-cycle launch
 // sequence, its MINRES hook and the ABI entries that expose S and one V-cycle.
 #include <algorithm>
+#include <unordered_map>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -46,18 +47,21 @@ static double finestDiv(int steps) { return 100.0 * (steps / 8.0) * (steps / 8.0
 constexpr double kLmaxSafety = 1.05;  // the estimate approaches lmax from below
 
 struct AmgLevel {
-  int n = 0, nnz = 0, nc = 0, nSlots = 0;
+  int n = 0, nnz = 0, nc = 0, nSlots = 0, nColSlots = 0;
   std::vector<int> h_rowPtr, h_col;
   DevBuf rowPtr, col, val, dinv, agg, memPtr, mem, gSeg, gSrc;
   DevBuf sOff, sCol, sCol16, sBase, sPos, sVal;  // the smoother's sliced-ELLPACK copy
-  bool narrow = false;
+  DevBuf sCOff, sDOff, sDPos, sDia;              // and the diagonals of its pattern slices
+  int nd = 0, ldd = 0;
+  bool narrow = false, patterned = false;
   DevBuf r, x, res, da, db, t, start;
   double lmax = 1.0;
   AmgCsr csr() const { return AmgCsr{n, rowPtr.as<int>(), col.as<int>(), val.as<double>()}; }
   AmgSell sell() const {
     return AmgSell{n, sOff.as<int>(), narrow ? nullptr : sCol.as<int>(),
                    narrow ? sCol16.as<unsigned short>() : nullptr, narrow ? sBase.as<int>() : nullptr,
-                   sVal.as<float>()};
+                   sVal.as<float>(), patterned ? sCOff.as<int>() : nullptr, sDOff.as<int>(), sDia.as<float>(), nd,
+                   ldd};
   }
 };
 
@@ -126,6 +130,41 @@ static uint64_t mix64(uint64_t x) {
   return x ^ (x >> 31);
 }
 
+// the most common list of column offsets (col - row) over the level's rows (empty when it has
+// more than 64 entries or covers fewer than a quarter of the rows)
+static std::vector<int> commonOffsets(const AmgLevel& A) {
+  std::unordered_map<uint64_t, std::pair<int, int>> count;  // hash -> (rows, first row)
+  for (int i = 0; i < A.n; i++) {
+    uint64_t hsh = mix64((uint64_t)(A.h_rowPtr[i + 1] - A.h_rowPtr[i]));
+    for (int k = A.h_rowPtr[i]; k < A.h_rowPtr[i + 1]; k++) hsh = mix64(hsh ^ (uint64_t)(uint32_t)(A.h_col[k] - i));
+    auto it = count.find(hsh);
+    if (it == count.end()) count.emplace(hsh, std::make_pair(1, i));
+    else it->second.first++;
+  }
+  int best = -1, rows = 0;
+  for (auto& kv : count)
+    if (kv.second.first > rows || (kv.second.first == rows && kv.second.second < best)) {
+      rows = kv.second.first;
+      best = kv.second.second;
+    }
+  std::vector<int> pat;
+  if (best < 0 || 4ll * rows < A.n) return pat;
+  for (int k = A.h_rowPtr[best]; k < A.h_rowPtr[best + 1]; k++) pat.push_back(A.h_col[k] - best);
+  if (pat.size() > 64) pat.clear();
+  return pat;
+}
+
+// row i's column offsets are all in the (sorted) pattern
+static bool rowFits(const AmgLevel& A, int i, const std::vector<int>& pat) {
+  size_t p = 0;
+  for (int k = A.h_rowPtr[i]; k < A.h_rowPtr[i + 1]; k++) {
+    const int o = A.h_col[k] - i;
+    while (p < pat.size() && pat[p] < o) p++;
+    if (p == pat.size() || pat[p] != o) return false;
+  }
+  return true;
+}
+
 static void allocLevel(AmgLevel& A, bool coarsest) {
   A.rowPtr.upload(A.h_rowPtr);
   A.col.upload(A.h_col);
@@ -134,20 +173,48 @@ static void allocLevel(AmgLevel& A, bool coarsest) {
   A.r.alloc(nb);
   A.x.alloc(nb);
   if (coarsest) return;
-  // sliced ELLPACK: 64-row slices, each as wide as its longest row
+  // sliced ELLPACK: 64-row slices, each as wide as its longest row, or as wide as the pattern
+  // for a pattern slice (rows whose column offsets col - row are all in the level's most common
+  // offset list, with every row + offset inside the matrix: the columns are not stored)
   const int nSl = (A.n + 63) / 64;
-  std::vector<int> off(nSl + 1, 0);
+  std::vector<int> pat = commonOffsets(A);
+  // the diagonal form needs a symmetric offset list holding 0
+  for (size_t k = 0; k < pat.size(); k++)
+    if (pat[k] != -pat[pat.size() - 1 - k]) pat.clear();
+  if (!pat.empty() && pat.size() % 2 == 0) pat.clear();
+  std::vector<char> patSlice(nSl, 0);
+  if (!pat.empty()) {
+    int nPat = 0;
+    for (int sl = 0; sl < nSl; sl++) {
+      bool fits = true;
+      for (int i = sl * 64; i < std::min(A.n, sl * 64 + 64) && fits; i++)
+        fits = i + pat.front() >= 0 && i + pat.back() < A.n && rowFits(A, i, pat);
+      patSlice[sl] = fits ? 1 : 0;
+      nPat += fits ? 1 : 0;
+    }
+    if (2 * nPat < nSl) std::fill(patSlice.begin(), patSlice.end(), 0);  // not worth a second path
+  }
+  std::vector<int> off(nSl + 1, 0), cOff(nSl, -1);
+  int nCol = 0;
   for (int sl = 0; sl < nSl; sl++) {
     int w = 0;
     for (int i = sl * 64; i < std::min(A.n, sl * 64 + 64); i++) w = std::max(w, A.h_rowPtr[i + 1] - A.h_rowPtr[i]);
+    if (patSlice[sl]) w = 0;  // held by the diagonals
     if ((int64_t)off[sl] + 64ll * w >= (1ll << 31)) throw ApiError(-1, "Schur level too large for its smoother copy");
     off[sl + 1] = off[sl] + 64 * w;
+    if (!patSlice[sl]) {
+      cOff[sl] = nCol;
+      nCol += 64 * w;
+    }
   }
   A.nSlots = off[nSl];
-  // base column of each slice; 16-bit offsets when every slice spans < 65,536 columns
+  A.nColSlots = nCol;
+  A.patterned = !pat.empty() && std::find(patSlice.begin(), patSlice.end(), 1) != patSlice.end();
+  // base column of each stored-column slice; 16-bit offsets when every one spans < 65,536 columns
   std::vector<int> base(nSl, 0);
   A.narrow = true;
   for (int sl = 0; sl < nSl; sl++) {
+    if (patSlice[sl]) continue;
     int lo = INT_MAX, hi = -1;
     for (int i = sl * 64; i < std::min(A.n, sl * 64 + 64); i++)
       for (int k = A.h_rowPtr[i]; k < A.h_rowPtr[i + 1]; k++) {
@@ -159,24 +226,47 @@ static void allocLevel(AmgLevel& A, bool coarsest) {
     if (hi - lo > 65535) A.narrow = false;
   }
   // padding: value 0 at the slice's base column (a valid index)
-  std::vector<int> scol(A.nSlots), spos(A.nSlots);
+  std::vector<int> scol(std::max(nCol, 1)), spos(std::max(A.nSlots, 1));
   for (int sl = 0; sl < nSl; sl++) {
     const int w = (off[sl + 1] - off[sl]) / 64;
+    if (patSlice[sl]) continue;
     for (int l = 0; l < 64; l++) {
       const int i = sl * 64 + l;
       for (int k = 0; k < w; k++) {
-        const int j = off[sl] + k * 64 + l;
         const bool real = i < A.n && k < A.h_rowPtr[i + 1] - A.h_rowPtr[i];
-        scol[j] = real ? A.h_col[A.h_rowPtr[i] + k] : base[sl];
-        spos[j] = real ? A.h_rowPtr[i] + k : -1;
+        scol[cOff[sl] + k * 64 + l] = real ? A.h_col[A.h_rowPtr[i] + k] : base[sl];
+        spos[off[sl] + k * 64 + l] = real ? A.h_rowPtr[i] + k : -1;
       }
     }
   }
   A.sOff.upload(off);
+  if (A.patterned) {
+    // diagonals 0, d_1 .. d_m of every row j: the CSR position of (j, j + d_k) or -1
+    const int m = (int)pat.size() / 2;
+    std::vector<int> dOff(m + 1);
+    for (int k = 0; k <= m; k++) dOff[k] = pat[m + k];
+    A.nd = m + 1;
+    A.ldd = (A.n + 63) / 64 * 64;
+    std::vector<int> dpos((size_t)A.nd * A.ldd, -1);
+    for (int j = 0; j < A.n; j++) {
+      int k = A.h_rowPtr[j];
+      for (int q = 0; q < A.nd; q++) {
+        const int c = j + dOff[q];
+        while (k < A.h_rowPtr[j + 1] && A.h_col[k] < c) k++;
+        if (k < A.h_rowPtr[j + 1] && A.h_col[k] == c) dpos[(size_t)q * A.ldd + j] = k;
+      }
+    }
+    A.sCOff.upload(cOff);
+    A.sDOff.upload(dOff);
+    A.sDPos.upload(dpos);
+    A.sDia.alloc(dpos.size() * 4);
+  }
   if (A.narrow) {
-    std::vector<unsigned short> c16(A.nSlots);
-    for (int sl = 0; sl < nSl; sl++)
-      for (int j = off[sl]; j < off[sl + 1]; j++) c16[j] = (unsigned short)(scol[j] - base[sl]);
+    std::vector<unsigned short> c16(std::max(nCol, 1));
+    for (int sl = 0; sl < nSl; sl++) {
+      if (patSlice[sl]) continue;
+      for (int j = cOff[sl]; j < cOff[sl] + (off[sl + 1] - off[sl]); j++) c16[j] = (unsigned short)(scol[j] - base[sl]);
+    }
     A.sCol16.upload(c16);
     A.sBase.upload(base);
   } else {
@@ -350,7 +440,7 @@ void setupSchur(Ctx& c, const std::vector<int64_t>& rp, const std::vector<int>& 
   S->cinv.alloc((size_t)ncst * ncst * 8);
   // (a range holding only the coarsest level stays a multi-workgroup dense product)
   for (size_t l = 1; l + 1 < S->L.size(); l++)
-    if (S->L[l]->nSlots <= kFuseSlots) {
+    if ((int64_t)S->L[l]->nSlots + (int64_t)S->L[l]->nd * S->L[l]->ldd <= kFuseSlots) {
       S->fuseFrom = (int)l;
       S->fused.alloc((S->L.size() - l) * sizeof(FusedLevel));
       break;
@@ -417,6 +507,8 @@ bool schurRefresh(Ctx& c, std::string* why) {
   for (size_t l = 0; l + 1 < S.L.size(); l++) {
     AmgLevel& A = *S.L[l];
     launch_sell_fill(s, A.nSlots, A.sPos.as<int>(), A.val.as<double>(), A.sVal.as<float>());
+    if (A.patterned)
+      launch_sell_fill(s, A.nd * A.ldd, A.sDPos.as<int>(), A.val.as<double>(), A.sDia.as<float>());
     launch_inv_diag(s, A.csr(), A.dinv.as<double>());
     // largest eigenvalue of D^-1 A by power iteration from a fixed pseudo-random start,
     // raised by a safety factor (the Chebyshev polynomial grows quickly above its interval)
@@ -707,8 +799,10 @@ int nla3d_gpu_time_smoother(nla3d_gpu_t h, int iters, double* ms, long long* byt
       // five vectors of every step (din, res, D^-1 read; res, dout written) and x, read and
       // written by the steps that add to it: the mean over the timed steps, which follow the
       // sweep's order of modes
+      // (pattern slices store values only)
       const long long nSl = (A.n + 63) / 64;
-      *bytes = (long long)A.nSlots * (4 + (A.narrow ? 2 : 4)) + (nSl + 1) * 4 + (A.narrow ? nSl * 4 : 0) +
+      *bytes = (long long)A.nSlots * 4 + (long long)A.nColSlots * (A.narrow ? 2 : 4) + (nSl + 1) * 4 +
+               (A.narrow ? nSl * 4 : 0) + (A.patterned ? nSl * 4 + (long long)A.nd * A.ldd * 4 : 0) +
                5ll * A.n * 8 + (xPasses * A.n * 8 + iters / 2) / iters;
     }
   });

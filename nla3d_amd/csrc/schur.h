@@ -36,6 +36,13 @@ struct AmgCsr {
 // (col16, base) when every slice spans fewer than 65,536 columns, else full indices (col).
 // The values are the FP64 matrix rounded once per tangent, so the smoother stays one fixed
 // symmetric operator.
+// A level's matrix as 64-row sliced ELLPACK: slice s holds its rows' values slot by slot,
+// val[off[s] + 64 k + lane], and their columns from cOff[s] (off[s] when cOff is null), as full
+// indices or as 16-bit offsets from base[s].  A "pattern slice" (cOff[s] < 0) stores nothing
+// there: its rows' columns are i + o for the offsets o of one symmetric list (0, +-d_1 ..
+// +-d_m), and the matrix being symmetric, row i reads the entry (i, i + d_k) from diagonal k,
+// dia[k * ldd + i], and the entry (i, i - d_k) as (i - d_k, i) from dia[k * ldd + i - d_k]:
+// each entry of the pattern rows is stored once.  Absent entries are zeros.
 struct AmgSell {
   int n;
   const int* off;
@@ -43,6 +50,10 @@ struct AmgSell {
   const unsigned short* col16;   // offsets from base[s]
   const int* base;               // [slices] smallest column of the slice
   const float* val;
+  const int* cOff;               // [slices] start of the slice's stored columns, -1 for a pattern slice; or null
+  const int* dOff;               // [nd] 0, d_1 .. d_m (d_k > 0)
+  const float* dia;              // [nd][ldd] dia[k * ldd + j] = A(j, j + d_k), over all rows j
+  int nd, ldd;
 };
 
 // One level of the fused coarse cycle (schur.hip coarse_cycle_kernel): the level's smoother

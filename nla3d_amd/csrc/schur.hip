@@ -22,19 +22,38 @@ __device__ __forceinline__ double row_dot(const AmgCsr& A, int i, int lane, cons
 // are the 64 lanes of one wave, so every load of values and columns is coalesced and the
 // loop trip count (the slice width) is uniform over the wave
 // (kNarrow: 16-bit column offsets from the slice's base column)
+// (pattern slices, cOff[s] < 0: the symmetric diagonals, 64 consecutive rows per wave, so every
+// value and x load of the wave is one contiguous segment)
 template <bool kNarrow>
 __device__ __forceinline__ double sell_dot(const AmgSell& A, int i, const double* __restrict__ x) {
   const int sl = i >> 6;
   const int b = A.off[sl], w = (A.off[sl + 1] - b) >> 6;
   const float* __restrict__ v = A.val + b + (i & 63);
   double acc = 0.0;
+  int cb = b;
+  if (A.cOff) {
+    cb = A.cOff[sl];
+    if (cb < 0) {
+      const float* __restrict__ u = A.dia + i;
+      const int* __restrict__ dOff = A.dOff;
+      acc = (double)u[0] * x[i];
+#pragma unroll 2
+      for (int k = 1; k < A.nd; k++) {
+        const int d = dOff[k];
+        const float* __restrict__ uk = u + (size_t)k * A.ldd;
+        acc += (double)uk[0] * x[i + d] + (double)uk[-d] * x[i - d];
+      }
+      return acc;
+    }
+  }
+  cb += i & 63;
   if constexpr (kNarrow) {
-    const unsigned short* __restrict__ c = A.col16 + b + (i & 63);
+    const unsigned short* __restrict__ c = A.col16 + cb;
     const double* __restrict__ xb = x + A.base[sl];
 #pragma unroll 4
     for (int k = 0; k < w; k++) acc += (double)v[k * 64] * xb[c[k * 64]];
   } else {
-    const int* __restrict__ c = A.col + b + (i & 63);
+    const int* __restrict__ c = A.col + cb;
 #pragma unroll 4
     for (int k = 0; k < w; k++) acc += (double)v[k * 64] * x[c[k * 64]];
   }

@@ -465,6 +465,13 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
     }
   }
   st.newton_iterations++;
+  // the Krylov tolerance of this solve: krylov_tol, or looser where the load step cannot use
+  // more accuracy (inexact Newton, criterion 1)
+  double ktol = o.krylov_tol;
+  if (o.criterion == 1 && o.krylov_tol_loose > 0.0) {
+    const double eta = it == 1 ? o.krylov_tol_loose : (rn > 0.0 ? o.krylov_forcing * o.convergence * rhs1 / rn : 0.0);
+    ktol = std::max(ktol, std::min(o.krylov_tol_loose, eta));
+  }
   // solve K.block(2) dU_sl = rhs (FESolver.cpp:457)
   t0 = nowMs();
   const bool learnedPc = o.preconditioner == 1 || o.preconditioner == 3;
@@ -480,15 +487,15 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
   if (learnedPc) learnedCorrect(c, t);
   double relres = 0;
   bool brk = false;
-  int kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), o.krylov_tol, o.krylov_max_iter, &relres, &brk,
+  int kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), ktol, o.krylov_max_iter, &relres, &brk,
                           useSchur ? &hook : nullptr);
-  if ((learnedPc || useSchur) && (brk || relres > o.krylov_tol)) {
+  if ((learnedPc || useSchur) && (brk || relres > ktol)) {
     // the learned corrections failed this solve: block Jacobi from a fresh start
     st.precond_fallbacks++;
     st.krylov_iterations += kit;
     krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
     brk = false;
-    kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), o.krylov_tol, o.krylov_max_iter, &relres, &brk);
+    kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), ktol, o.krylov_max_iter, &relres, &brk);
   }
   st.krylov_iterations += kit;
   st.linear_solves++;
@@ -572,6 +579,8 @@ int nla3d_gpu_newton_defaults(nla3d_gpu_newton_opts* o) {
   o->krylov_tol = 1.0e-12;
   o->krylov_max_iter = 20000;
   o->preconditioner = 0;
+  o->krylov_tol_loose = 0.0;
+  o->krylov_forcing = 0.0;
   return 0;
 }
 
