@@ -10,35 +10,40 @@ and a 20 degree ROTZ, ramped over 20 load steps (TimeControl).
 
 Preconditioner: block Jacobi on the node DoFs and one V-cycle of the Schur-row
 multigrid on the pressure and multiplier rows (default --preconditioner
-schur-multigrid).  A per-node MLP correction of the block-Jacobi node blocks is
-trained first (2,000 tangent/rhs pairs from Newton iterations of seeded
+schur-multigrid).  The system of the first (untimed) Newton iteration is also
+solved with plain block Jacobi and the Krylov iterations of both are reported
+("judge").  With --train, a per-node MLP correction of the block-Jacobi node
+blocks is trained first (2,000 tangent/rhs pairs from Newton iterations of seeded
 8..16-element-wide training boxes, checkpointed on a 24x24x12 validation
-system), then judged on the system of the first (untimed) Newton iteration
-against the same method without the network and against plain block Jacobi;
-the timed iterations use the network only if it saved more than
---judge-margin of the Krylov iterations there.  All counts are reported
-("judge").
+system) and judged on that system as well; the timed iterations use the network
+only if it saved more than --judge-margin of the Krylov iterations there (under
+the multigrid it has not, so training is off by default).
 
 A step (mode "newton", the default) is one equilibrium iteration of
 NonlinearFESolver::solve (FESolver.cpp:443-490), all on the device with the
 model resident in HBM: update + buildK of every element, deterministic
 scatter into the three CSR blocks and F, rigid-body MPC rows, boundary
 conditions, rhs with the eliminated constrained DoFs, the block(2) solve
-(MINRES to a 1e-9 relative preconditioned residual) and the increment update.
-Newton runs to a 1e-8 relative residual; iterations that only confirm
-convergence (an assembly without a solve) are timed but not counted as steps.
-value = elements / s = 864,000 / (Newton-iteration wall).
+(MINRES to a 1e-9 relative preconditioned residual; the first solve of each load
+step to 1e-5, which leaves the Newton iterates and the number of solves per load
+step as they are with 1e-9) and the increment update.  Newton runs to a 1e-8
+relative residual; iterations that only confirm convergence (an assembly without
+a solve) are timed but not counted as steps.  value = elements / s = 864,000 /
+(Newton-iteration wall); the load steps completed inside the timed window are
+reported beside it ("load_steps_timed", "ms_per_load_step").
 
 roofline: the dominant kernel of a Newton iteration, timed with HIP events on
 the library's stream: the finest-level Chebyshev smoother step of the Schur
 multigrid (HBM-bound; traffic from the committed PMC summary), or the block(2)
 SpMV without the multigrid.  The SpMV and the element kernel (FP64-bound) are
 reported beside it.
-cpu_baseline: one reference Newton iteration on one core = the reference
+cpu_baseline: one reference Newton iteration on the host = the reference
 assembly loop (oracle/nla3d_oracle.c: update + buildK + binary-search
-addValue) timed on a bounded sample of the same model's elements, plus the
-direct solve of K.block(2) (SuperLU, one thread, in place of the absent
-PARDISO) timed on C3-like boxes and extrapolated to this size (labelled).
+addValue; one thread, as the reference runs it) timed on a bounded sample of the
+same model's elements, plus the direct solve of K.block(2) in place of the
+absent PARDISO: a nested-dissection multifrontal LU (oracle/ndsolve.py) on all
+host threads, timed on C3-like boxes up to about 200k unknowns and extrapolated
+less than 20x in size to this one (labelled).
 
 Other modes: "assemble" (the assembly alone), "probe" (short kernel runs for
 hardware-counter passes), "thermal" (QUADTH transient, 1024x1024), "batch"
@@ -95,18 +100,21 @@ def c3_like_block2(mm):
     """K.block(2) of the C3 box at mesh 2mm x 2mm x mm (same material, rigid-body top) at the
     first tangent (U = 0), assembled by the oracle's reference loop: the element blocks,
     and the multiplier rows with their slave coefficient 1 (the master's DoFs are
-    prescribed, so its coefficients sit in block(1,2)).  Returns a CSC matrix."""
+    prescribed, so its coefficients sit in block(1,2)).  Returns (full symmetric CSR, the
+    nested-dissection row description of oracle/ndsolve.py)."""
     import scipy.sparse as sp
     import nla3d_amd
     from nla3d_amd import meshgen
-    from oracle import pyoracle
+    from oracle import ndsolve, pyoracle
+    nx, ny, nz = 2 * mm, 2 * mm, mm
     m = nla3d_amd.GpuModel(-1)
-    xyz, conn = meshgen.solid81_rigid_top_case(m, 2 * mm, 2 * mm, mm, 2.0, 2.0, 1.0, 0.15, 0.349,
+    xyz, conn = meshgen.solid81_rigid_top_case(m, nx, ny, nz, 2.0, 2.0, 1.0, 0.15, 0.349,
                                                material=(3, (0.5, 0.1, 1000.0)))
     m.set_max_in_row(0)
     m.init_solution()
     ne, ee = m.equation_numbers()
-    nE, nC, nU = len(conn), m.sizes["nConstrained"], m.sizes["nUnknown"]
+    sz = m.sizes
+    nE, nC, nU = len(conn), sz["nConstrained"], sz["nUnknown"]
     eq = np.zeros((nE, 25), dtype=np.uint32)
     eq[:, :24] = ne[conn - 1][:, :, :3].reshape(nE, 24)
     eq[:, 24] = ee[:, 6]
@@ -122,33 +130,71 @@ def c3_like_block2(mm):
     top = np.nonzero(xyz[:-1, 2] == 1.0)[0] + 1
     r = [int(ne[node - 1, d]) - nC - 1 for node in top for d in range(3)]
     B = sp.csr_matrix((np.ones(len(r)), (r, nU + np.arange(len(r)))), shape=(n2, n2))
-    return (A + B + B.T).tocsc()
+    node_rows, elem_rows = ndsolve.box_rows(ne, ee, nC, sz["nDofs"], sz["nMpc"], nx, ny, nz)
+    m.close()
+    return (A + B + B.T).tocsr(), node_rows, elem_rows
 
 
-def cpu_solve_fit(n2_target, budget_s):
-    """The direct solve of K.block(2) that a reference Newton iteration performs (PARDISO
-    there, absent here): SuperLU (scipy, one thread, COLAMD ordering) factor + solve on
-    C3-like boxes of growing size until the budget is spent, t = a n2^b fitted on the three
-    largest and extrapolated to n2_target (labelled as extrapolated)."""
-    import scipy.sparse.linalg as sla
-    sizes, secs = [], []
+def host_threads():
+    """the host threads this process may use: the box's CPU share (OMP_NUM_THREADS when set)"""
+    v = os.environ.get("OMP_NUM_THREADS")
+    return int(v) if v and v.isdigit() and int(v) > 0 else (os.cpu_count() or 1)
+
+
+def cpu_solve_fit(n2_target, budget_s, sizes_mm=(14, 18, 23)):
+    """The direct solve of K.block(2) that a reference Newton iteration performs
+    (solveEquations: PARDISO's numerical factorisation and substitution, EquationSolver.cpp:
+    141-251; absent here): the nested-dissection multifrontal LU of oracle/ndsolve.py on all
+    host threads, factorisation + solve timed on C3-like boxes (2mm x 2mm x mm), the ordering
+    (PARDISO's phase 11, once per pattern) left out.  Extrapolated to n2_target (labelled, with
+    its factor in n2) through the work: the exact flop counts of the timed factorisations fitted
+    as a n2^b (b near 2, nested dissection of a 3-D grid), at the rate measured on the largest
+    size; the same flops at the host's dense matrix-product rate are reported as a bound."""
+    from oracle import ndsolve
+    threads = host_threads()
+    sizes, secs, flops = [], [], []
     spent = 0.0
-    for mm in (3, 4, 5, 6, 7, 8, 9, 10, 12):
-        K2 = c3_like_block2(mm)
-        rhs = np.random.default_rng(mm).standard_normal(K2.shape[0])
+    for mm in sizes_mm:
+        A, nr, er = c3_like_block2(mm)
+        nd = ndsolve.NestedDissectionLU(nr, er, leaf_nodes=64)
+        rhs = np.random.default_rng(mm).standard_normal(A.shape[0])
         t0 = time.perf_counter()
-        sla.splu(K2).solve(rhs)
+        nd.factorize(A, threads=threads)
+        x = nd.solve(rhs)
         dt = time.perf_counter() - t0
-        sizes.append(K2.shape[0])
+        res = float(np.linalg.norm(A @ x - rhs) / np.linalg.norm(rhs))
+        sizes.append(A.shape[0])
         secs.append(dt)
+        flops.append(nd.flops)
         spent += dt
-        print(f"[bench] cpu solve sample 2x{mm}: n2 = {K2.shape[0]}, {dt:.2f} s", file=sys.stderr, flush=True)
-        if spent + 6.0 * dt > budget_s:  # the next size costs several times this one
+        print(f"[bench] cpu solve sample 2x{mm}: n2 = {A.shape[0]}, {dt:.2f} s, {nd.flops / dt / 1e9:.0f} GFLOP/s, "
+              f"residual {res:.1e}", file=sys.stderr, flush=True)
+        del nd, A
+        if spent + 3.0 * dt > budget_s:  # the next size costs about three times this one
             break
-    ln_n, ln_t = np.log(sizes[-3:]), np.log(secs[-3:])
-    b, ln_a = np.polyfit(ln_n, ln_t, 1)
-    return {"sizes": sizes, "seconds": secs, "exponent": float(b),
-            "extrapolated_seconds": float(np.exp(ln_a) * n2_target ** b), "target_n2": int(n2_target)}
+    b = float(np.polyfit(np.log(sizes), np.log(flops), 1)[0]) if len(sizes) > 1 else 2.0
+    flops_target = flops[-1] * (n2_target / sizes[-1]) ** b
+    rate = flops[-1] / secs[-1]
+    g = np.random.default_rng(0).standard_normal((4000, 4000))
+    t0 = time.perf_counter()
+    g @ g
+    gemm_rate = 2.0 * 4000 ** 3 / (time.perf_counter() - t0)
+    return {"sizes": sizes, "seconds": secs, "gflops": [f / t / 1e9 for f, t in zip(flops, secs)],
+            "flops": flops, "flops_exponent": b, "target_flops": flops_target,
+            "extrapolated_seconds": flops_target / rate, "seconds_at_dgemm_rate": flops_target / gemm_rate,
+            "dgemm_gflops": gemm_rate / 1e9, "target_n2": int(n2_target),
+            "extrapolation_factor_n2": n2_target / sizes[-1], "threads": threads}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(m, xyz, conn, U, n_sample):
@@ -168,16 +214,20 @@ def cpu_baseline(m, xyz, conn, U, n_sample):
     dt = time.perf_counter() - t0
     asm_rate = len(sel) / dt
     n2 = m.sizes["nUnknown"] + m.sizes["nMpc"]
-    fit = cpu_solve_fit(n2, 15.0)
+    fit = cpu_solve_fit(n2, 45.0)
     t_iter = len(conn) / asm_rate + fit["extrapolated_seconds"]
-    return {"value": len(conn) / t_iter, "unit": "elements/s", "cores": 1, "kind": "port",
-            "assembly_elements_per_s": asm_rate, "solve": fit,
-            "sample": f"one Newton iteration on 1 thread = the reference assembly loop (update + buildK + "
-                      f"binary-search addValue) timed over the first {len(sel)} elements of this model "
-                      f"({dt:.1f} s, {asm_rate:.0f} elements/s, scaled to {len(conn)}) + the direct solve of "
-                      f"K.block(2) (SuperLU, one thread) timed on C3-like boxes up to n2 = {fit['sizes'][-1]} "
-                      f"and extrapolated as n2^{fit['exponent']:.2f} to n2 = {n2}: "
-                      f"{fit['extrapolated_seconds']:.3g} s (extrapolated)"}
+    return {"value": len(conn) / t_iter, "unit": "elements/s", "cores": fit["threads"], "kind": "port",
+            "assembly_elements_per_s": asm_rate, "assembly_cores": 1, "solve": fit, "host_cpu": cpu_model(),
+            "sample": f"one Newton iteration = the reference assembly loop (update + buildK + binary-search "
+                      f"addValue) on 1 thread, as the reference runs it, timed over the first {len(sel)} elements "
+                      f"of this model ({dt:.1f} s, {asm_rate:.0f} elements/s, scaled to {len(conn)}) + the direct "
+                      f"solve of K.block(2) in place of PARDISO: nested-dissection multifrontal LU on "
+                      f"{fit['threads']} host threads, factorisation + solve timed on C3-like boxes up to "
+                      f"n2 = {fit['sizes'][-1]} ({fit['seconds'][-1]:.1f} s, {fit['gflops'][-1]:.0f} GFLOP/s) and "
+                      f"extrapolated by {fit['extrapolation_factor_n2']:.1f}x in n2 to n2 = {n2} through its flops "
+                      f"(n2^{fit['flops_exponent']:.2f}, {fit['target_flops']:.2e}) at that rate: "
+                      f"{fit['extrapolated_seconds']:.0f} s (extrapolated; {fit['seconds_at_dgemm_rate']:.0f} s at "
+                      f"the host's {fit['dgemm_gflops']:.0f} GFLOP/s matrix-product rate)"}
 
 
 def batch_mode(args):
@@ -438,9 +488,17 @@ def main():
     ap.add_argument("--shape", type=str, default="120,120,60")
     ap.add_argument("--load-steps", type=int, default=20)
     ap.add_argument("--krylov-tol", type=float, default=1e-9)
+    # inexact Newton: the first solve of each load step only to this tolerance (the later solves
+    # to --krylov-tol); 1e-5 keeps the Newton iterates and the solves per load step of exact
+    # first solves at this workload, 1e-4 adds a solve to most load steps
+    ap.add_argument("--krylov-tol-loose", type=float, default=1e-5)
     ap.add_argument("--preconditioner", choices=["schur-multigrid", "learned", "block-jacobi"],
                     default="schur-multigrid")
-    ap.add_argument("--no-train", action="store_true", help="skip the network (and its judge)")
+    ap.add_argument("--train", action="store_true",
+                    help="train the learned node-block correction first and let the judge decide whether the "
+                         "timed iterations use it (off by default: under the Schur multigrid it has not saved "
+                         "iterations at this workload)")
+    ap.add_argument("--no-train", action="store_true", help="the default; kept for older command lines")
     ap.add_argument("--train-pairs", type=int, default=2000)
     ap.add_argument("--train-k", type=int, default=100)
     ap.add_argument("--train-lr", type=float, default=1e-4)
@@ -494,6 +552,7 @@ def main():
 
     extra = {}
     opts = m.newton_defaults(criterion=1, convergence=1e-8, krylov_tol=args.krylov_tol, krylov_max_iter=100000)
+    opts.krylov_tol_loose = args.krylov_tol_loose
     if args.mode == "assemble":
         for _ in range(args.warmup):
             m.assemble()
@@ -515,7 +574,7 @@ def main():
         net = None
         base_pc = 2 if args.preconditioner == "schur-multigrid" else 0
         opts.preconditioner = base_pc
-        if args.preconditioner in ("learned", "schur-multigrid") and not args.no_train:
+        if args.preconditioner in ("learned", "schur-multigrid") and args.train:
             from nla3d_amd import learned
             net = nla3d_amd.Mlp(0, seed=20261203)
             val = learned.validation_system(24)
@@ -529,57 +588,95 @@ def main():
                                  "lr": args.train_lr, "validation_iterations": info["validation"],
                                  "validation_system": "24x24x12 rigid-top box, first Newton iteration at t=0.5"}
 
+        calls = []  # every Newton iteration: (load step, wall s, Krylov iterations or 0, timed)
+        timing = {"on": False}
+
         def one_solving_iteration():
             while True:
                 state["it"] += 1
                 before = st.linear_solves
-                conv = m.newton_iteration(sched[state["k"]], state["it"], rhs1, opts, st)
+                k = state["k"]
+                t1 = time.perf_counter()
+                conv = m.newton_iteration(sched[k], state["it"], rhs1, opts, st)
+                solved = st.linear_solves > before
+                calls.append((k, time.perf_counter() - t1, st.last_krylov_iterations if solved else 0, timing["on"]))
                 if conv:
                     state["k"] += 1
                     state["it"] = 0
                     if state["k"] >= len(sched):
                         raise RuntimeError("load schedule exhausted")
-                if st.linear_solves > before:
+                if solved:
                     return st.last_krylov_iterations
 
         for w in range(args.warmup):
             kit = one_solving_iteration()
             print(f"[bench] warmup iteration: {kit} Krylov iterations", file=sys.stderr, flush=True)
-            if net is not None and w == 0:
-                # the method without the network judges the learned corrections on the system just
-                # solved; plain block Jacobi is solved beside them for reference
+            if w == 0 and base_pc == 2:
+                # the system just solved, to the timed tolerance, with plain block Jacobi, with the
+                # Schur multigrid and (when trained) with the learned corrections beside it: the
+                # Krylov-iteration comparison of the metric; the network is kept only if it saves
+                # more than the judge's margin over the method without it
                 rhs = m.get_rhs()
                 tt = sched[state["k"]]
-                m.attach_mlp(net)
                 x0, it_bj, _ = m.solve_block2_pc(rhs, args.krylov_tol, 100000, preconditioner=0, t=tt)
-                its = {0: it_bj}
-                if base_pc == 2:
-                    x2, its[2], _ = m.solve_block2_pc(rhs, args.krylov_tol, 100000, preconditioner=2, t=tt)
-                xl, its[base_pc + 1], _ = m.solve_block2_pc(rhs, args.krylov_tol, 100000,
-                                                            preconditioner=base_pc + 1, t=tt)
-                it_without, it_with = its[base_pc], its[base_pc + 1]
-                print(f"[bench] judge: block Jacobi {it_bj}, " +
-                      (f"Schur multigrid {its[2]}, " if base_pc == 2 else "") +
-                      f"learned {it_with} Krylov iterations", file=sys.stderr, flush=True)
-                extra["judge"] = {"block_jacobi_iterations": it_bj,
-                                  "without_network_iterations": it_without, "learned_iterations": it_with,
-                                  "solution_difference": float(np.abs(xl - x0).max() / np.abs(x0).max()),
+                x2, it_mg, _ = m.solve_block2_pc(rhs, args.krylov_tol, 100000, preconditioner=2, t=tt)
+                extra["judge"] = {"block_jacobi_iterations": it_bj, "schur_multigrid_iterations": it_mg,
+                                  "block_jacobi_over_multigrid": it_bj / max(it_mg, 1),
+                                  "solution_difference_multigrid": float(np.abs(x2 - x0).max() / np.abs(x0).max()),
+                                  "system": "tangent and rhs of the first Newton iteration of load step 1, "
+                                            f"MINRES to {args.krylov_tol:g}"}
+                msg = f"[bench] judge: block Jacobi {it_bj}, Schur multigrid {it_mg}"
+                if net is not None:
+                    m.attach_mlp(net)
+                    xl, it_with, _ = m.solve_block2_pc(rhs, args.krylov_tol, 100000, preconditioner=3, t=tt)
+                    extra["judge"].update({"learned_iterations": it_with, "margin": args.judge_margin,
+                                           "solution_difference_learned":
+                                               float(np.abs(xl - x0).max() / np.abs(x0).max())})
+                    msg += f", learned + multigrid {it_with}"
+                    if it_with > (1.0 - args.judge_margin) * it_mg:
+                        opts.preconditioner = base_pc  # rejected: the method without the network
+                else:
+                    extra["judge"]["learned_iterations"] = None
+                    extra["judge"]["learned"] = "not trained (bench.py --train); see the learned-correction tests"
+                print(msg + " Krylov iterations", file=sys.stderr, flush=True)
+            elif w == 0 and net is not None:  # --preconditioner learned: block Jacobi judges the corrections
+                rhs = m.get_rhs()
+                tt = sched[state["k"]]
+                x0, it_bj, _ = m.solve_block2_pc(rhs, args.krylov_tol, 100000, preconditioner=0, t=tt)
+                m.attach_mlp(net)
+                xl, it_with, _ = m.solve_block2_pc(rhs, args.krylov_tol, 100000, preconditioner=1, t=tt)
+                extra["judge"] = {"block_jacobi_iterations": it_bj, "learned_iterations": it_with,
                                   "margin": args.judge_margin,
+                                  "solution_difference_learned": float(np.abs(xl - x0).max() / np.abs(x0).max()),
                                   "system": "tangent and rhs of the first Newton iteration of load step 1"}
-                if base_pc == 2:
-                    extra["judge"]["schur_multigrid_iterations"] = its[2]
-                if it_with > (1.0 - args.judge_margin) * it_without:
-                    opts.preconditioner = base_pc  # rejected: the method without the network
+                print(f"[bench] judge: block Jacobi {it_bj}, learned {it_with} Krylov iterations", file=sys.stderr,
+                      flush=True)
+                if it_with > (1.0 - args.judge_margin) * it_bj:
+                    opts.preconditioner = base_pc
         kits = []
         ms_asm0, ms_sol0 = st.ms_assemble, st.ms_solve
+        timing["on"] = True
         t0 = time.perf_counter()
         for _ in range(args.steps):
             kits.append(one_solving_iteration())
             print(f"[bench] iteration: {kits[-1]} Krylov iterations, criterion {st.last_criterion:.3e}",
                   file=sys.stderr, flush=True)
         t1 = time.perf_counter()
+        timing["on"] = False
         ms_step = (t1 - t0) * 1e3 / args.steps
-        extra.update({"krylov_iterations_per_newton": float(np.mean(kits)),
+        # the load steps whose every Newton iteration (the solving ones and the one that only
+        # confirms convergence) fell in the timed window
+        steps_in = sorted({k for k, _, _, on in calls if on})
+        whole = [k for k in steps_in if all(on for kk, _, _, on in calls if kk == k)
+                 and any(kk == k and not kr for kk, _, kr, _ in calls)]
+        per_step = [{"load_step": k + 1, "s": sum(dt for kk, dt, _, _ in calls if kk == k),
+                     "solves": sum(1 for kk, _, kr, _ in calls if kk == k and kr),
+                     "krylov_iterations": sum(kr for kk, _, kr, _ in calls if kk == k)} for k in whole]
+        extra["load_steps_timed"] = per_step
+        if per_step:
+            extra["ms_per_load_step"] = 1e3 * float(np.mean([p["s"] for p in per_step]))
+        extra.update({"krylov_tol": args.krylov_tol, "krylov_tol_first_solve": args.krylov_tol_loose,
+                      "krylov_iterations_per_newton": float(np.mean(kits)),
                  "ms_krylov_per_iteration": (st.ms_solve - ms_sol0) / max(1, sum(kits)),
                  "ms_assemble_per_newton": (st.ms_assemble - ms_asm0) / args.steps,
                  "load_step": state["k"] + 1,

@@ -112,7 +112,7 @@ struct Ctx {
   int n2 = 0;                 // nUnknown + nMpc
   // full-symmetric block 2 in grouped CSR (krylov.h)
   DevBuf d_sgRow0, d_sgSize, d_sgColPtr, d_sgCols, d_sgValPtr, d_rowGroup, d_diagpos2, d_full2;
-  int nSpmvGroups = 0;
+  int nSpmvGroups = 0, nLead1 = 0;  // groups, and the leading single-row ones
   DevBuf d_mir2;              // int [2*nnz2]: full positions of each upper entry of block 2
   int64_t fullNnz2 = 0;
   DevBuf d_gStart, d_gInfo, d_gDiagPos, d_binv;  // block-Jacobi groups

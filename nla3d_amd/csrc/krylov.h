@@ -21,6 +21,7 @@ struct KrylovMatrix {
   const double* val = nullptr;
   const int* diagpos = nullptr;  // [n] position of (i,i) in val, or -1 (multiplier rows have none)
   const int* rowGroup = nullptr; // [n] group of row i
+  int nLead1 = 0;                // groups [0, nLead1) are single rows (the element pressures come first)
 };
 
 // block-Jacobi groups: each row belongs to exactly one group

@@ -137,18 +137,28 @@ __device__ __forceinline__ void group_rowsums(const KrylovMatrix& M, int g, int 
   a0 = a1 = a2 = 0.0;
   const int* __restrict__ cols = M.gCols + c0;
   const double* __restrict__ v0 = M.val + vb;
-  if (sz == 3) {
-    for (int k = lane; k < L; k += LPG) {
-      const double xv = x[cols[k]];
-      a0 += v0[k] * xv;
-      a1 += v0[L + k] * xv;
-      a2 += v0[2 * L + k] * xv;
+  // kU entries of the lane at a time: their column indices, then the x values, then the
+  // products (one dependent load round trip per kU entries instead of per entry); each lane
+  // still sums its entries in increasing k
+  constexpr int kU = 4;
+  for (int k0 = lane; k0 < L; k0 += kU * LPG) {
+    int c[kU];
+    double xv[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int k = k0 + u * LPG;
+      c[u] = k < L ? cols[k] : -1;
     }
-  } else {
-    for (int k = lane; k < L; k += LPG) {
-      const double xv = x[cols[k]];
-      a0 += v0[k] * xv;
-      if (sz > 1) a1 += v0[L + k] * xv;
+#pragma unroll
+    for (int u = 0; u < kU; u++) xv[u] = c[u] >= 0 ? x[c[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int k = k0 + u * LPG;
+      if (k < L) {
+        a0 += v0[k] * xv[u];
+        if (sz > 1) a1 += v0[L + k] * xv[u];
+        if (sz > 2) a2 += v0[2 * L + k] * xv[u];
+      }
     }
   }
   for (int o = LPG / 2; o > 0; o >>= 1) {
@@ -158,18 +168,16 @@ __device__ __forceinline__ void group_rowsums(const KrylovMatrix& M, int g, int 
   }
 }
 
-// T = s * A Yp - c1 * R1 ; partial alfa = s * sum Yp . T
+// T = s * A Yp - c1 * R1 over groups [g0, g1), LPG lanes per group, grid-stride; returns this
+// thread's part of s * sum Yp . T
 template <int LPG>
-__global__ __launch_bounds__(256) void spmv_minres_kernel(KrylovMatrix M, const double* __restrict__ Yp,
-                                                         const double* __restrict__ R1, double* __restrict__ T,
-                                                         MinresState* __restrict__ st, double* __restrict__ partial) {
-  __shared__ double sh[4];
-  if (st->converged) return;
-  const double s = st->s, c1 = st->c_r1;
+__device__ __forceinline__ double spmv_minres_range(const KrylovMatrix& M, int g0, int g1,
+                                                    const double* __restrict__ Yp, const double* __restrict__ R1,
+                                                    double* __restrict__ T, double s, double c1) {
   const int lane = threadIdx.x % LPG;
   const int gpb = blockDim.x / LPG;
   double contrib = 0.0;
-  for (int g = blockIdx.x * gpb + threadIdx.x / LPG; g < M.nGroups; g += gridDim.x * gpb) {
+  for (int g = g0 + blockIdx.x * gpb + threadIdx.x / LPG; g < g1; g += gridDim.x * gpb) {
     double a0, a1, a2;
     int r0, sz;
     group_rowsums<LPG>(M, g, lane, Yp, a0, a1, a2, r0, sz);
@@ -181,22 +189,43 @@ __global__ __launch_bounds__(256) void spmv_minres_kernel(KrylovMatrix M, const 
       contrib += s * Yp[row] * t;
     }
   }
+  return contrib;
+}
+
+// T = s * A Yp - c1 * R1 ; partial alfa = s * sum Yp . T.  The leading single-row groups (the
+// element pressures: 25 entries each) take LPG1 lanes each, the others LPG lanes.
+template <int LPG1, int LPG>
+__global__ __launch_bounds__(256) void spmv_minres_kernel(KrylovMatrix M, const double* __restrict__ Yp,
+                                                         const double* __restrict__ R1, double* __restrict__ T,
+                                                         MinresState* __restrict__ st, double* __restrict__ partial) {
+  __shared__ double sh[4];
+  if (st->converged) return;
+  const double s = st->s, c1 = st->c_r1;
+  double contrib = spmv_minres_range<LPG1>(M, 0, M.nLead1, Yp, R1, T, s, c1);
+  contrib += spmv_minres_range<LPG>(M, M.nLead1, M.nGroups, Yp, R1, T, s, c1);
   const double b = block_sum<256>(contrib, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
-// plain y = A x
+// plain y = A x over groups [g0, g1)
 template <int LPG>
-__global__ __launch_bounds__(256) void spmv_kernel(KrylovMatrix M, const double* __restrict__ x,
-                                                  double* __restrict__ y) {
+__device__ __forceinline__ void spmv_range(const KrylovMatrix& M, int g0, int g1, const double* __restrict__ x,
+                                           double* __restrict__ y) {
   const int lane = threadIdx.x % LPG;
   const int gpb = blockDim.x / LPG;
-  for (int g = blockIdx.x * gpb + threadIdx.x / LPG; g < M.nGroups; g += gridDim.x * gpb) {
+  for (int g = g0 + blockIdx.x * gpb + threadIdx.x / LPG; g < g1; g += gridDim.x * gpb) {
     double a0, a1, a2;
     int r0, sz;
     group_rowsums<LPG>(M, g, lane, x, a0, a1, a2, r0, sz);
     if (lane < sz) y[r0 + lane] = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
   }
+}
+
+template <int LPG1, int LPG>
+__global__ __launch_bounds__(256) void spmv_kernel(KrylovMatrix M, const double* __restrict__ x,
+                                                  double* __restrict__ y) {
+  spmv_range<LPG1>(M, 0, M.nLead1, x, y);
+  spmv_range<LPG>(M, M.nLead1, M.nGroups, x, y);
 }
 
 // ---- fixed-order finish of a reduction + MINRES scalar steps --------------
@@ -402,10 +431,10 @@ void krylov_precond_setup(hipStream_t s, const KrylovMatrix& M, const Precond& P
                      P.gDiagPos, M, P.binv);
 }
 
-constexpr int kLPG = 16;
+constexpr int kLPG = 16, kLPG1 = 8;
 
 void krylov_spmv(hipStream_t s, const KrylovMatrix& M, const double* x, double* y) {
-  hipLaunchKernelGGL(spmv_kernel<kLPG>, dim3(kKrylovBlocks), dim3(256), 0, s, M, x, y);
+  hipLaunchKernelGGL((spmv_kernel<kLPG1, kLPG>), dim3(kKrylovBlocks), dim3(256), 0, s, M, x, y);
 }
 
 void krylov_dot(hipStream_t s, int n, const double* a, const double* b, double* partial, double* out) {
@@ -427,8 +456,8 @@ void minres_start(hipStream_t s, const KrylovMatrix& M, const Precond& P, Minres
 // one MINRES iteration; buffers rotate on the host (pure pointer swaps)
 void minres_iteration(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, double tol,
                       const PrecondHook* hook) {
-  hipLaunchKernelGGL(spmv_minres_kernel<kLPG>, dim3(kKrylovBlocks), dim3(256), 0, s, M, w.Yp, w.R1, w.T, w.st,
-                     w.partial);
+  hipLaunchKernelGGL((spmv_minres_kernel<kLPG1, kLPG>), dim3(kKrylovBlocks), dim3(256), 0, s, M, w.Yp, w.R1, w.T,
+                     w.st, w.partial);
   hipLaunchKernelGGL(minres_scalar1_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, kKrylovBlocks);
   hipLaunchKernelGGL(precond_kernel, dim3(kKrylovBlocks), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo, P.binv, w.T,
                      w.R2, w.Z, w.st, w.partial);

@@ -440,7 +440,8 @@ void setupSchur(Ctx& c, const std::vector<int64_t>& rp, const std::vector<int>& 
   S->cinv.alloc((size_t)ncst * ncst * 8);
   // (a range holding only the coarsest level stays a multi-workgroup dense product)
   for (size_t l = 1; l + 1 < S->L.size(); l++)
-    if ((int64_t)S->L[l]->nSlots + (int64_t)S->L[l]->nd * S->L[l]->ldd <= kFuseSlots) {
+    // (the diagonal form reads 2 nd - 1 entries a row: count those, as the slots they replace)
+    if ((int64_t)S->L[l]->nSlots + (2ll * S->L[l]->nd - 1) * S->L[l]->ldd <= kFuseSlots) {
       S->fuseFrom = (int)l;
       S->fused.alloc((S->L.size() - l) * sizeof(FusedLevel));
       break;

@@ -127,6 +127,8 @@ void Ctx::setupBlock2(const CsrPattern& U2, const std::vector<int8_t>& kind3, co
     for (int64_t k = rp[i]; k < rp[i + 1]; k++)
       if (col[k] == i) diagpos[i] = (int)fullToVal[k];
   nSpmvGroups = (int)sgRow0.size();
+  nLead1 = 0;
+  while (nLead1 < nSpmvGroups && sgSize[nLead1] == 1) nLead1++;
   d_sgRow0.upload(sgRow0);
   d_sgSize.upload(sgSize);
   d_sgColPtr.upload(sgColPtr);
@@ -311,6 +313,7 @@ KrylovMatrix Ctx::krylovMatrix() const {
   M.val = d_full2.as<double>();
   M.diagpos = d_diagpos2.as<int>();
   M.rowGroup = d_rowGroup.as<int>();
+  M.nLead1 = nLead1;
   return M;
 }
 

@@ -2,7 +2,8 @@
 Newton runs in seconds: the 2 x 2 x 1 box with a rigid-body top (+15 % UZ, 20 degree ROTZ),
 Mooney-Rivlin C10 = 0.5, C01 = 0.1, K = 1000, meshed 12 x 12 x 6 with the same jitter and
 seed, over the full 20-step schedule; Newton to a 1e-8 relative residual (criterion 1),
-MINRES to 1e-9 with block Jacobi + the Schur-row multigrid (preconditioner 2).  After every
+MINRES to 1e-9 with block Jacobi + the Schur-row multigrid (preconditioner 2), the first solve
+of each load step to 1e-5 (krylov_tol_loose, the inexact-Newton setting).  After every
 load step (FESolver.cpp:457-483) the displacements must agree with the oracle's (FP64
 direct solves in place of PARDISO) within 1e-8 relative (max-abs over the solution vector),
 with no fallback to plain block Jacobi."""
@@ -36,12 +37,13 @@ def c3_like(nx, ny, nz):
 
 @pytest.mark.timeout(600)
 def test_timed_settings_match_oracle_every_load_step(gpu_lib):
-    print("[gpu] timed settings (criterion 1e-8, MINRES 1e-9, preconditioner 2), 12x12x6, 20 load steps",
+    print("[gpu] timed settings (criterion 1e-8, MINRES 1e-9 / first solve 1e-5, preconditioner 2), 12x12x6, "
+          "20 load steps",
           flush=True)
     m, o = c3_like(12, 12, 6)
     o.newton_init()
     opts = m.newton_defaults(criterion=1, convergence=1e-8, krylov_tol=1e-9, krylov_max_iter=100000,
-                             preconditioner=2)
+                             preconditioner=2, krylov_tol_loose=1e-5)
     st = nla3d_amd.NewtonStats()
     worst = 0.0
     for t in load_steps(20):
