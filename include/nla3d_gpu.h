@@ -437,6 +437,15 @@ int nla3d_gpu_time_assembly(nla3d_gpu_t h, int iters, double* ms_element, double
 /* Mean duration in ms of the block(2) SpMV kernel of the Krylov solve (full-symmetric
  * CSR, y = K2 x) over `iters` launches, HIP events on the handle's stream. */
 int nla3d_gpu_time_spmv(nla3d_gpu_t h, int iters, double* ms);
+/* No counterpart in the reference (a check of the Krylov solve's operator): y = K.block(2) x
+ * with the currently assembled K through the solve's own SpMV kernel; x and y are host arrays
+ * of length nUnknown + nMpc. */
+int nla3d_gpu_apply_block2(nla3d_gpu_t h, const double* x, double* y);
+/* How the SpMV reads block(2): dia_nodes nodes through the symmetric node-diagonal copy with
+ * dia_offsets upper node offsets each (0, 0 when there is no copy: small or unstructured
+ * models, or NLA3D_GPU_NODE_DIA=0 in the environment at nla3d_gpu_init_solution), csr_groups
+ * groups of rows through the grouped full-symmetric CSR.  Any pointer may be NULL. */
+int nla3d_gpu_spmv_info(nla3d_gpu_t h, int* dia_nodes, int* dia_offsets, int* csr_groups);
 /* Mean duration in ms of one finest-level step of the Schur multigrid's Chebyshev
  * smoother (cheb_step_kernel: res -= A d, d' = c1 d + c2 D^-1 res over the FP32
  * sliced-ELLPACK copy; x += d + d' on odd steps and nothing on the even steps after them,

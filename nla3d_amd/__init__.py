@@ -121,6 +121,8 @@ ABI = {
     "nla3d_gpu_solid81_gp_state": [_P, _DP],
     "nla3d_gpu_time_assembly": [_P, ctypes.c_int, _DP, _DP],
     "nla3d_gpu_time_spmv": [_P, ctypes.c_int, _DP],
+    "nla3d_gpu_apply_block2": [_P, _DP, _DP],
+    "nla3d_gpu_spmv_info": [_P, _IP, _IP, _IP],
     "nla3d_gpu_copy_calibration": [_P, ctypes.c_uint64, _DP],
     "nla3d_gpu_solve_load_step": [_P, ctypes.c_double, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats)],
     "nla3d_gpu_solve_nonlinear": [_P, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats)],
@@ -563,6 +565,20 @@ class GpuModel:
         nb = ctypes.c_longlong()
         self._call("nla3d_gpu_time_smoother", int(iters), ctypes.byref(a), ctypes.byref(nb))
         return a.value, nb.value
+
+    def apply_block2(self, x):
+        """K.block(2) x through the Krylov solve's SpMV (host arrays)"""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self._call("nla3d_gpu_apply_block2", _ptr(x, ctypes.c_double), _ptr(y, ctypes.c_double))
+        return y
+
+    def spmv_info(self):
+        """(nodes read through the node-diagonal copy, its upper node offsets, row groups read
+        through the grouped CSR)"""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._call("nla3d_gpu_spmv_info", ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
 
     def time_spmv(self, iters):
         a = ctypes.c_double()

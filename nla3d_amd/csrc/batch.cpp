@@ -92,6 +92,7 @@ static void preSolveChain(Batch& b, double t) {
     newton_rhs(s, c.n2, nC, c.d_vals.as<double>() + md.nnzAll(), R, U, c.d_Ucprev.as<double>(), c.d_k12tPtr.as<int>(),
                c.d_k12tPos.as<int>(), c.d_k12tRow.as<int>(), c.d_vals.as<double>() + md.offB12(), c.d_rhs.as<double>());
     vec_sumsq(s, c.n2, c.d_rhs.as<double>(), c.d_partial.as<double>(), b.d_norms.as<double>() + k);
+    c.refreshSpmvCopy(s);
     krylov_precond_setup(s, c.krylovMatrix(), c.precond());
   }
 }

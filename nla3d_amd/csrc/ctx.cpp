@@ -122,6 +122,7 @@ void Ctx::applyMpc() {
                   d_full2.as<double>());
     hipCheck(hipGetLastError(), "mirror launch");
   }
+  refreshSpmvCopy(stream);
 }
 
 // FEStorage::assembleGlobalEqMatrices (FEStorage.cpp:26-78): the gather writes

@@ -113,6 +113,10 @@ struct Ctx {
   // full-symmetric block 2 in grouped CSR (krylov.h)
   DevBuf d_sgRow0, d_sgSize, d_sgColPtr, d_sgCols, d_sgValPtr, d_rowGroup, d_diagpos2, d_full2;
   int nSpmvGroups = 0, nLead1 = 0;  // groups, and the leading single-row ones
+  // node-diagonal copy of the SpMV (KrylovMatrix): values, node list, ELL columns / values,
+  // the grouped-row list, fill positions; shape in spmvDia (nDiaFit == 0: no copy)
+  DevBuf d_dia, d_diaFit, d_remCol, d_remVal, d_csrList, d_diaPos, d_remPos;
+  KrylovMatrix spmvDia;
   DevBuf d_mir2;              // int [2*nnz2]: full positions of each upper entry of block 2
   int64_t fullNnz2 = 0;
   DevBuf d_gStart, d_gInfo, d_gDiagPos, d_binv;  // block-Jacobi groups
@@ -174,12 +178,16 @@ struct Ctx {
   void scatter();
   void applyMpc();
   void assemble();
+  // the SpMV's node-diagonal copy from d_full2 (after every write of d_full2)
+  void refreshSpmvCopy(hipStream_t s) const;
 
   // solver.cpp
   void setupKrylov();
   void setupBlock2(const CsrPattern& U2, const std::vector<int8_t>& kind3, const std::vector<uint8_t>& rowType,
                    const std::vector<uint32_t>& rowNode);
   void setupModelRows();
+  void setupNodeDia(const std::vector<int>& sgRow0, const std::vector<int>& sgSize, const std::vector<int>& sgColPtr,
+                    const std::vector<int>& sgCols);
   KrylovMatrix krylovMatrix() const;
   Precond precond();
   // solve block(2) x = b with the current K; b, x device pointers [n2]; returns iterations

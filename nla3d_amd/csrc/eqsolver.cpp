@@ -129,6 +129,7 @@ static void factorize(EqSolver& S, uint32_t n, const double* values, const uint3
   const size_t nnz = S.pat.nnz();
   hipCheck(hipMemcpyAsync(S.d_up.p, values, nnz * 8, hipMemcpyHostToDevice, c.stream), "values");
   launch_mirror(c.stream, (int)nnz, S.d_up.as<double>(), nullptr, c.d_mir2.as<int>(), c.d_full2.as<double>());
+  c.refreshSpmvCopy(c.stream);
   krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
   hipCheck(hipGetLastError(), "factorize");
   S.schurOk = false;

@@ -22,6 +22,28 @@ struct KrylovMatrix {
   const int* diagpos = nullptr;  // [n] position of (i,i) in val, or -1 (multiplier rows have none)
   const int* rowGroup = nullptr; // [n] group of row i
   int nLead1 = 0;                // groups [0, nLead1) are single rows (the element pressures come first)
+
+  // Node-diagonal copy, read by the SpMV in place of the grouped rows of most nodes of a
+  // structured block (absent when nDiaFit == 0).  The run: diaNodes consecutive 3-row groups
+  // diaG0 + t with rows diaRow0 + 3t.  Node t's upper node couplings are the 3x3 blocks
+  // (t, t + diaOff[k]), k < nd, diaOff[0] = 0, stored once each (symmetric storage):
+  // dia[(9k + 3r + c) * diaNodes + t] = K(row(t) + r, row(t + diaOff[k]) + c).  The nDiaFit
+  // nodes fit[i] whose node couplings are exactly (t, t +- diaOff[k]) are read from there
+  // (the lower blocks as the transposes of the neighbours' upper blocks), with their other
+  // columns (element pressures, multipliers) in a width-remW ELL: column remCol[j * nDiaFit + i]
+  // (-1 = padding), values remVal[(3j + r) * nDiaFit + i].  All other groups after the leading
+  // single rows are csrList[0, nCsrList).  The grouped rows stay complete; the copy is refreshed
+  // from val by krylov_refresh (diaPos / remPos: positions in each group's column list).
+  static constexpr int kMaxDiaOffsets = 32;
+  int nDiaFit = 0, diaNodes = 0, diaG0 = 0, diaRow0 = 0, nd = 0, remW = 0, nCsrList = 0;
+  int diaOff[kMaxDiaOffsets] = {};
+  const int* fit = nullptr;
+  double* dia = nullptr;
+  const int* remCol = nullptr;
+  double* remVal = nullptr;
+  const int* csrList = nullptr;
+  const int* diaPos = nullptr;  // [k * diaNodes + t] or -1
+  const int* remPos = nullptr;  // [j * nDiaFit + i] or -1
 };
 
 // block-Jacobi groups: each row belongs to exactly one group
@@ -52,6 +74,9 @@ struct MinresWork {
 constexpr int kKrylovBlocks = 2048;
 
 void krylov_precond_setup(hipStream_t s, const KrylovMatrix& M, const Precond& P);
+// the node-diagonal copy from the current values (nothing when M has none); every write of
+// M.val must be followed by it before the next SpMV
+void krylov_refresh(hipStream_t s, const KrylovMatrix& M);
 void krylov_spmv(hipStream_t s, const KrylovMatrix& M, const double* x, double* y);
 void krylov_dot(hipStream_t s, int n, const double* a, const double* b, double* partial, double* out);
 // A preconditioner stage after the block-Jacobi pass: given r and z = P_bj r (device, n), it

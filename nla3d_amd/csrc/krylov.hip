@@ -168,16 +168,68 @@ __device__ __forceinline__ void group_rowsums(const KrylovMatrix& M, int g, int 
   }
 }
 
-// T = s * A Yp - c1 * R1 over groups [g0, g1), LPG lanes per group, grid-stride; returns this
-// thread's part of s * sum Yp . T
-template <int LPG>
-__device__ __forceinline__ double spmv_minres_range(const KrylovMatrix& M, int g0, int g1,
-                                                    const double* __restrict__ Yp, const double* __restrict__ R1,
-                                                    double* __restrict__ T, double s, double c1) {
+// the three rows of node-diagonal node i (see KrylovMatrix): the diagonal block, the upper
+// blocks (t, t + d) and the transposes of the upper blocks (t - d, t) of the neighbours, then
+// the ELL columns; consecutive i read consecutive addresses of every diagonal
+__device__ __forceinline__ void dia_rows(const KrylovMatrix& M, int i, const double* __restrict__ x, double& a0,
+                                         double& a1, double& a2, int& row) {
+  const int t = M.fit[i];
+  const size_t ld = (size_t)M.diaNodes;
+  const double* __restrict__ xr = x + M.diaRow0;
+  const double* __restrict__ D = M.dia + t;
+  {
+    const double x0 = xr[3 * t], x1 = xr[3 * t + 1], x2 = xr[3 * t + 2];
+    a0 = D[0] * x0 + D[ld] * x1 + D[2 * ld] * x2;
+    a1 = D[3 * ld] * x0 + D[4 * ld] * x1 + D[5 * ld] * x2;
+    a2 = D[6 * ld] * x0 + D[7 * ld] * x1 + D[8 * ld] * x2;
+  }
+#pragma unroll 1
+  for (int k = 1; k < M.nd; k++) {
+    const int d = M.diaOff[k];
+    const double* __restrict__ U = D + (size_t)k * 9 * ld;  // block (t, t + d)
+    const double* __restrict__ B = U - d;                   // block (t - d, t)
+    const double u0 = xr[3 * (t + d)], u1 = xr[3 * (t + d) + 1], u2 = xr[3 * (t + d) + 2];
+    const double l0 = xr[3 * (t - d)], l1 = xr[3 * (t - d) + 1], l2 = xr[3 * (t - d) + 2];
+    a0 += U[0] * u0 + U[ld] * u1 + U[2 * ld] * u2;
+    a1 += U[3 * ld] * u0 + U[4 * ld] * u1 + U[5 * ld] * u2;
+    a2 += U[6 * ld] * u0 + U[7 * ld] * u1 + U[8 * ld] * u2;
+    a0 += B[0] * l0 + B[3 * ld] * l1 + B[6 * ld] * l2;
+    a1 += B[ld] * l0 + B[4 * ld] * l1 + B[7 * ld] * l2;
+    a2 += B[2 * ld] * l0 + B[5 * ld] * l1 + B[8 * ld] * l2;
+  }
+  const int nf = M.nDiaFit;
+  for (int j = 0; j < M.remW; j++) {
+    const int c = M.remCol[j * nf + i];
+    if (c >= 0) {
+      const double xv = x[c];
+      const double* __restrict__ v = M.remVal + (size_t)3 * j * nf + i;
+      a0 += v[0] * xv;
+      a1 += v[nf] * xv;
+      a2 += v[2 * nf] * xv;
+    }
+  }
+  row = M.diaRow0 + 3 * t;
+}
+
+// the group of the j-th grouped-row item after the leading single rows
+__device__ __forceinline__ int csr_group(const KrylovMatrix& M, int j) {
+  return M.csrList ? M.csrList[j] : M.nLead1 + j;
+}
+__device__ __forceinline__ int csr_count(const KrylovMatrix& M) {
+  return M.csrList ? M.nCsrList : M.nGroups - M.nLead1;
+}
+
+// T = s * A Yp - c1 * R1 over grouped-row items [0, n) (group of item j: LIST ? csr_group :
+// j), LPG lanes per group, grid-stride; returns this thread's part of s * sum Yp . T
+template <int LPG, bool LIST>
+__device__ __forceinline__ double spmv_minres_range(const KrylovMatrix& M, int n, const double* __restrict__ Yp,
+                                                    const double* __restrict__ R1, double* __restrict__ T, double s,
+                                                    double c1) {
   const int lane = threadIdx.x % LPG;
   const int gpb = blockDim.x / LPG;
   double contrib = 0.0;
-  for (int g = g0 + blockIdx.x * gpb + threadIdx.x / LPG; g < g1; g += gridDim.x * gpb) {
+  for (int j = blockIdx.x * gpb + threadIdx.x / LPG; j < n; j += gridDim.x * gpb) {
+    const int g = LIST ? csr_group(M, j) : j;
     double a0, a1, a2;
     int r0, sz;
     group_rowsums<LPG>(M, g, lane, Yp, a0, a1, a2, r0, sz);
@@ -192,8 +244,9 @@ __device__ __forceinline__ double spmv_minres_range(const KrylovMatrix& M, int g
   return contrib;
 }
 
-// T = s * A Yp - c1 * R1 ; partial alfa = s * sum Yp . T.  The leading single-row groups (the
-// element pressures: 25 entries each) take LPG1 lanes each, the others LPG lanes.
+// T = s * A Yp - c1 * R1 ; partial alfa = s * sum Yp . T over the grouped rows: the leading
+// single-row groups (the element pressures: 25 entries each) take LPG1 lanes each, the others
+// LPG lanes
 template <int LPG1, int LPG>
 __global__ __launch_bounds__(256) void spmv_minres_kernel(KrylovMatrix M, const double* __restrict__ Yp,
                                                          const double* __restrict__ R1, double* __restrict__ T,
@@ -201,19 +254,46 @@ __global__ __launch_bounds__(256) void spmv_minres_kernel(KrylovMatrix M, const 
   __shared__ double sh[4];
   if (st->converged) return;
   const double s = st->s, c1 = st->c_r1;
-  double contrib = spmv_minres_range<LPG1>(M, 0, M.nLead1, Yp, R1, T, s, c1);
-  contrib += spmv_minres_range<LPG>(M, M.nLead1, M.nGroups, Yp, R1, T, s, c1);
+  double contrib = spmv_minres_range<LPG1, false>(M, M.nLead1, Yp, R1, T, s, c1);
+  contrib += spmv_minres_range<LPG, true>(M, csr_count(M), Yp, R1, T, s, c1);
   const double b = block_sum<256>(contrib, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
-// plain y = A x over groups [g0, g1)
-template <int LPG>
-__device__ __forceinline__ void spmv_range(const KrylovMatrix& M, int g0, int g1, const double* __restrict__ x,
+// the same over the node-diagonal nodes, one thread each, launched after spmv_minres_kernel on
+// the same grid: each block adds its sum to that kernel's partial of the same block
+__global__ __launch_bounds__(256) void spmv_minres_dia_kernel(KrylovMatrix M, const double* __restrict__ Yp,
+                                                             const double* __restrict__ R1, double* __restrict__ T,
+                                                             MinresState* __restrict__ st,
+                                                             double* __restrict__ partial) {
+  __shared__ double sh[4];
+  if (st->converged) return;
+  const double s = st->s, c1 = st->c_r1;
+  double contrib = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M.nDiaFit; i += gridDim.x * blockDim.x) {
+    double a0, a1, a2;
+    int row;
+    dia_rows(M, i, Yp, a0, a1, a2, row);
+    const double t0 = s * a0 - c1 * R1[row], t1 = s * a1 - c1 * R1[row + 1], t2 = s * a2 - c1 * R1[row + 2];
+    T[row] = t0;
+    T[row + 1] = t1;
+    T[row + 2] = t2;
+    contrib += s * Yp[row] * t0;
+    contrib += s * Yp[row + 1] * t1;
+    contrib += s * Yp[row + 2] * t2;
+  }
+  const double b = block_sum<256>(contrib, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] += b;
+}
+
+// plain y = A x over grouped-row items [0, n)
+template <int LPG, bool LIST>
+__device__ __forceinline__ void spmv_range(const KrylovMatrix& M, int n, const double* __restrict__ x,
                                            double* __restrict__ y) {
   const int lane = threadIdx.x % LPG;
   const int gpb = blockDim.x / LPG;
-  for (int g = g0 + blockIdx.x * gpb + threadIdx.x / LPG; g < g1; g += gridDim.x * gpb) {
+  for (int j = blockIdx.x * gpb + threadIdx.x / LPG; j < n; j += gridDim.x * gpb) {
+    const int g = LIST ? csr_group(M, j) : j;
     double a0, a1, a2;
     int r0, sz;
     group_rowsums<LPG>(M, g, lane, x, a0, a1, a2, r0, sz);
@@ -224,8 +304,52 @@ __device__ __forceinline__ void spmv_range(const KrylovMatrix& M, int g0, int g1
 template <int LPG1, int LPG>
 __global__ __launch_bounds__(256) void spmv_kernel(KrylovMatrix M, const double* __restrict__ x,
                                                   double* __restrict__ y) {
-  spmv_range<LPG1>(M, 0, M.nLead1, x, y);
-  spmv_range<LPG>(M, M.nLead1, M.nGroups, x, y);
+  spmv_range<LPG1, false>(M, M.nLead1, x, y);
+  spmv_range<LPG, true>(M, csr_count(M), x, y);
+}
+
+__global__ __launch_bounds__(256) void spmv_dia_kernel(KrylovMatrix M, const double* __restrict__ x,
+                                                      double* __restrict__ y) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M.nDiaFit; i += gridDim.x * blockDim.x) {
+    double a0, a1, a2;
+    int row;
+    dia_rows(M, i, x, a0, a1, a2, row);
+    y[row] = a0;
+    y[row + 1] = a1;
+    y[row + 2] = a2;
+  }
+}
+
+// the node-diagonal copy from the grouped values: upper blocks of every run node (absent
+// blocks are zero), then the ELL values of the node-diagonal nodes
+__global__ __launch_bounds__(256) void dia_fill_kernel(KrylovMatrix M) {
+  const long long nD = (long long)M.nd * M.diaNodes, nR = (long long)M.remW * M.nDiaFit;
+  const size_t ld = (size_t)M.diaNodes;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nD + nR;
+       e += (long long)gridDim.x * blockDim.x) {
+    if (e < nD) {
+      const int k = (int)(e / M.diaNodes), t = (int)(e % M.diaNodes);
+      const int p = M.diaPos[e];
+      const int g = M.diaG0 + t;
+      const int L = M.gColPtr[g + 1] - M.gColPtr[g];
+      const double* v = M.val + M.gValPtr[g];
+      double* D = M.dia + (size_t)k * 9 * ld + t;
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) D[(3 * r + c) * ld] = p >= 0 ? v[r * L + p + c] : 0.0;
+    } else {
+      const long long q = e - nD;
+      const int j = (int)(q / M.nDiaFit), i = (int)(q % M.nDiaFit);
+      const int p = M.remPos[q];
+      const int g = M.diaG0 + M.fit[i];
+      const int L = M.gColPtr[g + 1] - M.gColPtr[g];
+      const double* v = M.val + M.gValPtr[g];
+      double* R = M.remVal + (size_t)3 * j * M.nDiaFit + i;
+#pragma unroll
+      for (int r = 0; r < 3; r++) R[r * M.nDiaFit] = p >= 0 ? v[r * L + p] : 0.0;
+    }
+  }
 }
 
 // ---- fixed-order finish of a reduction + MINRES scalar steps --------------
@@ -433,8 +557,14 @@ void krylov_precond_setup(hipStream_t s, const KrylovMatrix& M, const Precond& P
 
 constexpr int kLPG = 16, kLPG1 = 8;
 
+void krylov_refresh(hipStream_t s, const KrylovMatrix& M) {
+  if (M.nDiaFit == 0) return;
+  hipLaunchKernelGGL(dia_fill_kernel, dim3(kKrylovBlocks), dim3(256), 0, s, M);
+}
+
 void krylov_spmv(hipStream_t s, const KrylovMatrix& M, const double* x, double* y) {
   hipLaunchKernelGGL((spmv_kernel<kLPG1, kLPG>), dim3(kKrylovBlocks), dim3(256), 0, s, M, x, y);
+  if (M.nDiaFit > 0) hipLaunchKernelGGL(spmv_dia_kernel, dim3(kKrylovBlocks), dim3(256), 0, s, M, x, y);
 }
 
 void krylov_dot(hipStream_t s, int n, const double* a, const double* b, double* partial, double* out) {
@@ -458,6 +588,9 @@ void minres_iteration(hipStream_t s, const KrylovMatrix& M, const Precond& P, Mi
                       const PrecondHook* hook) {
   hipLaunchKernelGGL((spmv_minres_kernel<kLPG1, kLPG>), dim3(kKrylovBlocks), dim3(256), 0, s, M, w.Yp, w.R1, w.T,
                      w.st, w.partial);
+  if (M.nDiaFit > 0)
+    hipLaunchKernelGGL(spmv_minres_dia_kernel, dim3(kKrylovBlocks), dim3(256), 0, s, M, w.Yp, w.R1, w.T, w.st,
+                       w.partial);
   hipLaunchKernelGGL(minres_scalar1_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, kKrylovBlocks);
   hipLaunchKernelGGL(precond_kernel, dim3(kKrylovBlocks), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo, P.binv, w.T,
                      w.R2, w.Z, w.st, w.partial);

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <unordered_map>
 #include <string>
 
 #include "ctx.h"
@@ -77,6 +78,157 @@ void fullSymmetricPattern(const CsrPattern& U2, std::vector<int64_t>& rp, std::v
   }
 }
 
+// The SpMV's node-diagonal copy (KrylovMatrix): on the longest run of consecutive 3-row
+// groups with consecutive rows (the free DoF triples of a block of nodes in node order), the
+// most common set of upper node offsets; the run nodes whose node couplings are exactly
+// (t, t +- offset) and whose other columns fit the most common count are read through the
+// diagonals, the rest through the grouped rows.  Worth it only on large structured blocks:
+// no copy under kMinDiaNodes such nodes or half the run.
+void Ctx::setupNodeDia(const std::vector<int>& sgRow0, const std::vector<int>& sgSize, const std::vector<int>& sgColPtr,
+                       const std::vector<int>& sgCols) {
+  constexpr int kMinDiaNodes = 4096;
+  constexpr int kMaxW = 16;
+  spmvDia = KrylovMatrix();
+  const int nG = (int)sgRow0.size();
+  if (std::getenv("NLA3D_GPU_NODE_DIA") && std::string(std::getenv("NLA3D_GPU_NODE_DIA")) == "0") return;
+  int bestG0 = 0, bestT = 0;
+  for (int g = nLead1; g < nG;) {
+    if (sgSize[g] != 3) {
+      g++;
+      continue;
+    }
+    int e = g + 1;
+    while (e < nG && sgSize[e] == 3 && sgRow0[e] == sgRow0[e - 1] + 3) e++;
+    if (e - g > bestT) bestG0 = g, bestT = e - g;
+    g = e;
+  }
+  if (bestT < kMinDiaNodes) return;
+  const int T = bestT, g0 = bestG0, R0 = sgRow0[g0], R1 = R0 + 3 * T;
+  // per run node: node offsets (upper, lower), the positions of the upper ones and of the
+  // other columns in the group's column list
+  std::vector<int> upPtr{0}, upOff, upPos, loPtr{0}, loOff, otPtr{0}, otPos;
+  std::vector<char> complete(T, 1);
+  for (int t = 0; t < T; t++) {
+    const int g = g0 + t, c0 = sgColPtr[g], L = sgColPtr[g + 1] - c0;
+    for (int p = 0; p < L;) {
+      const int c = sgCols[c0 + p];
+      if (c >= R0 && c < R1) {
+        if ((c - R0) % 3 != 0 || p + 2 >= L || sgCols[c0 + p + 1] != c + 1 || sgCols[c0 + p + 2] != c + 2) {
+          complete[t] = 0;
+          p++;
+          continue;
+        }
+        const int d = (c - R0) / 3 - t;
+        if (d >= 0) {
+          upOff.push_back(d);
+          upPos.push_back(p);
+        } else {
+          loOff.push_back(-d);
+        }
+        p += 3;
+      } else {
+        otPos.push_back(p);
+        p++;
+      }
+    }
+    upPtr.push_back((int)upOff.size());
+    loPtr.push_back((int)loOff.size());
+    otPtr.push_back((int)otPos.size());
+  }
+  // the most common upper offset list (columns are sorted, so the lists are ascending)
+  std::unordered_map<uint64_t, std::pair<int, int>> count;  // hash -> (count, first node)
+  auto hashOf = [&](int t) {
+    uint64_t h = 1469598103934665603ull;
+    for (int q = upPtr[t]; q < upPtr[t + 1]; q++) h = (h ^ (uint64_t)(uint32_t)upOff[q]) * 1099511628211ull;
+    return h ^ (uint64_t)(upPtr[t + 1] - upPtr[t]);
+  };
+  for (int t = 0; t < T; t++) {
+    if (!complete[t]) continue;
+    auto& e = count[hashOf(t)];
+    if (e.first++ == 0) e.second = t;
+  }
+  int ref = -1, best = 0;
+  for (auto& kv : count)
+    if (kv.second.first > best) best = kv.second.first, ref = kv.second.second;
+  if (ref < 0) return;
+  const int nd = upPtr[ref + 1] - upPtr[ref];
+  if (nd < 1 || nd > KrylovMatrix::kMaxDiaOffsets || upOff[upPtr[ref]] != 0) return;
+  std::vector<int> off(upOff.begin() + upPtr[ref], upOff.begin() + upPtr[ref + 1]);
+  auto patternFits = [&](int t) {
+    if (!complete[t] || upPtr[t + 1] - upPtr[t] != nd || loPtr[t + 1] - loPtr[t] != nd - 1) return false;
+    for (int k = 0; k < nd; k++)
+      if (upOff[upPtr[t] + k] != off[k]) return false;
+    // lower offsets come in ascending column order: the largest distance first
+    for (int k = 1; k < nd; k++)
+      if (loOff[loPtr[t] + k - 1] != off[nd - k]) return false;
+    return true;
+  };
+  std::vector<int> wCount(kMaxW + 1, 0);
+  for (int t = 0; t < T; t++)
+    if (patternFits(t) && otPtr[t + 1] - otPtr[t] <= kMaxW) wCount[otPtr[t + 1] - otPtr[t]]++;
+  const int W = (int)(std::max_element(wCount.begin(), wCount.end()) - wCount.begin());
+  std::vector<int> fit;
+  for (int t = 0; t < T; t++)
+    if (patternFits(t) && otPtr[t + 1] - otPtr[t] <= W) fit.push_back(t);
+  const int nf = (int)fit.size();
+  if (nf < kMinDiaNodes || 2 * nf < T) return;
+  // fill positions: every upper block of every run node (a fitting node's lower blocks are
+  // its neighbours' upper blocks, present by the pattern's symmetry)
+  std::vector<int> diaPos((size_t)nd * T, -1);
+  for (int t = 0; t < T; t++)
+    for (int q = upPtr[t], k = 0; q < upPtr[t + 1]; q++) {
+      while (k < nd && off[k] < upOff[q]) k++;
+      if (k < nd && off[k] == upOff[q]) diaPos[(size_t)k * T + t] = upPos[q];
+    }
+  for (int t : fit)
+    for (int k = 1; k < nd; k++)
+      if (diaPos[(size_t)k * T + t - off[k]] < 0) throw ApiError(-1, "node-diagonal copy: asymmetric block pattern");
+  std::vector<int> remCol((size_t)W * nf, -1), remPos((size_t)W * nf, -1);
+  for (int i = 0; i < nf; i++) {
+    const int t = fit[i], g = g0 + t;
+    for (int q = otPtr[t], j = 0; q < otPtr[t + 1]; q++, j++) {
+      remPos[(size_t)j * nf + i] = otPos[q];
+      remCol[(size_t)j * nf + i] = sgCols[sgColPtr[g] + otPos[q]];
+    }
+  }
+  std::vector<char> isFit(T, 0);
+  for (int t : fit) isFit[t] = 1;
+  std::vector<int> csrList;
+  for (int g = nLead1; g < nG; g++)
+    if (g < g0 || g >= g0 + T || !isFit[g - g0]) csrList.push_back(g);
+  d_dia.alloc((size_t)nd * 9 * T * 8);
+  hipCheck(hipMemset(d_dia.p, 0, d_dia.bytes), "memset node-diagonal copy");
+  d_remVal.alloc((size_t)W * 3 * nf * 8 + 8);
+  hipCheck(hipMemset(d_remVal.p, 0, d_remVal.bytes), "memset node-diagonal ELL");
+  d_diaFit.upload(fit);
+  d_remCol.upload(remCol);
+  d_remPos.upload(remPos);
+  d_diaPos.upload(diaPos);
+  d_csrList.upload(csrList);
+  KrylovMatrix& D = spmvDia;
+  D.nDiaFit = nf;
+  D.diaNodes = T;
+  D.diaG0 = g0;
+  D.diaRow0 = R0;
+  D.nd = nd;
+  D.remW = W;
+  D.nCsrList = (int)csrList.size();
+  for (int k = 0; k < nd; k++) D.diaOff[k] = off[k];
+  D.fit = d_diaFit.as<int>();
+  D.dia = d_dia.as<double>();
+  D.remCol = d_remCol.as<int>();
+  D.remVal = d_remVal.as<double>();
+  D.csrList = d_csrList.as<int>();
+  D.diaPos = d_diaPos.as<int>();
+  D.remPos = d_remPos.as<int>();
+}
+
+void Ctx::refreshSpmvCopy(hipStream_t s) const {
+  if (!krylovReady || spmvDia.nDiaFit == 0) return;
+  krylov_refresh(s, krylovMatrix());
+  hipCheck(hipGetLastError(), "node-diagonal fill launch");
+}
+
 // The block(2) solve's device data from its upper CSR pattern (1-based, the reference's
 // layout) and a description of each row (kind3: 0 node DoF, 1 element DoF, 2 multiplier;
 // the DoF type and node of a node-DoF row): the grouped full-symmetric copy the SpMV reads
@@ -139,6 +291,7 @@ void Ctx::setupBlock2(const CsrPattern& U2, const std::vector<int8_t>& kind3, co
   d_mir2.upload(mir);
   d_full2.alloc((size_t)valTotal * 8);
   hipCheck(hipMemset(d_full2.p, 0, d_full2.bytes), "memset full2");
+  setupNodeDia(sgRow0, sgSize, sgColPtr, sgCols);
 
   // block-Jacobi groups
   // a node DoF row without a diagonal entry (element-less node) is treated like a multiplier row
@@ -314,6 +467,24 @@ KrylovMatrix Ctx::krylovMatrix() const {
   M.diagpos = d_diagpos2.as<int>();
   M.rowGroup = d_rowGroup.as<int>();
   M.nLead1 = nLead1;
+  const KrylovMatrix& D = spmvDia;
+  if (D.nDiaFit > 0) {
+    M.nDiaFit = D.nDiaFit;
+    M.diaNodes = D.diaNodes;
+    M.diaG0 = D.diaG0;
+    M.diaRow0 = D.diaRow0;
+    M.nd = D.nd;
+    M.remW = D.remW;
+    M.nCsrList = D.nCsrList;
+    for (int k = 0; k < D.nd; k++) M.diaOff[k] = D.diaOff[k];
+    M.fit = D.fit;
+    M.dia = D.dia;
+    M.remCol = D.remCol;
+    M.remVal = D.remVal;
+    M.csrList = D.csrList;
+    M.diaPos = D.diaPos;
+    M.remPos = D.remPos;
+  }
   return M;
 }
 
@@ -647,6 +818,30 @@ int nla3d_gpu_time_spmv(nla3d_gpu_t h, int iters, double* ms) {
     *ms = t / iters;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
+    return 0;
+  });
+}
+
+int nla3d_gpu_apply_block2(nla3d_gpu_t h, const double* x, double* y) {
+  if (!x || !y) return -1;
+  return newtonGuard(h, [&](Ctx& c) {
+    double* dx = c.d_mw[7].as<double>();
+    double* dy = c.d_mw[6].as<double>();
+    hipCheck(hipMemcpyAsync(dx, x, (size_t)c.n2 * 8, hipMemcpyHostToDevice, c.stream), "x");
+    krylov_spmv(c.stream, c.krylovMatrix(), dx, dy);
+    hipCheck(hipGetLastError(), "spmv launch");
+    hipCheck(hipMemcpyAsync(y, dy, (size_t)c.n2 * 8, hipMemcpyDeviceToHost, c.stream), "y");
+    hipCheck(hipStreamSynchronize(c.stream), "sync");
+    return 0;
+  });
+}
+
+int nla3d_gpu_spmv_info(nla3d_gpu_t h, int* dia_nodes, int* dia_offsets, int* csr_groups) {
+  return newtonGuard(h, [&](Ctx& c) {
+    const KrylovMatrix M = c.krylovMatrix();
+    if (dia_nodes) *dia_nodes = M.nDiaFit;
+    if (dia_offsets) *dia_offsets = M.nDiaFit ? M.nd : 0;
+    if (csr_groups) *csr_groups = M.nLead1 + (M.csrList ? M.nCsrList : M.nGroups - M.nLead1);
     return 0;
   });
 }

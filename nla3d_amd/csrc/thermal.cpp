@@ -152,6 +152,7 @@ void assembleThermal(Ctx& c) {
   const double* C2 = c.d_vals.as<double>() + m.nnzAll() + m.nAll() + m.offB2();
   launch_combine_mirror(c.stream, n2u, 1.0, K2, 0.0, nullptr, c.d_mir2.as<int>(), c.d_full2.as<double>());
   launch_combine_mirror(c.stream, n2u, 1.0, C2, 0.0, nullptr, c.d_mir2.as<int>(), T.full2C.as<double>());
+  c.refreshSpmvCopy(c.stream);
   hipCheck(hipGetLastError(), "thermal mirror");
   T.assembled = true;
 }
@@ -333,6 +334,7 @@ void transientBegin(Ctx& c, const nla3d_gpu_transient_opts& o) {
   const double* K2 = c.d_vals.as<double>() + m.offB2();
   const double* C2 = c.d_vals.as<double>() + m.nnzAll() + m.nAll() + m.offB2();
   launch_combine_mirror(c.stream, n2u, 1.0, K2, T.a1, C2, c.d_mir2.as<int>(), c.d_full2.as<double>());
+  c.refreshSpmvCopy(c.stream);
   krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
   hipCheck(hipGetLastError(), "transient set-up");
   hipCheck(hipStreamSynchronize(c.stream), "sync");
@@ -352,6 +354,8 @@ bool transientStep(Ctx& c, nla3d_gpu_newton_stats& st) {
   launch_tr_comb(c.stream, c.n2, nC, T.a1, U, T.a4, T.DU.as<double>(), T.a5, T.DDU.as<double>(), T.v.as<double>());
   KrylovMatrix MC = c.krylovMatrix();
   MC.val = T.full2C.as<double>();
+  MC.nDiaFit = 0;  // C has no node-diagonal copy of its own: all rows through the grouped CSR
+  MC.csrList = nullptr;
   krylov_spmv(c.stream, MC, T.v.as<double>(), T.y.as<double>());
   launch_tr_rhs(c.stream, c.n2, nC, c.d_vals.as<double>() + m.nnzAll(), c.d_R.as<double>(), T.y.as<double>(), U,
                 c.d_k12tPtr.as<int>(), c.d_k12tPos.as<int>(), c.d_k12tRow.as<int>(),
