@@ -70,6 +70,18 @@ struct MinresWork {
   MinresState* st;
 };
 
+// The logical index of this workgroup for one-row-block-per-workgroup and chunked launches.
+// Workgroups go to the 8 XCDs in turn (workgroup b on XCD b % 8); renumbering them so that
+// each XCD takes one contiguous range of logical blocks keeps what a workgroup reads around
+// its rows (the lower entries i - d of symmetric diagonals, the neighbours' x) in its own
+// XCD's L2 instead of another XCD's.  A bijection of [0, gridDim.x).
+__device__ __forceinline__ int xcd_block() {
+  constexpr int kXcd = 8;
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int x = b % kXcd, q = b / kXcd, base = nb / kXcd, rem = nb % kXcd;
+  return x * base + (x < rem ? x : rem) + q;
+}
+
 // fixed grid of the grid-stride Krylov kernels (also the number of reduction partials)
 constexpr int kKrylovBlocks = 2048;
 

@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void cheb_start_x_kernel(AmgSell A, const doub
                                                          const double* __restrict__ dinv, double c0,
                                                          double* __restrict__ res, double* __restrict__ d,
                                                          const double* __restrict__ x) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
   const double ri = r[i] - sell_dot<kNarrow>(A, i, x);
   res[i] = ri;
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(AmgSell A, const double*
                                                       const double* __restrict__ dinv, double c1, double c2,
                                                       double* __restrict__ res, double* __restrict__ dout,
                                                       double* __restrict__ x, int xmode) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
   const double ri = res[i] - sell_dot<kNarrow>(A, i, din);
   res[i] = ri;
@@ -206,7 +206,7 @@ template <bool kNarrow>
 __global__ __launch_bounds__(256) void residual_kernel(AmgSell A, const double* __restrict__ res,
                                                      const double* __restrict__ dlast, double* __restrict__ t,
                                                      double* __restrict__ x) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
   t[i] = res[i] - sell_dot<kNarrow>(A, i, dlast);
   x[i] += dlast[i];
