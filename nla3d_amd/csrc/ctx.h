@@ -115,7 +115,7 @@ struct Ctx {
   int nSpmvGroups = 0, nLead1 = 0;  // groups, and the leading single-row ones
   // node-diagonal copy of the SpMV (KrylovMatrix): values, node list, ELL columns / values,
   // the grouped-row list, fill positions; shape in spmvDia (nDiaFit == 0: no copy)
-  DevBuf d_dia, d_diaFit, d_remCol, d_remVal, d_csrList, d_diaPos, d_remPos;
+  DevBuf d_dia, d_diaFit, d_remCol, d_remVal, d_csrList, d_diaPos, d_remPos, d_diaRank;
   KrylovMatrix spmvDia;
   DevBuf d_mir2;              // int [2*nnz2]: full positions of each upper entry of block 2
   int64_t fullNnz2 = 0;

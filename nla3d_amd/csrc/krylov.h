@@ -44,6 +44,10 @@ struct KrylovMatrix {
   const int* csrList = nullptr;
   const int* diaPos = nullptr;  // [k * diaNodes + t] or -1
   const int* remPos = nullptr;  // [j * nDiaFit + i] or -1
+  // the SpMV walks the run in chunks of diaChunk nodes, chunk c on XCD c % 8 (a chunk size
+  // that puts most pairs (t, t - diaOff[k]) on one XCD); diaRank[t] = i for fit[i] = t, else -1
+  int diaChunk = 0;
+  const int* diaRank = nullptr;
 };
 
 // block-Jacobi groups: each row belongs to exactly one group

@@ -171,9 +171,8 @@ __device__ __forceinline__ void group_rowsums(const KrylovMatrix& M, int g, int 
 // the three rows of node-diagonal node i (see KrylovMatrix): the diagonal block, the upper
 // blocks (t, t + d) and the transposes of the upper blocks (t - d, t) of the neighbours, then
 // the ELL columns; consecutive i read consecutive addresses of every diagonal
-__device__ __forceinline__ void dia_rows(const KrylovMatrix& M, int i, const double* __restrict__ x, double& a0,
-                                         double& a1, double& a2, int& row) {
-  const int t = M.fit[i];
+__device__ __forceinline__ void dia_rows(const KrylovMatrix& M, int i, int t, const double* __restrict__ x,
+                                         double& a0, double& a1, double& a2, int& row) {
   const size_t ld = (size_t)M.diaNodes;
   const double* __restrict__ xr = x + M.diaRow0;
   const double* __restrict__ D = M.dia + t;
@@ -209,6 +208,21 @@ __device__ __forceinline__ void dia_rows(const KrylovMatrix& M, int i, const dou
     }
   }
   row = M.diaRow0 + 3 * t;
+}
+
+// f(i, t) for every node-diagonal node (t = fit[i]): the run in chunks of M.diaChunk nodes,
+// chunk c walked by the workgroups of XCD c % 8 (workgroup b runs on XCD b % 8; gridDim.x is
+// a multiple of 8), each XCD's chunks in order, all XCDs at about the same place of the run
+template <class F>
+__device__ __forceinline__ void dia_walk(const KrylovMatrix& M, F&& f) {
+  const int x = (int)blockIdx.x & 7, q = (int)blockIdx.x >> 3, nq = (int)gridDim.x >> 3;
+  const int C = M.diaChunk;
+  for (int l = q * (int)blockDim.x + (int)threadIdx.x;; l += nq * (int)blockDim.x) {
+    const int t = (x + 8 * (l / C)) * C + l % C;
+    if (t >= M.diaNodes) break;
+    const int i = M.diaRank[t];
+    if (i >= 0) f(i, t);
+  }
 }
 
 // the group of the j-th grouped-row item after the leading single rows
@@ -270,10 +284,10 @@ __global__ __launch_bounds__(256) void spmv_minres_dia_kernel(KrylovMatrix M, co
   if (st->converged) return;
   const double s = st->s, c1 = st->c_r1;
   double contrib = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M.nDiaFit; i += gridDim.x * blockDim.x) {
+  dia_walk(M, [&](int i, int t) {
     double a0, a1, a2;
     int row;
-    dia_rows(M, i, Yp, a0, a1, a2, row);
+    dia_rows(M, i, t, Yp, a0, a1, a2, row);
     const double t0 = s * a0 - c1 * R1[row], t1 = s * a1 - c1 * R1[row + 1], t2 = s * a2 - c1 * R1[row + 2];
     T[row] = t0;
     T[row + 1] = t1;
@@ -281,7 +295,7 @@ __global__ __launch_bounds__(256) void spmv_minres_dia_kernel(KrylovMatrix M, co
     contrib += s * Yp[row] * t0;
     contrib += s * Yp[row + 1] * t1;
     contrib += s * Yp[row + 2] * t2;
-  }
+  });
   const double b = block_sum<256>(contrib, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] += b;
 }
@@ -310,14 +324,14 @@ __global__ __launch_bounds__(256) void spmv_kernel(KrylovMatrix M, const double*
 
 __global__ __launch_bounds__(256) void spmv_dia_kernel(KrylovMatrix M, const double* __restrict__ x,
                                                       double* __restrict__ y) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M.nDiaFit; i += gridDim.x * blockDim.x) {
+  dia_walk(M, [&](int i, int t) {
     double a0, a1, a2;
     int row;
-    dia_rows(M, i, x, a0, a1, a2, row);
+    dia_rows(M, i, t, x, a0, a1, a2, row);
     y[row] = a0;
     y[row + 1] = a1;
     y[row + 2] = a2;
-  }
+  });
 }
 
 // the node-diagonal copy from the grouped values: upper blocks of every run node (absent
@@ -556,6 +570,7 @@ void krylov_precond_setup(hipStream_t s, const KrylovMatrix& M, const Precond& P
 }
 
 constexpr int kLPG = 16, kLPG1 = 8;
+static_assert(kKrylovBlocks % 8 == 0, "the node-diagonal walk splits the grid over the 8 XCDs");
 
 void krylov_refresh(hipStream_t s, const KrylovMatrix& M) {
   if (M.nDiaFit == 0) return;

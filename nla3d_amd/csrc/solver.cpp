@@ -193,6 +193,21 @@ void Ctx::setupNodeDia(const std::vector<int>& sgRow0, const std::vector<int>& s
   }
   std::vector<char> isFit(T, 0);
   for (int t : fit) isFit[t] = 1;
+  std::vector<int> rank(T, -1);
+  for (int i = 0; i < nf; i++) rank[fit[i]] = i;
+  // the walk's chunk size C (a multiple of 64, at least 50 chunks per XCD): the one that puts
+  // the most pairs (t, t - d_k) on one XCD (chunk c on XCD c % 8).  With d = a C + b, 0 <= b < C,
+  // t - d lies a or a + 1 chunks back, the latter for a fraction b / C of the nodes t.
+  int chunk = 64;
+  double bestScore = -1.0;
+  for (int C = 64; C <= std::max(64, T / 400); C += 64) {
+    double score = 0.0;
+    for (int k = 1; k < nd; k++) {
+      const int a = off[k] / C, b = off[k] % C;
+      score += (double)b / C * ((a + 1) % 8 == 0) + (1.0 - (double)b / C) * (a % 8 == 0);
+    }
+    if (score > bestScore + 1e-12) bestScore = score, chunk = C;
+  }
   std::vector<int> csrList;
   for (int g = nLead1; g < nG; g++)
     if (g < g0 || g >= g0 + T || !isFit[g - g0]) csrList.push_back(g);
@@ -205,6 +220,7 @@ void Ctx::setupNodeDia(const std::vector<int>& sgRow0, const std::vector<int>& s
   d_remPos.upload(remPos);
   d_diaPos.upload(diaPos);
   d_csrList.upload(csrList);
+  d_diaRank.upload(rank);
   KrylovMatrix& D = spmvDia;
   D.nDiaFit = nf;
   D.diaNodes = T;
@@ -221,6 +237,8 @@ void Ctx::setupNodeDia(const std::vector<int>& sgRow0, const std::vector<int>& s
   D.csrList = d_csrList.as<int>();
   D.diaPos = d_diaPos.as<int>();
   D.remPos = d_remPos.as<int>();
+  D.diaChunk = chunk;
+  D.diaRank = d_diaRank.as<int>();
 }
 
 void Ctx::refreshSpmvCopy(hipStream_t s) const {
@@ -484,6 +502,8 @@ KrylovMatrix Ctx::krylovMatrix() const {
     M.csrList = D.csrList;
     M.diaPos = D.diaPos;
     M.remPos = D.remPos;
+    M.diaChunk = D.diaChunk;
+    M.diaRank = D.diaRank;
   }
   return M;
 }
