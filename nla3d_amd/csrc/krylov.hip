@@ -335,15 +335,17 @@ __global__ __launch_bounds__(256) void spmv_dia_kernel(KrylovMatrix M, const dou
 }
 
 // the node-diagonal copy from the grouped values: upper blocks of every run node (absent
-// blocks are zero), then the ELL values of the node-diagonal nodes
+// blocks are zero), then the ELL values of the node-diagonal nodes.  The diagonal (ELL slot)
+// is the fastest index, so the threads of a wave read neighbouring entries of a few nodes'
+// row blocks, each node's rows once, and the writes of one diagonal row merge in L2.
 __global__ __launch_bounds__(256) void dia_fill_kernel(KrylovMatrix M) {
   const long long nD = (long long)M.nd * M.diaNodes, nR = (long long)M.remW * M.nDiaFit;
   const size_t ld = (size_t)M.diaNodes;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nD + nR;
        e += (long long)gridDim.x * blockDim.x) {
     if (e < nD) {
-      const int k = (int)(e / M.diaNodes), t = (int)(e % M.diaNodes);
-      const int p = M.diaPos[e];
+      const int t = (int)(e / M.nd), k = (int)(e % M.nd);
+      const int p = M.diaPos[(size_t)k * M.diaNodes + t];
       const int g = M.diaG0 + t;
       const int L = M.gColPtr[g + 1] - M.gColPtr[g];
       const double* v = M.val + M.gValPtr[g];
@@ -354,8 +356,8 @@ __global__ __launch_bounds__(256) void dia_fill_kernel(KrylovMatrix M) {
         for (int c = 0; c < 3; c++) D[(3 * r + c) * ld] = p >= 0 ? v[r * L + p + c] : 0.0;
     } else {
       const long long q = e - nD;
-      const int j = (int)(q / M.nDiaFit), i = (int)(q % M.nDiaFit);
-      const int p = M.remPos[q];
+      const int i = (int)(q / M.remW), j = (int)(q % M.remW);
+      const int p = M.remPos[(size_t)j * M.nDiaFit + i];
       const int g = M.diaG0 + M.fit[i];
       const int L = M.gColPtr[g + 1] - M.gColPtr[g];
       const double* v = M.val + M.gValPtr[g];
