@@ -25,8 +25,8 @@ model resident in HBM: update + buildK of every element, deterministic
 scatter into the three CSR blocks and F, rigid-body MPC rows, boundary
 conditions, rhs with the eliminated constrained DoFs, the block(2) solve
 (MINRES to a 1e-9 relative preconditioned residual; the first solve of each load
-step to 1e-5, which leaves the Newton iterates and the number of solves per load
-step as they are with 1e-9) and the increment update.  Newton runs to a 1e-8
+step to 1e-7, which keeps the displacements after every load step within 1e-8 of a
+Newton run with every solve to 1e-11) and the increment update.  Newton runs to a 1e-8
 relative residual; iterations that only confirm convergence (an assembly without
 a solve) are timed but not counted as steps.  value = elements / s = 864,000 /
 (Newton-iteration wall); the load steps completed inside the timed window are
@@ -489,9 +489,9 @@ def main():
     ap.add_argument("--load-steps", type=int, default=20)
     ap.add_argument("--krylov-tol", type=float, default=1e-9)
     # inexact Newton: the first solve of each load step only to this tolerance (the later solves
-    # to --krylov-tol); 1e-5 keeps the Newton iterates and the solves per load step of exact
-    # first solves at this workload, 1e-4 adds a solve to most load steps
-    ap.add_argument("--krylov-tol-loose", type=float, default=1e-5)
+    # to --krylov-tol).  At 120x120x60 the displacements after 6 load steps lie 3.8e-9 from a
+    # Newton run with every solve to 1e-11 at 1e-7, 2.2e-8 at 1e-6 and 4.1e-7 at 1e-5
+    ap.add_argument("--krylov-tol-loose", type=float, default=1e-7)
     ap.add_argument("--preconditioner", choices=["schur-multigrid", "learned", "block-jacobi"],
                     default="schur-multigrid")
     ap.add_argument("--train", action="store_true",

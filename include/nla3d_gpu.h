@@ -147,13 +147,14 @@ typedef struct {
                             (S is formed from the plain block-Jacobi blocks).  A solve of 1, 2 or 3 that
                             breaks down or does not reach its tolerance is repeated with plain block Jacobi
                             (stats.precond_fallbacks) */
-  /* inexact Newton (criterion 1 only; the single-model solvers): a solve's Krylov tolerance is
-     max(krylov_tol, min(krylov_tol_loose, eta)), eta = infinity for the first solve of a load step
-     and, for a later one at residual ||rhs_k||, eta = krylov_forcing * convergence * ||rhs_1|| /
-     ||rhs_k|| (this factor times the residual reduction the load step still needs).  So the first
-     solve runs to krylov_tol_loose, the solve that only has to finish the load step to at most that,
-     and the others to krylov_tol.  0 (the defaults) leaves every solve at krylov_tol.  The
-     converged state is set by the residual criterion either way. */
+  /* inexact Newton (criterion 1 only; the single-model solvers).  krylov_tol_loose > 0: the first
+     solve of a load step runs to max(krylov_tol, krylov_tol_loose) (its update leaves a residual
+     set by the nonlinearity whatever its accuracy).  krylov_forcing > 0: every later solve also
+     stops, at the end of a 12-iteration group, once its true linear residual ||rhs - K2 dU||_2 is
+     at most krylov_forcing * convergence * ||rhs_1||, i.e. that fraction of the residual the
+     criterion accepts (one more SpMV per group); the next Newton residual is then that plus
+     Newton's quadratic term, in the norm the criterion measures.  0 (the defaults) leaves every
+     solve at krylov_tol.  The converged state is set by the residual criterion either way. */
   double krylov_tol_loose;
   double krylov_forcing;
 } nla3d_gpu_newton_opts;

@@ -130,6 +130,7 @@ struct Ctx {
   DevBuf d_Ucprev, d_fixEq, d_fixVal, d_loadEq, d_loadVal;
   DevBuf d_rhs, d_dU;         // rhs [n2], increment [nAll]
   DevBuf d_mw[8], d_partial, d_mstate, d_scal;
+  DevBuf d_trueY;  // K2 x of minresSolve's true-residual check (allocated on first use)
   DevBuf d_mpcMirIdx;         // block(2) upper indices of MPC entries
   int nMpcMir = 0;
   // learned block-Jacobi correction (learned.cpp)
@@ -191,8 +192,10 @@ struct Ctx {
   KrylovMatrix krylovMatrix() const;
   Precond precond();
   // solve block(2) x = b with the current K; b, x device pointers [n2]; returns iterations
+  // trueAbs > 0 also stops, at the end of a 12-iteration group, once ||b - K2 x||_2 <= trueAbs
+  // (*trueStop then set); the preconditioned test stays in force
   int minresSolve(const double* b, double* x, double tol, int maxit, double* relres_est, bool* breakdown,
-                  const PrecondHook* hook = nullptr);
+                  const PrecondHook* hook = nullptr, double trueAbs = 0.0, bool* trueStop = nullptr);
   // multigrid on the Schur rows (schur.cpp)
   std::shared_ptr<SchurAmg> schur;
 };

@@ -519,7 +519,7 @@ Precond Ctx::precond() {
 }
 
 int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* relres_est, bool* breakdown,
-                     const PrecondHook* hook) {
+                     const PrecondHook* hook, double trueAbs, bool* trueStop) {
   KrylovMatrix M = krylovMatrix();
   Precond P = precond();
   MinresWork w;
@@ -553,7 +553,10 @@ int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* 
     graph.end();
     if (w.R1 != w0.R1 || w.Yp != w0.Yp || w.W1 != w0.W1) throw ApiError(-2, "MINRES rotation is not periodic");
   }
+  if (trueStop) *trueStop = false;
+  if (trueAbs > 0.0 && d_trueY.bytes < (size_t)n2 * 8) d_trueY.alloc((size_t)n2 * 8);
   int it = 0;
+  double r2 = 0.0;
   for (;;) {
     if (graph && it + period <= maxit) {
       graph.launch(stream);
@@ -562,9 +565,19 @@ int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* 
       for (int k = 0; k < period && it < maxit; k++, it++) minres_iteration(stream, M, P, w, tol, hook);
     }
     hipCheck(hipGetLastError(), "minres iteration");
+    if (trueAbs > 0.0) {  // ||b - K2 x||^2 of the current iterate (the partials are free between iterations)
+      krylov_spmv(stream, M, x, d_trueY.as<double>());
+      vec_diffsq(stream, n2, b, d_trueY.as<double>(), w.partial, d_scal.as<double>() + 10);
+      hipCheck(hipGetLastError(), "true residual");
+      hipCheck(hipMemcpyAsync(&r2, d_scal.as<double>() + 10, 8, hipMemcpyDeviceToHost, stream), "true residual");
+    }
     hipCheck(hipMemcpyAsync(h_mstate, w.st, sizeof(MinresState), hipMemcpyDeviceToHost, stream), "state");
     hipCheck(hipStreamSynchronize(stream), "minres sync");
     if (h_mstate->converged || it >= maxit) break;
+    if (trueAbs > 0.0 && std::sqrt(r2) <= trueAbs) {
+      if (trueStop) *trueStop = true;
+      break;
+    }
   }
   if (relres_est) *relres_est = h_mstate->beta1 > 0 ? h_mstate->phibar / h_mstate->beta1 : 0.0;
   if (breakdown) *breakdown = h_mstate->breakdown != 0;
@@ -659,12 +672,16 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
     }
   }
   st.newton_iterations++;
-  // the Krylov tolerance of this solve: krylov_tol, or looser where the load step cannot use
-  // more accuracy (inexact Newton, criterion 1)
+  // the Krylov tolerance of this solve (inexact Newton, criterion 1): the first solve of a load
+  // step to krylov_tol_loose (its update leaves a residual set by the nonlinearity), the later
+  // ones to krylov_tol, and with krylov_forcing also until the linear residual
+  // ||rhs - K2 dU||_2 is krylov_forcing times the residual the criterion accepts: the next
+  // residual is then that plus Newton's quadratic term, in the norm the criterion measures
   double ktol = o.krylov_tol;
-  if (o.criterion == 1 && o.krylov_tol_loose > 0.0) {
-    const double eta = it == 1 ? o.krylov_tol_loose : (rn > 0.0 ? o.krylov_forcing * o.convergence * rhs1 / rn : 0.0);
-    ktol = std::max(ktol, std::min(o.krylov_tol_loose, eta));
+  double trueAbs = 0.0;
+  if (o.criterion == 1) {
+    if (it == 1 && o.krylov_tol_loose > 0.0) ktol = std::max(ktol, o.krylov_tol_loose);
+    if (it > 1 && o.krylov_forcing > 0.0) trueAbs = o.krylov_forcing * o.convergence * rhs1;
   }
   // solve K.block(2) dU_sl = rhs (FESolver.cpp:457)
   t0 = nowMs();
@@ -680,16 +697,17 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
   }
   if (learnedPc) learnedCorrect(c, t);
   double relres = 0;
-  bool brk = false;
+  bool brk = false, trueStop = false;
   int kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), ktol, o.krylov_max_iter, &relres, &brk,
-                          useSchur ? &hook : nullptr);
-  if ((learnedPc || useSchur) && (brk || relres > ktol)) {
+                          useSchur ? &hook : nullptr, trueAbs, &trueStop);
+  if ((learnedPc || useSchur) && (brk || (relres > ktol && !trueStop))) {
     // the learned corrections failed this solve: block Jacobi from a fresh start
     st.precond_fallbacks++;
     st.krylov_iterations += kit;
     krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
     brk = false;
-    kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), ktol, o.krylov_max_iter, &relres, &brk);
+    kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), ktol, o.krylov_max_iter, &relres, &brk, nullptr,
+                        trueAbs, &trueStop);
   }
   st.krylov_iterations += kit;
   st.linear_solves++;
