@@ -245,6 +245,7 @@ typedef struct {
   int precond_fallbacks;  /* 1 or 2 when preconditioner 2 was unavailable or failed */
   int breakdown;          /* the preconditioner was not positive definite */
   double ms_factorize, ms_substitute;  /* host wall time of the last calls */
+  int spmv_dia_nodes;     /* nodes the SpMV reads through its node-diagonal copy (nla3d_gpu_spmv_info) */
 } nla3d_gpu_eqsolver_stats;
 
 void nla3d_gpu_eqsolver_defaults(nla3d_gpu_eqsolver_opts* opts);

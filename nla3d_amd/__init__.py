@@ -85,7 +85,8 @@ class EqSolverOpts(ctypes.Structure):
 class EqSolverStats(ctypes.Structure):
     """nla3d_gpu_eqsolver_stats (include/nla3d_gpu.h)"""
     _fields_ = [("iterations", ctypes.c_int), ("relres", ctypes.c_double), ("precond_fallbacks", ctypes.c_int),
-                ("breakdown", ctypes.c_int), ("ms_factorize", ctypes.c_double), ("ms_substitute", ctypes.c_double)]
+                ("breakdown", ctypes.c_int), ("ms_factorize", ctypes.c_double), ("ms_substitute", ctypes.c_double),
+                ("spmv_dia_nodes", ctypes.c_int)]
 
 
 _P = ctypes.c_void_p

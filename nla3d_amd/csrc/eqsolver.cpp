@@ -283,7 +283,9 @@ int nla3d_gpu_eqsolver_solve(nla3d_gpu_eqsolver_t s, uint32_t n, const double* v
 
 int nla3d_gpu_eqsolver_get_stats(nla3d_gpu_eqsolver_t s, nla3d_gpu_eqsolver_stats* stats) {
   if (!s || !stats) return NLA3D_GPU_EINVAL;
-  *stats = reinterpret_cast<EqSolver*>(s)->stats;
+  EqSolver& S = *reinterpret_cast<EqSolver*>(s);
+  *stats = S.stats;
+  stats->spmv_dia_nodes = S.analysed ? S.c.krylovMatrix().nDiaFit : 0;
   return NLA3D_GPU_OK;
 }
 

@@ -98,3 +98,50 @@ def test_pattern_change_and_error_codes(gpu_lib):
     with pytest.raises(nla3d_amd.Nla3dGpuError) as e:
         s.solve(np.array([4.0, 1.0, 3.0, 1.0]), io, np.array([1, 2, 2, 3], dtype=np.uint32), np.ones(3))
     assert e.value.code == -5
+
+
+@pytest.mark.parametrize("use_map,node_dia", [(True, True), (False, True), (False, False)])
+def test_structured_interior_through_the_drop_in(gpu_lib, use_map, node_dia):
+    """a 24 x 24 x 12 rigid-top system assembled on the host (the oracle's C element loop; the
+    first tangent, then a state with 1e-3 noise): the drop-in builds the SpMV's node-diagonal
+    copy from the caller's CSR (the 23 x 23 x 9 interior nodes, with or without a DoF map; none
+    with NLA3D_GPU_NODE_DIA=0) and solves to the direct solve's answer within 1e-8, for two
+    right-hand sides and after new values"""
+    import os
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    print(f"[gpu] equation solver on a 24x24x12 system (DoF map {use_map}, node diagonals {node_dia})", flush=True)
+    o = rigid_oracle(24, 24, 12)
+    old = os.environ.get("NLA3D_GPU_NODE_DIA")
+    os.environ["NLA3D_GPU_NODE_DIA"] = "1" if node_dia else "0"
+    try:
+        s = nla3d_amd.EqSolver(0, tol=1e-12, max_iter=50000, preconditioner=2)
+        if use_map:
+            s.set_dof_map(*dof_map(o))
+        si = o.blocks[2]
+        n2 = len(si.iofeir) - 1
+        rows = np.repeat(np.arange(n2), np.diff(si.iofeir.astype(np.int64)))
+        its = []
+        for k, scale in enumerate((0.0, 1e-3)):
+            U = scale * np.random.default_rng(40 + k).standard_normal(o.nAll)
+            Kv, _ = o.assemble(U)
+            A = sp.csr_matrix((Kv[2], (rows, si.columns.astype(np.int64) - 1)), shape=(n2, n2))
+            A = (A + sp.triu(A, 1).T).tocsc()
+            lu = spla.splu(A)
+            s.factorize(Kv[2], si.iofeir, si.columns)  # the pattern is analysed here, on the first call
+            for j in range(2):
+                rhs = np.random.default_rng(50 + 2 * k + j).standard_normal(n2)
+                x = s.substitute(rhs)
+                xr = lu.solve(rhs)
+                st = s.stats()
+                err = np.abs(x - xr).max() / np.abs(xr).max()
+                its.append(st.iterations)
+                print(f"values {k}, rhs {j}: MINRES {st.iterations}, node-diagonal nodes {st.spmv_dia_nodes}, "
+                      f"error {err:.2e}", flush=True)
+                assert st.spmv_dia_nodes == (23 * 23 * 9 if node_dia else 0) and st.precond_fallbacks == 0
+                assert err <= 1e-8, (k, j, err)
+    finally:
+        if old is None:
+            del os.environ["NLA3D_GPU_NODE_DIA"]
+        else:
+            os.environ["NLA3D_GPU_NODE_DIA"] = old
