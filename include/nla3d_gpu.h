@@ -95,7 +95,9 @@ int nla3d_gpu_get_equation_numbers(nla3d_gpu_t h, uint32_t* node_eq, uint32_t* e
 int nla3d_gpu_get_pattern(nla3d_gpu_t h, int block, uint32_t* iofeir, uint32_t* columns);
 
 /* ---- solution vector and assembly ---------------------------------------- */
-/* vecU (FEStorage.h:250), length nDofs + nMpc, equation order */
+/* vecU (FEStorage.h:250), length nDofs + nMpc, equation order.  set_solution makes U the state
+ * the next load step starts from: its constrained values are the ones that step's prescribed
+ * increments are measured from, and the first-solve predictor's history restarts. */
 int nla3d_gpu_set_solution(nla3d_gpu_t h, const double* U);
 int nla3d_gpu_get_solution(nla3d_gpu_t h, double* U);
 /* FEStorage::assembleGlobalEqMatrices (FEStorage.cpp:26-78) on the device for
@@ -157,6 +159,12 @@ typedef struct {
      solve at krylov_tol.  The converged state is set by the residual criterion either way. */
   double krylov_tol_loose;
   double krylov_forcing;
+  /* 1: the first solve of a load step starts from the previous load step's increment (scaled by
+     the ratio of the time steps; the multipliers from their current values) instead of zero and
+     solves for the correction, to the same absolute accuracy (its tolerance is raised by
+     ||rhs|| / ||rhs - K2 x0||).  The history is the handle's: it restarts after
+     nla3d_gpu_set_solution.  0 (the default): start from zero. */
+  int first_solve_predictor;
 } nla3d_gpu_newton_opts;
 
 typedef struct {

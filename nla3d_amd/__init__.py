@@ -34,7 +34,8 @@ class NewtonOpts(ctypes.Structure):
     _fields_ = [("n_load_steps", ctypes.c_int), ("max_iterations", ctypes.c_int), ("criterion", ctypes.c_int),
                 ("convergence", ctypes.c_double), ("krylov_tol", ctypes.c_double),
                 ("krylov_max_iter", ctypes.c_int), ("preconditioner", ctypes.c_int),
-                ("krylov_tol_loose", ctypes.c_double), ("krylov_forcing", ctypes.c_double)]
+                ("krylov_tol_loose", ctypes.c_double), ("krylov_forcing", ctypes.c_double),
+                ("first_solve_predictor", ctypes.c_int)]
 
 
 class TransientOpts(ctypes.Structure):

@@ -349,7 +349,13 @@ int nla3d_gpu_set_solution(nla3d_gpu_t h, const double* U) {
     if (!c.initialized) throw ApiError(NLA3D_GPU_ESTATE, "call nla3d_gpu_init_solution first");
     if (!U) throw ApiError(NLA3D_GPU_EINVAL, "NULL");
     hipCheck(hipMemcpyAsync(c.d_U.p, U, (size_t)c.model.nAll() * 8, hipMemcpyHostToDevice, c.stream), "copy U");
+    // the next load step's prescribed increments are measured from this state's constrained values
+    if (c.model.nConstrainedDofs > 0)
+      hipCheck(hipMemcpyAsync(c.d_Ucprev.p, U, (size_t)c.model.nConstrainedDofs * 8, hipMemcpyHostToDevice, c.stream),
+               "copy Uc");
     hipCheck(hipStreamSynchronize(c.stream), "sync");
+    c.lsValid = false;  // the first-solve predictor's history no longer describes this state
+    c.lsStartT = c.lsTargetT = 0.0;
   });
 }
 

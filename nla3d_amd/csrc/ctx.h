@@ -131,6 +131,12 @@ struct Ctx {
   DevBuf d_rhs, d_dU;         // rhs [n2], increment [nAll]
   DevBuf d_mw[8], d_partial, d_mstate, d_scal;
   DevBuf d_trueY;  // K2 x of minresSolve's true-residual check (allocated on first use)
+  // the first-solve predictor (nla3d_gpu_newton_opts.first_solve_predictor): U[nC, nC + n2) at
+  // the start of the last load step and the times of that state and of its target; the
+  // starting point and the right-hand side of the correction
+  DevBuf d_lsStart, d_x0, d_rhsC;
+  bool lsValid = false;
+  double lsStartT = 0.0, lsTargetT = 0.0;
   DevBuf d_mpcMirIdx;         // block(2) upper indices of MPC entries
   int nMpcMir = 0;
   // learned block-Jacobi correction (learned.cpp)

@@ -20,5 +20,13 @@ void vec_abssum(hipStream_t s, int n, const double* x, double* partial, double* 
 void newton_lambda_vec(hipStream_t s, int nC, int nU, int nMpc, const double* U, double* v);
 void vec_diffsq(hipStream_t s, int n, const double* a, const double* b, double* partial, double* out);
 void vec_sumsq(hipStream_t s, int n, const double* x, double* partial, double* out);
+// the first solve's starting point at a new load step: x0 = [scale (U_s - Ustart) ; U_l] over the
+// block(2) unknowns (free DoFs, then the multipliers, which the solve replaces), then
+// Ustart = U[nC, nC + n2); with predict = false only the copy
+void newton_predictor(hipStream_t s, int nC, int nU, int nMpc, const double* U, double* Ustart, double scale,
+                      bool predict, double* x0);
+// y += x over n entries; y = b - y
+void vec_add(hipStream_t s, int n, const double* x, double* y);
+void vec_rsub(hipStream_t s, int n, const double* b, double* y);
 
 }  // namespace nla3d_amd

@@ -165,6 +165,40 @@ void newton_lambda_vec(hipStream_t s, int nC, int nU, int nMpc, const double* U,
   hipLaunchKernelGGL(lambda_vec_kernel, dim3(cdiv(std::max(nU + nMpc, 1), 256)), dim3(256), 0, s, nC, nU, nMpc, U, v);
 }
 
+__global__ __launch_bounds__(256) void predictor_kernel(int nC, int nU, int nMpc, const double* __restrict__ U,
+                                                       double* __restrict__ Ustart, double scale, int predict,
+                                                       double* __restrict__ x0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nU + nMpc) return;
+  const double u = U[nC + i];
+  if (predict) x0[i] = i < nU ? scale * (u - Ustart[i]) : u;
+  Ustart[i] = u;
+}
+
+__global__ __launch_bounds__(256) void add_kernel(int n, const double* __restrict__ x, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += x[i];
+}
+
+void newton_predictor(hipStream_t s, int nC, int nU, int nMpc, const double* U, double* Ustart, double scale,
+                      bool predict, double* x0) {
+  hipLaunchKernelGGL(predictor_kernel, dim3(cdiv(std::max(nU + nMpc, 1), 256)), dim3(256), 0, s, nC, nU, nMpc, U,
+                     Ustart, scale, predict ? 1 : 0, x0);
+}
+
+__global__ __launch_bounds__(256) void rsub_kernel(int n, const double* __restrict__ b, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = b[i] - y[i];
+}
+
+void vec_rsub(hipStream_t s, int n, const double* b, double* y) {
+  hipLaunchKernelGGL(rsub_kernel, dim3(cdiv(std::max(n, 1), 256)), dim3(256), 0, s, n, b, y);
+}
+
+void vec_add(hipStream_t s, int n, const double* x, double* y) {
+  hipLaunchKernelGGL(add_kernel, dim3(cdiv(std::max(n, 1), 256)), dim3(256), 0, s, n, x, y);
+}
+
 void vec_diffsq(hipStream_t s, int n, const double* a, const double* b, double* partial, double* out) {
   const int nb = cdiv(std::max(n, 1), 256);
   hipLaunchKernelGGL(diffsq_partial_kernel, dim3(nb), dim3(256), 0, s, n, a, b, partial);

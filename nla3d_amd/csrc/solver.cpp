@@ -683,6 +683,41 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
     if (it == 1 && o.krylov_tol_loose > 0.0) ktol = std::max(ktol, o.krylov_tol_loose);
     if (it > 1 && o.krylov_forcing > 0.0) trueAbs = o.krylov_forcing * o.convergence * rhs1;
   }
+  // the first solve of a load step from the previous load step's increment: solve for the
+  // correction of x0, with the tolerance raised so the absolute accuracy stays the same (a
+  // predictor that does not lower the residual is not used)
+  const double* b = c.d_rhs.as<double>();
+  bool predicted = false;
+  if (it == 1 && o.first_solve_predictor) {
+    const int n2 = c.n2;
+    if (c.d_lsStart.bytes < (size_t)n2 * 8) {
+      c.d_lsStart.alloc((size_t)n2 * 8);
+      c.d_x0.alloc((size_t)n2 * 8);
+      c.d_rhsC.alloc((size_t)n2 * 8);
+      c.lsValid = false;
+    }
+    const double dtPrev = c.lsTargetT - c.lsStartT;
+    const bool predict = c.lsValid && dtPrev > 0.0 && t > c.lsTargetT;
+    const double scale = predict ? (t - c.lsTargetT) / dtPrev : 0.0;
+    newton_predictor(c.stream, nC, nU, nM, U, c.d_lsStart.as<double>(), scale, predict, c.d_x0.as<double>());
+    c.lsStartT = c.lsTargetT;
+    c.lsTargetT = t;
+    c.lsValid = true;
+    if (predict) {
+      double* bc = c.d_rhsC.as<double>();
+      krylov_spmv(c.stream, c.krylovMatrix(), c.d_x0.as<double>(), bc);
+      vec_rsub(c.stream, n2, b, bc);  // bc = rhs - K2 x0
+      vec_sumsq(c.stream, n2, bc, c.d_partial.as<double>(), c.d_scal.as<double>() + 11);
+      vec_sumsq(c.stream, n2, b, c.d_partial.as<double>(), c.d_scal.as<double>() + 12);
+      hipCheck(hipGetLastError(), "predictor");
+      const double r0 = std::sqrt(readScalar(c, 11)), bn = std::sqrt(readScalar(c, 12));
+      if (r0 > 0.0 && r0 < bn) {
+        ktol = std::min(0.1, ktol * bn / r0);
+        b = bc;
+        predicted = true;
+      }
+    }
+  }
   // solve K.block(2) dU_sl = rhs (FESolver.cpp:457)
   t0 = nowMs();
   const bool learnedPc = o.preconditioner == 1 || o.preconditioner == 3;
@@ -698,7 +733,7 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
   if (learnedPc) learnedCorrect(c, t);
   double relres = 0;
   bool brk = false, trueStop = false;
-  int kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), ktol, o.krylov_max_iter, &relres, &brk,
+  int kit = c.minresSolve(b, c.d_dU.as<double>(), ktol, o.krylov_max_iter, &relres, &brk,
                           useSchur ? &hook : nullptr, trueAbs, &trueStop);
   if ((learnedPc || useSchur) && (brk || (relres > ktol && !trueStop))) {
     // the learned corrections failed this solve: block Jacobi from a fresh start
@@ -706,9 +741,10 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
     st.krylov_iterations += kit;
     krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
     brk = false;
-    kit = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), ktol, o.krylov_max_iter, &relres, &brk, nullptr,
+    kit = c.minresSolve(b, c.d_dU.as<double>(), ktol, o.krylov_max_iter, &relres, &brk, nullptr,
                         trueAbs, &trueStop);
   }
+  if (predicted) vec_add(c.stream, c.n2, c.d_x0.as<double>(), c.d_dU.as<double>());  // dU = x0 + correction
   st.krylov_iterations += kit;
   st.linear_solves++;
   st.last_krylov_iterations = kit;
@@ -793,6 +829,7 @@ int nla3d_gpu_newton_defaults(nla3d_gpu_newton_opts* o) {
   o->preconditioner = 0;
   o->krylov_tol_loose = 0.0;
   o->krylov_forcing = 0.0;
+  o->first_solve_predictor = 0;
   return 0;
 }
 

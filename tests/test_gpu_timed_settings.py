@@ -1,6 +1,7 @@
 """The metric workload's timed settings: Newton to a 1e-8 relative residual (criterion 1),
 MINRES to 1e-9 with block Jacobi + the Schur-row multigrid (preconditioner 2), the first solve
-of each load step to 1e-7 (krylov_tol_loose, the inexact-Newton setting), on the 2 x 2 x 1
+of each load step started from the previous load step's increment (first_solve_predictor) and
+run to 5e-8 of the right-hand side (krylov_tol_loose), on the 2 x 2 x 1
 box with a rigid-body top (+15 % UZ, 20 degree ROTZ), Mooney-Rivlin C10 = 0.5, C01 = 0.1,
 K = 1000, with the same jitter and seed.  After every load step (FESolver.cpp:457-483) the
 displacements must agree within 1e-8 relative (max-abs over the solution vector), with no
@@ -11,8 +12,9 @@ fallback to plain block Jacobi:
 * meshed 120 x 120 x 60, the timed shape itself, over the first 6 load steps, with the same
   library's Newton whose every MINRES solve runs to 1e-11 (the oracle cannot run this size;
   a 1e-10 run lies within 3.2e-10 of it, so it stands for the direct-solve Newton).  The
-  accuracy of the first solve matters only at this size: with it at 1e-5 the displacements
-  after 6 load steps lay 4.1e-7 from the tight run here, within 5e-9 at 60 x 60 x 30."""
+  accuracy of the first solve matters only at this size: with it at 1e-5 (no predictor) the
+  displacements after 6 load steps lay 4.1e-7 from the tight run here, within 5e-9 at
+  60 x 60 x 30."""
 import numpy as np
 import pytest
 
@@ -23,7 +25,7 @@ from oracle.pyoracle import OracleModel, load_steps
 pytestmark = pytest.mark.gpu
 
 UZ, ROTZ = 0.15, float(np.deg2rad(20.0))
-TIMED_FIRST_SOLVE_TOL = 1e-7  # bench.py --krylov-tol-loose
+TIMED_FIRST_SOLVE_TOL = 5e-8  # bench.py --krylov-tol-loose, with the first-solve predictor
 MAT = (3, [0.5, 0.1, 1000.0])
 
 
@@ -44,13 +46,14 @@ def c3_like(nx, ny, nz):
 
 @pytest.mark.timeout(600)
 def test_timed_settings_match_oracle_every_load_step(gpu_lib):
-    print("[gpu] timed settings (criterion 1e-8, MINRES 1e-9 / first solve 1e-7, preconditioner 2), 12x12x6, "
+    print("[gpu] timed settings (criterion 1e-8, MINRES 1e-9 / predicted first solve 5e-8, preconditioner 2), 12x12x6, "
           "20 load steps",
           flush=True)
     m, o = c3_like(12, 12, 6)
     o.newton_init()
     opts = m.newton_defaults(criterion=1, convergence=1e-8, krylov_tol=1e-9, krylov_max_iter=100000,
                              preconditioner=2, krylov_tol_loose=TIMED_FIRST_SOLVE_TOL)
+    opts.first_solve_predictor = 1
     st = nla3d_amd.NewtonStats()
     worst = 0.0
     for t in load_steps(20):
@@ -66,13 +69,14 @@ def test_timed_settings_match_oracle_every_load_step(gpu_lib):
     print("worst relative error over the schedule", worst, flush=True)
 
 
-def timed_shape(krylov_tol, first_tol):
+def timed_shape(krylov_tol, first_tol, predictor):
     m = nla3d_amd.GpuModel(0)
     meshgen.solid81_rigid_top_case(m, 120, 120, 60, 2.0, 2.0, 1.0, UZ, ROTZ, material=MAT)
     m.set_max_in_row(0)
     m.init_solution()
     opts = m.newton_defaults(criterion=1, convergence=1e-8, krylov_tol=krylov_tol, krylov_max_iter=200000,
                              preconditioner=2, krylov_tol_loose=first_tol, n_load_steps=20, max_iterations=15)
+    opts.first_solve_predictor = predictor
     return m, opts
 
 
@@ -80,7 +84,7 @@ def timed_shape(krylov_tol, first_tol):
 def test_timed_settings_match_tight_solves_at_the_timed_shape(gpu_lib):
     print("[gpu] timed settings vs MINRES 1e-11 at 120x120x60, 6 load steps", flush=True)
     steps = load_steps(20)[:6]
-    ref, opts = timed_shape(1e-11, 0.0)
+    ref, opts = timed_shape(1e-11, 0.0, 0)
     st = nla3d_amd.NewtonStats()
     Uref = []
     for t in steps:
@@ -90,7 +94,7 @@ def test_timed_settings_match_tight_solves_at_the_timed_shape(gpu_lib):
     assert st.precond_fallbacks == 0
     ref.close()
     del ref
-    m, opts = timed_shape(1e-9, TIMED_FIRST_SOLVE_TOL)
+    m, opts = timed_shape(1e-9, TIMED_FIRST_SOLVE_TOL, 1)
     st = nla3d_amd.NewtonStats()
     for t, U0 in zip(steps, Uref):
         m.solve_load_step(t, opts, st)
@@ -125,3 +129,41 @@ def test_true_residual_forcing_ends_the_later_solves_early(gpu_lib):
     (k0, s0, U0), (k1, s1, U1) = runs[0.0], runs[0.1]
     assert k1 < k0 and s1 == s0
     assert np.abs(U1 - U0).max() <= 1e-6 * np.abs(U0).max()
+
+
+def test_first_solve_predictor(gpu_lib):
+    """first_solve_predictor: from the second load step on, the first solve starts from the last
+    load step's increment and needs fewer MINRES iterations for the same absolute accuracy; the
+    solves per load step and the displacements (1e-8) are those of the unpredicted run; after
+    nla3d_gpu_set_solution the history restarts (the next first solve is unpredicted)"""
+    print("[gpu] first-solve predictor, 16x16x8, 4 load steps", flush=True)
+    steps = load_steps(20)[:4]
+    runs = {}
+    for pred in (0, 1):
+        m = nla3d_amd.GpuModel(0)
+        meshgen.solid81_rigid_top_case(m, 16, 16, 8, 2.0, 2.0, 1.0, UZ, ROTZ, material=MAT)
+        m.set_max_in_row(0)
+        m.init_solution()
+        opts = m.newton_defaults(criterion=1, convergence=1e-8, krylov_tol=1e-10, krylov_max_iter=100000,
+                                 preconditioner=2, krylov_tol_loose=1e-8, n_load_steps=20, max_iterations=15)
+        opts.first_solve_predictor = pred
+        firsts, solves, Us = [], [], []
+        for t in steps:
+            st = nla3d_amd.NewtonStats()
+            m.solve_load_step(t, opts, st)
+            solves.append(st.linear_solves)
+            Us.append(m.get_solution().copy())
+            firsts.append(st.krylov_iterations)
+        runs[pred] = (firsts, solves, Us, m, opts)
+        print(f"predictor {pred}: MINRES per load step {firsts}, solves {solves}", flush=True)
+    f0, s0, U0, _, _ = runs[0]
+    f1, s1, U1, m, opts = runs[1]
+    assert s0 == s1 and f1[0] == f0[0] and all(a < b for a, b in zip(f1[1:], f0[1:]))
+    for a, b in zip(U1, U0):
+        assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
+    # restart: back to the state after the first load step, then the second again, unpredicted
+    m.set_solution(U1[0])
+    st = nla3d_amd.NewtonStats()
+    m.solve_load_step(steps[1], opts, st)
+    assert st.krylov_iterations > f1[1] and abs(st.krylov_iterations - f0[1]) <= 0.02 * f0[1], \
+        (st.krylov_iterations, f0[1], f1[1])
