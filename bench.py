@@ -26,8 +26,10 @@ scatter into the three CSR blocks and F, rigid-body MPC rows, boundary
 conditions, rhs with the eliminated constrained DoFs, the block(2) solve
 (MINRES to a 1e-9 relative preconditioned residual; the first solve of each load
 step started from the previous load step's increment and run to 5e-8 of the
-right-hand side, which keeps the displacements after every load step within 1e-8
-of a Newton run with every solve to 1e-11) and the increment update.  Newton runs to a 1e-8
+right-hand side, the second started from the previous load step's second increment
+(scaled by least squares) and run to the same absolute accuracy as from zero, which
+keeps the displacements after every load step within 1e-8 of a Newton run with every
+solve to 1e-11) and the increment update.  Newton runs to a 1e-8
 relative residual; iterations that only confirm convergence (an assembly without
 a solve) are timed but not counted as steps.  value = elements / s = 864,000 /
 (Newton-iteration wall); the load steps completed inside the timed window are
@@ -495,7 +497,10 @@ def main():
     # (without the predictor: 3.8e-9 at 1e-7, 2.2e-8 at 1e-6, 4.1e-7 at 1e-5)
     ap.add_argument("--krylov-tol-loose", type=float, default=5e-8)
     ap.add_argument("--no-predictor", action="store_true",
-                    help="start the first solve of each load step from zero, not from the last increment")
+                    help="start every solve from zero, not from the last load step's increments")
+    ap.add_argument("--predictor", type=int, default=2, choices=[1, 2],
+                    help="1: the first solve of each load step starts from the last load step's increment; "
+                         "2: the second solve also from the last load step's second increment")
     ap.add_argument("--preconditioner", choices=["schur-multigrid", "learned", "block-jacobi"],
                     default="schur-multigrid")
     ap.add_argument("--train", action="store_true",
@@ -557,7 +562,7 @@ def main():
     extra = {}
     opts = m.newton_defaults(criterion=1, convergence=1e-8, krylov_tol=args.krylov_tol, krylov_max_iter=100000)
     opts.krylov_tol_loose = args.krylov_tol_loose
-    opts.first_solve_predictor = 0 if args.no_predictor else 1
+    opts.first_solve_predictor = 0 if args.no_predictor else args.predictor
     if args.mode == "assemble":
         for _ in range(args.warmup):
             m.assemble()
@@ -681,6 +686,7 @@ def main():
         if per_step:
             extra["ms_per_load_step"] = 1e3 * float(np.mean([p["s"] for p in per_step]))
         extra.update({"krylov_tol": args.krylov_tol, "krylov_tol_first_solve": args.krylov_tol_loose,
+                      "predictor": 0 if args.no_predictor else args.predictor,
                       "krylov_iterations_per_newton": float(np.mean(kits)),
                  "ms_krylov_per_iteration": (st.ms_solve - ms_sol0) / max(1, sum(kits)),
                  "ms_assemble_per_newton": (st.ms_assemble - ms_asm0) / args.steps,
@@ -711,10 +717,12 @@ def main():
     if args.mode == "newton" and smoother is not None and \
             smoother["ms"] * smoother["launches_per_krylov_iteration"] > ms_spmv:
         # with the multigrid, the finest-level smoother steps take the largest share
+        kname = "cheb_step_kernel<true>" if os.environ.get("NLA3D_GPU_SCHUR_FP32_SMOOTHER") == "1" \
+            else "half_step_kernel<true>"
         roofline = {"bound": "hbm", "achieved": smoother["GBps"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": smoother["GBps"] / PEAK_HBM_GBS,
-                    "traffic": profiled_traffic("cheb_step_kernel<true>", (nx, ny, nz)),
-                    "kernel": "cheb_step_kernel<true> (finest-level Chebyshev smoother step of the Schur multigrid)",
+                    "traffic": profiled_traffic(kname, (nx, ny, nz)),
+                    "kernel": kname + " (finest-level Chebyshev smoother step of the Schur multigrid)",
                     "kernel_ms": smoother["ms"], "algorithmic_bytes": smoother["bytes"]}
     elif args.mode == "newton":
         roofline = {"bound": "hbm", "achieved": spmv_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",

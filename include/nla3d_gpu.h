@@ -162,7 +162,10 @@ typedef struct {
   /* 1: the first solve of a load step starts from the previous load step's increment (scaled by
      the ratio of the time steps; the multipliers from their current values) instead of zero and
      solves for the correction, to the same absolute accuracy (its tolerance is raised by
-     ||rhs|| / ||rhs - K2 x0||).  The history is the handle's: it restarts after
+     ||rhs|| / ||rhs - K2 x0||).  2: also the second solve of a load step, from the previous
+     load step's second increment p: x0 = alpha [p_u ; 0] + [0 ; U_l] (node and pressure rows
+     scaled by the alpha that minimises ||rhs - K2 x0||_2, the multipliers from their current
+     values), to the same absolute accuracy.  The history is the handle's: it restarts after
      nla3d_gpu_set_solution.  0 (the default): start from zero. */
   int first_solve_predictor;
 } nla3d_gpu_newton_opts;
@@ -359,6 +362,10 @@ int nla3d_gpu_schur_matrix(nla3d_gpu_t h, int* qrow, int* rowPtr, int* col, doub
 int nla3d_gpu_schur_cycle_info(nla3d_gpu_t h, int* finest_steps, int* fuse_from);
 int nla3d_gpu_schur_level(nla3d_gpu_t h, int level, int* agg, double* lmax);
 int nla3d_gpu_schur_apply(nla3d_gpu_t h, const double* r, double* z, int refresh);
+/* the finest level's stored smoother operator for the current tangent: s[nq] = D^-1/2 and, in
+ * S's CSR pattern (val[nnz0]), the scaled entries as the smoother reads them (FP16 off the
+ * diagonal, 1 + Delta on it); -3 when the finest level smooths with its FP32 copy */
+int nla3d_gpu_schur_smoother_operator(nla3d_gpu_t h, double* s, double* val);
 
 /* ---- learned block-Jacobi correction --------------------------------------
  * No counterpart in the reference, whose only solver is PARDISO

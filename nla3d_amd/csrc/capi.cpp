@@ -37,22 +37,6 @@ void DevBuf::alloc(size_t nbytes) {
   bytes = nbytes;
 }
 
-static void parallelFor(size_t n, const std::function<void(size_t, size_t)>& fn) {
-  unsigned nt = hostThreads();
-  if (n < 4096 || nt == 1) {
-    fn(0, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  size_t chunk = (n + nt - 1) / nt;
-  for (unsigned t = 0; t < nt; t++) {
-    size_t b = t * chunk, e = std::min(n, b + chunk);
-    if (b >= e) break;
-    th.emplace_back([&fn, b, e] { fn(b, e); });
-  }
-  for (auto& x : th) x.join();
-}
-
 void setupThermal(Ctx& c);  // thermal.cpp
 
 // Build the u/p element (element, slot) -> target map and sort it by target (stable in
@@ -354,7 +338,8 @@ int nla3d_gpu_set_solution(nla3d_gpu_t h, const double* U) {
       hipCheck(hipMemcpyAsync(c.d_Ucprev.p, U, (size_t)c.model.nConstrainedDofs * 8, hipMemcpyHostToDevice, c.stream),
                "copy Uc");
     hipCheck(hipStreamSynchronize(c.stream), "sync");
-    c.lsValid = false;  // the first-solve predictor's history no longer describes this state
+    c.lsValid = false;  // the predictors' history no longer describes this state
+    c.it2Valid = false;
     c.lsStartT = c.lsTargetT = 0.0;
   });
 }

@@ -136,6 +136,9 @@ struct Ctx {
   // starting point and the right-hand side of the correction
   DevBuf d_lsStart, d_x0, d_rhsC;
   bool lsValid = false;
+  // the later-solve predictor: the last load step's second increment (node and pressure rows)
+  DevBuf d_it2, d_q, d_kq;
+  bool it2Valid = false;
   double lsStartT = 0.0, lsTargetT = 0.0;
   DevBuf d_mpcMirIdx;         // block(2) upper indices of MPC entries
   int nMpcMir = 0;

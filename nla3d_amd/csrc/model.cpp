@@ -16,7 +16,7 @@ unsigned hostThreads() {
   return std::min(n, 16u);
 }
 
-static void parallelFor(size_t n, const std::function<void(size_t, size_t)>& fn) {
+void parallelFor(size_t n, const std::function<void(size_t, size_t)>& fn) {
   unsigned nt = hostThreads();
   if (n < 4096 || nt == 1) {
     fn(0, n);

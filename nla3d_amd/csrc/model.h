@@ -11,6 +11,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -117,5 +118,7 @@ class FEStorage {
 
 // number of host threads used by the setup loops
 unsigned hostThreads();
+// fn(begin, end) over [0, n) split into contiguous chunks, one per host thread (inline below 4096)
+void parallelFor(size_t n, const std::function<void(size_t, size_t)>& fn);
 
 }  // namespace nla3d_amd

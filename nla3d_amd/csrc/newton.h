@@ -27,6 +27,12 @@ void newton_predictor(hipStream_t s, int nC, int nU, int nMpc, const double* U, 
                       bool predict, double* x0);
 // y += x over n entries; y = b - y
 void vec_add(hipStream_t s, int n, const double* x, double* y);
+// the later-solve predictor's pieces: P = [p_u ; 0], Q = [0 ; U_l]; then x0 = alpha P + Q (over
+// P) and bc = r0 - alpha KP (over KP)
+void newton_pred_split(hipStream_t s, int nC, int nU, int nMpc, const double* p, const double* U, double* P,
+                       double* Q);
+void newton_pred_combine(hipStream_t s, int n, double alpha, double* P, const double* Q, const double* r0,
+                         double* KP);
 void vec_rsub(hipStream_t s, int n, const double* b, double* y);
 
 }  // namespace nla3d_amd

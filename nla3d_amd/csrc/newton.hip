@@ -53,7 +53,9 @@ __global__ __launch_bounds__(256) void rhs_kernel(int n2, int nC, const double* 
 }
 
 // vecRc = K11 dUc + K12 dUsl - Fc (FESolver.cpp:464-467), dUc = Uc - Ucprev; K11 through
-// full-symmetric row lists (positions into its upper values), K12 by rows
+// full-symmetric row lists (positions into its upper values), K12 by rows.  One 64-lane wave per
+// constrained row: lane l sums the row's entries l, l + 64, ... and a butterfly adds the lanes (a
+// rigid-body master's row holds every slave row of its MPC, tens of thousands of entries)
 __global__ __launch_bounds__(256) void reaction_kernel(int nC, const int* __restrict__ k11Ptr,
                                                       const int* __restrict__ k11Col, const int* __restrict__ k11Pos,
                                                       const double* __restrict__ K11, const int* __restrict__ k12Ptr,
@@ -61,15 +63,16 @@ __global__ __launch_bounds__(256) void reaction_kernel(int nC, const int* __rest
                                                       const double* __restrict__ U, const double* __restrict__ Ucprev,
                                                       const double* __restrict__ dUsl, const double* __restrict__ F,
                                                       double* __restrict__ R) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nC) return;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (i >= nC) return;  // the 64 lanes of a row leave together
   double r = 0.0;
-  for (int k = k11Ptr[i]; k < k11Ptr[i + 1]; k++) {
+  for (int k = k11Ptr[i] + lane; k < k11Ptr[i + 1]; k += 64) {
     const int j = k11Col[k];
     r += K11[k11Pos[k]] * (U[j] - Ucprev[j]);
   }
-  for (int k = k12Ptr[i]; k < k12Ptr[i + 1]; k++) r += K12[k] * dUsl[k12Col[k]];
-  R[i] = r - F[i];
+  for (int k = k12Ptr[i] + lane; k < k12Ptr[i + 1]; k += 64) r += K12[k] * dUsl[k12Col[k]];
+  r = wsum(r);
+  if (lane == 0) R[i] = r - F[i];
 }
 
 // vecUs += dUs ; vecUl = dUl ; Ucprev = Uc (FESolver.cpp:460-469)
@@ -146,7 +149,7 @@ void newton_reactions(hipStream_t s, int nC, const int* k11Ptr, const int* k11Co
                       const int* k12Ptr, const int* k12Col, const double* K12, const double* U, const double* Ucprev,
                       const double* dUsl, const double* F, double* R) {
   if (nC <= 0) return;
-  hipLaunchKernelGGL(reaction_kernel, dim3(cdiv(nC, 256)), dim3(256), 0, s, nC, k11Ptr, k11Col, k11Pos, K11, k12Ptr,
+  hipLaunchKernelGGL(reaction_kernel, dim3(cdiv(nC, 4)), dim3(256), 0, s, nC, k11Ptr, k11Col, k11Pos, K11, k12Ptr,
                      k12Col, K12, U, Ucprev, dUsl, F, R);
 }
 
@@ -173,6 +176,26 @@ __global__ __launch_bounds__(256) void predictor_kernel(int nC, int nU, int nMpc
   const double u = U[nC + i];
   if (predict) x0[i] = i < nU ? scale * (u - Ustart[i]) : u;
   Ustart[i] = u;
+}
+
+// later-solve predictor: P = [p_u ; 0], Q = [0 ; U_l] (the multipliers' current values)
+__global__ __launch_bounds__(256) void pred_split_kernel(int nC, int nU, int nMpc, const double* __restrict__ p,
+                                                        const double* __restrict__ U, double* __restrict__ P,
+                                                        double* __restrict__ Q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nU + nMpc) return;
+  P[i] = i < nU ? p[i] : 0.0;
+  Q[i] = i < nU ? 0.0 : U[nC + i];
+}
+
+// x0 = alpha P + Q (over P) ; bc = r0 - alpha KP (over KP)
+__global__ __launch_bounds__(256) void pred_combine_kernel(int n, double alpha, double* __restrict__ P,
+                                                          const double* __restrict__ Q, const double* __restrict__ r0,
+                                                          double* __restrict__ KP) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  P[i] = alpha * P[i] + Q[i];
+  KP[i] = r0[i] - alpha * KP[i];
 }
 
 __global__ __launch_bounds__(256) void add_kernel(int n, const double* __restrict__ x, double* __restrict__ y) {
@@ -209,6 +232,21 @@ void vec_sumsq(hipStream_t s, int n, const double* x, double* partial, double* o
   const int nb = cdiv(std::max(n, 1), 256);
   hipLaunchKernelGGL(sumsq_partial_kernel, dim3(nb), dim3(256), 0, s, n, x, partial);
   hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, s, partial, nb, out);
+}
+
+}  // namespace nla3d_amd
+
+namespace nla3d_amd {
+
+void newton_pred_split(hipStream_t s, int nC, int nU, int nMpc, const double* p, const double* U, double* P,
+                       double* Q) {
+  hipLaunchKernelGGL(pred_split_kernel, dim3(cdiv(std::max(nU + nMpc, 1), 256)), dim3(256), 0, s, nC, nU, nMpc, p, U,
+                     P, Q);
+}
+
+void newton_pred_combine(hipStream_t s, int n, double alpha, double* P, const double* Q, const double* r0,
+                         double* KP) {
+  hipLaunchKernelGGL(pred_combine_kernel, dim3(cdiv(std::max(n, 1), 256)), dim3(256), 0, s, n, alpha, P, Q, r0, KP);
 }
 
 }  // namespace nla3d_amd

@@ -5,6 +5,7 @@
 #include <unordered_map>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -54,6 +55,10 @@ struct AmgLevel {
   DevBuf sCOff, sDOff, sDPos, sDia;              // and the diagonals of its pattern slices
   int nd = 0, ldd = 0;
   bool narrow = false, patterned = false;
+  // the finest level's half-precision smoother operator (AmgHalf): FP16 slots in the layout
+  // above, the compensated diagonal and s = D^-1/2
+  bool half = false;
+  DevBuf hVal, hDia, hDiag, hS;
   DevBuf r, x, res, da, db, t, start;
   double lmax = 1.0;
   AmgCsr csr() const { return AmgCsr{n, rowPtr.as<int>(), col.as<int>(), val.as<double>()}; }
@@ -62,6 +67,12 @@ struct AmgLevel {
                    narrow ? sCol16.as<unsigned short>() : nullptr, narrow ? sBase.as<int>() : nullptr,
                    sVal.as<float>(), patterned ? sCOff.as<int>() : nullptr, sDOff.as<int>(), sDia.as<float>(), nd,
                    ldd};
+  }
+  AmgHalf halfView() const {
+    return AmgHalf{n, sOff.as<int>(), narrow ? nullptr : sCol.as<int>(),
+                   narrow ? sCol16.as<unsigned short>() : nullptr, narrow ? sBase.as<int>() : nullptr,
+                   hVal.as<unsigned short>(), patterned ? sCOff.as<int>() : nullptr, sDOff.as<int>(),
+                   patterned ? hDia.as<unsigned short>() : nullptr, hDiag.as<float>(), hS.as<double>(), nd, ldd};
   }
 };
 
@@ -273,6 +284,12 @@ static void allocLevel(AmgLevel& A, bool coarsest) {
     A.sCol.upload(scol);
   }
   A.sPos.upload(spos);
+  if (A.half) {
+    A.hVal.alloc((size_t)std::max(A.nSlots, 1) * 2);
+    if (A.patterned) A.hDia.alloc((size_t)A.nd * A.ldd * 2);
+    A.hDiag.alloc((size_t)std::max(A.n, 1) * 4);
+    A.hS.alloc((size_t)std::max(A.n, 1) * 8);
+  }
   A.sVal.alloc((size_t)std::max(A.nSlots, 1) * 4);
   A.dinv.alloc(nb);
   A.res.alloc(nb);
@@ -435,6 +452,12 @@ void setupSchur(Ctx& c, const std::vector<int64_t>& rp, const std::vector<int>& 
     S->L.clear();
     return;
   }
+  // the finest level smooths in half precision unless NLA3D_GPU_SCHUR_FP32_SMOOTHER=1 (its FP32
+  // copy, as the coarser levels)
+  {
+    const char* e = std::getenv("NLA3D_GPU_SCHUR_FP32_SMOOTHER");
+    S->L[0]->half = S->L.size() > 1 && !(e && e[0] == '1');
+  }
   for (size_t l = 0; l < S->L.size(); l++) allocLevel(*S->L[l], l + 1 == S->L.size());
   const int ncst = S->L.back()->n;
   S->cinv.alloc((size_t)ncst * ncst * 8);
@@ -507,10 +530,16 @@ bool schurRefresh(Ctx& c, std::string* why) {
   launch_schur_sub_c(s, S.nq, S.sDiag.as<int>(), S.cPos.as<int>(), kval, L0.val.as<double>());
   for (size_t l = 0; l + 1 < S.L.size(); l++) {
     AmgLevel& A = *S.L[l];
-    launch_sell_fill(s, A.nSlots, A.sPos.as<int>(), A.val.as<double>(), A.sVal.as<float>());
-    if (A.patterned)
-      launch_sell_fill(s, A.nd * A.ldd, A.sDPos.as<int>(), A.val.as<double>(), A.sDia.as<float>());
+    if (!A.half) {
+      launch_sell_fill(s, A.nSlots, A.sPos.as<int>(), A.val.as<double>(), A.sVal.as<float>());
+      if (A.patterned)
+        launch_sell_fill(s, A.nd * A.ldd, A.sDPos.as<int>(), A.val.as<double>(), A.sDia.as<float>());
+    }
     launch_inv_diag(s, A.csr(), A.dinv.as<double>());
+    if (A.half)
+      launch_half_fill(s, A.halfView(), A.csr(), A.sPos.as<int>(), A.patterned ? A.sDPos.as<int>() : nullptr,
+                       A.dinv.as<double>(), A.hS.as<double>(), A.hVal.as<unsigned short>(),
+                       A.patterned ? A.hDia.as<unsigned short>() : nullptr, A.hDiag.as<float>());
     // largest eigenvalue of D^-1 A by power iteration from a fixed pseudo-random start,
     // raised by a safety factor (the Chebyshev polynomial grows quickly above its interval)
     double* x = A.da.as<double>();
@@ -628,6 +657,40 @@ static void vcycle(SchurAmg& S, hipStream_t s, size_t l, const double* rext = nu
   }
   const int steps = l == 0 ? finestSteps(A.n) : kStepsC;
   const ChebCoef k = chebCoef(A.lmax, steps, l == 0 ? finestDiv(steps) : kLoDivC);
+  if (A.half) {
+    // the scaled three-term sweep (schur.h, AmgHalf): r^ in res, the iterates in da / db,
+    // x^_(k+1) written over x^_(k-1); the post-smoother's last step writes z = s x^
+    const AmgHalf H = A.halfView();
+    const double* sc = A.hS.as<double>();
+    double* rhat = A.res.as<double>();
+    double* xa = A.da.as<double>();
+    double* xb = A.db.as<double>();
+    launch_half_start(s, A.n, rext ? S.qrow.as<int>() : nullptr, rext ? rext : A.r.as<double>(), sc, k.c0, rhat, xa,
+                      st);
+    double* cur = xa;
+    double* prev = nullptr;
+    for (int i = 1; i < steps; i++) {
+      double* out = prev ? prev : xb;
+      launch_half_step(s, H, cur, prev, rhat, k.c1[i], k.c2[i], out, nullptr, nullptr, st);
+      prev = cur;
+      cur = out;
+    }
+    AmgLevel& C = *S.L[l + 1];
+    launch_half_residual_restrict(s, H, cur, rhat, A.t.as<double>(), A.nc, A.memPtr.as<int>(), A.mem.as<int>(),
+                                  C.r.as<double>(), st);
+    vcycle(S, s, l + 1);
+    launch_half_prolong(s, A.n, A.agg.as<int>(), C.x.as<double>(), sc, cur, st);
+    prev = cur;
+    cur = cur == xa ? xb : xa;
+    launch_half_start_x(s, H, prev, rhat, k.c0, cur, st);
+    for (int i = 1; i < steps; i++) {
+      const bool last = i + 1 == steps;
+      launch_half_step(s, H, cur, prev, rhat, k.c1[i], k.c2[i], prev, last && zext ? S.qrow.as<int>() : nullptr,
+                       last ? (zext ? zext : A.x.as<double>()) : nullptr, st);
+      std::swap(prev, cur);
+    }
+    return;
+  }
   const AmgSell M = A.sell();
   double* dinv = A.dinv.as<double>();
   double* din = A.da.as<double>();
@@ -755,6 +818,62 @@ int nla3d_gpu_schur_level(nla3d_gpu_t h, int level, int* agg, double* lmax) {
   });
 }
 
+int nla3d_gpu_schur_smoother_operator(nla3d_gpu_t h, double* sc, double* val) {
+  return schurGuard(h, [&](Ctx& c) {
+    if (!sc || !val) throw ApiError(-1, "null arrays");
+    if (!c.schur || !c.schur->fresh) refreshOrThrow(c);
+    SchurAmg& S = *c.schur;
+    AmgLevel& A = *S.L[0];
+    if (!A.half) throw ApiError(-3, "the finest level has no half-precision smoother operator");
+    auto down = [&](const DevBuf& b, auto& v) {
+      v.resize(b.bytes / sizeof(v[0]));
+      if (b.bytes) hipCheck(hipMemcpy(v.data(), b.p, b.bytes, hipMemcpyDeviceToHost), "operator");
+    };
+    std::vector<int> off, cOff, dOff, pos;
+    std::vector<unsigned short> hv, hd;
+    std::vector<float> dg;
+    down(A.sOff, off);
+    down(A.sPos, pos);
+    down(A.hVal, hv);
+    down(A.hDiag, dg);
+    if (A.patterned) {
+      down(A.sCOff, cOff);
+      down(A.sDOff, dOff);
+      down(A.hDia, hd);
+    }
+    hipCheck(hipMemcpy(sc, A.hS.p, (size_t)A.n * 8, hipMemcpyDeviceToHost), "s");
+    auto half2d = [](unsigned short b) {
+      _Float16 f;
+      std::memcpy(&f, &b, 2);
+      return (double)f;
+    };
+    for (int i = 0; i < A.n; i++) {
+      const int sl = i >> 6;
+      const bool pat = A.patterned && cOff[sl] < 0;
+      for (int k = A.h_rowPtr[i]; k < A.h_rowPtr[i + 1]; k++) {
+        const int col = A.h_col[k], o = col - i;
+        if (o == 0) {
+          val[k] = dg[i];
+        } else if (pat) {
+          const int d = std::abs(o);
+          const int q = (int)(std::find(dOff.begin(), dOff.end(), d) - dOff.begin());
+          val[k] = half2d(hd[(size_t)q * A.ldd + (o > 0 ? i : col)]);
+        } else {
+          val[k] = 0.0;
+        }
+      }
+      if (!pat) {
+        const int w = (off[sl + 1] - off[sl]) / 64;
+        for (int k = 0; k < w; k++) {
+          const int slot = off[sl] + k * 64 + (i & 63);
+          const int p = pos[slot];
+          if (p >= 0 && A.h_col[p] != i) val[p] = half2d(hv[slot]);
+        }
+      }
+    }
+  });
+}
+
 int nla3d_gpu_time_smoother(nla3d_gpu_t h, int iters, double* ms, long long* bytes) {
   return schurGuard(h, [&](Ctx& c) {
     if (iters < 1 || !ms) throw ApiError(-1, "bad arguments");
@@ -764,8 +883,44 @@ int nla3d_gpu_time_smoother(nla3d_gpu_t h, int iters, double* ms, long long* byt
     AmgLevel& A = *S.L[0];
     const int steps0 = finestSteps(A.n);
     const ChebCoef k = chebCoef(A.lmax, steps0, finestDiv(steps0));
-    const AmgSell M = A.sell();
     hipStream_t s = c.stream;
+    if (A.half) {
+      const AmgHalf H = A.halfView();
+      double* rhat = A.res.as<double>();
+      double* cur = A.da.as<double>();
+      double* prev = A.db.as<double>();
+      // a finite start (the power-iteration start vector), one warm-up step, then the steps
+      launch_half_start(s, A.n, nullptr, A.start.as<double>(), A.hS.as<double>(), k.c0, rhat, cur, nullptr);
+      launch_half_step(s, H, cur, nullptr, rhat, k.c1[1], k.c2[1], prev, nullptr, nullptr, nullptr);
+      std::swap(prev, cur);
+      hipEvent_t e0, e1;
+      hipCheck(hipEventCreate(&e0), "event");
+      hipCheck(hipEventCreate(&e1), "event");
+      hipCheck(hipEventRecord(e0, s), "rec");
+      for (int i = 0; i < iters; i++) {
+        const int j = 1 + i % (steps0 - 1);
+        launch_half_step(s, H, cur, prev, rhat, k.c1[j], k.c2[j], prev, nullptr, nullptr, nullptr);
+        std::swap(prev, cur);
+      }
+      hipCheck(hipEventRecord(e1, s), "rec");
+      hipCheck(hipEventSynchronize(e1), "sync");
+      float t = 0;
+      hipCheck(hipEventElapsedTime(&t, e0, e1), "elapsed");
+      *ms = t / iters;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      if (bytes) {
+        // stored operator (FP16 slots, column offsets or indices, slice offsets and bases, the
+        // FP16 diagonals k >= 1 of the pattern slices, the FP32 diagonal) and four vectors per
+        // step (x^_k, x^_(k-1), r^ read; x^_(k+1) written)
+        const long long nSl = (A.n + 63) / 64;
+        *bytes = (long long)A.nSlots * 2 + (long long)A.nColSlots * (A.narrow ? 2 : 4) + (nSl + 1) * 4 +
+                 (A.narrow ? nSl * 4 : 0) + (A.patterned ? nSl * 4 + (long long)(A.nd - 1) * A.ldd * 2 : 0) +
+                 (long long)A.n * 4 + 4ll * A.n * 8;
+      }
+      return;
+    }
+    const AmgSell M = A.sell();
     double* din = A.da.as<double>();
     double* dout = A.db.as<double>();
     // a finite start (the power-iteration start vector), then the finest level's steps

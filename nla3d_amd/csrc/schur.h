@@ -73,6 +73,60 @@ struct FusedLevel {
 // workgroup: every level must be small (one workgroup walks all its rows per pass)
 void launch_coarse_cycle(hipStream_t s, const FusedLevel* lv, int nl);
 
+// The finest level's smoother operator in half precision.  With s = D^-1/2 (D the diagonal
+// of S), the symmetrically scaled matrix S^ = diag(s) S diag(s) has a unit diagonal and
+// off-diagonal entries of magnitude below one; those are stored as FP16 (the FP64 value
+// rounded once, to nearest even) in the slots of the level's ELLPACK and diagonal layout
+// above (AmgSell: the same off, cOff, columns, dOff, nd, ldd), the ELLPACK rows' diagonal slots
+// holding zero.  The diagonal is stored apart in FP32 as 1 + Delta_i rounded up, Delta_i the
+// sum over row i of the absolute rounding errors of its stored entries, so the stored
+// operator S^' = H + diag(1 + Delta) satisfies S^' - S^ >= 0 (E + diag(Delta), E the
+// rounding errors, is diagonally dominant with a non-negative diagonal): S^' is SPD whenever
+// S is, and the smoother stays one fixed symmetric positive-definite operator.  The
+// smoother runs in the scaled variables (x^ = x / s, r^ = s r), where the Jacobi step
+// D^-1 (r - S x) becomes r^ - S^' x^: no diagonal is read per step.
+struct AmgHalf {
+  int n;
+  const int* off;
+  const int* col;
+  const unsigned short* col16;
+  const int* base;
+  const unsigned short* val;  // FP16 bits of the ELLPACK slots
+  const int* cOff;
+  const int* dOff;
+  const unsigned short* dia;  // FP16 bits, dia[k * ldd + j] = S^(j, j + d_k) for k >= 1
+  const float* diag;          // 1 + Delta_i
+  const double* s;            // D^-1/2
+  int nd, ldd;
+};
+
+// the half-precision copy of a level from its FP64 CSR values: s = sqrt(dinv), the FP16
+// slots (ELLPACK rows, diagonals k >= 1 of every row), then the compensated diagonal
+void launch_half_fill(hipStream_t s, const AmgHalf& H, const AmgCsr& A, const int* sPos, const int* dPos,
+                      const double* dinv, double* sOut, unsigned short* val, unsigned short* dia, float* diag);
+// Chebyshev smoothing of the scaled system in three-term form: with x^_0 = 0 (pre-smoothing)
+// or the prolongated iterate (post-smoothing),
+//   x^_1 = x^_0 + c0 (r^ - S^' x^_0),
+//   x^_(k+1) = x^_k + c1_k (x^_k - x^_(k-1)) + c2_k (r^ - S^' x^_k),
+// the same polynomial as the two-term Jacobi-Chebyshev sweep (d_k = x_(k+1) - x_k).  Every
+// kernel returns at once when st says the MINRES solve has converged.
+// start from zero: r^ = s r (r = rsrc[qrow[i]], or rsrc[i] without qrow) ; x^_1 = c0 r^
+void launch_half_start(hipStream_t s, int n, const int* qrow, const double* rsrc, const double* sc, double c0,
+                       double* rhat, double* x1, const MinresState* st);
+// start from x^_0 (post-smoothing): x^_1 = x^_0 + c0 (r^ - S^' x^_0)
+void launch_half_start_x(hipStream_t s, const AmgHalf& H, const double* x0, const double* rhat, double c0, double* x1,
+                         const MinresState* st);
+// one step: xout = x^_(k+1) from xk = x^_k and xprev = x^_(k-1) (null: zero; xout may be xprev).
+// With z, the step is the sweep's last: z[qrow[i]] (z[i] without qrow) = s_i x^_(k+1) instead.
+void launch_half_step(hipStream_t s, const AmgHalf& H, const double* xk, const double* xprev, const double* rhat,
+                      double c1, double c2, double* xout, const int* qrow, double* z, const MinresState* st);
+// t = (r^ - S^' x^) / s, the unscaled residual, then the restriction rc = P^T t
+void launch_half_residual_restrict(hipStream_t s, const AmgHalf& H, const double* x, const double* rhat, double* t,
+                                   int nc, const int* memPtr, const int* mem, double* rc, const MinresState* st);
+// x^ += xc[agg] / s (the prolongated coarse correction in the scaled variables)
+void launch_half_prolong(hipStream_t s, int n, const int* agg, const double* xc, const double* sc, double* x,
+                         const MinresState* st);
+
 // level-0 assembly of S: one thread per (node, Schur column a) writes alpha B(:,a)^T D^-1 B(:,b)
 void launch_schur_contrib(hipStream_t s, int nPairs, const int* pairNode, const int* nodeGroup, const int* nodeSize,
                           const int* qPtr, const int* cPtr, const int* posB, const double* kval, const double* binv,

@@ -60,6 +60,192 @@ __device__ __forceinline__ double sell_dot(const AmgSell& A, int i, const double
   return acc;
 }
 
+// ---- the finest level's half-precision smoother (AmgHalf) ------------------------------
+// FP64 -> FP16 in one correctly rounded step (round to nearest even): no FP32 intermediate,
+// whose double rounding would turn values just off an FP16 midpoint into ties
+__device__ __forceinline__ unsigned short to_half_bits(double v) {
+  return __builtin_bit_cast(unsigned short, static_cast<_Float16>(v));
+}
+__device__ __forceinline__ double from_half_bits(unsigned short b) {
+  return (double)__builtin_bit_cast(_Float16, b);
+}
+
+// row i of S^' = H + diag(1 + Delta) times x, one thread per row (the layout of sell_dot).
+// kND > 0: the diagonal-form rows' loop over the kND - 1 offsets is unrolled, and all of its
+// loads (two FP16 entries and two x values per offset) are issued before the first product,
+// so a wave waits for memory once per row instead of once per pair of offsets
+template <bool kNarrow, int kND>
+__device__ __forceinline__ double half_dot(const AmgHalf& H, int i, const double* __restrict__ x) {
+  const int sl = i >> 6;
+  double acc = (double)H.diag[i] * x[i];
+  if (H.cOff && H.cOff[sl] < 0) {
+    const unsigned short* __restrict__ u = H.dia + i;
+    if constexpr (kND > 0) {
+      unsigned short hu[kND - 1], hl[kND - 1];
+      double xu[kND - 1], xl[kND - 1];
+#pragma unroll
+      for (int k = 1; k < kND; k++) {
+        const int d = H.dOff[k];
+        const unsigned short* __restrict__ uk = u + (size_t)k * H.ldd;
+        hu[k - 1] = uk[0];
+        hl[k - 1] = uk[-d];
+        xu[k - 1] = x[i + d];
+        xl[k - 1] = x[i - d];
+      }
+#pragma unroll
+      for (int k = 0; k < kND - 1; k++) acc += from_half_bits(hu[k]) * xu[k] + from_half_bits(hl[k]) * xl[k];
+    } else {
+#pragma unroll 2
+      for (int k = 1; k < H.nd; k++) {
+        const int d = H.dOff[k];
+        const unsigned short* __restrict__ uk = u + (size_t)k * H.ldd;
+        acc += from_half_bits(uk[0]) * x[i + d] + from_half_bits(uk[-d]) * x[i - d];
+      }
+    }
+    return acc;
+  }
+  const int b = H.off[sl], w = (H.off[sl + 1] - b) >> 6;
+  const unsigned short* __restrict__ v = H.val + b + (i & 63);
+  const int cb = (H.cOff ? H.cOff[sl] : b) + (i & 63);
+  if constexpr (kNarrow) {
+    const unsigned short* __restrict__ c = H.col16 + cb;
+    const double* __restrict__ xb = x + H.base[sl];
+#pragma unroll 4
+    for (int k = 0; k < w; k++) acc += from_half_bits(v[k * 64]) * xb[c[k * 64]];
+  } else {
+    const int* __restrict__ c = H.col + cb;
+#pragma unroll 4
+    for (int k = 0; k < w; k++) acc += from_half_bits(v[k * 64]) * x[c[k * 64]];
+  }
+  return acc;
+}
+
+__global__ __launch_bounds__(256) void half_scale_kernel(int n, const double* __restrict__ dinv,
+                                                       double* __restrict__ sc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) sc[i] = sqrt(dinv[i]);
+}
+
+// S^(i, c) = S(i, c) (s_i s_c) rounded to FP16 for the ELLPACK rows (the diagonal slot: 0)
+template <bool kNarrow>
+__global__ __launch_bounds__(256) void half_fill_ell_kernel(AmgHalf H, const double* __restrict__ csrVal,
+                                                          const int* __restrict__ sPos,
+                                                          unsigned short* __restrict__ val) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= H.n) return;
+  const int sl = i >> 6;
+  if (H.cOff && H.cOff[sl] < 0) return;
+  const int b = H.off[sl], w = (H.off[sl + 1] - b) >> 6;
+  const int cb = (H.cOff ? H.cOff[sl] : b) + (i & 63);
+  const double si = H.s[i];
+  for (int k = 0; k < w; k++) {
+    const int slot = b + k * 64 + (i & 63);
+    const int p = sPos[slot];
+    const int c = kNarrow ? H.base[sl] + H.col16[cb + k * 64] : H.col[cb + k * 64];
+    const double v = (p >= 0 && c != i) ? csrVal[p] * (si * H.s[c]) : 0.0;
+    val[slot] = to_half_bits(v);
+  }
+}
+
+// dia[k][j] = S^(j, j + d_k) rounded to FP16, k >= 1, every row j (0 where absent)
+__global__ __launch_bounds__(256) void half_fill_dia_kernel(AmgHalf H, const double* __restrict__ csrVal,
+                                                          const int* __restrict__ dPos,
+                                                          unsigned short* __restrict__ dia) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)(H.nd - 1) * H.ldd) return;
+  const int k = 1 + (int)(idx / H.ldd), j = (int)(idx - (long long)(k - 1) * H.ldd);
+  double v = 0.0;
+  if (j < H.n) {
+    const int p = dPos[(size_t)k * H.ldd + j];
+    if (p >= 0) v = csrVal[p] * (H.s[j] * H.s[j + H.dOff[k]]);
+  }
+  dia[(size_t)k * H.ldd + j] = to_half_bits(v);
+}
+
+// diag_i = 1 + Delta_i rounded up to FP32, Delta_i = |1 - S^(i,i)| + the absolute FP16 rounding
+// errors of the row's other entries, summed in the row's CSR order
+__global__ __launch_bounds__(256) void half_diag_kernel(AmgCsr A, const double* __restrict__ sc,
+                                                      float* __restrict__ diag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  const double si = sc[i];
+  double delta = 0.0;
+  for (int k = A.rowPtr[i]; k < A.rowPtr[i + 1]; k++) {
+    const int c = A.col[k];
+    const double v = A.val[k] * (si * sc[c]);
+    delta += c == i ? fabs(1.0 - v) : fabs(from_half_bits(to_half_bits(v)) - v);
+  }
+  const double d = 1.0 + delta;
+  float f = (float)d;
+  if ((double)f < d) f = nextafterf(f, INFINITY);
+  diag[i] = f;
+}
+
+// r^ = s r ; x^_1 = c0 r^
+__global__ __launch_bounds__(256) void half_start_kernel(int n, const int* __restrict__ qrow,
+                                                       const double* __restrict__ rsrc, const double* __restrict__ sc,
+                                                       double c0, double* __restrict__ rhat, double* __restrict__ x1,
+                                                       const MinresState* __restrict__ st) {
+  if (st && st->converged) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double rh = sc[i] * rsrc[qrow ? qrow[i] : i];
+    rhat[i] = rh;
+    x1[i] = c0 * rh;
+  }
+}
+
+template <bool kNarrow, int kND>
+__global__ __launch_bounds__(256) void half_start_x_kernel(AmgHalf H, const double* __restrict__ x0,
+                                                         const double* __restrict__ rhat, double c0,
+                                                         double* __restrict__ x1, const MinresState* __restrict__ st) {
+  if (st && st->converged) return;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= H.n) return;
+  x1[i] = x0[i] + c0 * (rhat[i] - half_dot<kNarrow, kND>(H, i, x0));
+}
+
+// x^_(k+1) = x^_k + c1 (x^_k - x^_(k-1)) + c2 (r^ - S^' x^_k); xout may be xprev (each thread
+// reads its own entry of xprev before writing it)
+template <bool kNarrow, int kND>
+__global__ __launch_bounds__(256) void half_step_kernel(AmgHalf H, const double* __restrict__ xk, const double* xprev,
+                                                      const double* __restrict__ rhat, double c1, double c2,
+                                                      double* xout, const int* __restrict__ qrow,
+                                                      double* __restrict__ z, const MinresState* __restrict__ st) {
+  if (st && st->converged) return;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= H.n) return;
+  const double ax = half_dot<kNarrow, kND>(H, i, xk);
+  const double xi = xk[i];
+  const double xp = xprev ? xprev[i] : 0.0;
+  const double xn = xi + c1 * (xi - xp) + c2 * (rhat[i] - ax);
+  if (z)
+    z[qrow ? qrow[i] : i] = H.s[i] * xn;
+  else
+    xout[i] = xn;
+}
+
+template <bool kNarrow, int kND>
+__global__ __launch_bounds__(256) void half_residual_kernel(AmgHalf H, const double* __restrict__ x,
+                                                          const double* __restrict__ rhat, double* __restrict__ t,
+                                                          const MinresState* __restrict__ st) {
+  if (st && st->converged) return;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= H.n) return;
+  const double si = H.s[i];
+  const double th = rhat[i] - half_dot<kNarrow, kND>(H, i, x);
+  t[i] = si > 0.0 ? th / si : 0.0;
+}
+
+__global__ __launch_bounds__(256) void half_prolong_kernel(int n, const int* __restrict__ agg,
+                                                         const double* __restrict__ xc, const double* __restrict__ sc,
+                                                         double* __restrict__ x, const MinresState* __restrict__ st) {
+  if (st && st->converged) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double si = sc[i];
+    if (si > 0.0) x[i] += xc[agg[i]] / si;
+  }
+}
+
 __global__ __launch_bounds__(256) void sell_fill_kernel(int nSlots, const int* __restrict__ pos,
                                                       const double* __restrict__ csrVal, float* __restrict__ val) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -396,6 +582,85 @@ void launch_gather_sum(hipStream_t s, int n, const int* seg, const int* src, con
 void launch_inv_diag(hipStream_t s, const AmgCsr& A, double* dinv) {
   if (A.n <= 0) return;
   hipLaunchKernelGGL(inv_diag_kernel, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, dinv);
+}
+
+void launch_half_fill(hipStream_t s, const AmgHalf& H, const AmgCsr& A, const int* sPos, const int* dPos,
+                      const double* dinv, double* sOut, unsigned short* val, unsigned short* dia, float* diag) {
+  if (H.n <= 0) return;
+  hipLaunchKernelGGL(half_scale_kernel, dim3(cdiv(H.n, 256)), dim3(256), 0, s, H.n, dinv, sOut);
+  AmgHalf W = H;
+  W.s = sOut;
+  if (W.col16)
+    hipLaunchKernelGGL(half_fill_ell_kernel<true>, dim3(cdiv(W.n, 256)), dim3(256), 0, s, W, A.val, sPos, val);
+  else
+    hipLaunchKernelGGL(half_fill_ell_kernel<false>, dim3(cdiv(W.n, 256)), dim3(256), 0, s, W, A.val, sPos, val);
+  if (dia && W.nd > 1)
+    hipLaunchKernelGGL(half_fill_dia_kernel, dim3(cdiv((long long)(W.nd - 1) * W.ldd, 256)), dim3(256), 0, s, W,
+                       A.val, dPos, dia);
+  hipLaunchKernelGGL(half_diag_kernel, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, sOut, diag);
+}
+
+void launch_half_start(hipStream_t s, int n, const int* qrow, const double* rsrc, const double* sc, double c0,
+                       double* rhat, double* x1, const MinresState* st) {
+  hipLaunchKernelGGL(half_start_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, qrow, rsrc, sc, c0, rhat, x1, st);
+}
+
+// the diagonal-form width the kernels unroll: 14 offsets (a hexahedral mesh's element-pressure
+// Schur rows: the element and its 13 upper neighbours), 5 (quadrilaterals), else a loop
+template <template <bool, int> class Launch, typename... Args>
+static void half_dispatch(const AmgHalf& H, Args... args) {
+  const bool narrow = H.col16 != nullptr;
+  const int nd = H.cOff ? H.nd : 0;
+  if (nd == 14) narrow ? Launch<true, 14>::go(H, args...) : Launch<false, 14>::go(H, args...);
+  else if (nd == 5) narrow ? Launch<true, 5>::go(H, args...) : Launch<false, 5>::go(H, args...);
+  else narrow ? Launch<true, 0>::go(H, args...) : Launch<false, 0>::go(H, args...);
+}
+
+template <bool kNarrow, int kND>
+struct StartXLaunch {
+  static void go(const AmgHalf& H, hipStream_t s, const double* x0, const double* rhat, double c0, double* x1,
+                 const MinresState* st) {
+    hipLaunchKernelGGL((half_start_x_kernel<kNarrow, kND>), dim3(cdiv(H.n, 256)), dim3(256), 0, s, H, x0, rhat, c0,
+                       x1, st);
+  }
+};
+template <bool kNarrow, int kND>
+struct StepLaunch {
+  static void go(const AmgHalf& H, hipStream_t s, const double* xk, const double* xprev, const double* rhat, double c1,
+                 double c2, double* xout, const int* qrow, double* z, const MinresState* st) {
+    hipLaunchKernelGGL((half_step_kernel<kNarrow, kND>), dim3(cdiv(H.n, 256)), dim3(256), 0, s, H, xk, xprev, rhat,
+                       c1, c2, xout, qrow, z, st);
+  }
+};
+template <bool kNarrow, int kND>
+struct ResidualLaunch {
+  static void go(const AmgHalf& H, hipStream_t s, const double* x, const double* rhat, double* t,
+                 const MinresState* st) {
+    hipLaunchKernelGGL((half_residual_kernel<kNarrow, kND>), dim3(cdiv(H.n, 256)), dim3(256), 0, s, H, x, rhat, t,
+                       st);
+  }
+};
+
+void launch_half_start_x(hipStream_t s, const AmgHalf& H, const double* x0, const double* rhat, double c0, double* x1,
+                         const MinresState* st) {
+  half_dispatch<StartXLaunch>(H, s, x0, rhat, c0, x1, st);
+}
+
+void launch_half_step(hipStream_t s, const AmgHalf& H, const double* xk, const double* xprev, const double* rhat,
+                      double c1, double c2, double* xout, const int* qrow, double* z, const MinresState* st) {
+  half_dispatch<StepLaunch>(H, s, xk, xprev, rhat, c1, c2, xout, qrow, z, st);
+}
+
+void launch_half_residual_restrict(hipStream_t s, const AmgHalf& H, const double* x, const double* rhat, double* t,
+                                   int nc, const int* memPtr, const int* mem, double* rc, const MinresState* st) {
+  half_dispatch<ResidualLaunch>(H, s, x, rhat, t, st);
+  hipLaunchKernelGGL(restrict_kernel, dim3(cdiv((long long)nc * kRestrictLanes, 256)), dim3(256), 0, s, nc, memPtr,
+                     mem, t, rc);
+}
+
+void launch_half_prolong(hipStream_t s, int n, const int* agg, const double* xc, const double* sc, double* x,
+                         const MinresState* st) {
+  hipLaunchKernelGGL(half_prolong_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, agg, xc, sc, x, st);
 }
 
 void launch_cheb_start(hipStream_t s, int n, const double* r, const double* dinv, double c0, double* res, double* d,

@@ -718,6 +718,50 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
       }
     }
   }
+  // the second solve of a load step from the last load step's second increment p: x0 = alpha
+  // [p_u ; 0] + [0 ; U_l], alpha minimising ||rhs - K2 x0||_2, and the correction solved for
+  // to the same absolute accuracy, as above
+  const bool predict2 = it == 2 && o.first_solve_predictor >= 2;
+  if (predict2) {
+    const int n2 = c.n2;
+    if (c.d_it2.bytes < (size_t)n2 * 8) {
+      c.d_it2.alloc((size_t)n2 * 8);
+      c.d_q.alloc((size_t)n2 * 8);
+      c.d_kq.alloc((size_t)n2 * 8);
+      c.it2Valid = false;
+    }
+    if (c.d_x0.bytes < (size_t)n2 * 8) {
+      c.d_x0.alloc((size_t)n2 * 8);
+      c.d_rhsC.alloc((size_t)n2 * 8);
+    }
+    if (c.it2Valid) {
+      double* P = c.d_x0.as<double>();
+      double* KP = c.d_rhsC.as<double>();
+      double* Q = c.d_q.as<double>();
+      double* R0 = c.d_kq.as<double>();
+      double* part = c.d_partial.as<double>();
+      double* sc = c.d_scal.as<double>();
+      newton_pred_split(c.stream, nC, nU, nM, c.d_it2.as<double>(), U, P, Q);
+      krylov_spmv(c.stream, c.krylovMatrix(), P, KP);
+      krylov_spmv(c.stream, c.krylovMatrix(), Q, R0);
+      vec_rsub(c.stream, n2, b, R0);  // R0 = rhs - K2 Q
+      krylov_dot(c.stream, n2, R0, KP, part, sc + 11);
+      vec_sumsq(c.stream, n2, KP, part, sc + 12);
+      vec_sumsq(c.stream, n2, b, part, sc + 13);
+      hipCheck(hipGetLastError(), "predictor");
+      const double rk = readScalar(c, 11), kk = readScalar(c, 12), bn = std::sqrt(readScalar(c, 13));
+      const double alpha = kk > 0.0 ? rk / kk : 0.0;
+      newton_pred_combine(c.stream, n2, alpha, P, Q, R0, KP);  // P = x0, KP = rhs - K2 x0
+      vec_sumsq(c.stream, n2, KP, part, sc + 11);
+      hipCheck(hipGetLastError(), "predictor");
+      const double r0 = std::sqrt(readScalar(c, 11));
+      if (r0 > 0.0 && r0 < bn) {
+        ktol = std::min(0.1, ktol * bn / r0);
+        b = KP;
+        predicted = true;
+      }
+    }
+  }
   // solve K.block(2) dU_sl = rhs (FESolver.cpp:457)
   t0 = nowMs();
   const bool learnedPc = o.preconditioner == 1 || o.preconditioner == 3;
@@ -745,6 +789,10 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
                         trueAbs, &trueStop);
   }
   if (predicted) vec_add(c.stream, c.n2, c.d_x0.as<double>(), c.d_dU.as<double>());  // dU = x0 + correction
+  if (predict2) {
+    hipCheck(hipMemcpyAsync(c.d_it2.p, c.d_dU.p, (size_t)c.n2 * 8, hipMemcpyDeviceToDevice, c.stream), "it2");
+    c.it2Valid = true;
+  }
   st.krylov_iterations += kit;
   st.linear_solves++;
   st.last_krylov_iterations = kit;

@@ -9,15 +9,23 @@ the device path implements:
 * vcycle: one V-cycle over given aggregation maps with Jacobi-Chebyshev smoothing
   on [lmax/div, 1.05 lmax], the same polynomial before and after (finest level:
   16 terms, div 400; coarser levels: 3 terms, div 10), exact solve on the
-  coarsest level.  The smoother's and the residual's products use each level's
-  matrix with its entries rounded to FP32 (products and sums in FP64), the diagonal
-  scaling and the coarse matrices the FP64 values.
+  coarsest level.  The coarser levels' smoother and residual products use the
+  level's matrix with its entries rounded to FP32 (products and sums in FP64), the
+  diagonal scaling and the coarse matrices the FP64 values.  The finest level of a
+  hierarchy with more than one level smooths with its half-precision operator
+  (half_operator: the symmetrically scaled matrix with FP16 off-diagonal entries and
+  a diagonal raised by their rounding errors) in the scaled variables, by the
+  three-term form of the same polynomial (smooth_scaled).
 
 Pinned by: schur_matrix equals minus the exact Schur complement (dense elimination) of
 block(2) with its displacement block replaced by its block diagonal over alpha; a
 one-level hierarchy gives S^-1 exactly; the cycle is a symmetric
 positive-definite operator; the smoother alone reduces the energy-norm error; S is
-symmetric positive definite on the models of the tests.
+symmetric positive definite on the models of the tests; the two-term smoother is the
+scaled minimax Chebyshev polynomial in closed form; half_operator is exact on a matrix
+whose scaled entries FP16 holds, dominates the scaled matrix, and stays positive definite
+where plain FP16 rounding does not; the scaled three-term sweep on it equals the two-term
+Jacobi-Chebyshev sweep on the unscaled operator.
 """
 import math
 
@@ -104,16 +112,74 @@ def rounded_f32(A):
     return A
 
 
-def vcycle(mats, aggs, lmaxs, r, level=0):
+def round_half(v):
+    """FP64 values rounded to FP16 in one step, to nearest even (kept as FP64)"""
+    return np.asarray(v, dtype=np.float64).astype(np.float16).astype(np.float64)
+
+
+def round_up_f32(v):
+    """the smallest FP32 values >= v (kept as FP64)"""
+    v = np.asarray(v, dtype=np.float64)
+    f = v.astype(np.float32)
+    low = f.astype(np.float64) < v
+    f[low] = np.nextafter(f[low], np.float32(np.inf))
+    return f.astype(np.float64)
+
+
+def half_operator(A):
+    """(s, H) for the finest level's half-precision smoother: s = D^-1/2 (D = diag A, s = 0 where
+    D <= 0), the scaled matrix A^ = diag(s) A diag(s) (entry (i, c) computed as A_ic (s_i s_c)),
+    H = its off-diagonal entries rounded to FP16 plus the diagonal 1 + Delta_i rounded up to
+    FP32, Delta_i = |1 - A^_ii| + sum over the row's other entries of |round_half(A^_ic) - A^_ic|"""
+    A = sp.coo_matrix(A)
+    n = A.shape[0]
+    d = sp.csr_matrix(A).diagonal()
+    s = np.sqrt(np.where(d > 0, 1.0 / np.where(d > 0, d, 1.0), 0.0))
+    rows, cols = A.row.astype(np.int64), A.col.astype(np.int64)
+    v = A.data * (s[rows] * s[cols])
+    off = rows != cols
+    h = round_half(v[off])
+    err = np.zeros(len(v))
+    err[off] = np.abs(h - v[off])
+    err[~off] = np.abs(1.0 - v[~off])
+    delta = np.bincount(rows, weights=err, minlength=n)
+    H = sp.csr_matrix((h, (rows[off], cols[off])), shape=(n, n)) + sp.diags(round_up_f32(1.0 + delta))
+    return s, H.tocsr()
+
+
+def smooth_scaled(H, rhat, x0, lmax, steps, div):
+    """the Chebyshev sweep of `smooth` in the scaled variables, three-term form:
+    x_1 = x_0 + c0 (rhat - H x_0), x_(k+1) = x_k + c1_k (x_k - x_(k-1)) + c2_k (rhat - H x_k)"""
+    c0, c1, c2 = cheb_coef(lmax, steps, div)
+    prev, x = x0, x0 + c0 * (rhat - H @ x0)
+    for i in range(1, steps):
+        prev, x = x, x + c1[i] * (x - prev) + c2[i] * (rhat - H @ x)
+    return x
+
+
+def vcycle(mats, aggs, lmaxs, r, level=0, half=None):
+    """one V-cycle from `level`; half = (s, H) replaces half_operator(mats[0]) on the finest
+    level (to run the cycle on a given stored operator)"""
     A = mats[level]
     if level == len(mats) - 1:
         return np.linalg.solve(A.toarray(), r)
     steps = finest_steps(A.shape[0]) if level == 0 else STEPSC
     div = finest_div(steps) if level == 0 else DIVC
+    agg = aggs[level]
+    if level == 0:
+        s, H = half_operator(A) if half is None else half
+        rhat = s * r
+        x = smooth_scaled(H, rhat, np.zeros_like(r), lmaxs[0], steps, div)
+        pos = s > 0
+        t = np.zeros_like(r)
+        t[pos] = (rhat - H @ x)[pos] / s[pos]
+        rc = np.bincount(agg, weights=t, minlength=mats[1].shape[0])
+        xc = vcycle(mats, aggs, lmaxs, rc, 1)[agg]
+        x[pos] += xc[pos] / s[pos]
+        return s * smooth_scaled(H, rhat, x, lmaxs[0], steps, div)
     dinv = 1.0 / A.diagonal()
     As = rounded_f32(A)
     x = smooth(As, dinv, r, np.zeros_like(r), lmaxs[level], steps, div)
-    agg = aggs[level]
     rc = np.bincount(agg, weights=r - As @ x, minlength=mats[level + 1].shape[0])
     x = x + vcycle(mats, aggs, lmaxs, rc, level + 1)[agg]
     return smooth(As, dinv, r, x, lmaxs[level], steps, div)

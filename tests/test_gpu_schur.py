@@ -50,6 +50,28 @@ def gpu_s_dense(m):
     return qrow, sp.csr_matrix((val, col, rp), shape=(n, n)).toarray()
 
 
+def check_half_operator(m):
+    """the finest level's stored smoother operator against the oracle's half_operator(S): s
+    bitwise, the FP16 entries and the compensated diagonal bitwise except (at most two) entries
+    whose FP64 inputs differ because the device reads a diagonal-form row's lower entries from
+    its neighbour's upper ones (S is symmetric to rounding, not bitwise); returns (s, H)"""
+    qrow, rp, col, val = m.schur_matrix()
+    n = len(qrow)
+    S = sp.csr_matrix((val, col, rp), shape=(n, n))
+    sd, hv = m.schur_smoother_operator()
+    so, Ho = ps.half_operator(S)
+    assert np.array_equal(sd, so)
+    Hd = sp.csr_matrix((hv, S.indices, S.indptr), shape=(n, n))
+    D = (Hd - Ho).tocoo()
+    bad = D.data != 0
+    assert bad.sum() <= 2, bad.sum()
+    ref = Ho[D.row, D.col]
+    ref = ref.toarray().ravel() if sp.issparse(ref) else np.asarray(ref).ravel()
+    assert np.all(np.abs(D.data) <= 2.0 ** -10 * np.abs(ref) + 2.0 ** -24)
+    assert abs(Hd - Hd.T).max() == 0.0
+    return sd, Hd
+
+
 def test_schur_matrix_matches_oracle(gpu_lib):
     print("[gpu] schur matrix", flush=True)
     m = rigid_model(4, 4, 2)
@@ -74,13 +96,15 @@ def test_vcycle_matches_oracle_and_is_symmetric(gpu_lib):
         aggs.append(agg.astype(np.int64))
         lmaxs.append(lm)
     mats = ps.hierarchy(S, aggs)
+    half = check_half_operator(m)
     rng = np.random.default_rng(3)
     y, z = rng.standard_normal(len(qrow)), rng.standard_normal(len(qrow))
     by, bz = m.schur_apply(y), m.schur_apply(z)
+    assert np.abs(by - ps.vcycle(mats, aggs, lmaxs, y, half=half)).max() <= 1e-12 * np.abs(by).max()
     bo = ps.vcycle(mats, aggs, lmaxs, y)
-    # the smoother's FP32 copies are rounded from coarse values that the device sums by a
-    # gather and the oracle by P^T A P: an entry an ulp apart can round to the neighbouring
-    # FP32 value, which moves the cycle by far less than 1e-9
+    # the coarse levels' FP32 copies are rounded from values that the device sums by a gather
+    # and the oracle by P^T A P: an entry an ulp apart can round to the neighbouring FP32
+    # value, which moves the cycle by far less than 1e-9
     assert np.abs(by - bo).max() <= 1e-9 * np.abs(bo).max()
     assert abs(z @ by - y @ bz) <= 1e-10 * abs(z @ by)
     assert y @ by > 0 and z @ bz > 0
@@ -119,8 +143,9 @@ def test_smoother_timing_entry(gpu_lib):
     ms, nbytes = m.time_smoother(4)
     info = m.schur_info()
     assert ms > 0.0
-    # at least the FP32 values of every entry and seven vectors
-    assert nbytes >= 4 * info["nnz0"] + 7 * 8 * info["nq"]
+    # at least one byte per entry (the half-precision operator stores each off-diagonal entry of
+    # the diagonal-form rows once, in two bytes) and the step's four vectors
+    assert nbytes >= info["nnz0"] + 4 * 8 * info["nq"]
 
 
 def a2000_damper_model():
@@ -164,9 +189,11 @@ def test_vcycle_matches_oracle_on_deep_hierarchies(gpu_lib, case):
         lmaxs.append(lm)
     mats = ps.hierarchy(S, aggs)
     assert [A.shape[0] for A in mats] == info["sizes"]
+    half = check_half_operator(m)
     rng = np.random.default_rng(9)
     y, z = rng.standard_normal(len(qrow)), rng.standard_normal(len(qrow))
     by, bz = m.schur_apply(y), m.schur_apply(z)
+    assert np.abs(by - ps.vcycle(mats, aggs, lmaxs, y, half=half)).max() <= 1e-12 * np.abs(by).max()
     bo = ps.vcycle(mats, aggs, lmaxs, y)
     assert np.abs(by - bo).max() <= 1e-9 * np.abs(bo).max()
     assert abs(z @ by - y @ bz) <= 1e-10 * abs(z @ by)

@@ -1,7 +1,8 @@
 """The metric workload's timed settings: Newton to a 1e-8 relative residual (criterion 1),
 MINRES to 1e-9 with block Jacobi + the Schur-row multigrid (preconditioner 2), the first solve
-of each load step started from the previous load step's increment (first_solve_predictor) and
-run to 5e-8 of the right-hand side (krylov_tol_loose), on the 2 x 2 x 1
+of each load step started from the previous load step's increment and run to 5e-8 of the
+right-hand side (krylov_tol_loose), the second from the previous load step's second increment
+(first_solve_predictor = 2), on the 2 x 2 x 1
 box with a rigid-body top (+15 % UZ, 20 degree ROTZ), Mooney-Rivlin C10 = 0.5, C01 = 0.1,
 K = 1000, with the same jitter and seed.  After every load step (FESolver.cpp:457-483) the
 displacements must agree within 1e-8 relative (max-abs over the solution vector), with no
@@ -26,6 +27,7 @@ pytestmark = pytest.mark.gpu
 
 UZ, ROTZ = 0.15, float(np.deg2rad(20.0))
 TIMED_FIRST_SOLVE_TOL = 5e-8  # bench.py --krylov-tol-loose, with the first-solve predictor
+TIMED_PREDICTOR = 2           # bench.py --predictor: the first and the second solve predicted
 MAT = (3, [0.5, 0.1, 1000.0])
 
 
@@ -53,7 +55,7 @@ def test_timed_settings_match_oracle_every_load_step(gpu_lib):
     o.newton_init()
     opts = m.newton_defaults(criterion=1, convergence=1e-8, krylov_tol=1e-9, krylov_max_iter=100000,
                              preconditioner=2, krylov_tol_loose=TIMED_FIRST_SOLVE_TOL)
-    opts.first_solve_predictor = 1
+    opts.first_solve_predictor = TIMED_PREDICTOR
     st = nla3d_amd.NewtonStats()
     worst = 0.0
     for t in load_steps(20):
@@ -94,7 +96,7 @@ def test_timed_settings_match_tight_solves_at_the_timed_shape(gpu_lib):
     assert st.precond_fallbacks == 0
     ref.close()
     del ref
-    m, opts = timed_shape(1e-9, TIMED_FIRST_SOLVE_TOL, 1)
+    m, opts = timed_shape(1e-9, TIMED_FIRST_SOLVE_TOL, TIMED_PREDICTOR)
     st = nla3d_amd.NewtonStats()
     for t, U0 in zip(steps, Uref):
         m.solve_load_step(t, opts, st)
@@ -133,13 +135,14 @@ def test_true_residual_forcing_ends_the_later_solves_early(gpu_lib):
 
 def test_first_solve_predictor(gpu_lib):
     """first_solve_predictor: from the second load step on, the first solve starts from the last
-    load step's increment and needs fewer MINRES iterations for the same absolute accuracy; the
-    solves per load step and the displacements (1e-8) are those of the unpredicted run; after
-    nla3d_gpu_set_solution the history restarts (the next first solve is unpredicted)"""
+    load step's increment (1), and the second from the last load step's second increment (2),
+    each needing fewer MINRES iterations for the same absolute accuracy; the solves per load step
+    and the displacements (1e-8) are those of the unpredicted run; after nla3d_gpu_set_solution
+    the history restarts (the next load step's solves are unpredicted)"""
     print("[gpu] first-solve predictor, 16x16x8, 4 load steps", flush=True)
     steps = load_steps(20)[:4]
     runs = {}
-    for pred in (0, 1):
+    for pred in (0, 1, 2):
         m = nla3d_amd.GpuModel(0)
         meshgen.solid81_rigid_top_case(m, 16, 16, 8, 2.0, 2.0, 1.0, UZ, ROTZ, material=MAT)
         m.set_max_in_row(0)
@@ -157,12 +160,15 @@ def test_first_solve_predictor(gpu_lib):
         runs[pred] = (firsts, solves, Us, m, opts)
         print(f"predictor {pred}: MINRES per load step {firsts}, solves {solves}", flush=True)
     f0, s0, U0, _, _ = runs[0]
-    f1, s1, U1, m, opts = runs[1]
-    assert s0 == s1 and f1[0] == f0[0] and all(a < b for a, b in zip(f1[1:], f0[1:]))
-    for a, b in zip(U1, U0):
-        assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
+    f1, s1, U1, _, _ = runs[1]
+    f2, s2, U2, m, opts = runs[2]
+    assert s0 == s1 == s2 and f1[0] == f0[0] == f2[0]
+    assert all(a < b for a, b in zip(f1[1:], f0[1:])) and all(a < b for a, b in zip(f2[1:], f1[1:]))
+    for a1, a2, b in zip(U1, U2, U0):
+        assert np.abs(a1 - b).max() <= 1e-8 * np.abs(b).max()
+        assert np.abs(a2 - b).max() <= 1e-8 * np.abs(b).max()
     # restart: back to the state after the first load step, then the second again, unpredicted
-    m.set_solution(U1[0])
+    m.set_solution(U2[0])
     st = nla3d_amd.NewtonStats()
     m.solve_load_step(steps[1], opts, st)
     assert st.krylov_iterations > f1[1] and abs(st.krylov_iterations - f0[1]) <= 0.02 * f0[1], \

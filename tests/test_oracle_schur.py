@@ -68,3 +68,77 @@ def test_finest_smoother_length_follows_the_size():
     assert [ps.finest_steps(n) for n in (1, 3101, 7513, 10000, 32768, 100000, 907923, 10 ** 7)] == \
         [8, 8, 8, 8, 10, 12, 16, 16]
     assert ps.finest_div(8) == 100.0 and ps.finest_div(16) == ps.DIV0 == 400.0
+
+
+def jittered_spd(n, lam_min, seed):
+    """a dense SPD matrix with a unit-free spread of entries and smallest eigenvalue lam_min
+    relative to its diagonal scale (entries not representable in FP16)"""
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.concatenate([[lam_min], rng.uniform(0.05, 2.0, n - 1)])
+    A = (Q * lam) @ Q.T
+    d = 1.0 + rng.uniform(0.0, 3.0, n)  # a non-trivial diagonal scaling
+    return sp.csr_matrix(np.sqrt(d)[:, None] * A * np.sqrt(d)[None, :])
+
+
+def test_half_operator_is_exact_when_fp16_holds_the_scaled_entries():
+    """the 2-D Laplacian without shift: every diagonal 4, s = 1/2, scaled entries -1/4 and 1,
+    all exact in FP16 and FP64, so Delta = 0 and H is the scaled matrix itself"""
+    n = 8
+    T = sp.diags([-np.ones(n - 1), 2 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1])
+    A = (sp.kron(T, sp.identity(n)) + sp.kron(sp.identity(n), T)).tocsr()
+    s, H = ps.half_operator(A)
+    assert np.all(s == 0.5)
+    assert np.array_equal(H.toarray(), 0.25 * A.toarray())
+
+
+def test_half_operator_dominates_the_scaled_matrix_and_stays_definite():
+    """H - A^ is positive semi-definite (diagonally dominant by construction), so H is SPD
+    with A; on a matrix whose smallest scaled eigenvalue lies below the FP16 rounding, plain
+    rounding of the off-diagonal entries (unit diagonal) gives an indefinite matrix and H
+    does not.  The seed is fixed where plain rounding fails (asserted)."""
+    A = jittered_spd(40, 2e-6, seed=3)
+    s, H = ps.half_operator(A)
+    Ad = A.toarray()
+    Ah = s[:, None] * Ad * s[None, :]
+    Hd = H.toarray()
+    assert np.array_equal(Hd, Hd.T)
+    off = ~np.eye(40, dtype=bool)
+    # within half an FP16 spacing (relative 2^-11, absolute 2^-25 among the subnormals)
+    assert np.all(np.abs(Hd - Ah)[off] <= 2.0 ** -11 * np.abs(Ah)[off] + 2.0 ** -25)
+    assert np.linalg.eigvalsh(Hd - Ah).min() >= -1e-14
+    plain = np.where(off, ps.round_half(Ah), 1.0)
+    assert np.linalg.eigvalsh(0.5 * (plain + plain.T)).min() < 0
+    assert np.linalg.eigvalsh(Hd).min() > 0
+    # dropping the compensation (a unit diagonal) is what fails: the diagonal carries all of it
+    assert np.all(np.diag(Hd) > 1.0)
+
+
+def test_round_half_is_one_correctly_rounded_step():
+    """FP64 -> FP16 rounds once: a value just above an FP16 midpoint goes up (two roundings
+    through FP32 would land on the midpoint and go to the even neighbour below), ties go to the
+    even neighbour, and the subnormal spacing is 2^-24"""
+    lo, hi = 1.0, 1.0 + 2.0 ** -10          # neighbouring FP16 values, lo even
+    mid = 0.5 * (lo + hi)
+    assert ps.round_half(mid + 2.0 ** -40) == hi
+    assert np.float64(np.float32(mid + 2.0 ** -40)) == mid  # (FP32 alone would make it a tie)
+    assert ps.round_half(mid) == lo and ps.round_half(mid - 2.0 ** -40) == lo
+    assert ps.round_half(0.5 * (hi + hi + 2.0 ** -10)) == hi + 2.0 ** -10  # tie up to the even one
+    assert ps.round_half(3 * 2.0 ** -24) == 3 * 2.0 ** -24 and ps.round_half(2.0 ** -26) == 0.0
+
+
+def test_scaled_three_term_sweep_is_the_two_term_sweep():
+    """smooth_scaled on H in the scaled variables equals smooth (Jacobi-Chebyshev, two-term)
+    on the unscaled operator diag(1/s) H diag(1/s) with D^-1 = s^2, from zero and from a
+    non-zero start"""
+    A = jittered_spd(30, 1e-2, seed=5)
+    s, H = ps.half_operator(A)
+    Sp = sp.diags(1 / s) @ H @ sp.diags(1 / s)
+    rng = np.random.default_rng(7)
+    r, x0 = rng.standard_normal(30), rng.standard_normal(30)
+    lmax = float(np.max(np.linalg.eigvalsh((sp.diags(s) @ Sp @ sp.diags(s)).toarray())))
+    for steps, div in ((16, 400.0), (9, ps.finest_div(9))):
+        for start in (np.zeros(30), x0):
+            two = ps.smooth(Sp, s * s, r, start, lmax, steps, div)
+            three = s * ps.smooth_scaled(H, s * r, start / s, lmax, steps, div)
+            assert np.abs(two - three).max() <= 1e-10 * np.abs(two).max()
