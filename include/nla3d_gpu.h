@@ -159,14 +159,14 @@ typedef struct {
      solve at krylov_tol.  The converged state is set by the residual criterion either way. */
   double krylov_tol_loose;
   double krylov_forcing;
-  /* 1: the first solve of a load step starts from the previous load step's increment (scaled by
-     the ratio of the time steps; the multipliers from their current values) instead of zero and
-     solves for the correction, to the same absolute accuracy (its tolerance is raised by
-     ||rhs|| / ||rhs - K2 x0||).  2: also the second solve of a load step, from the previous
-     load step's second increment p: x0 = alpha [p_u ; 0] + [0 ; U_l] (node and pressure rows
-     scaled by the alpha that minimises ||rhs - K2 x0||_2, the multipliers from their current
-     values), to the same absolute accuracy.  The history is the handle's: it restarts after
-     nla3d_gpu_set_solution.  0 (the default): start from zero. */
+  /* 1: the first solve of a load step starts from a prediction instead of zero and solves for
+     the correction, to the same absolute accuracy (its tolerance is raised by
+     ||rhs|| / ||rhs - K2 x0||): x0 = sum_i a_i [p_i,u ; 0] + [0 ; U_l] over the increments p_i of
+     the last two load steps (node and pressure rows; the multipliers from their current
+     values), the a_i minimising ||rhs - K2 x0||_2.  2: also the second solve of a load step,
+     the same way from the last two load steps' second Newton increments.  The history is the
+     handle's: it restarts after nla3d_gpu_set_solution and when a load step does not follow the
+     last one.  0 (the default): start from zero. */
   int first_solve_predictor;
 } nla3d_gpu_newton_opts;
 

@@ -339,8 +339,8 @@ int nla3d_gpu_set_solution(nla3d_gpu_t h, const double* U) {
                "copy Uc");
     hipCheck(hipStreamSynchronize(c.stream), "sync");
     c.lsValid = false;  // the predictors' history no longer describes this state
-    c.it2Valid = false;
-    c.lsStartT = c.lsTargetT = 0.0;
+    c.predN[0] = c.predN[1] = 0;
+    c.lsTargetT = 0.0;
   });
 }
 

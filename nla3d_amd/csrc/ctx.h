@@ -136,10 +136,12 @@ struct Ctx {
   // starting point and the right-hand side of the correction
   DevBuf d_lsStart, d_x0, d_rhsC;
   bool lsValid = false;
-  // the later-solve predictor: the last load step's second increment (node and pressure rows)
-  DevBuf d_it2, d_q, d_kq;
-  bool it2Valid = false;
-  double lsStartT = 0.0, lsTargetT = 0.0;
+  // the predictors' history: for the first (j = 0) and second (j = 1) solve of a load step, the
+  // increments of the last two load steps (whole load step for j = 0, the second Newton
+  // increment for j = 1), newest at predHead[j]; work space of the least-squares fit
+  DevBuf d_pred[2][2], d_q, d_kq, d_pw[2], d_yw[2];
+  int predN[2] = {0, 0}, predHead[2] = {0, 0};
+  double lsTargetT = 0.0;  // the time of the last load step the predictors saw
   DevBuf d_mpcMirIdx;         // block(2) upper indices of MPC entries
   int nMpcMir = 0;
   // learned block-Jacobi correction (learned.cpp)
@@ -202,9 +204,14 @@ struct Ctx {
   Precond precond();
   // solve block(2) x = b with the current K; b, x device pointers [n2]; returns iterations
   // trueAbs > 0 also stops, at the end of a 12-iteration group, once ||b - K2 x||_2 <= trueAbs
-  // (*trueStop then set); the preconditioned test stays in force
+  // (*trueStop then set); the preconditioned test stays in force.  beta1Ref > 0: tol is taken
+  // relative to beta1Ref instead of b's own preconditioned norm beta1 = sqrt(b^T M b) (the
+  // relative tolerance becomes tol * beta1Ref / beta1, at most 0.1)
   int minresSolve(const double* b, double* x, double tol, int maxit, double* relres_est, bool* breakdown,
-                  const PrecondHook* hook = nullptr, double trueAbs = 0.0, bool* trueStop = nullptr);
+                  const PrecondHook* hook = nullptr, double trueAbs = 0.0, bool* trueStop = nullptr,
+                  double beta1Ref = 0.0);
+  // sqrt(b^T M b) for the current preconditioner M (block Jacobi, and the hook's stage)
+  double minresBeta1(const double* b, const PrecondHook* hook = nullptr);
   // multigrid on the Schur rows (schur.cpp)
   std::shared_ptr<SchurAmg> schur;
 };

@@ -23,16 +23,17 @@ void vec_sumsq(hipStream_t s, int n, const double* x, double* partial, double* o
 // the first solve's starting point at a new load step: x0 = [scale (U_s - Ustart) ; U_l] over the
 // block(2) unknowns (free DoFs, then the multipliers, which the solve replaces), then
 // Ustart = U[nC, nC + n2); with predict = false only the copy
-void newton_predictor(hipStream_t s, int nC, int nU, int nMpc, const double* U, double* Ustart, double scale,
-                      bool predict, double* x0);
+void newton_predictor(hipStream_t s, int nC, int nU, int nMpc, const double* U, double* Ustart, bool record,
+                      double* inc);
 // y += x over n entries; y = b - y
 void vec_add(hipStream_t s, int n, const double* x, double* y);
-// the later-solve predictor's pieces: P = [p_u ; 0], Q = [0 ; U_l]; then x0 = alpha P + Q (over
-// P) and bc = r0 - alpha KP (over KP)
+// the predictors' pieces: P = [p_u ; 0] and (Q non-null) Q = [0 ; U_l]; then x0 = Q + a1 P1 +
+// a2 P2 and bc = R0 - a1 Y1 - a2 Y2 (P2 = Y2 = null: one vector)
 void newton_pred_split(hipStream_t s, int nC, int nU, int nMpc, const double* p, const double* U, double* P,
                        double* Q);
-void newton_pred_combine(hipStream_t s, int n, double alpha, double* P, const double* Q, const double* r0,
-                         double* KP);
+void newton_pred_combine(hipStream_t s, int n, double a1, const double* P1, const double* Y1, double a2,
+                         const double* P2, const double* Y2, const double* Q, const double* R0, double* x0,
+                         double* bc);
 void vec_rsub(hipStream_t s, int n, const double* b, double* y);
 
 }  // namespace nla3d_amd

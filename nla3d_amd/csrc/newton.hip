@@ -168,34 +168,45 @@ void newton_lambda_vec(hipStream_t s, int nC, int nU, int nMpc, const double* U,
   hipLaunchKernelGGL(lambda_vec_kernel, dim3(cdiv(std::max(nU + nMpc, 1), 256)), dim3(256), 0, s, nC, nU, nMpc, U, v);
 }
 
+// the load step's start: inc = U_sl - Ustart on the node and pressure rows (0 on the
+// multipliers) when record, then Ustart = U_sl
 __global__ __launch_bounds__(256) void predictor_kernel(int nC, int nU, int nMpc, const double* __restrict__ U,
-                                                       double* __restrict__ Ustart, double scale, int predict,
-                                                       double* __restrict__ x0) {
+                                                       double* __restrict__ Ustart, int record,
+                                                       double* __restrict__ inc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nU + nMpc) return;
   const double u = U[nC + i];
-  if (predict) x0[i] = i < nU ? scale * (u - Ustart[i]) : u;
+  if (record) inc[i] = i < nU ? u - Ustart[i] : 0.0;
   Ustart[i] = u;
 }
 
-// later-solve predictor: P = [p_u ; 0], Q = [0 ; U_l] (the multipliers' current values)
+// predictor basis: P = [p_u ; 0] and, with Q, Q = [0 ; U_l] (the multipliers' current values)
 __global__ __launch_bounds__(256) void pred_split_kernel(int nC, int nU, int nMpc, const double* __restrict__ p,
                                                         const double* __restrict__ U, double* __restrict__ P,
                                                         double* __restrict__ Q) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nU + nMpc) return;
   P[i] = i < nU ? p[i] : 0.0;
-  Q[i] = i < nU ? 0.0 : U[nC + i];
+  if (Q) Q[i] = i < nU ? 0.0 : U[nC + i];
 }
 
-// x0 = alpha P + Q (over P) ; bc = r0 - alpha KP (over KP)
-__global__ __launch_bounds__(256) void pred_combine_kernel(int n, double alpha, double* __restrict__ P,
-                                                          const double* __restrict__ Q, const double* __restrict__ r0,
-                                                          double* __restrict__ KP) {
+// x0 = Q + a1 P1 + a2 P2 ; bc = R0 - a1 Y1 - a2 Y2 (P2, Y2 null: one vector)
+__global__ __launch_bounds__(256) void pred_combine_kernel(int n, double a1, const double* __restrict__ P1,
+                                                          const double* __restrict__ Y1, double a2,
+                                                          const double* __restrict__ P2,
+                                                          const double* __restrict__ Y2,
+                                                          const double* __restrict__ Q,
+                                                          const double* __restrict__ R0, double* __restrict__ x0,
+                                                          double* __restrict__ bc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  P[i] = alpha * P[i] + Q[i];
-  KP[i] = r0[i] - alpha * KP[i];
+  double x = Q[i] + a1 * P1[i], r = R0[i] - a1 * Y1[i];
+  if (P2) {
+    x += a2 * P2[i];
+    r -= a2 * Y2[i];
+  }
+  x0[i] = x;
+  bc[i] = r;
 }
 
 __global__ __launch_bounds__(256) void add_kernel(int n, const double* __restrict__ x, double* __restrict__ y) {
@@ -203,10 +214,10 @@ __global__ __launch_bounds__(256) void add_kernel(int n, const double* __restric
   if (i < n) y[i] += x[i];
 }
 
-void newton_predictor(hipStream_t s, int nC, int nU, int nMpc, const double* U, double* Ustart, double scale,
-                      bool predict, double* x0) {
+void newton_predictor(hipStream_t s, int nC, int nU, int nMpc, const double* U, double* Ustart, bool record,
+                      double* inc) {
   hipLaunchKernelGGL(predictor_kernel, dim3(cdiv(std::max(nU + nMpc, 1), 256)), dim3(256), 0, s, nC, nU, nMpc, U,
-                     Ustart, scale, predict ? 1 : 0, x0);
+                     Ustart, record ? 1 : 0, inc);
 }
 
 __global__ __launch_bounds__(256) void rsub_kernel(int n, const double* __restrict__ b, double* __restrict__ y) {
@@ -244,9 +255,11 @@ void newton_pred_split(hipStream_t s, int nC, int nU, int nMpc, const double* p,
                      P, Q);
 }
 
-void newton_pred_combine(hipStream_t s, int n, double alpha, double* P, const double* Q, const double* r0,
-                         double* KP) {
-  hipLaunchKernelGGL(pred_combine_kernel, dim3(cdiv(std::max(n, 1), 256)), dim3(256), 0, s, n, alpha, P, Q, r0, KP);
+void newton_pred_combine(hipStream_t s, int n, double a1, const double* P1, const double* Y1, double a2,
+                         const double* P2, const double* Y2, const double* Q, const double* R0, double* x0,
+                         double* bc) {
+  hipLaunchKernelGGL(pred_combine_kernel, dim3(cdiv(std::max(n, 1), 256)), dim3(256), 0, s, n, a1, P1, Y1, a2, P2, Y2,
+                     Q, R0, x0, bc);
 }
 
 }  // namespace nla3d_amd

@@ -518,8 +518,27 @@ Precond Ctx::precond() {
   return P;
 }
 
+double Ctx::minresBeta1(const double* b, const PrecondHook* hook) {
+  MinresWork w;
+  w.X = d_mw[6].as<double>();  // (zeroed by the start; overwritten by the solve that follows)
+  w.R1 = d_mw[0].as<double>();
+  w.R2 = d_mw[1].as<double>();
+  w.T = d_mw[2].as<double>();
+  w.Yp = d_mw[3].as<double>();
+  w.Z = d_mw[4].as<double>();
+  w.W1 = d_mw[5].as<double>();
+  w.W2 = d_mw[6].as<double>();
+  w.partial = d_partial.as<double>();
+  w.st = d_mstate.as<MinresState>();
+  minres_start(stream, krylovMatrix(), precond(), w, b, hook);
+  hipCheck(hipGetLastError(), "minres start");
+  hipCheck(hipMemcpyAsync(h_mstate, w.st, sizeof(MinresState), hipMemcpyDeviceToHost, stream), "state");
+  hipCheck(hipStreamSynchronize(stream), "minres sync");
+  return h_mstate->beta1;
+}
+
 int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* relres_est, bool* breakdown,
-                     const PrecondHook* hook, double trueAbs, bool* trueStop) {
+                     const PrecondHook* hook, double trueAbs, bool* trueStop, double beta1Ref) {
   KrylovMatrix M = krylovMatrix();
   Precond P = precond();
   MinresWork w;
@@ -535,6 +554,11 @@ int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* 
   w.st = d_mstate.as<MinresState>();
   minres_start(stream, M, P, w, b, hook);
   hipCheck(hipGetLastError(), "minres start");
+  if (beta1Ref > 0.0) {  // the tolerance relative to beta1Ref
+    hipCheck(hipMemcpyAsync(h_mstate, w.st, sizeof(MinresState), hipMemcpyDeviceToHost, stream), "state");
+    hipCheck(hipStreamSynchronize(stream), "minres sync");
+    if (h_mstate->beta1 > 0.0) tol = std::min(0.1, tol * beta1Ref / h_mstate->beta1);
+  }
   // The host rotates the work-vector pointers with period 6, so 12 iterations form a
   // launch sequence with fixed arguments: capture it once per solve and replay it;
   // iterations after convergence do nothing (every kernel tests the state).
@@ -621,6 +645,72 @@ void computeReactions(Ctx& c, hipStream_t stream) {
                    c.d_dU.as<double>(), c.d_vals.as<double>() + m.nnzAll(), c.d_R.as<double>());
 }
 
+// beta1 of the last MINRES start (the host copy of its state)
+static double h_beta1(Ctx& c) { return c.h_mstate->beta1; }
+
+// the predictors' device buffers (n2 doubles each), allocated on first use
+static void predictorBuffers(Ctx& c) {
+  const size_t nb = (size_t)c.n2 * 8;
+  if (c.d_lsStart.bytes >= nb) return;
+  for (DevBuf* d : {&c.d_lsStart, &c.d_x0, &c.d_rhsC, &c.d_q, &c.d_kq, &c.d_pw[0], &c.d_pw[1], &c.d_yw[0], &c.d_yw[1],
+                    &c.d_pred[0][0], &c.d_pred[0][1], &c.d_pred[1][0], &c.d_pred[1][1]})
+    d->alloc(nb);
+  c.lsValid = false;
+  c.predN[0] = c.predN[1] = 0;
+}
+
+// the vector just written at d_pred[j][predHead[j] ^ 1] becomes the newest of history j
+static void pushHistory(Ctx& c, int j) {
+  c.predHead[j] ^= 1;
+  c.predN[j] = std::min(2, c.predN[j] + 1);
+}
+
+// x0 = [0 ; U_l] + sum_i a_i [h_i,u ; 0] over history j (newest first, at most two vectors), the
+// a_i minimising ||b - K2 x0||_2 (normal equations; the older vector is dropped when the pair is
+// numerically dependent).  Leaves x0 in d_x0 and b - K2 x0 in d_rhsC; returns
+// ||b - K2 x0||_2 / ||b||_2 (0 when b = 0).
+static double lsPredict(Ctx& c, int j, const double* b) {
+  const int nC = (int)c.model.nConstrainedDofs, nU = (int)c.model.nUnknownDofs, nM = (int)c.model.nMpc(), n2 = c.n2;
+  const double* U = c.d_U.as<double>();
+  double* part = c.d_partial.as<double>();
+  double* sc = c.d_scal.as<double>();
+  double* Q = c.d_q.as<double>();
+  double* R0 = c.d_kq.as<double>();
+  int m = c.predN[j];
+  double* P[2] = {c.d_pw[0].as<double>(), c.d_pw[1].as<double>()};
+  double* Y[2] = {c.d_yw[0].as<double>(), c.d_yw[1].as<double>()};
+  for (int i = 0; i < m; i++) {
+    const double* h = c.d_pred[j][c.predHead[j] ^ i].as<double>();
+    newton_pred_split(c.stream, nC, nU, nM, h, U, P[i], i == 0 ? Q : nullptr);
+    krylov_spmv(c.stream, c.krylovMatrix(), P[i], Y[i]);
+  }
+  krylov_spmv(c.stream, c.krylovMatrix(), Q, R0);
+  vec_rsub(c.stream, n2, b, R0);  // R0 = b - K2 Q
+  auto dot = [&](const double* x, const double* y) {
+    krylov_dot(c.stream, n2, x, y, part, sc + 11);
+    return readScalar(c, 11);
+  };
+  const double bb = dot(b, b);
+  if (!(bb > 0.0)) return 0.0;
+  const double g00 = dot(Y[0], Y[0]), r0 = dot(Y[0], R0);
+  double a0 = g00 > 0.0 ? r0 / g00 : 0.0, a1 = 0.0;
+  if (m == 2) {
+    const double g01 = dot(Y[0], Y[1]), g11 = dot(Y[1], Y[1]), r1 = dot(Y[1], R0);
+    const double det = g00 * g11 - g01 * g01;
+    if (det > 1e-12 * g00 * g11) {
+      a0 = (r0 * g11 - r1 * g01) / det;
+      a1 = (g00 * r1 - g01 * r0) / det;
+    } else {
+      m = 1;
+    }
+  }
+  newton_pred_combine(c.stream, n2, a0, P[0], Y[0], a1, m == 2 ? P[1] : nullptr, m == 2 ? Y[1] : nullptr, Q, R0,
+                      c.d_x0.as<double>(), c.d_rhsC.as<double>());
+  hipCheck(hipGetLastError(), "predictor");
+  const double rr = dot(c.d_rhsC.as<double>(), c.d_rhsC.as<double>());
+  return std::sqrt(rr / bb);
+}
+
 // One equilibrium iteration of the Newton loop at normalised time t
 // (FESolver.cpp:443-490).  it is the 1-based iteration number within the load
 // step; rhs1 carries ||rhs|| of iteration 1.  Returns 1 when the criterion
@@ -683,83 +773,33 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
     if (it == 1 && o.krylov_tol_loose > 0.0) ktol = std::max(ktol, o.krylov_tol_loose);
     if (it > 1 && o.krylov_forcing > 0.0) trueAbs = o.krylov_forcing * o.convergence * rhs1;
   }
-  // the first solve of a load step from the previous load step's increment: solve for the
-  // correction of x0, with the tolerance raised so the absolute accuracy stays the same (a
-  // predictor that does not lower the residual is not used)
+  // predicted starts (first_solve_predictor): the first solve of a load step from the last two
+  // load steps' increments, the second from their second Newton increments, fitted by least
+  // squares (lsPredict); the correction of x0 is solved for to the same absolute accuracy in
+  // MINRES's own norm: its tolerance is taken relative to the preconditioned norm of rhs, not of
+  // rhs - K2 x0 (a fit that does not lower the residual is not used)
   const double* b = c.d_rhs.as<double>();
   bool predicted = false;
+  const int n2 = c.n2;
+  const int slot = it - 1;
+  const bool usePred = (it == 1 && o.first_solve_predictor >= 1) || (it == 2 && o.first_solve_predictor >= 2);
+  if (usePred) predictorBuffers(c);
   if (it == 1 && o.first_solve_predictor) {
-    const int n2 = c.n2;
-    if (c.d_lsStart.bytes < (size_t)n2 * 8) {
-      c.d_lsStart.alloc((size_t)n2 * 8);
-      c.d_x0.alloc((size_t)n2 * 8);
-      c.d_rhsC.alloc((size_t)n2 * 8);
-      c.lsValid = false;
-    }
-    const double dtPrev = c.lsTargetT - c.lsStartT;
-    const bool predict = c.lsValid && dtPrev > 0.0 && t > c.lsTargetT;
-    const double scale = predict ? (t - c.lsTargetT) / dtPrev : 0.0;
-    newton_predictor(c.stream, nC, nU, nM, U, c.d_lsStart.as<double>(), scale, predict, c.d_x0.as<double>());
-    c.lsStartT = c.lsTargetT;
+    // the last load step's increment (U_sl now minus at its start) enters the history when this
+    // load step follows it; otherwise the history restarts
+    const bool next = c.lsValid && t > c.lsTargetT;
+    if (!next) c.predN[0] = c.predN[1] = 0;
+    newton_predictor(c.stream, nC, nU, nM, U, c.d_lsStart.as<double>(), next,
+                     c.d_pred[0][c.predHead[0] ^ 1].as<double>());
+    if (next) pushHistory(c, 0);
     c.lsTargetT = t;
     c.lsValid = true;
-    if (predict) {
-      double* bc = c.d_rhsC.as<double>();
-      krylov_spmv(c.stream, c.krylovMatrix(), c.d_x0.as<double>(), bc);
-      vec_rsub(c.stream, n2, b, bc);  // bc = rhs - K2 x0
-      vec_sumsq(c.stream, n2, bc, c.d_partial.as<double>(), c.d_scal.as<double>() + 11);
-      vec_sumsq(c.stream, n2, b, c.d_partial.as<double>(), c.d_scal.as<double>() + 12);
-      hipCheck(hipGetLastError(), "predictor");
-      const double r0 = std::sqrt(readScalar(c, 11)), bn = std::sqrt(readScalar(c, 12));
-      if (r0 > 0.0 && r0 < bn) {
-        ktol = std::min(0.1, ktol * bn / r0);
-        b = bc;
-        predicted = true;
-      }
-    }
   }
-  // the second solve of a load step from the last load step's second increment p: x0 = alpha
-  // [p_u ; 0] + [0 ; U_l], alpha minimising ||rhs - K2 x0||_2, and the correction solved for
-  // to the same absolute accuracy, as above
-  const bool predict2 = it == 2 && o.first_solve_predictor >= 2;
-  if (predict2) {
-    const int n2 = c.n2;
-    if (c.d_it2.bytes < (size_t)n2 * 8) {
-      c.d_it2.alloc((size_t)n2 * 8);
-      c.d_q.alloc((size_t)n2 * 8);
-      c.d_kq.alloc((size_t)n2 * 8);
-      c.it2Valid = false;
-    }
-    if (c.d_x0.bytes < (size_t)n2 * 8) {
-      c.d_x0.alloc((size_t)n2 * 8);
-      c.d_rhsC.alloc((size_t)n2 * 8);
-    }
-    if (c.it2Valid) {
-      double* P = c.d_x0.as<double>();
-      double* KP = c.d_rhsC.as<double>();
-      double* Q = c.d_q.as<double>();
-      double* R0 = c.d_kq.as<double>();
-      double* part = c.d_partial.as<double>();
-      double* sc = c.d_scal.as<double>();
-      newton_pred_split(c.stream, nC, nU, nM, c.d_it2.as<double>(), U, P, Q);
-      krylov_spmv(c.stream, c.krylovMatrix(), P, KP);
-      krylov_spmv(c.stream, c.krylovMatrix(), Q, R0);
-      vec_rsub(c.stream, n2, b, R0);  // R0 = rhs - K2 Q
-      krylov_dot(c.stream, n2, R0, KP, part, sc + 11);
-      vec_sumsq(c.stream, n2, KP, part, sc + 12);
-      vec_sumsq(c.stream, n2, b, part, sc + 13);
-      hipCheck(hipGetLastError(), "predictor");
-      const double rk = readScalar(c, 11), kk = readScalar(c, 12), bn = std::sqrt(readScalar(c, 13));
-      const double alpha = kk > 0.0 ? rk / kk : 0.0;
-      newton_pred_combine(c.stream, n2, alpha, P, Q, R0, KP);  // P = x0, KP = rhs - K2 x0
-      vec_sumsq(c.stream, n2, KP, part, sc + 11);
-      hipCheck(hipGetLastError(), "predictor");
-      const double r0 = std::sqrt(readScalar(c, 11));
-      if (r0 > 0.0 && r0 < bn) {
-        ktol = std::min(0.1, ktol * bn / r0);
-        b = KP;
-        predicted = true;
-      }
+  if (usePred && c.predN[slot] > 0) {
+    const double rel = lsPredict(c, slot, b);
+    if (rel > 0.0 && rel < 1.0) {
+      b = c.d_rhsC.as<double>();
+      predicted = true;
     }
   }
   // solve K.block(2) dU_sl = rhs (FESolver.cpp:457)
@@ -777,21 +817,27 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
   if (learnedPc) learnedCorrect(c, t);
   double relres = 0;
   bool brk = false, trueStop = false;
+  const double* rhs = c.d_rhs.as<double>();
+  double beta1Ref = predicted ? c.minresBeta1(rhs, useSchur ? &hook : nullptr) : 0.0;
   int kit = c.minresSolve(b, c.d_dU.as<double>(), ktol, o.krylov_max_iter, &relres, &brk,
-                          useSchur ? &hook : nullptr, trueAbs, &trueStop);
-  if ((learnedPc || useSchur) && (brk || (relres > ktol && !trueStop))) {
+                          useSchur ? &hook : nullptr, trueAbs, &trueStop, beta1Ref);
+  const double reached = predicted ? std::min(0.1, ktol * beta1Ref / std::max(h_beta1(c), 1e-300)) : ktol;
+  if ((learnedPc || useSchur) && (brk || (relres > reached && !trueStop))) {
     // the learned corrections failed this solve: block Jacobi from a fresh start
     st.precond_fallbacks++;
     st.krylov_iterations += kit;
     krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
     brk = false;
+    if (predicted) beta1Ref = c.minresBeta1(rhs, nullptr);
     kit = c.minresSolve(b, c.d_dU.as<double>(), ktol, o.krylov_max_iter, &relres, &brk, nullptr,
-                        trueAbs, &trueStop);
+                        trueAbs, &trueStop, beta1Ref);
   }
   if (predicted) vec_add(c.stream, c.n2, c.d_x0.as<double>(), c.d_dU.as<double>());  // dU = x0 + correction
-  if (predict2) {
-    hipCheck(hipMemcpyAsync(c.d_it2.p, c.d_dU.p, (size_t)c.n2 * 8, hipMemcpyDeviceToDevice, c.stream), "it2");
-    c.it2Valid = true;
+  if (it == 2 && o.first_solve_predictor >= 2) {  // this second increment into the history
+    hipCheck(hipMemcpyAsync(c.d_pred[1][c.predHead[1] ^ 1].p, c.d_dU.p, (size_t)n2 * 8, hipMemcpyDeviceToDevice,
+                            c.stream),
+             "predictor history");
+    pushHistory(c, 1);
   }
   st.krylov_iterations += kit;
   st.linear_solves++;
