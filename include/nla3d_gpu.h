@@ -160,11 +160,12 @@ typedef struct {
   double krylov_tol_loose;
   double krylov_forcing;
   /* 1: the first solve of a load step starts from a prediction instead of zero and solves for
-     the correction, to the same absolute accuracy (its tolerance is raised by
-     ||rhs|| / ||rhs - K2 x0||): x0 = sum_i a_i [p_i,u ; 0] + [0 ; U_l] over the increments p_i of
-     the last two load steps (node and pressure rows; the multipliers from their current
+     the correction to the same absolute accuracy in MINRES's norm (its tolerance is taken
+     relative to sqrt(rhs^T M rhs), M the preconditioner, instead of the correction's right-hand
+     side): x0 = sum_i a_i [p_i,u ; 0] + [0 ; U_l] over the increments p_i of
+     the last three load steps (node and pressure rows; the multipliers from their current
      values), the a_i minimising ||rhs - K2 x0||_2.  2: also the second solve of a load step,
-     the same way from the last two load steps' second Newton increments.  The history is the
+     the same way from the last three load steps' second Newton increments.  The history is the
      handle's: it restarts after nla3d_gpu_set_solution and when a load step does not follow the
      last one.  0 (the default): start from zero. */
   int first_solve_predictor;

@@ -137,9 +137,10 @@ struct Ctx {
   DevBuf d_lsStart, d_x0, d_rhsC;
   bool lsValid = false;
   // the predictors' history: for the first (j = 0) and second (j = 1) solve of a load step, the
-  // increments of the last two load steps (whole load step for j = 0, the second Newton
-  // increment for j = 1), newest at predHead[j]; work space of the least-squares fit
-  DevBuf d_pred[2][2], d_q, d_kq, d_pw[2], d_yw[2];
+  // increments of the last kPredDepth load steps (whole load step for j = 0, the second Newton
+  // increment for j = 1) in a ring, newest at predHead[j]; work space of the least-squares fit
+  static constexpr int kPredDepth = 3;
+  DevBuf d_pred[2][kPredDepth], d_q, d_kq, d_pw[kPredDepth], d_yw[kPredDepth];
   int predN[2] = {0, 0}, predHead[2] = {0, 0};
   double lsTargetT = 0.0;  // the time of the last load step the predictors saw
   DevBuf d_mpcMirIdx;         // block(2) upper indices of MPC entries

@@ -27,13 +27,12 @@ void newton_predictor(hipStream_t s, int nC, int nU, int nMpc, const double* U, 
                       double* inc);
 // y += x over n entries; y = b - y
 void vec_add(hipStream_t s, int n, const double* x, double* y);
-// the predictors' pieces: P = [p_u ; 0] and (Q non-null) Q = [0 ; U_l]; then x0 = Q + a1 P1 +
-// a2 P2 and bc = R0 - a1 Y1 - a2 Y2 (P2 = Y2 = null: one vector)
+// the predictors' pieces: P = [p_u ; 0] and (Q non-null) Q = [0 ; U_l]; then x0 += a P and
+// bc -= a Y per basis vector, the first call (Q, R0 non-null) starting from x0 = Q, bc = R0
 void newton_pred_split(hipStream_t s, int nC, int nU, int nMpc, const double* p, const double* U, double* P,
                        double* Q);
-void newton_pred_combine(hipStream_t s, int n, double a1, const double* P1, const double* Y1, double a2,
-                         const double* P2, const double* Y2, const double* Q, const double* R0, double* x0,
-                         double* bc);
+void newton_pred_axpy(hipStream_t s, int n, double a, const double* P, const double* Y, const double* Q,
+                      const double* R0, double* x0, double* bc);
 void vec_rsub(hipStream_t s, int n, const double* b, double* y);
 
 }  // namespace nla3d_amd

@@ -190,23 +190,16 @@ __global__ __launch_bounds__(256) void pred_split_kernel(int nC, int nU, int nMp
   if (Q) Q[i] = i < nU ? 0.0 : U[nC + i];
 }
 
-// x0 = Q + a1 P1 + a2 P2 ; bc = R0 - a1 Y1 - a2 Y2 (P2, Y2 null: one vector)
-__global__ __launch_bounds__(256) void pred_combine_kernel(int n, double a1, const double* __restrict__ P1,
-                                                          const double* __restrict__ Y1, double a2,
-                                                          const double* __restrict__ P2,
-                                                          const double* __restrict__ Y2,
-                                                          const double* __restrict__ Q,
-                                                          const double* __restrict__ R0, double* __restrict__ x0,
-                                                          double* __restrict__ bc) {
+// x0 += a P ; bc -= a Y (one basis vector of the predictor at a time; the first call starts
+// from x0 = Q, bc = R0)
+__global__ __launch_bounds__(256) void pred_axpy_kernel(int n, double a, const double* __restrict__ P,
+                                                       const double* __restrict__ Y, const double* __restrict__ Q,
+                                                       const double* __restrict__ R0, double* __restrict__ x0,
+                                                       double* __restrict__ bc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double x = Q[i] + a1 * P1[i], r = R0[i] - a1 * Y1[i];
-  if (P2) {
-    x += a2 * P2[i];
-    r -= a2 * Y2[i];
-  }
-  x0[i] = x;
-  bc[i] = r;
+  x0[i] = (Q ? Q[i] : x0[i]) + a * P[i];
+  bc[i] = (R0 ? R0[i] : bc[i]) - a * Y[i];
 }
 
 __global__ __launch_bounds__(256) void add_kernel(int n, const double* __restrict__ x, double* __restrict__ y) {
@@ -255,11 +248,9 @@ void newton_pred_split(hipStream_t s, int nC, int nU, int nMpc, const double* p,
                      P, Q);
 }
 
-void newton_pred_combine(hipStream_t s, int n, double a1, const double* P1, const double* Y1, double a2,
-                         const double* P2, const double* Y2, const double* Q, const double* R0, double* x0,
-                         double* bc) {
-  hipLaunchKernelGGL(pred_combine_kernel, dim3(cdiv(std::max(n, 1), 256)), dim3(256), 0, s, n, a1, P1, Y1, a2, P2, Y2,
-                     Q, R0, x0, bc);
+void newton_pred_axpy(hipStream_t s, int n, double a, const double* P, const double* Y, const double* Q,
+                      const double* R0, double* x0, double* bc) {
+  hipLaunchKernelGGL(pred_axpy_kernel, dim3(cdiv(std::max(n, 1), 256)), dim3(256), 0, s, n, a, P, Y, Q, R0, x0, bc);
 }
 
 }  // namespace nla3d_amd

@@ -652,24 +652,32 @@ static double h_beta1(Ctx& c) { return c.h_mstate->beta1; }
 static void predictorBuffers(Ctx& c) {
   const size_t nb = (size_t)c.n2 * 8;
   if (c.d_lsStart.bytes >= nb) return;
-  for (DevBuf* d : {&c.d_lsStart, &c.d_x0, &c.d_rhsC, &c.d_q, &c.d_kq, &c.d_pw[0], &c.d_pw[1], &c.d_yw[0], &c.d_yw[1],
-                    &c.d_pred[0][0], &c.d_pred[0][1], &c.d_pred[1][0], &c.d_pred[1][1]})
-    d->alloc(nb);
+  for (DevBuf* d : {&c.d_lsStart, &c.d_x0, &c.d_rhsC, &c.d_q, &c.d_kq}) d->alloc(nb);
+  for (int i = 0; i < Ctx::kPredDepth; i++) {
+    c.d_pw[i].alloc(nb);
+    c.d_yw[i].alloc(nb);
+    c.d_pred[0][i].alloc(nb);
+    c.d_pred[1][i].alloc(nb);
+  }
   c.lsValid = false;
   c.predN[0] = c.predN[1] = 0;
 }
 
-// the vector just written at d_pred[j][predHead[j] ^ 1] becomes the newest of history j
+// history j's slot for its next vector, and the vector written there becoming the newest
+static double* nextHistory(Ctx& c, int j) {
+  return c.d_pred[j][(c.predHead[j] + 1) % Ctx::kPredDepth].as<double>();
+}
 static void pushHistory(Ctx& c, int j) {
-  c.predHead[j] ^= 1;
-  c.predN[j] = std::min(2, c.predN[j] + 1);
+  c.predHead[j] = (c.predHead[j] + 1) % Ctx::kPredDepth;
+  c.predN[j] = std::min(Ctx::kPredDepth, c.predN[j] + 1);
 }
 
-// x0 = [0 ; U_l] + sum_i a_i [h_i,u ; 0] over history j (newest first, at most two vectors), the
-// a_i minimising ||b - K2 x0||_2 (normal equations; the older vector is dropped when the pair is
-// numerically dependent).  Leaves x0 in d_x0 and b - K2 x0 in d_rhsC; returns
-// ||b - K2 x0||_2 / ||b||_2 (0 when b = 0).
+// x0 = [0 ; U_l] + sum_i a_i [h_i,u ; 0] over history j (newest first), the a_i minimising
+// ||b - K2 x0||_2: the normal equations by Cholesky, newest vector first, stopping at the first
+// vector numerically dependent on the newer ones.  Leaves x0 in d_x0 and b - K2 x0 in d_rhsC;
+// returns ||b - K2 x0||_2 / ||b||_2 (0 when b = 0).
 static double lsPredict(Ctx& c, int j, const double* b) {
+  constexpr int D = Ctx::kPredDepth;
   const int nC = (int)c.model.nConstrainedDofs, nU = (int)c.model.nUnknownDofs, nM = (int)c.model.nMpc(), n2 = c.n2;
   const double* U = c.d_U.as<double>();
   double* part = c.d_partial.as<double>();
@@ -677,10 +685,12 @@ static double lsPredict(Ctx& c, int j, const double* b) {
   double* Q = c.d_q.as<double>();
   double* R0 = c.d_kq.as<double>();
   int m = c.predN[j];
-  double* P[2] = {c.d_pw[0].as<double>(), c.d_pw[1].as<double>()};
-  double* Y[2] = {c.d_yw[0].as<double>(), c.d_yw[1].as<double>()};
+  double* P[D];
+  double* Y[D];
   for (int i = 0; i < m; i++) {
-    const double* h = c.d_pred[j][c.predHead[j] ^ i].as<double>();
+    P[i] = c.d_pw[i].as<double>();
+    Y[i] = c.d_yw[i].as<double>();
+    const double* h = c.d_pred[j][(c.predHead[j] - i + D) % D].as<double>();
     newton_pred_split(c.stream, nC, nU, nM, h, U, P[i], i == 0 ? Q : nullptr);
     krylov_spmv(c.stream, c.krylovMatrix(), P[i], Y[i]);
   }
@@ -692,20 +702,39 @@ static double lsPredict(Ctx& c, int j, const double* b) {
   };
   const double bb = dot(b, b);
   if (!(bb > 0.0)) return 0.0;
-  const double g00 = dot(Y[0], Y[0]), r0 = dot(Y[0], R0);
-  double a0 = g00 > 0.0 ? r0 / g00 : 0.0, a1 = 0.0;
-  if (m == 2) {
-    const double g01 = dot(Y[0], Y[1]), g11 = dot(Y[1], Y[1]), r1 = dot(Y[1], R0);
-    const double det = g00 * g11 - g01 * g01;
-    if (det > 1e-12 * g00 * g11) {
-      a0 = (r0 * g11 - r1 * g01) / det;
-      a1 = (g00 * r1 - g01 * r0) / det;
-    } else {
-      m = 1;
-    }
+  double G[D][D], g[D], L[D][D] = {};
+  for (int i = 0; i < m; i++) {
+    g[i] = dot(Y[i], R0);
+    for (int k = 0; k <= i; k++) G[i][k] = dot(Y[i], Y[k]);
   }
-  newton_pred_combine(c.stream, n2, a0, P[0], Y[0], a1, m == 2 ? P[1] : nullptr, m == 2 ? Y[1] : nullptr, Q, R0,
-                      c.d_x0.as<double>(), c.d_rhsC.as<double>());
+  int r = 0;  // Cholesky of G, rows in order, up to the first dependent vector
+  for (; r < m; r++) {
+    double d = G[r][r];
+    for (int k = 0; k < r; k++) {
+      double t = G[r][k];
+      for (int q = 0; q < k; q++) t -= L[r][q] * L[k][q];
+      L[r][k] = t / L[k][k];
+      d -= L[r][k] * L[r][k];
+    }
+    if (!(d > 1e-12 * G[r][r])) break;
+    L[r][r] = std::sqrt(d);
+  }
+  m = r;
+  if (m == 0) return 1.0;
+  double a[D];
+  for (int i = 0; i < m; i++) {  // L y = g, then L^T a = y
+    double t = g[i];
+    for (int k = 0; k < i; k++) t -= L[i][k] * a[k];
+    a[i] = t / L[i][i];
+  }
+  for (int i = m - 1; i >= 0; i--) {
+    double t = a[i];
+    for (int k = i + 1; k < m; k++) t -= L[k][i] * a[k];
+    a[i] = t / L[i][i];
+  }
+  for (int i = 0; i < m; i++)
+    newton_pred_axpy(c.stream, n2, a[i], P[i], Y[i], i == 0 ? Q : nullptr, i == 0 ? R0 : nullptr,
+                     c.d_x0.as<double>(), c.d_rhsC.as<double>());
   hipCheck(hipGetLastError(), "predictor");
   const double rr = dot(c.d_rhsC.as<double>(), c.d_rhsC.as<double>());
   return std::sqrt(rr / bb);
@@ -773,7 +802,7 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
     if (it == 1 && o.krylov_tol_loose > 0.0) ktol = std::max(ktol, o.krylov_tol_loose);
     if (it > 1 && o.krylov_forcing > 0.0) trueAbs = o.krylov_forcing * o.convergence * rhs1;
   }
-  // predicted starts (first_solve_predictor): the first solve of a load step from the last two
+  // predicted starts (first_solve_predictor): the first solve of a load step from the last
   // load steps' increments, the second from their second Newton increments, fitted by least
   // squares (lsPredict); the correction of x0 is solved for to the same absolute accuracy in
   // MINRES's own norm: its tolerance is taken relative to the preconditioned norm of rhs, not of
@@ -789,8 +818,7 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
     // load step follows it; otherwise the history restarts
     const bool next = c.lsValid && t > c.lsTargetT;
     if (!next) c.predN[0] = c.predN[1] = 0;
-    newton_predictor(c.stream, nC, nU, nM, U, c.d_lsStart.as<double>(), next,
-                     c.d_pred[0][c.predHead[0] ^ 1].as<double>());
+    newton_predictor(c.stream, nC, nU, nM, U, c.d_lsStart.as<double>(), next, nextHistory(c, 0));
     if (next) pushHistory(c, 0);
     c.lsTargetT = t;
     c.lsValid = true;
@@ -834,8 +862,7 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
   }
   if (predicted) vec_add(c.stream, c.n2, c.d_x0.as<double>(), c.d_dU.as<double>());  // dU = x0 + correction
   if (it == 2 && o.first_solve_predictor >= 2) {  // this second increment into the history
-    hipCheck(hipMemcpyAsync(c.d_pred[1][c.predHead[1] ^ 1].p, c.d_dU.p, (size_t)n2 * 8, hipMemcpyDeviceToDevice,
-                            c.stream),
+    hipCheck(hipMemcpyAsync(nextHistory(c, 1), c.d_dU.p, (size_t)n2 * 8, hipMemcpyDeviceToDevice, c.stream),
              "predictor history");
     pushHistory(c, 1);
   }
