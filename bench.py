@@ -515,6 +515,9 @@ def main():
     # system than this fraction (the spread of iteration counts between consecutive Newton
     # systems of this workload is about 3%)
     ap.add_argument("--judge-margin", type=float, default=0.05)
+    ap.add_argument("--no-judge", action="store_true",
+                    help="skip the judge's solves of the first warm-up system (for kernel traces of the timed "
+                         "iterations alone)")
     ap.add_argument("--cpu-sample", type=int, default=250000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -621,7 +624,7 @@ def main():
         for w in range(args.warmup):
             kit = one_solving_iteration()
             print(f"[bench] warmup iteration: {kit} Krylov iterations", file=sys.stderr, flush=True)
-            if w == 0 and base_pc == 2:
+            if w == 0 and base_pc == 2 and not args.no_judge:
                 # the system just solved, to the timed tolerance, with plain block Jacobi, with the
                 # Schur multigrid and (when trained) with the learned corrections beside it: the
                 # Krylov-iteration comparison of the metric; the network is kept only if it saves

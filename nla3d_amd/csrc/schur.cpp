@@ -58,7 +58,7 @@ struct AmgLevel {
   // the finest level's half-precision smoother operator (AmgHalf): FP16 slots in the layout
   // above, the compensated diagonal and s = D^-1/2
   bool half = false;
-  DevBuf hVal, hDia, hDiag, hS;
+  DevBuf hVal, hDia, hDiag, hS, hBlk;
   DevBuf r, x, res, da, db, t, start;
   double lmax = 1.0;
   AmgCsr csr() const { return AmgCsr{n, rowPtr.as<int>(), col.as<int>(), val.as<double>()}; }
@@ -72,7 +72,8 @@ struct AmgLevel {
     return AmgHalf{n, sOff.as<int>(), narrow ? nullptr : sCol.as<int>(),
                    narrow ? sCol16.as<unsigned short>() : nullptr, narrow ? sBase.as<int>() : nullptr,
                    hVal.as<unsigned short>(), patterned ? sCOff.as<int>() : nullptr, sDOff.as<int>(),
-                   patterned ? hDia.as<unsigned short>() : nullptr, hDiag.as<float>(), hS.as<double>(), nd, ldd};
+                   patterned ? hDia.as<unsigned short>() : nullptr, hDiag.as<float>(), hS.as<double>(), nd, ldd,
+                   hBlk.as<int>()};
   }
 };
 
@@ -285,6 +286,30 @@ static void allocLevel(AmgLevel& A, bool coarsest) {
   }
   A.sPos.upload(spos);
   if (A.half) {
+    // dispatch order of the 512-row blocks (AmgHalf::blk): blocks with an ELLPACK slice first,
+    // round-robin over the XCDs, then contiguous ranges of the others
+    const int nB = (A.n + 511) / 512;
+    std::vector<int> ell, plain;
+    for (int b = 0; b < nB; b++) {
+      bool any = false;
+      for (int sl = 8 * b; sl < std::min(nSl, 8 * b + 8); sl++) any = any || !patSlice[sl];
+      (any ? ell : plain).push_back(b);
+    }
+    std::vector<std::vector<int>> seq(8);
+    std::vector<int> cap(8);
+    for (int x = 0; x < 8; x++) cap[x] = nB / 8 + (x < nB % 8 ? 1 : 0);
+    int x = 0;
+    for (int b : ell) {
+      while ((int)seq[x].size() >= cap[x]) x = (x + 1) % 8;
+      seq[x].push_back(b);
+      x = (x + 1) % 8;
+    }
+    size_t p = 0;
+    for (int y = 0; y < 8; y++)
+      while ((int)seq[y].size() < cap[y]) seq[y].push_back(plain[p++]);
+    std::vector<int> blk(std::max(nB, 1));
+    for (int b = 0; b < nB; b++) blk[b] = seq[b % 8][b / 8];
+    A.hBlk.upload(blk);
     A.hVal.alloc((size_t)std::max(A.nSlots, 1) * 2);
     if (A.patterned) A.hDia.alloc((size_t)A.nd * A.ldd * 2);
     A.hDiag.alloc((size_t)std::max(A.n, 1) * 4);

@@ -98,6 +98,10 @@ struct AmgHalf {
   const float* diag;          // 1 + Delta_i
   const double* s;            // D^-1/2
   int nd, ldd;
+  // dispatch order of the product kernels' 512-row blocks: hardware workgroup b (on XCD b mod 8)
+  // takes block blk[b].  The blocks holding ELLPACK slices (the rows slower per row) come first
+  // on every XCD, spread over the eight, then each XCD takes one contiguous range of the rest
+  const int* blk;
 };
 
 // the half-precision copy of a level from its FP64 CSR values: s = sqrt(dinv), the FP16

@@ -70,37 +70,18 @@ __device__ __forceinline__ double from_half_bits(unsigned short b) {
   return (double)__builtin_bit_cast(_Float16, b);
 }
 
-// row i of S^' = H + diag(1 + Delta) times x, one thread per row (the layout of sell_dot).
-// kND > 0: the diagonal-form rows' loop over the kND - 1 offsets is unrolled, and all of its
-// loads (two FP16 entries and two x values per offset) are issued before the first product,
-// so a wave waits for memory once per row instead of once per pair of offsets
-template <bool kNarrow, int kND>
-__device__ __forceinline__ double half_dot(const AmgHalf& H, int i, const double* __restrict__ x) {
-  const int sl = i >> 6;
+// row i of S^' = H + diag(1 + Delta) times x, i in slice sl (the layout of sell_dot); sl is
+// the same for every lane of the wave, so the slice's descriptors are scalar loads
+template <bool kNarrow>
+__device__ __forceinline__ double half_dot(const AmgHalf& H, int i, int sl, const double* __restrict__ x) {
   double acc = (double)H.diag[i] * x[i];
   if (H.cOff && H.cOff[sl] < 0) {
     const unsigned short* __restrict__ u = H.dia + i;
-    if constexpr (kND > 0) {
-      unsigned short hu[kND - 1], hl[kND - 1];
-      double xu[kND - 1], xl[kND - 1];
-#pragma unroll
-      for (int k = 1; k < kND; k++) {
-        const int d = H.dOff[k];
-        const unsigned short* __restrict__ uk = u + (size_t)k * H.ldd;
-        hu[k - 1] = uk[0];
-        hl[k - 1] = uk[-d];
-        xu[k - 1] = x[i + d];
-        xl[k - 1] = x[i - d];
-      }
-#pragma unroll
-      for (int k = 0; k < kND - 1; k++) acc += from_half_bits(hu[k]) * xu[k] + from_half_bits(hl[k]) * xl[k];
-    } else {
 #pragma unroll 2
-      for (int k = 1; k < H.nd; k++) {
-        const int d = H.dOff[k];
-        const unsigned short* __restrict__ uk = u + (size_t)k * H.ldd;
-        acc += from_half_bits(uk[0]) * x[i + d] + from_half_bits(uk[-d]) * x[i - d];
-      }
+    for (int k = 1; k < H.nd; k++) {
+      const int d = H.dOff[k];
+      const unsigned short* __restrict__ uk = u + (size_t)k * H.ldd;
+      acc += from_half_bits(uk[0]) * x[i + d] + from_half_bits(uk[-d]) * x[i - d];
     }
     return acc;
   }
@@ -118,6 +99,58 @@ __device__ __forceinline__ double half_dot(const AmgHalf& H, int i, const double
     for (int k = 0; k < w; k++) acc += from_half_bits(v[k * 64]) * x[c[k * 64]];
   }
   return acc;
+}
+
+// unaligned loads of 2 FP16 entries or 2 doubles (the hardware takes them as one access)
+struct Pair64 {
+  double a, b;
+};
+template <typename T>
+__device__ __forceinline__ T load_any(const void* p) {
+  T v;
+  __builtin_memcpy(&v, p, sizeof(T));
+  return v;
+}
+
+// Two rows of S^' x per thread (256-thread workgroups), a wave covering the 128 rows of two
+// slices: where both slices are in diagonal form (and kND > 0), lane l takes rows 2l and
+// 2l + 1, so each FP16 load takes two neighbouring entries of a diagonal and each x load two
+// neighbouring values (half the loads of one row per thread); otherwise lane l takes row l of
+// each slice, one at a time, in that slice's own layout.  Rows j0, j1 (-1 past the matrix)
+// and their products a0, a1.
+template <bool kNarrow, int kND>
+__device__ __forceinline__ void half_dot_wave(const AmgHalf& H, const double* __restrict__ x, int& j0, double& a0,
+                                              int& j1, double& a1) {
+  // the workgroup's 512-row block from the dispatch order (AmgHalf::blk); the wave index is
+  // uniform, so it lives in a scalar register
+  const int w = __builtin_amdgcn_readfirstlane(H.blk[blockIdx.x] * 4 + ((int)threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int r0 = w * 128;
+  const int sA = 2 * w;
+  const bool pairForm = kND > 0 && H.cOff && r0 + 128 <= H.n && H.cOff[sA] < 0 && H.cOff[sA + 1] < 0;
+  if (pairForm) {
+    const int i = r0 + 2 * lane;
+    const Pair64 xc = load_any<Pair64>(x + i);
+    a0 = (double)H.diag[i] * xc.a;
+    a1 = (double)H.diag[i + 1] * xc.b;
+#pragma unroll
+    for (int k = 1; k < (kND > 0 ? kND : 1); k++) {
+      const int d = H.dOff[k];
+      const unsigned short* __restrict__ uk = H.dia + (size_t)k * H.ldd + i;
+      const unsigned int up = load_any<unsigned int>(uk);
+      const unsigned int lo = load_any<unsigned int>(uk - d);
+      const Pair64 xu = load_any<Pair64>(x + i + d), xl = load_any<Pair64>(x + i - d);
+      a0 += from_half_bits(up & 0xffffu) * xu.a + from_half_bits(lo & 0xffffu) * xl.a;
+      a1 += from_half_bits(up >> 16) * xu.b + from_half_bits(lo >> 16) * xl.b;
+    }
+    j0 = i;
+    j1 = i + 1;
+    return;
+  }
+  j0 = r0 + lane < H.n ? r0 + lane : -1;
+  j1 = r0 + 64 + lane < H.n ? r0 + 64 + lane : -1;
+  a0 = j0 >= 0 ? half_dot<kNarrow>(H, j0, sA, x) : 0.0;
+  a1 = j1 >= 0 ? half_dot<kNarrow>(H, j1, sA + 1, x) : 0.0;
 }
 
 __global__ __launch_bounds__(256) void half_scale_kernel(int n, const double* __restrict__ dinv,
@@ -194,34 +227,43 @@ __global__ __launch_bounds__(256) void half_start_kernel(int n, const int* __res
   }
 }
 
+// The three product kernels take two rows per thread (half_dot_wave).
 template <bool kNarrow, int kND>
 __global__ __launch_bounds__(256) void half_start_x_kernel(AmgHalf H, const double* __restrict__ x0,
                                                          const double* __restrict__ rhat, double c0,
                                                          double* __restrict__ x1, const MinresState* __restrict__ st) {
   if (st && st->converged) return;
-  const int i = xcd_block() * blockDim.x + threadIdx.x;
-  if (i >= H.n) return;
-  x1[i] = x0[i] + c0 * (rhat[i] - half_dot<kNarrow, kND>(H, i, x0));
+  int j[2];
+  double a[2];
+  half_dot_wave<kNarrow, kND>(H, x0, j[0], a[0], j[1], a[1]);
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+    if (j[r] >= 0) x1[j[r]] = x0[j[r]] + c0 * (rhat[j[r]] - a[r]);
 }
 
 // x^_(k+1) = x^_k + c1 (x^_k - x^_(k-1)) + c2 (r^ - S^' x^_k); xout may be xprev (each thread
-// reads its own entry of xprev before writing it)
+// reads its own entries of xprev before writing them)
 template <bool kNarrow, int kND>
 __global__ __launch_bounds__(256) void half_step_kernel(AmgHalf H, const double* __restrict__ xk, const double* xprev,
                                                       const double* __restrict__ rhat, double c1, double c2,
                                                       double* xout, const int* __restrict__ qrow,
                                                       double* __restrict__ z, const MinresState* __restrict__ st) {
   if (st && st->converged) return;
-  const int i = xcd_block() * blockDim.x + threadIdx.x;
-  if (i >= H.n) return;
-  const double ax = half_dot<kNarrow, kND>(H, i, xk);
-  const double xi = xk[i];
-  const double xp = xprev ? xprev[i] : 0.0;
-  const double xn = xi + c1 * (xi - xp) + c2 * (rhat[i] - ax);
-  if (z)
-    z[qrow ? qrow[i] : i] = H.s[i] * xn;
-  else
-    xout[i] = xn;
+  int j[2];
+  double a[2];
+  half_dot_wave<kNarrow, kND>(H, xk, j[0], a[0], j[1], a[1]);
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int q = j[r];
+    if (q < 0) continue;
+    const double xq = xk[q];
+    const double xp = xprev ? xprev[q] : 0.0;
+    const double xn = xq + c1 * (xq - xp) + c2 * (rhat[q] - a[r]);
+    if (z)
+      z[qrow ? qrow[q] : q] = H.s[q] * xn;
+    else
+      xout[q] = xn;
+  }
 }
 
 template <bool kNarrow, int kND>
@@ -229,11 +271,16 @@ __global__ __launch_bounds__(256) void half_residual_kernel(AmgHalf H, const dou
                                                           const double* __restrict__ rhat, double* __restrict__ t,
                                                           const MinresState* __restrict__ st) {
   if (st && st->converged) return;
-  const int i = xcd_block() * blockDim.x + threadIdx.x;
-  if (i >= H.n) return;
-  const double si = H.s[i];
-  const double th = rhat[i] - half_dot<kNarrow, kND>(H, i, x);
-  t[i] = si > 0.0 ? th / si : 0.0;
+  int j[2];
+  double a[2];
+  half_dot_wave<kNarrow, kND>(H, x, j[0], a[0], j[1], a[1]);
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int q = j[r];
+    if (q < 0) continue;
+    const double sq = H.s[q];
+    t[q] = sq > 0.0 ? (rhat[q] - a[r]) / sq : 0.0;
+  }
 }
 
 __global__ __launch_bounds__(256) void half_prolong_kernel(int n, const int* __restrict__ agg,
@@ -620,7 +667,7 @@ template <bool kNarrow, int kND>
 struct StartXLaunch {
   static void go(const AmgHalf& H, hipStream_t s, const double* x0, const double* rhat, double c0, double* x1,
                  const MinresState* st) {
-    hipLaunchKernelGGL((half_start_x_kernel<kNarrow, kND>), dim3(cdiv(H.n, 256)), dim3(256), 0, s, H, x0, rhat, c0,
+    hipLaunchKernelGGL((half_start_x_kernel<kNarrow, kND>), dim3(cdiv(H.n, 512)), dim3(256), 0, s, H, x0, rhat, c0,
                        x1, st);
   }
 };
@@ -628,7 +675,7 @@ template <bool kNarrow, int kND>
 struct StepLaunch {
   static void go(const AmgHalf& H, hipStream_t s, const double* xk, const double* xprev, const double* rhat, double c1,
                  double c2, double* xout, const int* qrow, double* z, const MinresState* st) {
-    hipLaunchKernelGGL((half_step_kernel<kNarrow, kND>), dim3(cdiv(H.n, 256)), dim3(256), 0, s, H, xk, xprev, rhat,
+    hipLaunchKernelGGL((half_step_kernel<kNarrow, kND>), dim3(cdiv(H.n, 512)), dim3(256), 0, s, H, xk, xprev, rhat,
                        c1, c2, xout, qrow, z, st);
   }
 };
@@ -636,7 +683,7 @@ template <bool kNarrow, int kND>
 struct ResidualLaunch {
   static void go(const AmgHalf& H, hipStream_t s, const double* x, const double* rhat, double* t,
                  const MinresState* st) {
-    hipLaunchKernelGGL((half_residual_kernel<kNarrow, kND>), dim3(cdiv(H.n, 256)), dim3(256), 0, s, H, x, rhat, t,
+    hipLaunchKernelGGL((half_residual_kernel<kNarrow, kND>), dim3(cdiv(H.n, 512)), dim3(256), 0, s, H, x, rhat, t,
                        st);
   }
 };
