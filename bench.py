@@ -721,7 +721,7 @@ def main():
             smoother["ms"] * smoother["launches_per_krylov_iteration"] > ms_spmv:
         # with the multigrid, the finest-level smoother steps take the largest share
         kname = "cheb_step_kernel<true>" if os.environ.get("NLA3D_GPU_SCHUR_FP32_SMOOTHER") == "1" \
-            else "half_step_kernel<true>"
+            else "half_step_kernel<true, 14>"
         roofline = {"bound": "hbm", "achieved": smoother["GBps"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": smoother["GBps"] / PEAK_HBM_GBS,
                     "traffic": profiled_traffic(kname, (nx, ny, nz)),
