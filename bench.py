@@ -369,6 +369,7 @@ def damper_mode(args):
         m.init_solution()
         opts = m.newton_defaults(criterion=0, convergence=1e-3, max_iterations=15, krylov_tol=1e-10,
                                  krylov_max_iter=20000, preconditioner=2)
+        opts.first_solve_predictor = 0 if args.no_predictor else args.predictor
         ne, _ = m.equation_numbers()
         eqs = np.array([int(ne[n - 1, 1]) for n in d["comps"][comp]]) - 1
         curve, newton, krylov = [0.0], 0, 0

@@ -137,9 +137,11 @@ struct Ctx {
   DevBuf d_lsStart, d_x0, d_rhsC;
   bool lsValid = false;
   // the predictors' history: for the first (j = 0) and second (j = 1) solve of a load step, the
-  // increments of the last kPredDepth load steps (whole load step for j = 0, the second Newton
-  // increment for j = 1) in a ring, newest at predHead[j]; work space of the least-squares fit
-  static constexpr int kPredDepth = 3;
+  // increments of the last load steps (whole load step for j = 0, the second Newton increment
+  // for j = 1) in a ring of kPredDepth, newest at predHead[j]; work space of the least-squares
+  // fit.  The fit uses the newest predDepth() of them (NLA3D_GPU_PREDICTOR_DEPTH, 1 .. kPredDepth,
+  // default 3)
+  static constexpr int kPredDepth = 4;
   DevBuf d_pred[2][kPredDepth], d_q, d_kq, d_pw[kPredDepth], d_yw[kPredDepth];
   int predN[2] = {0, 0}, predHead[2] = {0, 0};
   double lsTargetT = 0.0;  // the time of the last load step the predictors saw

@@ -664,24 +664,44 @@ bool schurRefresh(Ctx& c, std::string* why) {
   return true;
 }
 
+// the coarse level l + 1's cheb_start for its restriction to do, when that level is smoothed by
+// the one-level kernels (not the coarsest, not the one-workgroup range)
+static ChebStartNext startNext(SchurAmg& S, size_t l) {
+  ChebStartNext nx;
+  const size_t c = l + 1;
+  if (c + 1 >= S.L.size() || (int)c == S.fuseFrom) return nx;
+  AmgLevel& C = *S.L[c];
+  nx.dinv = C.dinv.as<double>();
+  nx.c0 = chebCoef(C.lmax, kStepsC, kLoDivC).c0;
+  nx.res = C.res.as<double>();
+  nx.d = C.da.as<double>();
+  nx.x = C.x.as<double>();
+  return nx;
+}
+
 // one V-cycle from level l.  At level 0 of the MINRES stage (rext != null) the right-hand side
 // is gathered from the Krylov vector rext and the result scattered into zext (the Schur rows,
 // S.qrow; st gates the write) inside the first and last passes; otherwise it is A.r -> A.x.
-static void vcycle(SchurAmg& S, hipStream_t s, size_t l, const double* rext = nullptr, double* zext = nullptr,
-                   const MinresState* st = nullptr) {
+// started: the restriction that produced A.r already did this level's cheb_start.  Returns
+// the last smoothing increment still to be added to A.x (the prolongation above adds it), or
+// null when A.x is complete.
+static const double* vcycle(SchurAmg& S, hipStream_t s, size_t l, const double* rext = nullptr,
+                            double* zext = nullptr, const MinresState* st = nullptr, bool started = false) {
   if (l > 0 && (int)l == S.fuseFrom) {
     launch_coarse_cycle(s, S.fused.as<FusedLevel>(), (int)(S.L.size() - l));
-    return;
+    return nullptr;
   }
   AmgLevel& A = *S.L[l];
   if (l + 1 == S.L.size()) {
     if (rext) launch_q_gather(s, S.nq, S.qrow.as<int>(), rext, st, A.r.as<double>());
     launch_dense_matvec(s, A.n, S.cinv.as<double>(), A.r.as<double>(), A.x.as<double>());
     if (zext) launch_q_scatter(s, S.nq, S.qrow.as<int>(), A.x.as<double>(), st, zext);
-    return;
+    return nullptr;
   }
   const int steps = l == 0 ? finestSteps(A.n) : kStepsC;
   const ChebCoef k = chebCoef(A.lmax, steps, l == 0 ? finestDiv(steps) : kLoDivC);
+  AmgLevel& C = *S.L[l + 1];
+  const ChebStartNext nx = startNext(S, l);
   if (A.half) {
     // the scaled three-term sweep (schur.h, AmgHalf): r^ in res, the iterates in da / db,
     // x^_(k+1) written over x^_(k-1); the post-smoother's last step writes z = s x^
@@ -700,11 +720,10 @@ static void vcycle(SchurAmg& S, hipStream_t s, size_t l, const double* rext = nu
       prev = cur;
       cur = out;
     }
-    AmgLevel& C = *S.L[l + 1];
     launch_half_residual_restrict(s, H, cur, rhat, A.t.as<double>(), A.nc, A.memPtr.as<int>(), A.mem.as<int>(),
-                                  C.r.as<double>(), st);
-    vcycle(S, s, l + 1);
-    launch_half_prolong(s, A.n, A.agg.as<int>(), C.x.as<double>(), sc, cur, st);
+                                  C.r.as<double>(), st, nx);
+    const double* pend = vcycle(S, s, l + 1, nullptr, nullptr, nullptr, nx.res != nullptr);
+    launch_half_prolong(s, A.n, A.agg.as<int>(), C.x.as<double>(), pend, sc, cur, st);
     prev = cur;
     cur = cur == xa ? xb : xa;
     launch_half_start_x(s, H, prev, rhat, k.c0, cur, st);
@@ -714,7 +733,7 @@ static void vcycle(SchurAmg& S, hipStream_t s, size_t l, const double* rext = nu
                        last ? (zext ? zext : A.x.as<double>()) : nullptr, st);
       std::swap(prev, cur);
     }
-    return;
+    return nullptr;
   }
   const AmgSell M = A.sell();
   double* dinv = A.dinv.as<double>();
@@ -723,18 +742,17 @@ static void vcycle(SchurAmg& S, hipStream_t s, size_t l, const double* rext = nu
   if (rext)
     launch_cheb_start_gather(s, A.n, S.qrow.as<int>(), rext, dinv, k.c0, A.r.as<double>(), A.res.as<double>(), din,
                              A.x.as<double>());
-  else
+  else if (!started)
     launch_cheb_start(s, A.n, A.r.as<double>(), dinv, k.c0, A.res.as<double>(), din, A.x.as<double>());
   for (int i = 1; i < steps; i++) {
     launch_cheb_step(s, M, din, dinv, k.c1[i], k.c2[i], A.res.as<double>(), dout, A.x.as<double>(),
                      chebXMode(i, steps));
     std::swap(din, dout);
   }
-  AmgLevel& C = *S.L[l + 1];
   launch_residual_restrict(s, M, A.res.as<double>(), din, A.t.as<double>(), A.nc, A.memPtr.as<int>(),
-                           A.mem.as<int>(), C.r.as<double>(), A.x.as<double>());
-  vcycle(S, s, l + 1);
-  launch_prolong(s, A.n, A.agg.as<int>(), C.x.as<double>(), A.x.as<double>());
+                           A.mem.as<int>(), C.r.as<double>(), A.x.as<double>(), nx);
+  const double* pend = vcycle(S, s, l + 1, nullptr, nullptr, nullptr, nx.res != nullptr);
+  launch_prolong(s, A.n, A.agg.as<int>(), C.x.as<double>(), pend, A.x.as<double>());
   din = A.da.as<double>();
   dout = A.db.as<double>();
   launch_cheb_start_x(s, M, A.r.as<double>(), dinv, k.c0, A.res.as<double>(), din, A.x.as<double>());
@@ -743,10 +761,15 @@ static void vcycle(SchurAmg& S, hipStream_t s, size_t l, const double* rext = nu
                      chebXMode(i, steps));
     std::swap(din, dout);
   }
-  if (zext)
+  if (zext) {
     launch_finish_scatter(s, A.n, S.qrow.as<int>(), din, A.x.as<double>(), st, zext);
-  else
+    return nullptr;
+  }
+  if (l == 0) {  // the cycle's result in A.x
     launch_axpby(s, A.n, 1.0, din, 1.0, A.x.as<double>(), A.x.as<double>());
+    return nullptr;
+  }
+  return din;  // x + din: the prolongation above adds both
 }
 
 // the MINRES stage: z[Schur rows] = V-cycle(r[Schur rows]); partials of r . z

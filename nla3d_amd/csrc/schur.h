@@ -124,12 +124,23 @@ void launch_half_start_x(hipStream_t s, const AmgHalf& H, const double* x0, cons
 // With z, the step is the sweep's last: z[qrow[i]] (z[i] without qrow) = s_i x^_(k+1) instead.
 void launch_half_step(hipStream_t s, const AmgHalf& H, const double* xk, const double* xprev, const double* rhat,
                       double c1, double c2, double* xout, const int* qrow, double* z, const MinresState* st);
+// The coarse level's cheb_start done by the restriction that feeds it (res = r, d = c0 D^-1 r,
+// x = 0 for every coarse row; res == nullptr: none, the coarse level starts itself)
+struct ChebStartNext {
+  const double* dinv = nullptr;
+  double c0 = 0.0;
+  double* res = nullptr;
+  double* d = nullptr;
+  double* x = nullptr;
+};
 // t = (r^ - S^' x^) / s, the unscaled residual, then the restriction rc = P^T t
 void launch_half_residual_restrict(hipStream_t s, const AmgHalf& H, const double* x, const double* rhat, double* t,
-                                   int nc, const int* memPtr, const int* mem, double* rc, const MinresState* st);
-// x^ += xc[agg] / s (the prolongated coarse correction in the scaled variables)
-void launch_half_prolong(hipStream_t s, int n, const int* agg, const double* xc, const double* sc, double* x,
-                         const MinresState* st);
+                                   int nc, const int* memPtr, const int* mem, double* rc, const MinresState* st,
+                                   const ChebStartNext& nx);
+// x^ += (xc + dc)[agg] / s (the prolongated coarse correction in the scaled variables; dc: the
+// coarse level's last smoothing increment not yet added to xc, or null)
+void launch_half_prolong(hipStream_t s, int n, const int* agg, const double* xc, const double* dc, const double* sc,
+                         double* x, const MinresState* st);
 
 // level-0 assembly of S: one thread per (node, Schur column a) writes alpha B(:,a)^T D^-1 B(:,b)
 void launch_schur_contrib(hipStream_t s, int nPairs, const int* pairNode, const int* nodeGroup, const int* nodeSize,
@@ -163,8 +174,10 @@ inline int chebXMode(int i, int steps) { return (i & 1) ? (i + 1 < steps ? kCheb
 void launch_cheb_step(hipStream_t s, const AmgSell& A, const double* din, const double* dinv, double c1, double c2,
                       double* res, double* dout, double* x, int xmode);
 void launch_residual_restrict(hipStream_t s, const AmgSell& A, const double* res, const double* dlast, double* t,
-                              int nc, const int* memPtr, const int* mem, double* rc, double* x);
-void launch_prolong(hipStream_t s, int n, const int* agg, const double* xc, double* x);
+                              int nc, const int* memPtr, const int* mem, double* rc, double* x,
+                              const ChebStartNext& nx);
+// x += (xc + dc)[agg] (dc: the coarse level's pending last increment, or null)
+void launch_prolong(hipStream_t s, int n, const int* agg, const double* xc, const double* dc, double* x);
 void launch_dense_matvec(hipStream_t s, int n, const double* Ainv, const double* r, double* x);
 void launch_q_gather(hipStream_t s, int nq, const int* qrow, const double* v, const MinresState* st, double* out);
 void launch_q_scatter(hipStream_t s, int nq, const int* qrow, const double* v, const MinresState* st, double* out);

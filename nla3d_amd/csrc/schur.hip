@@ -284,12 +284,14 @@ __global__ __launch_bounds__(256) void half_residual_kernel(AmgHalf H, const dou
 }
 
 __global__ __launch_bounds__(256) void half_prolong_kernel(int n, const int* __restrict__ agg,
-                                                         const double* __restrict__ xc, const double* __restrict__ sc,
-                                                         double* __restrict__ x, const MinresState* __restrict__ st) {
+                                                         const double* __restrict__ xc, const double* __restrict__ dc,
+                                                         const double* __restrict__ sc, double* __restrict__ x,
+                                                         const MinresState* __restrict__ st) {
   if (st && st->converged) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double si = sc[i];
-    if (si > 0.0) x[i] += xc[agg[i]] / si;
+    const int a = agg[i];
+    if (si > 0.0) x[i] += (dc ? 1.0 * dc[a] + 1.0 * xc[a] : xc[a]) / si;
   }
 }
 
@@ -449,21 +451,33 @@ __global__ __launch_bounds__(256) void residual_kernel(AmgSell A, const double* 
 // 16th member in member order, then a fixed butterfly over the lanes (an aggregate holds ~25
 // rows, so one thread per aggregate left the launch latency-bound with few threads)
 constexpr int kRestrictLanes = 16;
+// (with nx.res: also the coarse level's cheb_start, res = r, d = c0 D^-1 r, x = 0)
 __global__ __launch_bounds__(256) void restrict_kernel(int nc, const int* __restrict__ memPtr,
                                                      const int* __restrict__ mem, const double* __restrict__ t,
-                                                     double* __restrict__ rc) {
+                                                     double* __restrict__ rc, ChebStartNext nx) {
   const int lane = threadIdx.x & (kRestrictLanes - 1);
   const int I = (blockIdx.x * blockDim.x + threadIdx.x) / kRestrictLanes;
   if (I >= nc) return;  // the 16 lanes of an aggregate leave together
   double s = 0.0;
   for (int k = memPtr[I] + lane; k < memPtr[I + 1]; k += kRestrictLanes) s += t[mem[k]];
   for (int o = kRestrictLanes / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kRestrictLanes);
-  if (lane == 0) rc[I] = s;
+  if (lane == 0) {
+    rc[I] = s;
+    if (nx.res) {
+      nx.res[I] = s;
+      nx.d[I] = nx.c0 * nx.dinv[I] * s;
+      nx.x[I] = 0.0;
+    }
+  }
 }
 
+// x += xc[agg] (+ dc[agg]: the coarse level's last smoothing increment, not yet added to xc)
 __global__ __launch_bounds__(256) void prolong_kernel(int n, const int* __restrict__ agg, const double* __restrict__ xc,
-                                                    double* __restrict__ x) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += xc[agg[i]];
+                                                    const double* __restrict__ dc, double* __restrict__ x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int a = agg[i];
+    x[i] += dc ? 1.0 * dc[a] + 1.0 * xc[a] : xc[a];
+  }
 }
 
 // x = Ainv r, one 64-lane wave per row (4 rows per block)
@@ -699,15 +713,16 @@ void launch_half_step(hipStream_t s, const AmgHalf& H, const double* xk, const d
 }
 
 void launch_half_residual_restrict(hipStream_t s, const AmgHalf& H, const double* x, const double* rhat, double* t,
-                                   int nc, const int* memPtr, const int* mem, double* rc, const MinresState* st) {
+                                   int nc, const int* memPtr, const int* mem, double* rc, const MinresState* st,
+                                   const ChebStartNext& nx) {
   half_dispatch<ResidualLaunch>(H, s, x, rhat, t, st);
   hipLaunchKernelGGL(restrict_kernel, dim3(cdiv((long long)nc * kRestrictLanes, 256)), dim3(256), 0, s, nc, memPtr,
-                     mem, t, rc);
+                     mem, t, rc, nx);
 }
 
-void launch_half_prolong(hipStream_t s, int n, const int* agg, const double* xc, const double* sc, double* x,
-                         const MinresState* st) {
-  hipLaunchKernelGGL(half_prolong_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, agg, xc, sc, x, st);
+void launch_half_prolong(hipStream_t s, int n, const int* agg, const double* xc, const double* dc, const double* sc,
+                         double* x, const MinresState* st) {
+  hipLaunchKernelGGL(half_prolong_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, agg, xc, dc, sc, x, st);
 }
 
 void launch_cheb_start(hipStream_t s, int n, const double* r, const double* dinv, double c0, double* res, double* d,
@@ -750,17 +765,18 @@ void launch_cheb_step(hipStream_t s, const AmgSell& A, const double* din, const 
 }
 
 void launch_residual_restrict(hipStream_t s, const AmgSell& A, const double* res, const double* dlast, double* t,
-                              int nc, const int* memPtr, const int* mem, double* rc, double* x) {
+                              int nc, const int* memPtr, const int* mem, double* rc, double* x,
+                              const ChebStartNext& nx) {
   if (A.col16)
     hipLaunchKernelGGL(residual_kernel<true>, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, res, dlast, t, x);
   else
     hipLaunchKernelGGL(residual_kernel<false>, dim3(cdiv(A.n, 256)), dim3(256), 0, s, A, res, dlast, t, x);
   hipLaunchKernelGGL(restrict_kernel, dim3(cdiv((long long)nc * kRestrictLanes, 256)), dim3(256), 0, s, nc, memPtr,
-                     mem, t, rc);
+                     mem, t, rc, nx);
 }
 
-void launch_prolong(hipStream_t s, int n, const int* agg, const double* xc, double* x) {
-  hipLaunchKernelGGL(prolong_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, agg, xc, x);
+void launch_prolong(hipStream_t s, int n, const int* agg, const double* xc, const double* dc, double* x) {
+  hipLaunchKernelGGL(prolong_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, agg, xc, dc, x);
 }
 
 void launch_coarse_cycle(hipStream_t s, const FusedLevel* lv, int nl) {

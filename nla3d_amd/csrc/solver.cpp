@@ -663,6 +663,16 @@ static void predictorBuffers(Ctx& c) {
   c.predN[0] = c.predN[1] = 0;
 }
 
+// how many of the newest history vectors the fit uses
+static int predDepth() {
+  static const int d = [] {
+    const char* e = std::getenv("NLA3D_GPU_PREDICTOR_DEPTH");
+    const int v = e ? std::atoi(e) : 3;
+    return std::max(1, std::min(Ctx::kPredDepth, v));
+  }();
+  return d;
+}
+
 // history j's slot for its next vector, and the vector written there becoming the newest
 static double* nextHistory(Ctx& c, int j) {
   return c.d_pred[j][(c.predHead[j] + 1) % Ctx::kPredDepth].as<double>();
@@ -684,7 +694,7 @@ static double lsPredict(Ctx& c, int j, const double* b) {
   double* sc = c.d_scal.as<double>();
   double* Q = c.d_q.as<double>();
   double* R0 = c.d_kq.as<double>();
-  int m = c.predN[j];
+  int m = std::min(c.predN[j], predDepth());
   double* P[D];
   double* Y[D];
   for (int i = 0; i < m; i++) {
