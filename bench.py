@@ -94,8 +94,9 @@ def profiled_traffic(kernel, shape):
     with open(path) as f:
         for line in f:
             parts = line.split()
-            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
-                return float(parts[2])
+            # traffic_bytes_per_launch <kernel name, may hold spaces> <bytes>
+            if len(parts) >= 3 and parts[0] == "traffic_bytes_per_launch" and " ".join(parts[1:-1]) == kernel:
+                return float(parts[-1])
     return None
 
 
@@ -721,8 +722,7 @@ def main():
     if args.mode == "newton" and smoother is not None and \
             smoother["ms"] * smoother["launches_per_krylov_iteration"] > ms_spmv:
         # with the multigrid, the finest-level smoother steps take the largest share
-        kname = "cheb_step_kernel<true>" if os.environ.get("NLA3D_GPU_SCHUR_FP32_SMOOTHER") == "1" \
-            else "half_step_kernel<true, 14>"
+        kname = "half_step_kernel<true, 14>" if m.schur_finest_precision() == 16 else "cheb_step_kernel<true>"
         roofline = {"bound": "hbm", "achieved": smoother["GBps"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": smoother["GBps"] / PEAK_HBM_GBS,
                     "traffic": profiled_traffic(kname, (nx, ny, nz)),

@@ -363,6 +363,10 @@ int nla3d_gpu_schur_matrix(nla3d_gpu_t h, int* qrow, int* rowPtr, int* col, doub
 int nla3d_gpu_schur_cycle_info(nla3d_gpu_t h, int* finest_steps, int* fuse_from);
 int nla3d_gpu_schur_level(nla3d_gpu_t h, int level, int* agg, double* lmax);
 int nla3d_gpu_schur_apply(nla3d_gpu_t h, const double* r, double* z, int refresh);
+/* the precision of the finest level's smoother operator: 16 (the half-precision operator, levels
+ * of at least 131,072 rows), 32 (an FP32 copy) or 64 (one level: its dense FP64 inverse);
+ * NLA3D_GPU_SCHUR_SMOOTHER=half / fp32 in the environment at set-up forces 16 / 32 */
+int nla3d_gpu_schur_finest_precision(nla3d_gpu_t h, int* bits);
 /* the finest level's stored smoother operator for the current tangent: s[nq] = D^-1/2 and, in
  * S's CSR pattern (val[nnz0]), the scaled entries as the smoother reads them (FP16 off the
  * diagonal, 1 + Delta on it); -3 when the finest level smooths with its FP32 copy */

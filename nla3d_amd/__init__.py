@@ -166,6 +166,7 @@ ABI.update({
     "nla3d_gpu_schur_cycle_info": [_P, _IP, _IP],
     "nla3d_gpu_schur_apply": [_P, _DP, _DP, ctypes.c_int],
     "nla3d_gpu_schur_smoother_operator": [_P, _DP, _DP],
+    "nla3d_gpu_schur_finest_precision": [_P, _IP],
     "nla3d_gpu_time_smoother": [_P, ctypes.c_int, _DP, ctypes.POINTER(ctypes.c_longlong)],
     "nla3d_gpu_get_R": [_P, _DP],
     "nla3d_gpu_solve_block2_pc": [_P, _DP, _DP, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, _IP,
@@ -470,6 +471,12 @@ class GpuModel:
         lm = ctypes.c_double()
         self._call("nla3d_gpu_schur_level", int(level), _ptr(agg, ctypes.c_int), ctypes.byref(lm))
         return agg[:n], lm.value
+
+    def schur_finest_precision(self):
+        """16 (half-precision finest smoother operator), 32 (FP32 copy) or 64 (one dense level)"""
+        b = ctypes.c_int()
+        self._call("nla3d_gpu_schur_finest_precision", ctypes.byref(b))
+        return b.value
 
     def schur_smoother_operator(self):
         """(s, CSR values in schur_matrix()'s pattern) of the finest level's stored half-precision

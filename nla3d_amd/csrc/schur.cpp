@@ -46,6 +46,7 @@ static int finestSteps(int n) {
 }
 static double finestDiv(int steps) { return 100.0 * (steps / 8.0) * (steps / 8.0); }
 constexpr double kLmaxSafety = 1.05;  // the estimate approaches lmax from below
+constexpr int kHalfRows = 1 << 17;    // finest levels at least this large smooth in half precision
 
 struct AmgLevel {
   int n = 0, nnz = 0, nc = 0, nSlots = 0, nColSlots = 0;
@@ -477,11 +478,16 @@ void setupSchur(Ctx& c, const std::vector<int64_t>& rp, const std::vector<int>& 
     S->L.clear();
     return;
   }
-  // the finest level smooths in half precision unless NLA3D_GPU_SCHUR_FP32_SMOOTHER=1 (its FP32
-  // copy, as the coarser levels)
+  // the finest level smooths in half precision when it has at least kHalfRows rows (below that
+  // its launches are bound by latency, not bytes, and the two-rows-per-thread kernels only add
+  // latency: the reference's 3d_damper, 7,513 Schur rows, took 15.9 s with the FP32 copy and
+  // 21.6 s with the half-precision one); NLA3D_GPU_SCHUR_SMOOTHER=half or =fp32 forces either
   {
-    const char* e = std::getenv("NLA3D_GPU_SCHUR_FP32_SMOOTHER");
-    S->L[0]->half = S->L.size() > 1 && !(e && e[0] == '1');
+    const char* e = std::getenv("NLA3D_GPU_SCHUR_SMOOTHER");
+    bool half = S->L[0]->n >= kHalfRows;
+    if (e && std::strcmp(e, "half") == 0) half = true;
+    if (e && std::strcmp(e, "fp32") == 0) half = false;
+    S->L[0]->half = S->L.size() > 1 && half;
   }
   for (size_t l = 0; l < S->L.size(); l++) allocLevel(*S->L[l], l + 1 == S->L.size());
   const int ncst = S->L.back()->n;
@@ -840,6 +846,14 @@ int nla3d_gpu_schur_cycle_info(nla3d_gpu_t h, int* finest_steps, int* fuse_from)
     if (!S || !S->available) throw ApiError(-3, "the Schur multigrid is unavailable for this model");
     if (finest_steps) *finest_steps = S->L.size() > 1 ? finestSteps(S->L[0]->n) : 0;
     if (fuse_from) *fuse_from = S->fuseFrom;
+  });
+}
+
+int nla3d_gpu_schur_finest_precision(nla3d_gpu_t h, int* bits) {
+  return schurGuard(h, [&](Ctx& c) {
+    SchurAmg* S = c.schur.get();
+    if (!S || !S->available) throw ApiError(-3, "the Schur multigrid is unavailable for this model");
+    if (bits) *bits = S->L[0]->half ? 16 : (S->L.size() > 1 ? 32 : 64);
   });
 }
 

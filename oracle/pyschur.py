@@ -12,7 +12,8 @@ the device path implements:
   coarsest level.  The coarser levels' smoother and residual products use the
   level's matrix with its entries rounded to FP32 (products and sums in FP64), the
   diagonal scaling and the coarse matrices the FP64 values.  The finest level of a
-  hierarchy with more than one level smooths with its half-precision operator
+  hierarchy with more than one level smooths either the same way or (finest_half,
+  the library's choice for large levels) with its half-precision operator
   (half_operator: the symmetrically scaled matrix with FP16 off-diagonal entries and
   a diagonal raised by their rounding errors) in the scaled variables, by the
   three-term form of the same polynomial (smooth_scaled).
@@ -157,16 +158,17 @@ def smooth_scaled(H, rhat, x0, lmax, steps, div):
     return x
 
 
-def vcycle(mats, aggs, lmaxs, r, level=0, half=None):
-    """one V-cycle from `level`; half = (s, H) replaces half_operator(mats[0]) on the finest
-    level (to run the cycle on a given stored operator)"""
+def vcycle(mats, aggs, lmaxs, r, level=0, half=None, finest_half=True):
+    """one V-cycle from `level`; finest_half: the finest level smooths with its half-precision
+    operator (else with its FP32-rounded matrix, as the coarser levels); half = (s, H) replaces
+    half_operator(mats[0]) (to run the cycle on a given stored operator)"""
     A = mats[level]
     if level == len(mats) - 1:
         return np.linalg.solve(A.toarray(), r)
     steps = finest_steps(A.shape[0]) if level == 0 else STEPSC
     div = finest_div(steps) if level == 0 else DIVC
     agg = aggs[level]
-    if level == 0:
+    if level == 0 and finest_half:
         s, H = half_operator(A) if half is None else half
         rhat = s * r
         x = smooth_scaled(H, rhat, np.zeros_like(r), lmaxs[0], steps, div)
@@ -174,12 +176,12 @@ def vcycle(mats, aggs, lmaxs, r, level=0, half=None):
         t = np.zeros_like(r)
         t[pos] = (rhat - H @ x)[pos] / s[pos]
         rc = np.bincount(agg, weights=t, minlength=mats[1].shape[0])
-        xc = vcycle(mats, aggs, lmaxs, rc, 1)[agg]
+        xc = vcycle(mats, aggs, lmaxs, rc, 1, finest_half=finest_half)[agg]
         x[pos] += xc[pos] / s[pos]
         return s * smooth_scaled(H, rhat, x, lmaxs[0], steps, div)
     dinv = 1.0 / A.diagonal()
     As = rounded_f32(A)
     x = smooth(As, dinv, r, np.zeros_like(r), lmaxs[level], steps, div)
     rc = np.bincount(agg, weights=r - As @ x, minlength=mats[level + 1].shape[0])
-    x = x + vcycle(mats, aggs, lmaxs, rc, level + 1)[agg]
+    x = x + vcycle(mats, aggs, lmaxs, rc, level + 1, finest_half=finest_half)[agg]
     return smooth(As, dinv, r, x, lmaxs[level], steps, div)

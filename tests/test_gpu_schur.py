@@ -83,9 +83,18 @@ def test_schur_matrix_matches_oracle(gpu_lib):
     assert np.linalg.eigvalsh(So).min() > 0
 
 
-def test_vcycle_matches_oracle_and_is_symmetric(gpu_lib):
-    print("[gpu] schur v-cycle", flush=True)
+@pytest.mark.parametrize("smoother", ["auto", "half"])
+def test_vcycle_matches_oracle_and_is_symmetric(gpu_lib, smoother, monkeypatch):
+    """auto: this small finest level smooths with its FP32 copy; half: forced to the
+    half-precision operator"""
+    print(f"[gpu] schur v-cycle ({smoother})", flush=True)
+    if smoother == "half":
+        monkeypatch.setenv("NLA3D_GPU_SCHUR_SMOOTHER", "half")
+    else:
+        monkeypatch.delenv("NLA3D_GPU_SCHUR_SMOOTHER", raising=False)
     m = rigid_model(8, 8, 4)
+    fh = m.schur_finest_precision() == 16
+    assert fh == (smoother == "half")
     info = m.schur_info()
     print("hierarchy", info, flush=True)
     assert info["levels"] >= 2
@@ -96,12 +105,13 @@ def test_vcycle_matches_oracle_and_is_symmetric(gpu_lib):
         aggs.append(agg.astype(np.int64))
         lmaxs.append(lm)
     mats = ps.hierarchy(S, aggs)
-    half = check_half_operator(m)
     rng = np.random.default_rng(3)
     y, z = rng.standard_normal(len(qrow)), rng.standard_normal(len(qrow))
     by, bz = m.schur_apply(y), m.schur_apply(z)
-    assert np.abs(by - ps.vcycle(mats, aggs, lmaxs, y, half=half)).max() <= 1e-12 * np.abs(by).max()
-    bo = ps.vcycle(mats, aggs, lmaxs, y)
+    if fh:
+        half = check_half_operator(m)
+        assert np.abs(by - ps.vcycle(mats, aggs, lmaxs, y, half=half)).max() <= 1e-12 * np.abs(by).max()
+    bo = ps.vcycle(mats, aggs, lmaxs, y, finest_half=fh)
     # the coarse levels' FP32 copies are rounded from values that the device sums by a gather
     # and the oracle by P^T A P: an entry an ulp apart can round to the neighbouring FP32
     # value, which moves the cycle by far less than 1e-9
@@ -164,14 +174,23 @@ def a2000_damper_model():
     return m
 
 
-@pytest.mark.parametrize("case", ["box_30x30x15", "a2000_damper"])
-def test_vcycle_matches_oracle_on_deep_hierarchies(gpu_lib, case):
+@pytest.mark.parametrize("case,smoother", [("box_30x30x15", "auto"), ("box_30x30x15", "half"),
+                                           ("a2000_damper", "auto"), ("a2000_damper", "half")])
+def test_vcycle_matches_oracle_on_deep_hierarchies(gpu_lib, case, smoother, monkeypatch):
     """The V-cycle against the oracle where the short tests do not reach: a 30x30x15
     rigid-top box (16,383 Schur rows: an odd, 9-term finest sweep, which ends on a step that
     leaves x alone) and the reference's a2000_damper model (PLANE41, 3,101 Schur rows: the
-    coarse levels below a smoothed one run as one workgroup launch, fuse_from >= 1)"""
-    print(f"[gpu] schur v-cycle, {case}", flush=True)
+    coarse levels below a smoothed one run as one workgroup launch, fuse_from >= 1); each with
+    the finest level's FP32 copy (the choice at these sizes) and forced to the half-precision
+    operator"""
+    print(f"[gpu] schur v-cycle, {case} ({smoother})", flush=True)
+    if smoother == "half":
+        monkeypatch.setenv("NLA3D_GPU_SCHUR_SMOOTHER", "half")
+    else:
+        monkeypatch.delenv("NLA3D_GPU_SCHUR_SMOOTHER", raising=False)
     m = rigid_model(30, 30, 15) if case == "box_30x30x15" else a2000_damper_model()
+    fh = m.schur_finest_precision() == 16
+    assert fh == (smoother == "half")
     info = m.schur_info()
     steps, fuse = m.schur_cycle_info()
     print("hierarchy", info, "finest steps", steps, "fused from", fuse, flush=True)
@@ -189,12 +208,13 @@ def test_vcycle_matches_oracle_on_deep_hierarchies(gpu_lib, case):
         lmaxs.append(lm)
     mats = ps.hierarchy(S, aggs)
     assert [A.shape[0] for A in mats] == info["sizes"]
-    half = check_half_operator(m)
     rng = np.random.default_rng(9)
     y, z = rng.standard_normal(len(qrow)), rng.standard_normal(len(qrow))
     by, bz = m.schur_apply(y), m.schur_apply(z)
-    assert np.abs(by - ps.vcycle(mats, aggs, lmaxs, y, half=half)).max() <= 1e-12 * np.abs(by).max()
-    bo = ps.vcycle(mats, aggs, lmaxs, y)
+    if fh:
+        half = check_half_operator(m)
+        assert np.abs(by - ps.vcycle(mats, aggs, lmaxs, y, half=half)).max() <= 1e-12 * np.abs(by).max()
+    bo = ps.vcycle(mats, aggs, lmaxs, y, finest_half=fh)
     assert np.abs(by - bo).max() <= 1e-9 * np.abs(bo).max()
     assert abs(z @ by - y @ bz) <= 1e-10 * abs(z @ by)
     assert y @ by > 0 and z @ bz > 0
