@@ -443,7 +443,7 @@ def thermal_mode(args):
     meshgen.quadth_transient_case(m, n)
     sizes = m.init_solution()
     opts = m.transient_defaults(time1=0.2, n_timesteps=200, init_value=0.0, dirichlet_coupling=1, krylov_tol=1e-10,
-                                krylov_max_iter=200000)
+                                krylov_max_iter=200000, predictor=0 if args.no_predictor else 1)
     m.transient_begin(opts)
     print(f"[bench] setup {time.perf_counter() - t0:.1f}s sizes {sizes}", file=sys.stderr, flush=True)
     st = nla3d_amd.NewtonStats()

@@ -433,6 +433,11 @@ typedef struct {
                                1: the coupling of prescribed temperatures is included */
   double krylov_tol;        /* CG: preconditioned residual <= tol * ||b||_M */
   int krylov_max_iter;
+  int predictor;            /* 1 (the default): each step's CG starts from the current unknowns
+                               plus the least-squares combination of the last three steps'
+                               increments (minimising ||b - A x0||_2); 0: from the current unknowns.
+                               The tolerance is relative to ||b||_M either way, so the accuracy is
+                               the same */
 } nla3d_gpu_transient_opts;
 
 /* FEStorage::assembleGlobalEqMatrices with transient matrices for QUADTH and

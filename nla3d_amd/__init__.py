@@ -43,7 +43,7 @@ class TransientOpts(ctypes.Structure):
     _fields_ = [("alpha", ctypes.c_double), ("delta", ctypes.c_double), ("time0", ctypes.c_double),
                 ("time1", ctypes.c_double), ("n_timesteps", ctypes.c_int), ("init_value", ctypes.c_double),
                 ("dirichlet_coupling", ctypes.c_int), ("krylov_tol", ctypes.c_double),
-                ("krylov_max_iter", ctypes.c_int)]
+                ("krylov_max_iter", ctypes.c_int), ("predictor", ctypes.c_int)]
 
 
 class NewtonStats(ctypes.Structure):

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -108,5 +109,39 @@ void launch_calibration_copy(hipStream_t s, size_t n, const double* a, double* b
 // Library slot k (0..349) of the SOLID81 kernel output -> index in the
 // reference assemble3 order (SOLID81.h:54-94).
 void solid81_slot_to_assemble3(int* map350);
+
+// Least squares by the normal equations for the predicted starts: given G[i][k] = y_i . y_k
+// (k <= i read) and g[i] = y_i . r for m <= kMaxLsBasis vectors, the a minimising
+// ||r - sum_i a_i y_i||_2.  Cholesky of G row by row, stopping at the first vector numerically
+// dependent on the earlier ones (so the earlier ones should be the more important); returns how
+// many vectors are used, the a_i beyond them are 0.
+constexpr int kMaxLsBasis = 4;
+inline int lsNormalSolve(const double G[kMaxLsBasis][kMaxLsBasis], const double* g, int m, double* a) {
+  double L[kMaxLsBasis][kMaxLsBasis] = {};
+  int r = 0;
+  for (; r < m; r++) {
+    double d = G[r][r];
+    for (int k = 0; k < r; k++) {
+      double t = G[r][k];
+      for (int q = 0; q < k; q++) t -= L[r][q] * L[k][q];
+      L[r][k] = t / L[k][k];
+      d -= L[r][k] * L[r][k];
+    }
+    if (!(d > 1e-12 * G[r][r])) break;
+    L[r][r] = std::sqrt(d);
+  }
+  for (int i = 0; i < kMaxLsBasis; i++) a[i] = 0.0;
+  for (int i = 0; i < r; i++) {  // L y = g, then L^T a = y
+    double t = g[i];
+    for (int k = 0; k < i; k++) t -= L[i][k] * a[k];
+    a[i] = t / L[i][i];
+  }
+  for (int i = r - 1; i >= 0; i--) {
+    double t = a[i];
+    for (int k = i + 1; k < r; k++) t -= L[k][i] * a[k];
+    a[i] = t / L[i][i];
+  }
+  return r;
+}
 
 }  // namespace nla3d_amd

@@ -712,36 +712,14 @@ static double lsPredict(Ctx& c, int j, const double* b) {
   };
   const double bb = dot(b, b);
   if (!(bb > 0.0)) return 0.0;
-  double G[D][D], g[D], L[D][D] = {};
+  static_assert(D <= kMaxLsBasis, "predictor history longer than the least-squares basis");
+  double G[kMaxLsBasis][kMaxLsBasis] = {}, g[kMaxLsBasis] = {}, a[kMaxLsBasis];
   for (int i = 0; i < m; i++) {
     g[i] = dot(Y[i], R0);
     for (int k = 0; k <= i; k++) G[i][k] = dot(Y[i], Y[k]);
   }
-  int r = 0;  // Cholesky of G, rows in order, up to the first dependent vector
-  for (; r < m; r++) {
-    double d = G[r][r];
-    for (int k = 0; k < r; k++) {
-      double t = G[r][k];
-      for (int q = 0; q < k; q++) t -= L[r][q] * L[k][q];
-      L[r][k] = t / L[k][k];
-      d -= L[r][k] * L[r][k];
-    }
-    if (!(d > 1e-12 * G[r][r])) break;
-    L[r][r] = std::sqrt(d);
-  }
-  m = r;
+  m = lsNormalSolve(G, g, m, a);
   if (m == 0) return 1.0;
-  double a[D];
-  for (int i = 0; i < m; i++) {  // L y = g, then L^T a = y
-    double t = g[i];
-    for (int k = 0; k < i; k++) t -= L[i][k] * a[k];
-    a[i] = t / L[i][i];
-  }
-  for (int i = m - 1; i >= 0; i--) {
-    double t = a[i];
-    for (int k = i + 1; k < m; k++) t -= L[k][i] * a[k];
-    a[i] = t / L[i][i];
-  }
   for (int i = 0; i < m; i++)
     newton_pred_axpy(c.stream, n2, a[i], P[i], Y[i], i == 0 ? Q : nullptr, i == 0 ? R0 : nullptr,
                      c.d_x0.as<double>(), c.d_rhsC.as<double>());

@@ -35,6 +35,11 @@ struct ThermalData {
   int curStep = 0;
   bool running = false;
   DevBuf DU, DDU, v, y, x;
+  // the predictor: the last steps' increments of the unknowns (a ring, newest at histHead),
+  // their products with the CG matrix, the residual of the current unknowns, a scratch vector
+  static constexpr int kHist = 3;
+  DevBuf hist[kHist], hy[kHist], r0, scratch;
+  int histN = 0, histHead = 0;
   DevBuf cgw[5], cgPart2, cgSt, cgSt0;
   CgState* hst = nullptr;
   // scalar-row CG copy (built on the first solve when every row is its own group)
@@ -321,6 +326,15 @@ void transientBegin(Ctx& c, const nla3d_gpu_transient_opts& o) {
   T.v.alloc((size_t)c.n2 * 8);
   T.y.alloc((size_t)c.n2 * 8);
   T.x.alloc((size_t)c.n2 * 8);
+  T.histN = T.histHead = 0;
+  if (o.predictor) {
+    for (int i = 0; i < ThermalData::kHist; i++) {
+      T.hist[i].alloc((size_t)c.n2 * 8);
+      T.hy[i].alloc((size_t)c.n2 * 8);
+    }
+    T.r0.alloc((size_t)c.n2 * 8);
+    T.scratch.alloc((size_t)c.n2 * 8);
+  }
   hipCheck(hipMemsetAsync(c.d_R.p, 0, c.d_R.bytes, c.stream), "zero R");
   double* U = c.d_U.as<double>();
   launch_fill(c.stream, nAll, o.init_value, U);  // vecU = initValue (FESolver.cpp:577-579)
@@ -339,6 +353,44 @@ void transientBegin(Ctx& c, const nla3d_gpu_transient_opts& o) {
   hipCheck(hipGetLastError(), "transient set-up");
   hipCheck(hipStreamSynchronize(c.stream), "sync");
   T.running = true;
+}
+
+// T.x = Us + sum_i a_i d_i over the history (newest first), the a_i minimising ||b - A x||_2 for
+// the CG matrix A; CG's tolerance is relative to ||b||_M, so a better start only saves
+// iterations
+static void transientPredict(Ctx& c, ThermalData& T, const double* Us) {
+  const int n = c.n2, m0 = T.histN;
+  const KrylovMatrix A = c.krylovMatrix();
+  double* part = c.d_partial.as<double>();
+  double* sc = c.d_scal.as<double>();
+  auto dot = [&](const double* x, const double* y) {
+    krylov_dot(c.stream, n, x, y, part, sc + 11);
+    double v;
+    hipCheck(hipMemcpyAsync(&v, sc + 11, 8, hipMemcpyDeviceToHost, c.stream), "dot");
+    hipCheck(hipStreamSynchronize(c.stream), "sync");
+    return v;
+  };
+  const double* d[ThermalData::kHist];
+  double* y[ThermalData::kHist];
+  for (int i = 0; i < m0; i++) {
+    d[i] = T.hist[(T.histHead - i + ThermalData::kHist) % ThermalData::kHist].as<double>();
+    y[i] = T.hy[i].as<double>();
+    krylov_spmv(c.stream, A, d[i], y[i]);
+  }
+  double* r0 = T.r0.as<double>();
+  krylov_spmv(c.stream, A, Us, r0);
+  vec_rsub(c.stream, n, c.d_rhs.as<double>(), r0);  // r0 = b - A Us
+  double G[kMaxLsBasis][kMaxLsBasis] = {}, g[kMaxLsBasis] = {}, a[kMaxLsBasis];
+  for (int i = 0; i < m0; i++) {
+    g[i] = dot(y[i], r0);
+    for (int k = 0; k <= i; k++) G[i][k] = dot(y[i], y[k]);
+  }
+  const int m = lsNormalSolve(G, g, m0, a);
+  // x = Us + sum a_i d_i (the residual update into a scratch vector is not needed)
+  for (int i = 0; i < m; i++)
+    newton_pred_axpy(c.stream, n, a[i], d[i], y[i], i == 0 ? Us : nullptr, i == 0 ? r0 : nullptr, T.x.as<double>(),
+                     T.scratch.as<double>());
+  hipCheck(hipGetLastError(), "transient predictor");
 }
 
 // one pass of the time loop (FESolver.cpp:604-636); returns false when curTime > time1
@@ -360,11 +412,21 @@ bool transientStep(Ctx& c, nla3d_gpu_newton_stats& st) {
   launch_tr_rhs(c.stream, c.n2, nC, c.d_vals.as<double>() + m.nnzAll(), c.d_R.as<double>(), T.y.as<double>(), U,
                 c.d_k12tPtr.as<int>(), c.d_k12tPos.as<int>(), c.d_k12tRow.as<int>(),
                 c.d_vals.as<double>() + m.offB12(), T.o.dirichlet_coupling, c.d_rhs.as<double>());
-  // initial guess: the current unknowns
+  // initial guess: the current unknowns, plus with the predictor the least-squares combination
+  // of the last steps' increments
   launch_copy(c.stream, c.n2, U + nC, T.x.as<double>());
   st.ms_assemble += nowMsT() - t0;
   const double t1 = nowMsT();
+  if (T.o.predictor && T.histN > 0) transientPredict(c, T, U + nC);
   cgSolve(c, T, c.d_rhs.as<double>(), T.x.as<double>(), T.o.krylov_tol, T.o.krylov_max_iter, st);
+  if (T.o.predictor) {  // this step's increment into the history
+    const int slot = (T.histHead + 1) % ThermalData::kHist;
+    double* d = T.hist[slot].as<double>();
+    launch_copy(c.stream, c.n2, U + nC, d);
+    vec_rsub(c.stream, c.n2, T.x.as<double>(), d);  // d = x - U_s
+    T.histHead = slot;
+    T.histN = std::min(ThermalData::kHist, T.histN + 1);
+  }
   launch_tr_update(c.stream, (int)m.nAll(), nC, T.x.as<double>(), T.a0, T.a2, T.a3, T.a6, T.a7, U, T.DU.as<double>(),
                    T.DDU.as<double>());
   hipCheck(hipGetLastError(), "transient update");
@@ -445,6 +507,7 @@ int nla3d_gpu_transient_defaults(nla3d_gpu_transient_opts* o) {
   o->dirichlet_coupling = 0;
   o->krylov_tol = 1.0e-12;
   o->krylov_max_iter = 100000;
+  o->predictor = 1;
   return 0;
 }
 
