@@ -349,18 +349,28 @@ int nla3d_gpu_batch_solve_nonlinear(nla3d_gpu_batch_t batch, const nla3d_gpu_new
 int nla3d_gpu_solve_block2_pc(nla3d_gpu_t h, const double* rhs, double* x, double tol, int max_iter,
                               int preconditioner, double t, int* iters, double* relres);
 
-/* ---- Schur-row multigrid (preconditioner 2) ---------------------------------
- * No counterpart in the reference.  Inspection entries: sizes of the hierarchy
- * (sizes[l] = rows of level l, up to cap); S of level 0 for the current tangent
- * (qrow[nq] = block(2) row of each Schur row; CSR rowPtr[nq+1], col[nnz0], val[nnz0]);
- * the aggregate of each row of a level and its power-iteration estimate of
- * lambda_max(D^-1 A); one V-cycle z = B r on host vectors of length nq (refresh = 1
- * recomputes the hierarchy's values from the current tangent first). */
+/* ---- Schur-row stage of preconditioner 2 -----------------------------------
+ * No counterpart in the reference.  The rows that block Jacobi treats by a scalar Schur
+ * estimate (element pressures, multipliers) get z = B r with S = 1/2 B^T D^-1 B - C
+ * (schur.h): by default one Jacobi-Chebyshev sweep on S (8 terms up to 10^4 rows, 8 more
+ * per decade, at most 24), or a V-cycle over an aggregation hierarchy of S
+ * (nla3d_gpu_schur_set_cycle; NLA3D_GPU_SCHUR_CYCLE=vcycle in the environment at set-up
+ * makes the V-cycle the default).  A level-0 matrix of at most 256 rows is inverted
+ * exactly either way.  Inspection entries: sizes of the hierarchy (sizes[l] = rows of level l,
+ * up to cap); S of level 0 for the current tangent (qrow[nq] = block(2) row of each Schur row;
+ * CSR rowPtr[nq+1], col[nnz0], val[nnz0]); the aggregate of each row of a level and its
+ * power-iteration estimate of lambda_max(D^-1 A); one application z = B r of the active
+ * stage on host vectors of length nq (refresh = 1 recomputes its values from the current
+ * tangent first). */
 int nla3d_gpu_schur_info(nla3d_gpu_t h, int* available, int* nq, int* nnz0, int* levels, int* sizes, int cap);
 int nla3d_gpu_schur_matrix(nla3d_gpu_t h, int* qrow, int* rowPtr, int* col, double* val);
-/* the cycle's shape: the finest level's Chebyshev length (terms; 0 with one level) and the first
- * level of the coarse range run as one workgroup launch (-1: none) */
+/* the stage's shape: the finest level's Chebyshev length in the active stage (terms; 0 with one
+ * level) and the first level of the V-cycle's coarse range run as one workgroup launch (-1: none) */
 int nla3d_gpu_schur_cycle_info(nla3d_gpu_t h, int* finest_steps, int* fuse_from);
+/* the active stage: vcycle = 0 the Chebyshev sweep on level 0, 1 the V-cycle; -3 when the model
+ * has no Schur stage */
+int nla3d_gpu_schur_set_cycle(nla3d_gpu_t h, int vcycle);
+int nla3d_gpu_schur_get_cycle(nla3d_gpu_t h, int* vcycle);
 int nla3d_gpu_schur_level(nla3d_gpu_t h, int level, int* agg, double* lmax);
 int nla3d_gpu_schur_apply(nla3d_gpu_t h, const double* r, double* z, int refresh);
 /* the precision of the finest level's smoother operator: 16 (the half-precision operator, levels

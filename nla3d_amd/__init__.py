@@ -164,6 +164,8 @@ ABI.update({
     "nla3d_gpu_schur_matrix": [_P, _IP, _IP, _IP, _DP],
     "nla3d_gpu_schur_level": [_P, ctypes.c_int, _IP, _DP],
     "nla3d_gpu_schur_cycle_info": [_P, _IP, _IP],
+    "nla3d_gpu_schur_set_cycle": [_P, ctypes.c_int],
+    "nla3d_gpu_schur_get_cycle": [_P, _IP],
     "nla3d_gpu_schur_apply": [_P, _DP, _DP, ctypes.c_int],
     "nla3d_gpu_schur_smoother_operator": [_P, _DP, _DP],
     "nla3d_gpu_schur_finest_precision": [_P, _IP],
@@ -465,6 +467,16 @@ class GpuModel:
         st, ff = ctypes.c_int(), ctypes.c_int()
         self._call("nla3d_gpu_schur_cycle_info", ctypes.byref(st), ctypes.byref(ff))
         return st.value, ff.value
+
+    def schur_set_cycle(self, vcycle):
+        """the Schur stage: False (default) one Chebyshev sweep on S, True the V-cycle"""
+        self._call("nla3d_gpu_schur_set_cycle", int(bool(vcycle)))
+
+    def schur_vcycle(self):
+        """True when the Schur stage is the V-cycle"""
+        v = ctypes.c_int()
+        self._call("nla3d_gpu_schur_get_cycle", ctypes.byref(v))
+        return bool(v.value)
 
     def schur_level(self, level, n):
         agg = np.zeros(max(n, 1), dtype=np.int32)

@@ -45,6 +45,22 @@ static int finestSteps(int n) {
   return std::max(kSteps0Min, std::min(kSteps0, (int)std::lround(s)));
 }
 static double finestDiv(int steps) { return 100.0 * (steps / 8.0) * (steps / 8.0); }
+// The default Schur stage is no V-cycle but one Jacobi-Chebyshev sweep on the finest level,
+// z = p(D^-1 S) D^-1 r, without coarse levels: at 120x120x60 (907,923 Schur rows) the
+// V-cycle's coarse correction (35,721 / 1,372 / 75 rows) changed the MINRES count by nothing
+// measurable (337.8 against 338.3 per Newton iteration with two 16-term sweeps) while its
+// launches cost 10 % of an iteration, and one longer sweep beats two short ones.  The length
+// grows with the matrix, 8 terms up to 10^4 rows and 8 more per decade up to 24 (the 907,923-row
+// matrix: 24 terms on [lmax / 400, .]; 20, 28 and 32 terms, or the interval's lower end at
+// lmax / 200 or / 900, all measured slower); the interval widens with the length to lmax / 400.
+constexpr int kPolyMin = 8, kPolyMax = 24;
+static int polySteps(int n) {
+  const double s = kPolyMin + 8.0 * std::log10(std::max(n, 1) / 1.0e4);
+  return std::max(kPolyMin, std::min(kPolyMax, (int)std::lround(s)));
+}
+static double polyDiv(int steps) { return std::min(400.0, 25.0 * steps); }
+constexpr int kCoefMax = kPolyMax > kSteps0 ? kPolyMax : kSteps0;  // longest sweep (coefficient arrays)
+static_assert(kStepsC <= kCoefMax, "coefficient arrays sized by the longest sweep");
 constexpr double kLmaxSafety = 1.05;  // the estimate approaches lmax from below
 constexpr int kHalfRows = 1 << 17;    // finest levels at least this large smooth in half precision
 
@@ -93,6 +109,10 @@ struct SchurAmg {
   // levels fuseFrom.. (each at most kFuseSlots smoother slots) run as one workgroup launch; -1: none
   int fuseFrom = -1;
   DevBuf fused;  // FusedLevel[L.size() - fuseFrom], refreshed with the values
+  // the Schur stage: one Chebyshev sweep on level 0 (true) or the V-cycle over every level; a
+  // hierarchy of one level (at most kCoarsest rows) is inverted exactly either way
+  bool poly = true;
+  bool polynomial() const { return poly && L.size() > 1; }
 };
 
 // stable counting sort of (target, source) pairs by target: seg[t]..seg[t+1] lists the sources
@@ -489,6 +509,10 @@ void setupSchur(Ctx& c, const std::vector<int64_t>& rp, const std::vector<int>& 
     if (e && std::strcmp(e, "fp32") == 0) half = false;
     S->L[0]->half = S->L.size() > 1 && half;
   }
+  {
+    const char* e = std::getenv("NLA3D_GPU_SCHUR_CYCLE");
+    S->poly = !(e && std::strcmp(e, "vcycle") == 0);
+  }
   for (size_t l = 0; l < S->L.size(); l++) allocLevel(*S->L[l], l + 1 == S->L.size());
   const int ncst = S->L.back()->n;
   S->cinv.alloc((size_t)ncst * ncst * 8);
@@ -529,16 +553,15 @@ static double hostDot(Ctx& c, int n, const double* a, const double* b) {
 // Chebyshev coefficients on [lmax/div, 1.05 lmax] (Jacobi-preconditioned, steps terms)
 struct ChebCoef {
   double c0;
-  double c1[kSteps0], c2[kSteps0];
+  double c1[kCoefMax], c2[kCoefMax];
 };
-static_assert(kStepsC <= kSteps0, "coefficient arrays sized by the longest smoother");
 static ChebCoef chebCoef(double lmax, int steps, double div) {
   const double hi = 1.05 * lmax, lo = lmax / div;
   const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
   ChebCoef k{};
   k.c0 = 1.0 / theta;
   double rho = 1.0 / sigma;
-  for (int i = 1; i < steps && i < kSteps0; i++) {
+  for (int i = 1; i < steps && i < kCoefMax; i++) {
     const double rn = 1.0 / (2.0 * sigma - rho);
     k.c1[i] = rn * rho;
     k.c2[i] = 2.0 * rn / delta;
@@ -559,7 +582,9 @@ bool schurRefresh(Ctx& c, std::string* why) {
                        kSchurAlpha, S.contrib.as<double>());
   launch_gather_sum(s, L0.nnz, S.sSeg.as<int>(), S.sSrc.as<int>(), S.contrib.as<double>(), L0.val.as<double>());
   launch_schur_sub_c(s, S.nq, S.sDiag.as<int>(), S.cPos.as<int>(), kval, L0.val.as<double>());
-  for (size_t l = 0; l + 1 < S.L.size(); l++) {
+  // the polynomial stage smooths level 0 only and needs no coarse matrices
+  const size_t nSmoothed = S.polynomial() ? 1 : S.L.size() - 1;
+  for (size_t l = 0; l < nSmoothed; l++) {
     AmgLevel& A = *S.L[l];
     if (!A.half) {
       launch_sell_fill(s, A.nSlots, A.sPos.as<int>(), A.val.as<double>(), A.sVal.as<float>());
@@ -585,10 +610,16 @@ bool schurRefresh(Ctx& c, std::string* why) {
       if (it == kPowerSteps - 1) lam = std::sqrt(hostDot(c, A.n, x, x));
     }
     A.lmax = kLmaxSafety * lam;
+    if (S.polynomial()) break;
     AmgLevel& C = *S.L[l + 1];
     launch_gather_sum(s, C.nnz, A.gSeg.as<int>(), A.gSrc.as<int>(), A.val.as<double>(), C.val.as<double>());
   }
   hipCheck(hipGetLastError(), "schur refresh");
+  if (S.polynomial()) {
+    hipCheck(hipStreamSynchronize(s), "sync");
+    S.fresh = true;
+    return true;
+  }
   // coarsest: explicit inverse by Cholesky on the host
   AmgLevel& Z = *S.L.back();
   const int n = Z.n;
@@ -685,6 +716,51 @@ static ChebStartNext startNext(SchurAmg& S, size_t l) {
   return nx;
 }
 
+// the polynomial stage: one Chebyshev sweep of polySteps() terms on level 0, from zero; the MINRES
+// stage (rext != null) gathers the right-hand side from rext and scatters the result into zext
+// (st gates the write), otherwise A.r -> A.x
+static void polySweep(SchurAmg& S, hipStream_t s, const double* rext, double* zext, const MinresState* st) {
+  AmgLevel& A = *S.L[0];
+  const int steps = polySteps(A.n);
+  const ChebCoef k = chebCoef(A.lmax, steps, polyDiv(steps));
+  const int* qrow = rext ? S.qrow.as<int>() : nullptr;
+  if (A.half) {
+    // the scaled three-term sweep (schur.h, AmgHalf); the last step writes z = s x^
+    const AmgHalf H = A.halfView();
+    double* rhat = A.res.as<double>();
+    double* cur = A.da.as<double>();
+    double* prev = nullptr;
+    launch_half_start(s, A.n, qrow, rext ? rext : A.r.as<double>(), A.hS.as<double>(), k.c0, rhat, cur, st);
+    for (int i = 1; i < steps; i++) {
+      const bool last = i + 1 == steps;
+      double* out = prev ? prev : A.db.as<double>();
+      launch_half_step(s, H, cur, prev, rhat, k.c1[i], k.c2[i], out, last ? qrow : nullptr,
+                       last ? (zext ? zext : A.x.as<double>()) : nullptr, st);
+      prev = cur;
+      cur = out;
+    }
+    return;
+  }
+  const AmgSell M = A.sell();
+  double* dinv = A.dinv.as<double>();
+  double* din = A.da.as<double>();
+  double* dout = A.db.as<double>();
+  if (rext)
+    launch_cheb_start_gather(s, A.n, qrow, rext, dinv, k.c0, A.r.as<double>(), A.res.as<double>(), din,
+                             A.x.as<double>());
+  else
+    launch_cheb_start(s, A.n, A.r.as<double>(), dinv, k.c0, A.res.as<double>(), din, A.x.as<double>());
+  for (int i = 1; i < steps; i++) {
+    launch_cheb_step(s, M, din, dinv, k.c1[i], k.c2[i], A.res.as<double>(), dout, A.x.as<double>(),
+                     chebXMode(i, steps));
+    std::swap(din, dout);
+  }
+  if (zext)
+    launch_finish_scatter(s, A.n, qrow, din, A.x.as<double>(), st, zext);
+  else
+    launch_axpby(s, A.n, 1.0, din, 1.0, A.x.as<double>(), A.x.as<double>());
+}
+
 // one V-cycle from level l.  At level 0 of the MINRES stage (rext != null) the right-hand side
 // is gathered from the Krylov vector rext and the result scattered into zext (the Schur rows,
 // S.qrow; st gates the write) inside the first and last passes; otherwise it is A.r -> A.x.
@@ -693,6 +769,10 @@ static ChebStartNext startNext(SchurAmg& S, size_t l) {
 // null when A.x is complete.
 static const double* vcycle(SchurAmg& S, hipStream_t s, size_t l, const double* rext = nullptr,
                             double* zext = nullptr, const MinresState* st = nullptr, bool started = false) {
+  if (l == 0 && S.polynomial()) {
+    polySweep(S, s, rext, zext, st);
+    return nullptr;
+  }
   if (l > 0 && (int)l == S.fuseFrom) {
     launch_coarse_cycle(s, S.fused.as<FusedLevel>(), (int)(S.L.size() - l));
     return nullptr;
@@ -844,8 +924,29 @@ int nla3d_gpu_schur_cycle_info(nla3d_gpu_t h, int* finest_steps, int* fuse_from)
   return schurGuard(h, [&](Ctx& c) {
     SchurAmg* S = c.schur.get();
     if (!S || !S->available) throw ApiError(-3, "the Schur multigrid is unavailable for this model");
-    if (finest_steps) *finest_steps = S->L.size() > 1 ? finestSteps(S->L[0]->n) : 0;
+    if (finest_steps)
+      *finest_steps = S->L.size() < 2 ? 0 : S->polynomial() ? polySteps(S->L[0]->n) : finestSteps(S->L[0]->n);
     if (fuse_from) *fuse_from = S->fuseFrom;
+  });
+}
+
+int nla3d_gpu_schur_set_cycle(nla3d_gpu_t h, int vcycle) {
+  return schurGuard(h, [&](Ctx& c) {
+    SchurAmg* S = c.schur.get();
+    if (!S || !S->available) throw ApiError(-3, "the Schur multigrid is unavailable for this model");
+    if (vcycle != 0 && vcycle != 1) throw ApiError(-1, "vcycle must be 0 or 1");
+    if (S->poly != (vcycle == 0)) {
+      S->poly = vcycle == 0;
+      S->fresh = false;  // the V-cycle needs the coarse matrices of the current values
+    }
+  });
+}
+
+int nla3d_gpu_schur_get_cycle(nla3d_gpu_t h, int* vcycle) {
+  return schurGuard(h, [&](Ctx& c) {
+    SchurAmg* S = c.schur.get();
+    if (!S || !S->available) throw ApiError(-3, "the Schur multigrid is unavailable for this model");
+    if (vcycle) *vcycle = S->poly ? 0 : 1;
   });
 }
 
@@ -943,8 +1044,9 @@ int nla3d_gpu_time_smoother(nla3d_gpu_t h, int iters, double* ms, long long* byt
     SchurAmg& S = *c.schur;
     if (S.L.size() < 2) throw ApiError(-3, "the hierarchy has no smoothed level");
     AmgLevel& A = *S.L[0];
-    const int steps0 = finestSteps(A.n);
-    const ChebCoef k = chebCoef(A.lmax, steps0, finestDiv(steps0));
+    // the coefficients of the active stage's finest-level sweep
+    const int steps0 = S.polynomial() ? polySteps(A.n) : finestSteps(A.n);
+    const ChebCoef k = chebCoef(A.lmax, steps0, S.polynomial() ? polyDiv(steps0) : finestDiv(steps0));
     hipStream_t s = c.stream;
     if (A.half) {
       const AmgHalf H = A.halfView();

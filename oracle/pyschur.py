@@ -6,6 +6,10 @@ the device path implements:
 * schur_matrix: S = B^T D^-1 B - C on the rows that block Jacobi treats by a scalar
   Schur estimate (element pressures, multipliers), B their couplings to the node
   rows, D^-1 the block-Jacobi inverse node blocks, C their own block;
+* polynomial: the default stage, one Jacobi-Chebyshev sweep on S from zero (no coarse
+  levels), 8 terms up to 10^4 rows and 8 more per decade to at most 24, on
+  [lmax/div, 1.05 lmax] with div = 25 terms (at most 400), on S's FP32-rounded entries
+  or, for large matrices, on its half-precision operator (below);
 * vcycle: one V-cycle over given aggregation maps with Jacobi-Chebyshev smoothing
   on [lmax/div, 1.05 lmax], the same polynomial before and after (finest level:
   16 terms, div 400; coarser levels: 3 terms, div 10), exact solve on the
@@ -26,7 +30,9 @@ symmetric positive definite on the models of the tests; the two-term smoother is
 scaled minimax Chebyshev polynomial in closed form; half_operator is exact on a matrix
 whose scaled entries FP16 holds, dominates the scaled matrix, and stays positive definite
 where plain FP16 rounding does not; the scaled three-term sweep on it equals the two-term
-Jacobi-Chebyshev sweep on the unscaled operator.
+Jacobi-Chebyshev sweep on the unscaled operator; the polynomial stage is a symmetric
+positive-definite operator that reduces the energy-norm error, and on a diagonal matrix it is
+the closed-form minimax polynomial (the sweep above).
 """
 import math
 
@@ -67,6 +73,21 @@ def finest_steps(n):
 def finest_div(steps):
     """the interval [lmax / div, 1.05 lmax] widens with the square of the length"""
     return 100.0 * (steps / 8.0) ** 2
+
+
+POLY_MIN, POLY_MAX = 8, 24  # the polynomial stage's shortest and longest sweep
+
+
+def poly_steps(n):
+    """length of the polynomial stage's one sweep for n rows: 8 terms up to 10^4 rows, + 8 per
+    decade, at most POLY_MAX (the library's polySteps)"""
+    s = POLY_MIN + 8.0 * math.log10(max(n, 1) / 1.0e4)
+    return max(POLY_MIN, min(POLY_MAX, int(math.floor(s + 0.5))))
+
+
+def poly_div(steps):
+    """its interval [lmax / div, 1.05 lmax]: div = 25 steps, at most 400"""
+    return min(400.0, 25.0 * steps)
 
 
 def cheb_coef(lmax, steps, div=10.0):
@@ -156,6 +177,20 @@ def smooth_scaled(H, rhat, x0, lmax, steps, div):
     for i in range(1, steps):
         prev, x = x, x + c1[i] * (x - prev) + c2[i] * (rhat - H @ x)
     return x
+
+
+def polynomial(S, lmax, r, half=None, use_half=False):
+    """the default Schur stage: one Jacobi-Chebyshev sweep from zero on S, z = p(D^-1 S) D^-1 r,
+    poly_steps(rows) terms on [lmax / poly_div, 1.05 lmax]; use_half: on the half-precision
+    operator in the scaled variables (half = (s, H) replaces half_operator(S)), else with S's
+    entries rounded to FP32 (products and sums in FP64, D the FP64 diagonal)"""
+    S = sp.csr_matrix(S)
+    steps = poly_steps(S.shape[0])
+    div = poly_div(steps)
+    if use_half:
+        s, H = half_operator(S) if half is None else half
+        return s * smooth_scaled(H, s * r, np.zeros_like(r), lmax, steps, div)
+    return smooth(rounded_f32(S), 1.0 / S.diagonal(), r, np.zeros_like(r), lmax, steps, div)
 
 
 def vcycle(mats, aggs, lmaxs, r, level=0, half=None, finest_half=True):

@@ -1,7 +1,8 @@
-"""Schur-row multigrid on the device against oracle/pyschur.py: S (1e-12 of its
-largest entry), one V-cycle over the device's aggregation maps and eigenvalue
-estimates (1e-9), the cycle's symmetry, and MINRES / Newton with it: the same
-solution as a dense solve and as block Jacobi (1e-8), in fewer iterations."""
+"""The Schur-row stage on the device against oracle/pyschur.py: S (1e-12 of its
+largest entry), the default one-sweep Chebyshev stage (1e-12 of the oracle's sweep on the
+device's stored operator and eigenvalue estimate), one V-cycle over the device's aggregation
+maps and eigenvalue estimates (1e-9), both stages' symmetry, and MINRES / Newton with each: the
+same solution as a dense solve and as block Jacobi (1e-8), in fewer iterations."""
 import ctypes
 
 import numpy as np
@@ -93,6 +94,8 @@ def test_vcycle_matches_oracle_and_is_symmetric(gpu_lib, smoother, monkeypatch):
     else:
         monkeypatch.delenv("NLA3D_GPU_SCHUR_SMOOTHER", raising=False)
     m = rigid_model(8, 8, 4)
+    m.schur_set_cycle(True)
+    assert m.schur_vcycle()
     fh = m.schur_finest_precision() == 16
     assert fh == (smoother == "half")
     info = m.schur_info()
@@ -120,9 +123,15 @@ def test_vcycle_matches_oracle_and_is_symmetric(gpu_lib, smoother, monkeypatch):
     assert y @ by > 0 and z @ bz > 0
 
 
-def test_minres_and_newton_with_schur_multigrid(gpu_lib):
-    print("[gpu] minres / newton with schur multigrid", flush=True)
+@pytest.mark.parametrize("cycle", ["polynomial", "vcycle"])
+def test_minres_and_newton_with_schur_multigrid(gpu_lib, cycle, monkeypatch):
+    print(f"[gpu] minres / newton with the schur stage ({cycle})", flush=True)
+    if cycle == "vcycle":
+        monkeypatch.setenv("NLA3D_GPU_SCHUR_CYCLE", "vcycle")
+    else:
+        monkeypatch.delenv("NLA3D_GPU_SCHUR_CYCLE", raising=False)
     m = rigid_model(8, 8, 4)
+    assert m.schur_vcycle() == (cycle == "vcycle")
     K2, n2 = full_k2(m)
     rhs = np.random.default_rng(4).standard_normal(n2)
     x0, it0, _ = m.solve_block2_pc(rhs, 1e-10, 50000, preconditioner=0)
@@ -189,6 +198,7 @@ def test_vcycle_matches_oracle_on_deep_hierarchies(gpu_lib, case, smoother, monk
     else:
         monkeypatch.delenv("NLA3D_GPU_SCHUR_SMOOTHER", raising=False)
     m = rigid_model(30, 30, 15) if case == "box_30x30x15" else a2000_damper_model()
+    m.schur_set_cycle(True)
     fh = m.schur_finest_precision() == 16
     assert fh == (smoother == "half")
     info = m.schur_info()
@@ -216,5 +226,44 @@ def test_vcycle_matches_oracle_on_deep_hierarchies(gpu_lib, case, smoother, monk
         assert np.abs(by - ps.vcycle(mats, aggs, lmaxs, y, half=half)).max() <= 1e-12 * np.abs(by).max()
     bo = ps.vcycle(mats, aggs, lmaxs, y, finest_half=fh)
     assert np.abs(by - bo).max() <= 1e-9 * np.abs(bo).max()
+    assert abs(z @ by - y @ bz) <= 1e-10 * abs(z @ by)
+    assert y @ by > 0 and z @ bz > 0
+
+
+@pytest.mark.parametrize("case,smoother", [("box_8x8x4", "auto"), ("box_8x8x4", "half"), ("box_30x30x15", "auto"),
+                                           ("box_30x30x15", "half"), ("a2000_damper", "auto")])
+def test_polynomial_stage_matches_oracle(gpu_lib, case, smoother, monkeypatch):
+    """the default Schur stage, one Chebyshev sweep on S (no coarse levels), against the
+    oracle's sweep on the device's S and eigenvalue estimate: its length follows the rows
+    (8 terms at 1,280 and 3,101 rows, 10 at 16,383), on the FP32 copy (the choice at these
+    sizes) and forced to the half-precision operator (then on the stored operator, 1e-12);
+    symmetric and positive"""
+    print(f"[gpu] schur polynomial stage, {case} ({smoother})", flush=True)
+    if smoother == "half":
+        monkeypatch.setenv("NLA3D_GPU_SCHUR_SMOOTHER", "half")
+    else:
+        monkeypatch.delenv("NLA3D_GPU_SCHUR_SMOOTHER", raising=False)
+    monkeypatch.delenv("NLA3D_GPU_SCHUR_CYCLE", raising=False)
+    m = a2000_damper_model() if case == "a2000_damper" else \
+        rigid_model(*[int(v) for v in case.split("_")[1].split("x")])
+    assert not m.schur_vcycle()
+    fh = m.schur_finest_precision() == 16
+    assert fh == (smoother == "half")
+    info = m.schur_info()
+    steps, _ = m.schur_cycle_info()
+    print("rows", info["nq"], "terms", steps, flush=True)
+    assert info["levels"] >= 2 and steps == ps.poly_steps(info["nq"])
+    qrow, rp, col, val = m.schur_matrix()
+    n = len(qrow)
+    S = sp.csr_matrix((val, col, rp), shape=(n, n))
+    _, lmax = m.schur_level(0, n)
+    rng = np.random.default_rng(21)
+    y, z = rng.standard_normal(n), rng.standard_normal(n)
+    by, bz = m.schur_apply(y), m.schur_apply(z)
+    if fh:
+        bo = ps.polynomial(S, lmax, y, half=check_half_operator(m), use_half=True)
+    else:
+        bo = ps.polynomial(S, lmax, y)
+    assert np.abs(by - bo).max() <= 1e-12 * np.abs(bo).max()
     assert abs(z @ by - y @ bz) <= 1e-10 * abs(z @ by)
     assert y @ by > 0 and z @ bz > 0

@@ -142,3 +142,47 @@ def test_scaled_three_term_sweep_is_the_two_term_sweep():
             two = ps.smooth(Sp, s * s, r, start, lmax, steps, div)
             three = s * ps.smooth_scaled(H, s * r, start / s, lmax, steps, div)
             assert np.abs(two - three).max() <= 1e-10 * np.abs(two).max()
+
+
+def test_polynomial_length_follows_the_size():
+    """8 terms on [lmax/200, .] up to 10^4 rows, 8 more per decade, 24 on [lmax/400, .] from
+    about 9 10^5 rows (the 120x120x60 box's Schur matrix has 907,923 rows)"""
+    assert [ps.poly_steps(n) for n in (1, 3101, 10000, 31623, 100000, 907923, 10 ** 7)] == \
+        [8, 8, 8, 12, 16, 24, 24]
+    assert [ps.poly_div(k) for k in (8, 12, 16, 24)] == [200.0, 300.0, 400.0, 400.0]
+
+
+def test_polynomial_stage_is_spd_and_contracts():
+    """z = B r of the one-sweep stage, both forms: B symmetric positive definite, and I - B A
+    contracts in the energy norm (the minimax polynomial on [lmax/div, 1.05 lmax] with lmax
+    the largest eigenvalue of D^-1 A); on the half-precision form A is its operator's unscaled
+    matrix"""
+    A = laplacian(8)
+    n = A.shape[0]
+    lmax = float(np.max(np.linalg.eigvals((sp.diags(1 / A.diagonal()) @ A).toarray()).real))
+    B = np.stack([ps.polynomial(A, lmax, e) for e in np.eye(n)], axis=1)
+    assert np.abs(B - B.T).max() <= 1e-12 * np.abs(B).max()
+    assert np.linalg.eigvalsh(0.5 * (B + B.T)).min() > 0
+    Ad = ps.rounded_f32(A).toarray()
+    L = np.linalg.cholesky(Ad)
+    E = L.T @ (np.eye(n) - B @ Ad) @ np.linalg.inv(L.T)
+    assert np.linalg.norm(E, 2) < 1.0
+    s, H = ps.half_operator(A)
+    lh = float(np.max(np.linalg.eigvalsh(H.toarray())))
+    Bh = np.stack([ps.polynomial(A, lh, e, half=(s, H), use_half=True) for e in np.eye(n)], axis=1)
+    assert np.abs(Bh - Bh.T).max() <= 1e-12 * np.abs(Bh).max()
+    assert np.linalg.eigvalsh(0.5 * (Bh + Bh.T)).min() > 0
+
+
+def test_polynomial_stage_is_the_sweep_from_zero():
+    """the one-sweep stage equals `smooth` from x = 0 with the stage's length and interval on the
+    FP32-rounded matrix, and the scaled three-term sweep on the half-precision operator"""
+    A = jittered_spd(30, 1e-2, seed=11)
+    r = np.random.default_rng(2).standard_normal(30)
+    lmax = 1.7
+    k = ps.poly_steps(30)
+    ref = ps.smooth(ps.rounded_f32(A), 1 / A.diagonal(), r, np.zeros(30), lmax, k, ps.poly_div(k))
+    assert np.array_equal(ps.polynomial(A, lmax, r), ref)
+    s, H = ps.half_operator(A)
+    refh = s * ps.smooth_scaled(H, s * r, np.zeros(30), lmax, k, ps.poly_div(k))
+    assert np.array_equal(ps.polynomial(A, lmax, r, use_half=True), refh)
