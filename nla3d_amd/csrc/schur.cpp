@@ -36,7 +36,12 @@ constexpr int kSteps0 = 16;  // longest finest-level smoother
 constexpr int kSteps0Min = 8;
 constexpr int kStepsC = 3;
 constexpr double kLoDivC = 10.0;
-constexpr double kSchurAlpha = 0.5;  // S = alpha B^T D^-1 B - C (see schur.h)
+// S = alpha B^T D^-1 B - C (see schur.h): alpha = 1/2 under the V-cycle (120x120x60, MINRES to
+// 1e-9: 1/4, 0.35, 1/2, 1 -> 840, 804, 795, 843 iterations); the one-sweep stage wants less
+// (Newton iterations of the driver's bench: 0.2, 0.25, 0.3, 0.35, 0.5, 0.75 -> 361.0, 358.0,
+// 359.4, 363.9, 369.6, 400.2 MINRES iterations each)
+constexpr double kSchurAlpha = 0.5;
+constexpr double kSchurAlphaSweep = 0.3;
 constexpr int kPowerSteps = 40;     // power iteration for lmax of D^-1 A
 
 // finest-level smoother length and interval for a Schur matrix of n rows
@@ -579,7 +584,7 @@ bool schurRefresh(Ctx& c, std::string* why) {
   AmgLevel& L0 = *S.L[0];
   launch_schur_contrib(s, S.nPairs, S.pairNode.as<int>(), S.nodeGroup.as<int>(), S.nodeSize.as<int>(),
                        S.qPtr.as<int>(), S.cPtr.as<int>(), S.posB.as<int>(), kval, c.d_binv.as<double>(),
-                       kSchurAlpha, S.contrib.as<double>());
+                       S.polynomial() ? kSchurAlphaSweep : kSchurAlpha, S.contrib.as<double>());
   launch_gather_sum(s, L0.nnz, S.sSeg.as<int>(), S.sSrc.as<int>(), S.contrib.as<double>(), L0.val.as<double>());
   launch_schur_sub_c(s, S.nq, S.sDiag.as<int>(), S.cPos.as<int>(), kval, L0.val.as<double>());
   // the polynomial stage smooths level 0 only and needs no coarse matrices

@@ -1,19 +1,21 @@
-// Multigrid preconditioner for the Schur rows of block(2) (schur.hip, schur.cpp).
+// Preconditioner stage for the Schur rows of block(2) (schur.hip, schur.cpp).
 //
 // The rows of block(2) that block Jacobi treats by a diagonal Schur estimate (element
-// pressures, MPC multipliers) are preconditioned instead by one V-cycle of
-// aggregation multigrid on the sparse Schur approximation
-//     S = alpha B^T D^-1 B - C,   alpha = 1/2,
+// pressures, MPC multipliers) are preconditioned instead through the sparse Schur
+// approximation
+//     S = alpha B^T D^-1 B - C,
 // B the couplings of those rows to the node DoFs, D^-1 the block-Jacobi inverse node
 // blocks, C their own block (the pressure rows' -V/k).  S is SPD when the node blocks
 // are.  alpha is empirical: the block-Jacobi blocks misjudge the displacement block's
-// smooth modes, and halving their Schur term cut MINRES iterations by 6 % at 3.5M
-// unknowns and 9 % at 60k (1/4 was best for the small, 1/2 .. 1/3 for the large case).  Levels: piecewise-constant aggregation of S's graph (greedy, distance one),
-// Galerkin coarse matrices P^T A P (FP64), Jacobi-Chebyshev smoothing (the same
-// polynomial before and after, so the cycle is symmetric; a long one on the finest level)
-// whose products use an FP32 sliced-ELLPACK copy of each level, an explicit dense FP64
-// inverse on the coarsest level.  All structure is built once per model; values, smoother intervals
-// (power iteration) and the coarsest inverse are refreshed with every tangent.
+// smooth modes, and shrinking their Schur term cuts MINRES iterations (0.3 for the default
+// stage, 1/2 for the V-cycle; 1/4 was best for a small case under the V-cycle).  The default
+// stage is one Jacobi-Chebyshev sweep on S from zero, its products on an FP32 or
+// half-precision copy of S.  The V-cycle form: piecewise-constant aggregation of S's graph
+// (greedy, distance one), Galerkin coarse matrices P^T A P (FP64), Jacobi-Chebyshev smoothing
+// (the same polynomial before and after, so the cycle is symmetric; a long one on the finest
+// level) whose products use an FP32 sliced-ELLPACK copy of each level, an explicit dense FP64
+// inverse on the coarsest level.  All structure is built once per model; values, smoother
+// intervals (power iteration) and the coarsest inverse are refreshed with every tangent.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -58,7 +58,8 @@ def schur_matrix(K2, groups, bj, alpha=None):
     return q, (alpha * (Bq.T @ Dinv @ Bq) - C).toarray()
 
 
-SCHUR_ALPHA = 0.5  # S = alpha B^T D^-1 B - C: the weight of the block-Jacobi Schur term
+SCHUR_ALPHA = 0.5  # S = alpha B^T D^-1 B - C: the weight of the block-Jacobi Schur term (V-cycle)
+SCHUR_ALPHA_SWEEP = 0.3  # the same under the default one-sweep stage
 STEPS0, DIV0 = 16, 400.0   # finest level of a 10^6-row matrix (the longest smoother)
 STEPSC, DIVC = 3, 10.0     # coarser levels
 

@@ -84,6 +84,22 @@ def test_schur_matrix_matches_oracle(gpu_lib):
     assert np.linalg.eigvalsh(So).min() > 0
 
 
+def test_schur_weight_follows_the_stage(gpu_lib, monkeypatch):
+    """S = alpha B^T D^-1 B - C with alpha = 0.3 under the default one-sweep stage and 1/2
+    under the V-cycle (the switch rebuilds S), against the oracle's dense S at 1e-12"""
+    print("[gpu] schur weight per stage", flush=True)
+    monkeypatch.delenv("NLA3D_GPU_SCHUR_CYCLE", raising=False)
+    m = rigid_model(8, 8, 4)
+    assert m.schur_info()["levels"] >= 2 and not m.schur_vcycle()
+    K2, n2, groups, bj = oracle_parts(m)
+    for vcycle, alpha in ((False, ps.SCHUR_ALPHA_SWEEP), (True, ps.SCHUR_ALPHA)):
+        m.schur_set_cycle(vcycle)
+        q, So = ps.schur_matrix(K2, groups, bj, alpha)
+        qrow, S = gpu_s_dense(m)
+        assert np.array_equal(qrow, q)
+        assert np.abs(S - So).max() <= 1e-12 * np.abs(So).max(), alpha
+
+
 @pytest.mark.parametrize("smoother", ["auto", "half"])
 def test_vcycle_matches_oracle_and_is_symmetric(gpu_lib, smoother, monkeypatch):
     """auto: this small finest level smooths with its FP32 copy; half: forced to the
