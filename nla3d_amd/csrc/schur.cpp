@@ -118,6 +118,7 @@ struct SchurAmg {
   // hierarchy of one level (at most kCoarsest rows) is inverted exactly either way
   bool poly = true;
   bool polynomial() const { return poly && L.size() > 1; }
+  DevBuf powerNorm;  // ||x||^2 of the power iteration's current vector
 };
 
 // stable counting sort of (target, source) pairs by target: seg[t]..seg[t+1] lists the sources
@@ -530,6 +531,7 @@ void setupSchur(Ctx& c, const std::vector<int64_t>& rp, const std::vector<int>& 
       break;
     }
   S->qrow.upload(S->h_qrow);
+  S->powerNorm.alloc(8);
   S->nNodes = nNodes;
   S->nPairs = (int)qlist.size();
   S->nContrib = cPtr[nNodes];
@@ -601,19 +603,24 @@ bool schurRefresh(Ctx& c, std::string* why) {
       launch_half_fill(s, A.halfView(), A.csr(), A.sPos.as<int>(), A.patterned ? A.sDPos.as<int>() : nullptr,
                        A.dinv.as<double>(), A.hS.as<double>(), A.hVal.as<unsigned short>(),
                        A.patterned ? A.hDia.as<unsigned short>() : nullptr, A.hDiag.as<float>());
-    // largest eigenvalue of D^-1 A by power iteration from a fixed pseudo-random start,
-    // raised by a safety factor (the Chebyshev polynomial grows quickly above its interval)
+    // largest eigenvalue of D^-1 A (of the stored operator S^' on a half-precision level, which
+    // the sweep applies) by power iteration from a fixed pseudo-random start, each step scaled
+    // by the norm of its input on the device, raised by a safety factor (the Chebyshev
+    // polynomial grows quickly above its interval)
     double* x = A.da.as<double>();
     double* y = A.db.as<double>();
+    double* nsq = S.powerNorm.as<double>();
     hipCheck(hipMemcpyAsync(x, A.start.p, (size_t)A.n * 8, hipMemcpyDeviceToDevice, s), "start");
-    double lam = 1.0;
     for (int it = 0; it < kPowerSteps; it++) {
-      const double nx = std::sqrt(hostDot(c, A.n, x, x));
-      if (!(nx > 0.0)) break;
-      launch_dinv_spmv(s, A.csr(), A.dinv.as<double>(), x, 1.0 / nx, y);
+      krylov_dot(s, A.n, x, x, c.d_partial.as<double>(), nsq);
+      if (A.half)
+        launch_half_power(s, A.halfView(), x, nsq, y);
+      else
+        launch_dinv_spmv(s, A.csr(), A.dinv.as<double>(), x, nsq, y);
       std::swap(x, y);
-      if (it == kPowerSteps - 1) lam = std::sqrt(hostDot(c, A.n, x, x));
     }
+    double lam = std::sqrt(hostDot(c, A.n, x, x));
+    if (!(lam > 0.0) || !std::isfinite(lam)) lam = 1.0;  // (a zero product: any interval serves)
     A.lmax = kLmaxSafety * lam;
     if (S.polynomial()) break;
     AmgLevel& C = *S.L[l + 1];

@@ -185,7 +185,11 @@ void launch_q_gather(hipStream_t s, int nq, const int* qrow, const double* v, co
 void launch_q_scatter(hipStream_t s, int nq, const int* qrow, const double* v, const MinresState* st, double* out);
 void launch_dot_partials(hipStream_t s, int n, const double* a, const double* b, const MinresState* st,
                          double* partial);
-// y = scale * D^-1 A x (power iteration)
-void launch_dinv_spmv(hipStream_t s, const AmgCsr& A, const double* dinv, const double* x, double scale, double* y);
+// power-iteration steps, the scale 1 / sqrt(*nsq) read on the device (no host round trip per
+// step): y = D^-1 A x / ||x|| on the FP64 values, and y = S^' x / ||x|| on the stored
+// half-precision operator (its largest eigenvalue is the sweep's, in the scaled variables)
+void launch_dinv_spmv(hipStream_t s, const AmgCsr& A, const double* dinv, const double* x, const double* nsq,
+                      double* y);
+void launch_half_power(hipStream_t s, const AmgHalf& H, const double* x, const double* nsq, double* y);
 
 }  // namespace nla3d_amd

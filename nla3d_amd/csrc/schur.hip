@@ -283,6 +283,19 @@ __global__ __launch_bounds__(256) void half_residual_kernel(AmgHalf H, const dou
   }
 }
 
+// power-iteration step on the stored operator: y = S^' x / ||x||, ||x||^2 read from the device
+template <bool kNarrow, int kND>
+__global__ __launch_bounds__(256) void half_power_kernel(AmgHalf H, const double* __restrict__ x,
+                                                       const double* __restrict__ nsq, double* __restrict__ y) {
+  int j[2];
+  double a[2];
+  half_dot_wave<kNarrow, kND>(H, x, j[0], a[0], j[1], a[1]);
+  const double scale = *nsq > 0.0 ? 1.0 / sqrt(*nsq) : 0.0;
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+    if (j[r] >= 0) y[j[r]] = scale * a[r];
+}
+
 __global__ __launch_bounds__(256) void half_prolong_kernel(int n, const int* __restrict__ agg,
                                                          const double* __restrict__ xc, const double* __restrict__ dc,
                                                          const double* __restrict__ sc, double* __restrict__ x,
@@ -526,8 +539,9 @@ __global__ __launch_bounds__(256) void dot_partials_kernel(int n, const double* 
 }
 
 __global__ __launch_bounds__(256) void dinv_spmv_kernel(AmgCsr A, const double* __restrict__ dinv,
-                                                      const double* __restrict__ x, double scale,
+                                                      const double* __restrict__ x, const double* __restrict__ nsq,
                                                       double* __restrict__ y) {
+  const double scale = *nsq > 0.0 ? 1.0 / sqrt(*nsq) : 0.0;
   const int lane = threadIdx.x % kLanes, rpb = blockDim.x / kLanes;
   for (int i = blockIdx.x * rpb + threadIdx.x / kLanes; i < A.n; i += gridDim.x * rpb) {
     const double ax = row_dot(A, i, lane, x);
@@ -686,6 +700,12 @@ struct StartXLaunch {
   }
 };
 template <bool kNarrow, int kND>
+struct PowerLaunch {
+  static void go(const AmgHalf& H, hipStream_t s, const double* x, const double* nsq, double* y) {
+    hipLaunchKernelGGL((half_power_kernel<kNarrow, kND>), dim3(cdiv(H.n, 512)), dim3(256), 0, s, H, x, nsq, y);
+  }
+};
+template <bool kNarrow, int kND>
 struct StepLaunch {
   static void go(const AmgHalf& H, hipStream_t s, const double* xk, const double* xprev, const double* rhat, double c1,
                  double c2, double* xout, const int* qrow, double* z, const MinresState* st) {
@@ -705,6 +725,10 @@ struct ResidualLaunch {
 void launch_half_start_x(hipStream_t s, const AmgHalf& H, const double* x0, const double* rhat, double c0, double* x1,
                          const MinresState* st) {
   half_dispatch<StartXLaunch>(H, s, x0, rhat, c0, x1, st);
+}
+
+void launch_half_power(hipStream_t s, const AmgHalf& H, const double* x, const double* nsq, double* y) {
+  half_dispatch<PowerLaunch>(H, s, x, nsq, y);
 }
 
 void launch_half_step(hipStream_t s, const AmgHalf& H, const double* xk, const double* xprev, const double* rhat,
@@ -800,8 +824,9 @@ void launch_dot_partials(hipStream_t s, int n, const double* a, const double* b,
   hipLaunchKernelGGL(dot_partials_kernel, dim3(kGrid), dim3(256), 0, s, n, a, b, st, partial);
 }
 
-void launch_dinv_spmv(hipStream_t s, const AmgCsr& A, const double* dinv, const double* x, double scale, double* y) {
-  hipLaunchKernelGGL(dinv_spmv_kernel, dim3(rowGrid(A.n)), dim3(256), 0, s, A, dinv, x, scale, y);
+void launch_dinv_spmv(hipStream_t s, const AmgCsr& A, const double* dinv, const double* x, const double* nsq,
+                      double* y) {
+  hipLaunchKernelGGL(dinv_spmv_kernel, dim3(rowGrid(A.n)), dim3(256), 0, s, A, dinv, x, nsq, y);
 }
 
 }  // namespace nla3d_amd

@@ -247,13 +247,15 @@ def test_vcycle_matches_oracle_on_deep_hierarchies(gpu_lib, case, smoother, monk
 
 
 @pytest.mark.parametrize("case,smoother", [("box_8x8x4", "auto"), ("box_8x8x4", "half"), ("box_30x30x15", "auto"),
-                                           ("box_30x30x15", "half"), ("a2000_damper", "auto")])
+                                           ("box_30x30x15", "half"), ("box_28x28x14", "auto"),
+                                           ("box_28x28x14", "half"), ("a2000_damper", "auto")])
 def test_polynomial_stage_matches_oracle(gpu_lib, case, smoother, monkeypatch):
     """the default Schur stage, one Chebyshev sweep on S (no coarse levels), against the
     oracle's sweep on the device's S and eigenvalue estimate: its length follows the rows
-    (8 terms at 1,280 and 3,101 rows, 10 at 16,383), on the FP32 copy (the choice at these
-    sizes) and forced to the half-precision operator (then on the stored operator, 1e-12);
-    symmetric and positive"""
+    (8 terms at 499 and 3,101 rows, an odd 9 at 13,499, 10 at 16,383), on the FP32 copy (the
+    choice at these sizes) and forced to the half-precision operator (then on the stored
+    operator, 1e-12); symmetric and positive; the eigenvalue estimate bounds the operator the
+    sweep applies (its power iteration runs on that operator)"""
     print(f"[gpu] schur polynomial stage, {case} ({smoother})", flush=True)
     if smoother == "half":
         monkeypatch.setenv("NLA3D_GPU_SCHUR_SMOOTHER", "half")
@@ -283,3 +285,12 @@ def test_polynomial_stage_matches_oracle(gpu_lib, case, smoother, monkeypatch):
     assert np.abs(by - bo).max() <= 1e-12 * np.abs(bo).max()
     assert abs(z @ by - y @ bz) <= 1e-10 * abs(z @ by)
     assert y @ by > 0 and z @ bz > 0
+    if case == "box_8x8x4":  # small enough for a dense eigenvalue check
+        if fh:
+            sh, Hh = check_half_operator(m)
+            lam = np.linalg.eigvalsh(Hh.toarray()).max()
+        else:
+            d = S.diagonal()
+            lam = np.linalg.eigvalsh((S / np.sqrt(np.outer(d, d))).toarray()).max()
+        # 40 power steps approach lambda_max from below; the stored estimate is 1.05 times theirs
+        assert lam * 0.999 <= lmax <= 1.05 * lam * (1 + 1e-12), (lam, lmax)
