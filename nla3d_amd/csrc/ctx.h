@@ -210,9 +210,11 @@ struct Ctx {
   // (*trueStop then set); the preconditioned test stays in force.  beta1Ref > 0: tol is taken
   // relative to beta1Ref instead of b's own preconditioned norm beta1 = sqrt(b^T M b) (the
   // relative tolerance becomes tol * beta1Ref / beta1, at most 0.1)
+  // fused: the iterations run the Schur rows' stage fused with the preconditioner pass instead
+  // of hook (FusedStage; the start still runs hook)
   int minresSolve(const double* b, double* x, double tol, int maxit, double* relres_est, bool* breakdown,
                   const PrecondHook* hook = nullptr, double trueAbs = 0.0, bool* trueStop = nullptr,
-                  double beta1Ref = 0.0);
+                  double beta1Ref = 0.0, const FusedStage* fused = nullptr);
   // sqrt(b^T M b) for the current preconditioner M (block Jacobi, and the hook's stage)
   double minresBeta1(const double* b, const PrecondHook* hook = nullptr);
   // multigrid on the Schur rows (schur.cpp)
@@ -229,6 +231,9 @@ void setupSchur(Ctx& c, const std::vector<int64_t>& rp, const std::vector<int>& 
 bool schurUsable(Ctx& c, std::string* why);
 bool schurRefresh(Ctx& c, std::string* why);
 PrecondHook schurHook(Ctx& c);
+// the stage as a FusedStage (the one-sweep stage on a half-precision level; partials = 0 when
+// the stage cannot run fused and schurHook must be used alone)
+FusedStage schurFusedStage(Ctx& c);
 // constrained-DoF reactions of the last solve into vecR (solver.cpp), on `stream` (null: the handle's)
 void computeReactions(Ctx& c, hipStream_t stream = nullptr);
 // block-Jacobi blocks with the attached network's corrections (learned.cpp); t = load-step time

@@ -54,6 +54,7 @@ struct KrylovMatrix {
 //   gInfo = size | kind << 2 ; kind 0 = node block (size 1..3), 1 = Schur-estimate scalar row
 struct Precond {
   int nGroups = 0;
+  int skipQ = 0;                  // 1: the pass leaves the scalar Schur groups to a fused stage
   const int* gStart = nullptr;    // first row of the group
   const int* gInfo = nullptr;
   const int* gDiagPos = nullptr;  // [nGroups*9] positions of the block entries in the full CSR (-1 absent)
@@ -101,9 +102,22 @@ void krylov_dot(hipStream_t s, int n, const double* a, const double* b, double* 
 // st->converged is set.
 using PrecondHook = std::function<void(hipStream_t, const double* r, double* z, double* partial,
                                        const MinresState* st)>;
+// A stage on the scalar Schur groups' rows that takes over their part of the preconditioner
+// pass: the block-Jacobi pass leaves those rows alone (Precond::skipQ), then `run` does
+// T -= c_r2 R2 on them, writes Z there and `partials` sums of T . Z over them to
+// partial[kKrylovBlocks ..), so no separate pass over r . z follows; every kernel it launches
+// must do nothing once st->converged is set.  (A second stream running the block-Jacobi pass
+// beside the stage was slower: the pass took the compute units the stage's short,
+// latency-bound launches wait for.)
+struct FusedStage {
+  std::function<void(hipStream_t, const double* R2, double* T, double* Z, double* partial,
+                     const MinresState* st)>
+      run;
+  int partials = 0;
+};
 void minres_start(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, const double* b,
                   const PrecondHook* hook = nullptr);
 void minres_iteration(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, double tol,
-                      const PrecondHook* hook = nullptr);
+                      const PrecondHook* hook = nullptr, const FusedStage* fused = nullptr);
 
 }  // namespace nla3d_amd

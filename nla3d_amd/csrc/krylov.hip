@@ -435,11 +435,12 @@ __global__ __launch_bounds__(kRedThreads) void minres_scalar2_kernel(MinresState
 }
 
 // T -= c_r2 * R2 ; Z = M T ; partial beta2 = T . Z   (one thread per group, grid-stride)
+// (skipQ: the scalar Schur groups are left alone, a fused stage takes them)
 __global__ __launch_bounds__(256) void precond_kernel(int nGroups, const int* __restrict__ gStart,
                                                      const int* __restrict__ gInfo, const double* __restrict__ binv,
                                                      double* __restrict__ T, const double* __restrict__ R2,
                                                      double* __restrict__ Z, MinresState* __restrict__ st,
-                                                     double* __restrict__ partial) {
+                                                     double* __restrict__ partial, int skipQ) {
   __shared__ double sh[4];
   __shared__ double sB[256 * 9];  // this pass's 256 inverse blocks, loaded with coalesced reads
   if (st->converged) return;
@@ -450,7 +451,7 @@ __global__ __launch_bounds__(256) void precond_kernel(int nGroups, const int* __
     for (int k = threadIdx.x; k < ng * 9; k += 256) sB[k] = binv[(size_t)g0 * 9 + k];
     __syncthreads();
     const int g = g0 + threadIdx.x;
-    if (g < nGroups) {
+    if (g < nGroups && !(skipQ && (gInfo[g] >> 2) == 1)) {
       const int r0 = gStart[g], size = gInfo[g] & 3;
       double t[3] = {0, 0, 0};
       for (int i = 0; i < size; i++) {
@@ -602,17 +603,27 @@ void minres_start(hipStream_t s, const KrylovMatrix& M, const Precond& P, Minres
 
 // one MINRES iteration; buffers rotate on the host (pure pointer swaps)
 void minres_iteration(hipStream_t s, const KrylovMatrix& M, const Precond& P, MinresWork& w, double tol,
-                      const PrecondHook* hook) {
+                      const PrecondHook* hook, const FusedStage* fused) {
   hipLaunchKernelGGL((spmv_minres_kernel<kLPG1, kLPG>), dim3(kKrylovBlocks), dim3(256), 0, s, M, w.Yp, w.R1, w.T,
                      w.st, w.partial);
   if (M.nDiaFit > 0)
     hipLaunchKernelGGL(spmv_minres_dia_kernel, dim3(kKrylovBlocks), dim3(256), 0, s, M, w.Yp, w.R1, w.T, w.st,
                        w.partial);
   hipLaunchKernelGGL(minres_scalar1_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, kKrylovBlocks);
-  hipLaunchKernelGGL(precond_kernel, dim3(kKrylovBlocks), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo, P.binv, w.T,
-                     w.R2, w.Z, w.st, w.partial);
-  if (hook) (*hook)(s, w.T, w.Z, w.partial, w.st);
-  hipLaunchKernelGGL(minres_scalar2_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, kKrylovBlocks, tol);
+  if (fused) {
+    // the block-Jacobi pass of the other rows, then the stage on the Schur rows
+    hipLaunchKernelGGL(precond_kernel, dim3(kKrylovBlocks), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo, P.binv,
+                       w.T, w.R2, w.Z, w.st, w.partial, 1);
+    fused->run(s, w.R2, w.T, w.Z, w.partial + kKrylovBlocks, w.st);
+    hipLaunchKernelGGL(minres_scalar2_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial,
+                       kKrylovBlocks + fused->partials, tol);
+  } else {
+    hipLaunchKernelGGL(precond_kernel, dim3(kKrylovBlocks), dim3(256), 0, s, P.nGroups, P.gStart, P.gInfo, P.binv,
+                       w.T, w.R2, w.Z, w.st, w.partial, P.skipQ);
+    if (hook) (*hook)(s, w.T, w.Z, w.partial, w.st);
+    hipLaunchKernelGGL(minres_scalar2_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, kKrylovBlocks,
+                       tol);
+  }
   hipLaunchKernelGGL(minres_update_kernel, dim3(cdiv(M.n, 256)), dim3(256), 0, s, M.n, w.Yp, w.W1, w.W2, w.X, w.st);
   // rotations: R1 <- R2, R2 <- T, T <- old R1 ; Yp <- Z ; (W1, W2) <- (W2, Wn)
   double* t = w.R1;

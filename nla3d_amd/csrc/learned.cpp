@@ -632,6 +632,7 @@ int nla3d_gpu_solve_block2_pc(nla3d_gpu_t h, const double* rhs, double* x, doubl
     requireLearnable(c);
     if (!rhs || !x) throw ApiError(-1, "null vectors");
     PrecondHook hook;
+    FusedStage fused;
     if (preconditioner < 0 || preconditioner > 3)
       throw ApiError(-1, "preconditioner must be 0 (block Jacobi), 1 (learned), 2 (Schur multigrid) or 3 (both)");
     krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
@@ -640,13 +641,15 @@ int nla3d_gpu_solve_block2_pc(nla3d_gpu_t h, const double* rhs, double* x, doubl
       std::string why;
       if (!schurUsable(c, &why) || !schurRefresh(c, &why)) throw ApiError(-3, "Schur multigrid: " + why);
       hook = schurHook(c);
+      fused = schurFusedStage(c);
     }
     if (preconditioner == 1 || preconditioner == 3) learnedCorrect(c, t);
     double* b = c.d_mw[7].as<double>();
     hipCheck(hipMemcpyAsync(b, rhs, (size_t)c.n2 * 8, hipMemcpyHostToDevice, c.stream), "rhs");
     double rr = 0;
     bool brk = false;
-    const int it = c.minresSolve(b, c.d_dU.as<double>(), tol, max_iter, &rr, &brk, mg ? &hook : nullptr);
+    const int it = c.minresSolve(b, c.d_dU.as<double>(), tol, max_iter, &rr, &brk, mg ? &hook : nullptr, 0.0,
+                                 nullptr, 0.0, mg ? &fused : nullptr);
     hipCheck(hipMemcpy(x, c.d_dU.p, (size_t)c.n2 * 8, hipMemcpyDeviceToHost), "x");
     if (iterations) *iterations = it;
     if (relres) *relres = rr;

@@ -773,6 +773,29 @@ static void polySweep(SchurAmg& S, hipStream_t s, const double* rext, double* ze
     launch_axpby(s, A.n, 1.0, din, 1.0, A.x.as<double>(), A.x.as<double>());
 }
 
+// the same sweep fused with MINRES's preconditioner pass (FusedStage): the start also does the
+// pass's T -= c_r2 R2 on the Schur rows, the last step writes Z there and the partials of T . Z
+static void polySweepFused(SchurAmg& S, hipStream_t s, const double* R2, double* T, double* Z, double* partial,
+                           const MinresState* st) {
+  AmgLevel& A = *S.L[0];
+  const int steps = polySteps(A.n);
+  const ChebCoef k = chebCoef(A.lmax, steps, polyDiv(steps));
+  const int* qrow = S.qrow.as<int>();
+  const AmgHalf H = A.halfView();
+  double* rhat = A.res.as<double>();
+  double* cur = A.da.as<double>();
+  double* prev = nullptr;
+  launch_half_start_fused(s, A.n, qrow, R2, T, A.hS.as<double>(), k.c0, rhat, cur, st);
+  for (int i = 1; i < steps; i++) {
+    const bool last = i + 1 == steps;
+    double* out = prev ? prev : A.db.as<double>();
+    launch_half_step(s, H, cur, prev, rhat, k.c1[i], k.c2[i], out, last ? qrow : nullptr, last ? Z : nullptr, st,
+                     last ? partial : nullptr);
+    prev = cur;
+    cur = out;
+  }
+}
+
 // one V-cycle from level l.  At level 0 of the MINRES stage (rext != null) the right-hand side
 // is gathered from the Krylov vector rext and the result scattered into zext (the Schur rows,
 // S.qrow; st gates the write) inside the first and last passes; otherwise it is A.r -> A.x.
@@ -878,6 +901,17 @@ PrecondHook schurHook(Ctx& c) {
     vcycle(*S, s, 0, r, z, st);
     launch_dot_partials(s, n2, r, z, st, partial);
   };
+}
+
+FusedStage schurFusedStage(Ctx& c) {
+  FusedStage f;
+  std::shared_ptr<SchurAmg> S = c.schur;
+  if (!S || !S->available || !S->polynomial() || !S->L[0]->half) return f;  // partials = 0: not fused
+  f.partials = (S->L[0]->n + 511) / 512;  // the step kernel's workgroups
+  f.run = [S](hipStream_t s, const double* R2, double* T, double* Z, double* partial, const MinresState* st) {
+    polySweepFused(*S, s, R2, T, Z, partial, st);
+  };
+  return f;
 }
 
 bool schurUsable(Ctx& c, std::string* why) {

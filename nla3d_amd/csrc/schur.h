@@ -119,13 +119,20 @@ void launch_half_fill(hipStream_t s, const AmgHalf& H, const AmgCsr& A, const in
 // start from zero: r^ = s r (r = rsrc[qrow[i]], or rsrc[i] without qrow) ; x^_1 = c0 r^
 void launch_half_start(hipStream_t s, int n, const int* qrow, const double* rsrc, const double* sc, double c0,
                        double* rhat, double* x1, const MinresState* st);
+// the start fused with MINRES's preconditioner pass on the Schur rows: t = T[qrow] - c_r2 R2[qrow]
+// written back to T[qrow], then r^ = s t, x^_1 = c0 r^
+void launch_half_start_fused(hipStream_t s, int n, const int* qrow, const double* R2, double* T, const double* sc,
+                             double c0, double* rhat, double* x1, const MinresState* st);
 // start from x^_0 (post-smoothing): x^_1 = x^_0 + c0 (r^ - S^' x^_0)
 void launch_half_start_x(hipStream_t s, const AmgHalf& H, const double* x0, const double* rhat, double c0, double* x1,
                          const MinresState* st);
 // one step: xout = x^_(k+1) from xk = x^_k and xprev = x^_(k-1) (null: zero; xout may be xprev).
-// With z, the step is the sweep's last: z[qrow[i]] (z[i] without qrow) = s_i x^_(k+1) instead.
+// With z, the step is the sweep's last: z[qrow[i]] (z[i] without qrow) = s_i x^_(k+1) instead;
+// with partial too, workgroup b also writes partial[b] = its rows' sum of r^_i x^_(k+1),i (the
+// rows' share of r . z), one per workgroup of cdiv(n, 512).
 void launch_half_step(hipStream_t s, const AmgHalf& H, const double* xk, const double* xprev, const double* rhat,
-                      double c1, double c2, double* xout, const int* qrow, double* z, const MinresState* st);
+                      double c1, double c2, double* xout, const int* qrow, double* z, const MinresState* st,
+                      double* partial = nullptr);
 // The coarse level's cheb_start done by the restriction that feeds it (res = r, d = c0 D^-1 r,
 // x = 0 for every coarse row; res == nullptr: none, the coarse level starts itself)
 struct ChebStartNext {

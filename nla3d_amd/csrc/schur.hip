@@ -227,6 +227,25 @@ __global__ __launch_bounds__(256) void half_start_kernel(int n, const int* __res
   }
 }
 
+// the same start when the stage runs fused with MINRES's preconditioner pass: first the
+// pass's own update of the Schur rows, t = T - c_r2 R2 (T[qrow] = t), then r^ = s t
+__global__ __launch_bounds__(256) void half_start_fused_kernel(int n, const int* __restrict__ qrow,
+                                                             const double* __restrict__ R2, double* __restrict__ T,
+                                                             const double* __restrict__ sc, double c0,
+                                                             double* __restrict__ rhat, double* __restrict__ x1,
+                                                             const MinresState* __restrict__ st) {
+  if (st->converged) return;
+  const double c2 = st->c_r2;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int r = qrow[i];
+    const double t = T[r] - c2 * R2[r];
+    T[r] = t;
+    const double rh = sc[i] * t;
+    rhat[i] = rh;
+    x1[i] = c0 * rh;
+  }
+}
+
 // The three product kernels take two rows per thread (half_dot_wave).
 template <bool kNarrow, int kND>
 __global__ __launch_bounds__(256) void half_start_x_kernel(AmgHalf H, const double* __restrict__ x0,
@@ -247,22 +266,33 @@ template <bool kNarrow, int kND>
 __global__ __launch_bounds__(256) void half_step_kernel(AmgHalf H, const double* __restrict__ xk, const double* xprev,
                                                       const double* __restrict__ rhat, double c1, double c2,
                                                       double* xout, const int* __restrict__ qrow,
-                                                      double* __restrict__ z, const MinresState* __restrict__ st) {
+                                                      double* __restrict__ z, const MinresState* __restrict__ st,
+                                                      double* __restrict__ partial) {
   if (st && st->converged) return;
   int j[2];
   double a[2];
   half_dot_wave<kNarrow, kND>(H, xk, j[0], a[0], j[1], a[1]);
+  double contrib = 0.0;
 #pragma unroll
   for (int r = 0; r < 2; r++) {
     const int q = j[r];
     if (q < 0) continue;
     const double xq = xk[q];
     const double xp = xprev ? xprev[q] : 0.0;
-    const double xn = xq + c1 * (xq - xp) + c2 * (rhat[q] - a[r]);
+    const double rq = rhat[q];
+    const double xn = xq + c1 * (xq - xp) + c2 * (rq - a[r]);
     if (z)
       z[qrow ? qrow[q] : q] = H.s[q] * xn;
     else
       xout[q] = xn;
+    contrib += rq * xn;  // r . z on this row: (s t) x^ = t (s x^)
+  }
+  if (partial) {  // the workgroup's sum of r . z (the fused stage's partials)
+    __shared__ double sh[4];
+    for (int o = 32; o > 0; o >>= 1) contrib += __shfl_down(contrib, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = contrib;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
   }
 }
 
@@ -680,6 +710,12 @@ void launch_half_start(hipStream_t s, int n, const int* qrow, const double* rsrc
   hipLaunchKernelGGL(half_start_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, qrow, rsrc, sc, c0, rhat, x1, st);
 }
 
+void launch_half_start_fused(hipStream_t s, int n, const int* qrow, const double* R2, double* T, const double* sc,
+                             double c0, double* rhat, double* x1, const MinresState* st) {
+  hipLaunchKernelGGL(half_start_fused_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, qrow, R2, T, sc, c0, rhat, x1,
+                     st);
+}
+
 // the diagonal-form width the kernels unroll: 14 offsets (a hexahedral mesh's element-pressure
 // Schur rows: the element and its 13 upper neighbours), 5 (quadrilaterals), else a loop
 template <template <bool, int> class Launch, typename... Args>
@@ -708,9 +744,9 @@ struct PowerLaunch {
 template <bool kNarrow, int kND>
 struct StepLaunch {
   static void go(const AmgHalf& H, hipStream_t s, const double* xk, const double* xprev, const double* rhat, double c1,
-                 double c2, double* xout, const int* qrow, double* z, const MinresState* st) {
+                 double c2, double* xout, const int* qrow, double* z, const MinresState* st, double* partial) {
     hipLaunchKernelGGL((half_step_kernel<kNarrow, kND>), dim3(cdiv(H.n, 512)), dim3(256), 0, s, H, xk, xprev, rhat,
-                       c1, c2, xout, qrow, z, st);
+                       c1, c2, xout, qrow, z, st, partial);
   }
 };
 template <bool kNarrow, int kND>
@@ -732,8 +768,9 @@ void launch_half_power(hipStream_t s, const AmgHalf& H, const double* x, const d
 }
 
 void launch_half_step(hipStream_t s, const AmgHalf& H, const double* xk, const double* xprev, const double* rhat,
-                      double c1, double c2, double* xout, const int* qrow, double* z, const MinresState* st) {
-  half_dispatch<StepLaunch>(H, s, xk, xprev, rhat, c1, c2, xout, qrow, z, st);
+                      double c1, double c2, double* xout, const int* qrow, double* z, const MinresState* st,
+                      double* partial) {
+  half_dispatch<StepLaunch>(H, s, xk, xprev, rhat, c1, c2, xout, qrow, z, st, partial);
 }
 
 void launch_half_residual_restrict(hipStream_t s, const AmgHalf& H, const double* x, const double* rhat, double* t,

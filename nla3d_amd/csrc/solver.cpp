@@ -369,7 +369,8 @@ void Ctx::setupBlock2(const CsrPattern& U2, const std::vector<int8_t>& kind3, co
   // MINRES work space
   d_rhs.alloc((size_t)n2 * 8);
   for (auto& b : d_mw) b.alloc((size_t)n2 * 8);
-  const int nPart = std::max(kKrylovBlocks, (std::max(n2, nGroups) + 255) / 256) + 16;
+  // (room for a fused Schur stage's partials after the first kKrylovBlocks: one per 512 rows)
+  const int nPart = std::max(kKrylovBlocks, (std::max(n2, nGroups) + 255) / 256) + (n2 + 511) / 512 + 16;
   d_partial.alloc((size_t)nPart * 8);
   d_mstate.alloc(sizeof(MinresState));
   d_scal.alloc(16 * 8);
@@ -538,7 +539,9 @@ double Ctx::minresBeta1(const double* b, const PrecondHook* hook) {
 }
 
 int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* relres_est, bool* breakdown,
-                     const PrecondHook* hook, double trueAbs, bool* trueStop, double beta1Ref) {
+                     const PrecondHook* hook, double trueAbs, bool* trueStop, double beta1Ref,
+                     const FusedStage* fused) {
+  if (fused && fused->partials <= 0) fused = nullptr;
   KrylovMatrix M = krylovMatrix();
   Precond P = precond();
   MinresWork w;
@@ -573,7 +576,7 @@ int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* 
   if (graphs && maxit >= 4 * period) {
     MinresWork w0 = w;
     graph.begin(stream);
-    for (int k = 0; k < period; k++) minres_iteration(stream, M, P, w, tol, hook);
+    for (int k = 0; k < period; k++) minres_iteration(stream, M, P, w, tol, hook, fused);
     graph.end();
     if (w.R1 != w0.R1 || w.Yp != w0.Yp || w.W1 != w0.W1) throw ApiError(-2, "MINRES rotation is not periodic");
   }
@@ -586,7 +589,7 @@ int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* 
       graph.launch(stream);
       it += period;
     } else {
-      for (int k = 0; k < period && it < maxit; k++, it++) minres_iteration(stream, M, P, w, tol, hook);
+      for (int k = 0; k < period && it < maxit; k++, it++) minres_iteration(stream, M, P, w, tol, hook, fused);
     }
     hipCheck(hipGetLastError(), "minres iteration");
     if (trueAbs > 0.0) {  // ||b - K2 x||^2 of the current iterate (the partials are free between iterations)
@@ -823,12 +826,17 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
   const bool learnedPc = o.preconditioner == 1 || o.preconditioner == 3;
   krylov_precond_setup(c.stream, c.krylovMatrix(), c.precond());
   PrecondHook hook;
+  FusedStage fused;
   bool useSchur = false;
   if (o.preconditioner == 2 || o.preconditioner == 3) {  // S from the plain block-Jacobi blocks
     std::string why;
     useSchur = schurUsable(c, &why) && schurRefresh(c, &why);
-    if (useSchur) hook = schurHook(c);
-    else st.precond_fallbacks++;  // plain block Jacobi for this solve
+    if (useSchur) {
+      hook = schurHook(c);
+      fused = schurFusedStage(c);
+    } else {
+      st.precond_fallbacks++;  // plain block Jacobi for this solve
+    }
   }
   if (learnedPc) learnedCorrect(c, t);
   double relres = 0;
@@ -836,7 +844,7 @@ int newtonIteration(Ctx& c, double t, int it, double& rhs1, const nla3d_gpu_newt
   const double* rhs = c.d_rhs.as<double>();
   double beta1Ref = predicted ? c.minresBeta1(rhs, useSchur ? &hook : nullptr) : 0.0;
   int kit = c.minresSolve(b, c.d_dU.as<double>(), ktol, o.krylov_max_iter, &relres, &brk,
-                          useSchur ? &hook : nullptr, trueAbs, &trueStop, beta1Ref);
+                          useSchur ? &hook : nullptr, trueAbs, &trueStop, beta1Ref, useSchur ? &fused : nullptr);
   const double reached = predicted ? std::min(0.1, ktol * beta1Ref / std::max(h_beta1(c), 1e-300)) : ktol;
   if ((learnedPc || useSchur) && (brk || (relres > reached && !trueStop))) {
     // the learned corrections failed this solve: block Jacobi from a fresh start

@@ -139,15 +139,23 @@ def test_vcycle_matches_oracle_and_is_symmetric(gpu_lib, smoother, monkeypatch):
     assert y @ by > 0 and z @ bz > 0
 
 
-@pytest.mark.parametrize("cycle", ["polynomial", "vcycle"])
+@pytest.mark.parametrize("cycle", ["polynomial", "polynomial-half", "vcycle"])
 def test_minres_and_newton_with_schur_multigrid(gpu_lib, cycle, monkeypatch):
+    """polynomial-half: the sweep on the half-precision operator, which runs fused with MINRES's
+    preconditioner pass (the start takes the Schur rows' update from the block-Jacobi pass, the
+    last step writes their share of the r . z partials)"""
     print(f"[gpu] minres / newton with the schur stage ({cycle})", flush=True)
     if cycle == "vcycle":
         monkeypatch.setenv("NLA3D_GPU_SCHUR_CYCLE", "vcycle")
     else:
         monkeypatch.delenv("NLA3D_GPU_SCHUR_CYCLE", raising=False)
+    if cycle == "polynomial-half":
+        monkeypatch.setenv("NLA3D_GPU_SCHUR_SMOOTHER", "half")
+    else:
+        monkeypatch.delenv("NLA3D_GPU_SCHUR_SMOOTHER", raising=False)
     m = rigid_model(8, 8, 4)
     assert m.schur_vcycle() == (cycle == "vcycle")
+    assert m.schur_finest_precision() == (16 if cycle == "polynomial-half" else 32)
     K2, n2 = full_k2(m)
     rhs = np.random.default_rng(4).standard_normal(n2)
     x0, it0, _ = m.solve_block2_pc(rhs, 1e-10, 50000, preconditioner=0)
