@@ -448,10 +448,14 @@ __global__ __launch_bounds__(256) void precond_kernel(int nGroups, const int* __
   double contrib = 0.0;
   for (int g0 = blockIdx.x * 256; g0 < nGroups; g0 += gridDim.x * 256) {
     const int ng = min(256, nGroups - g0);
+    const int g = g0 + threadIdx.x;
+    const bool mine = g < nGroups && !(skipQ && (gInfo[g] >> 2) == 1);
+    // a run of 256 groups all left to the fused stage (the element pressures come first, the
+    // multipliers last) is skipped whole, its inverse blocks unread
+    if (!__syncthreads_or(mine)) continue;
     for (int k = threadIdx.x; k < ng * 9; k += 256) sB[k] = binv[(size_t)g0 * 9 + k];
     __syncthreads();
-    const int g = g0 + threadIdx.x;
-    if (g < nGroups && !(skipQ && (gInfo[g] >> 2) == 1)) {
+    if (mine) {
       const int r0 = gStart[g], size = gInfo[g] & 3;
       double t[3] = {0, 0, 0};
       for (int i = 0; i < size; i++) {
