@@ -101,6 +101,54 @@ __device__ __forceinline__ double half_dot(const AmgHalf& H, int i, int sl, cons
   return acc;
 }
 
+// rows j0 (slice sA) and j1 (slice sA + 1, which exists) of a wave whose two slices are both
+// in ELLPACK form: one loop over the wider slice's slots with the two rows' loads interleaved,
+// so the two dependent chains (column index, then x) run side by side instead of one after the
+// other (the ELLPACK waves were the step's longest); each row summed in half_dot's order
+template <bool kNarrow>
+__device__ __forceinline__ void half_dot_ell2(const AmgHalf& H, int sA, int lane, int j0, int j1,
+                                              const double* __restrict__ x, double& a0, double& a1) {
+  const int bA = H.off[sA], bB = H.off[sA + 1];
+  const int wA = (bB - bA) >> 6, wB = (H.off[sA + 2] - bB) >> 6;
+  const unsigned short* __restrict__ vA = H.val + bA + lane;
+  const unsigned short* __restrict__ vB = H.val + bB + lane;
+  const int cA = (H.cOff ? H.cOff[sA] : bA) + lane, cB = (H.cOff ? H.cOff[sA + 1] : bB) + lane;
+  double acc0 = (double)H.diag[j0] * x[j0];
+  double acc1 = j1 >= 0 ? (double)H.diag[j1] * x[j1] : 0.0;
+  // the slots both rows have (no branch inside, so the loads of an unrolled group issue
+  // together), then the longer row's rest
+  const int wm = wA < wB ? wA : wB;
+  if constexpr (kNarrow) {
+    const unsigned short* __restrict__ kA = H.col16 + cA;
+    const unsigned short* __restrict__ kB = H.col16 + cB;
+    const double* __restrict__ xA = x + H.base[sA];
+    const double* __restrict__ xB = x + H.base[sA + 1];
+#pragma unroll 4
+    for (int k = 0; k < wm; k++) {
+      acc0 += from_half_bits(vA[k * 64]) * xA[kA[k * 64]];
+      acc1 += from_half_bits(vB[k * 64]) * xB[kB[k * 64]];
+    }
+#pragma unroll 4
+    for (int k = wm; k < wA; k++) acc0 += from_half_bits(vA[k * 64]) * xA[kA[k * 64]];
+#pragma unroll 4
+    for (int k = wm; k < wB; k++) acc1 += from_half_bits(vB[k * 64]) * xB[kB[k * 64]];
+  } else {
+    const int* __restrict__ kA = H.col + cA;
+    const int* __restrict__ kB = H.col + cB;
+#pragma unroll 4
+    for (int k = 0; k < wm; k++) {
+      acc0 += from_half_bits(vA[k * 64]) * x[kA[k * 64]];
+      acc1 += from_half_bits(vB[k * 64]) * x[kB[k * 64]];
+    }
+#pragma unroll 4
+    for (int k = wm; k < wA; k++) acc0 += from_half_bits(vA[k * 64]) * x[kA[k * 64]];
+#pragma unroll 4
+    for (int k = wm; k < wB; k++) acc1 += from_half_bits(vB[k * 64]) * x[kB[k * 64]];
+  }
+  a0 = acc0;
+  a1 = acc1;
+}
+
 // unaligned loads of 2 FP16 entries or 2 doubles (the hardware takes them as one access)
 struct Pair64 {
   double a, b;
@@ -149,6 +197,10 @@ __device__ __forceinline__ void half_dot_wave(const AmgHalf& H, const double* __
   }
   j0 = r0 + lane < H.n ? r0 + lane : -1;
   j1 = r0 + 64 + lane < H.n ? r0 + 64 + lane : -1;
+  if (r0 + 64 < H.n && (!H.cOff || (H.cOff[sA] >= 0 && H.cOff[sA + 1] >= 0))) {
+    half_dot_ell2<kNarrow>(H, sA, lane, j0, j1, x, a0, a1);  // (every row of slice sA exists)
+    return;
+  }
   a0 = j0 >= 0 ? half_dot<kNarrow>(H, j0, sA, x) : 0.0;
   a1 = j1 >= 0 ? half_dot<kNarrow>(H, j1, sA + 1, x) : 0.0;
 }
