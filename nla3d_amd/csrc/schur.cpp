@@ -334,8 +334,19 @@ static void allocLevel(AmgLevel& A, bool coarsest) {
     size_t p = 0;
     for (int y = 0; y < 8; y++)
       while ((int)seq[y].size() < cap[y]) seq[y].push_back(plain[p++]);
+    // dispatch word: the block, and in bit 24 + v whether wave v's two slices are both in
+    // diagonal form (the kernels' paired-row form)
+    if (nB >= (1 << 24)) throw ApiError(-1, "Schur level too large for its dispatch order");
     std::vector<int> blk(std::max(nB, 1));
-    for (int b = 0; b < nB; b++) blk[b] = seq[b % 8][b / 8];
+    for (int b = 0; b < nB; b++) {
+      const int blkIdx = seq[b % 8][b / 8];
+      int flags = 0;
+      for (int v = 0; v < 4; v++) {
+        const long long w = 4ll * blkIdx + v;
+        if (A.patterned && w * 128 + 128 <= A.n && patSlice[2 * w] && patSlice[2 * w + 1]) flags |= 1 << v;
+      }
+      blk[b] = blkIdx | (flags << 24);
+    }
     A.hBlk.upload(blk);
     A.hVal.alloc((size_t)std::max(A.nSlots, 1) * 2);
     if (A.patterned) A.hDia.alloc((size_t)A.nd * A.ldd * 2);

@@ -101,8 +101,10 @@ struct AmgHalf {
   const double* s;            // D^-1/2
   int nd, ldd;
   // dispatch order of the product kernels' 512-row blocks: hardware workgroup b (on XCD b mod 8)
-  // takes block blk[b].  The blocks holding ELLPACK slices (the rows slower per row) come first
-  // on every XCD, spread over the eight, then each XCD takes one contiguous range of the rest
+  // takes block blk[b] & 0xffffff; bit 24 + v of blk[b] is set when both slices of the block's
+  // wave v (rows 128 v .. 128 v + 127) are in diagonal form.  The blocks holding ELLPACK slices
+  // (the rows slower per row) come first on every XCD, spread over the eight, then each XCD
+  // takes one contiguous range of the rest
   const int* blk;
 };
 

@@ -169,13 +169,16 @@ __device__ __forceinline__ T load_any(const void* p) {
 template <bool kNarrow, int kND>
 __device__ __forceinline__ void half_dot_wave(const AmgHalf& H, const double* __restrict__ x, int& j0, double& a0,
                                               int& j1, double& a1) {
-  // the workgroup's 512-row block from the dispatch order (AmgHalf::blk); the wave index is
-  // uniform, so it lives in a scalar register
-  const int w = __builtin_amdgcn_readfirstlane(H.blk[blockIdx.x] * 4 + ((int)threadIdx.x >> 6));
+  // the workgroup's dispatch word (AmgHalf::blk: its 512-row block and, per wave, whether both
+  // slices are in diagonal form) is the one load before the rows are known; block and wave
+  // index are uniform, so they live in scalar registers
+  const int word = __builtin_amdgcn_readfirstlane(H.blk[blockIdx.x]);
+  const int v = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int w = (word & 0xffffff) * 4 + v;
   const int lane = threadIdx.x & 63;
   const int r0 = w * 128;
   const int sA = 2 * w;
-  const bool pairForm = kND > 0 && H.cOff && r0 + 128 <= H.n && H.cOff[sA] < 0 && H.cOff[sA + 1] < 0;
+  const bool pairForm = kND > 0 && ((word >> (24 + v)) & 1);
   if (pairForm) {
     const int i = r0 + 2 * lane;
     const Pair64 xc = load_any<Pair64>(x + i);
@@ -320,10 +323,12 @@ __global__ __launch_bounds__(256) void half_step_kernel(AmgHalf H, const double*
                                                       double* xout, const int* __restrict__ qrow,
                                                       double* __restrict__ z, const MinresState* __restrict__ st,
                                                       double* __restrict__ partial) {
-  if (st && st->converged) return;
+  // the state is tested before the stores only, so that no load waits for it
+  const bool done = st && st->converged;
   int j[2];
   double a[2];
   half_dot_wave<kNarrow, kND>(H, xk, j[0], a[0], j[1], a[1]);
+  if (done) return;
   double contrib = 0.0;
 #pragma unroll
   for (int r = 0; r < 2; r++) {
