@@ -330,19 +330,39 @@ __global__ __launch_bounds__(256) void half_step_kernel(AmgHalf H, const double*
   half_dot_wave<kNarrow, kND>(H, xk, j[0], a[0], j[1], a[1]);
   if (done) return;
   double contrib = 0.0;
+  if (kND > 0 && j[1] == j[0] + 1) {
+    // the paired form's two neighbouring rows: one 16-byte access per vector (the same
+    // arithmetic as the loop below)
+    const int i = j[0];
+    const Pair64 xc = load_any<Pair64>(xk + i);
+    const Pair64 xp = xprev ? load_any<Pair64>(xprev + i) : Pair64{0.0, 0.0};
+    const Pair64 rh = load_any<Pair64>(rhat + i);
+    Pair64 xn;
+    xn.a = xc.a + c1 * (xc.a - xp.a) + c2 * (rh.a - a[0]);
+    xn.b = xc.b + c1 * (xc.b - xp.b) + c2 * (rh.b - a[1]);
+    if (z) {
+      z[qrow ? qrow[i] : i] = H.s[i] * xn.a;
+      z[qrow ? qrow[i + 1] : i + 1] = H.s[i + 1] * xn.b;
+      contrib += rh.a * xn.a;
+      contrib += rh.b * xn.b;
+    } else {
+      __builtin_memcpy(xout + i, &xn, sizeof(Pair64));
+    }
+  } else {
 #pragma unroll
-  for (int r = 0; r < 2; r++) {
-    const int q = j[r];
-    if (q < 0) continue;
-    const double xq = xk[q];
-    const double xp = xprev ? xprev[q] : 0.0;
-    const double rq = rhat[q];
-    const double xn = xq + c1 * (xq - xp) + c2 * (rq - a[r]);
-    if (z)
-      z[qrow ? qrow[q] : q] = H.s[q] * xn;
-    else
-      xout[q] = xn;
-    contrib += rq * xn;  // r . z on this row: (s t) x^ = t (s x^)
+    for (int r = 0; r < 2; r++) {
+      const int q = j[r];
+      if (q < 0) continue;
+      const double xq = xk[q];
+      const double xp = xprev ? xprev[q] : 0.0;
+      const double rq = rhat[q];
+      const double xn = xq + c1 * (xq - xp) + c2 * (rq - a[r]);
+      if (z)
+        z[qrow ? qrow[q] : q] = H.s[q] * xn;
+      else
+        xout[q] = xn;
+      contrib += rq * xn;  // r . z on this row: (s t) x^ = t (s x^)
+    }
   }
   if (partial) {  // the workgroup's sum of r . z (the fused stage's partials)
     __shared__ double sh[4];
