@@ -476,16 +476,35 @@ __global__ __launch_bounds__(256) void precond_kernel(int nGroups, const int* __
   if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
-// Wn = (s Yp - oldeps W1 - delta W2) * denom ; X += phi Wn   (Wn overwrites W1)
+// Wn = (s Yp - oldeps W1 - delta W2) * denom ; X += phi Wn   (Wn overwrites W1); two
+// neighbouring entries per thread, each vector read and written 16 bytes at a time
+struct Dbl2u {
+  double a, b;
+};
 __global__ __launch_bounds__(256) void minres_update_kernel(int n, const double* __restrict__ Yp,
                                                            double* __restrict__ W1, const double* __restrict__ W2,
                                                            double* __restrict__ X, const MinresState* __restrict__ st) {
   if (st->skip_update) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= n) return;
-  const double wn = (st->s_update * Yp[i] - st->oldeps * W1[i] - st->delta * W2[i]) * st->denom;
+  const double su = st->s_update, oe = st->oldeps, de = st->delta, dn = st->denom, ph = st->phi;
+  if (i + 1 < n) {
+    Dbl2u y, w1, w2, x, wn;
+    __builtin_memcpy(&y, Yp + i, 16);
+    __builtin_memcpy(&w1, W1 + i, 16);
+    __builtin_memcpy(&w2, W2 + i, 16);
+    __builtin_memcpy(&x, X + i, 16);
+    wn.a = (su * y.a - oe * w1.a - de * w2.a) * dn;
+    wn.b = (su * y.b - oe * w1.b - de * w2.b) * dn;
+    x.a += ph * wn.a;
+    x.b += ph * wn.b;
+    __builtin_memcpy(W1 + i, &wn, 16);
+    __builtin_memcpy(X + i, &x, 16);
+    return;
+  }
+  const double wn = (su * Yp[i] - oe * W1[i] - de * W2[i]) * dn;
   W1[i] = wn;
-  X[i] += st->phi * wn;
+  X[i] += ph * wn;
 }
 
 // start: R1 = R2 = b (x0 = 0), Z = M b, partial = b . Z   (grid-stride)
@@ -628,7 +647,7 @@ void minres_iteration(hipStream_t s, const KrylovMatrix& M, const Precond& P, Mi
     hipLaunchKernelGGL(minres_scalar2_kernel, dim3(1), dim3(kRedThreads), 0, s, w.st, w.partial, kKrylovBlocks,
                        tol);
   }
-  hipLaunchKernelGGL(minres_update_kernel, dim3(cdiv(M.n, 256)), dim3(256), 0, s, M.n, w.Yp, w.W1, w.W2, w.X, w.st);
+  hipLaunchKernelGGL(minres_update_kernel, dim3(cdiv(M.n, 512)), dim3(256), 0, s, M.n, w.Yp, w.W1, w.W2, w.X, w.st);
   // rotations: R1 <- R2, R2 <- T, T <- old R1 ; Yp <- Z ; (W1, W2) <- (W2, Wn)
   double* t = w.R1;
   w.R1 = w.R2;

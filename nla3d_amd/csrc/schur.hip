@@ -291,13 +291,38 @@ __global__ __launch_bounds__(256) void half_start_fused_kernel(int n, const int*
                                                              const MinresState* __restrict__ st) {
   if (st->converged) return;
   const double c2 = st->c_r2;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int r = qrow[i];
-    const double t = T[r] - c2 * R2[r];
-    T[r] = t;
-    const double rh = sc[i] * t;
-    rhat[i] = rh;
-    x1[i] = c0 * rh;
+  // two neighbouring Schur rows per thread, 16-byte accesses where their block(2) rows are
+  // neighbours too (the element pressures)
+  for (int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x); i < n; i += 2 * gridDim.x * blockDim.x) {
+    if (i + 1 < n) {
+      const int r0 = qrow[i], r1 = qrow[i + 1];
+      const Pair64 s2 = load_any<Pair64>(sc + i);
+      Pair64 t2, rh, xo;
+      if (r1 == r0 + 1) {
+        const Pair64 tv = load_any<Pair64>(T + r0), rv = load_any<Pair64>(R2 + r0);
+        t2.a = tv.a - c2 * rv.a;
+        t2.b = tv.b - c2 * rv.b;
+        __builtin_memcpy(T + r0, &t2, sizeof(Pair64));
+      } else {
+        t2.a = T[r0] - c2 * R2[r0];
+        t2.b = T[r1] - c2 * R2[r1];
+        T[r0] = t2.a;
+        T[r1] = t2.b;
+      }
+      rh.a = s2.a * t2.a;
+      rh.b = s2.b * t2.b;
+      xo.a = c0 * rh.a;
+      xo.b = c0 * rh.b;
+      __builtin_memcpy(rhat + i, &rh, sizeof(Pair64));
+      __builtin_memcpy(x1 + i, &xo, sizeof(Pair64));
+    } else {
+      const int r = qrow[i];
+      const double t = T[r] - c2 * R2[r];
+      T[r] = t;
+      const double rh1 = sc[i] * t;
+      rhat[i] = rh1;
+      x1[i] = c0 * rh1;
+    }
   }
 }
 
@@ -789,8 +814,8 @@ void launch_half_start(hipStream_t s, int n, const int* qrow, const double* rsrc
 
 void launch_half_start_fused(hipStream_t s, int n, const int* qrow, const double* R2, double* T, const double* sc,
                              double c0, double* rhat, double* x1, const MinresState* st) {
-  hipLaunchKernelGGL(half_start_fused_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, qrow, R2, T, sc, c0, rhat, x1,
-                     st);
+  hipLaunchKernelGGL(half_start_fused_kernel, dim3(elemGrid((n + 1) / 2)), dim3(256), 0, s, n, qrow, R2, T, sc, c0,
+                     rhat, x1, st);
 }
 
 // the diagonal-form width the kernels unroll: 14 offsets (a hexahedral mesh's element-pressure
