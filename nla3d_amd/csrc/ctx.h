@@ -231,8 +231,9 @@ void setupSchur(Ctx& c, const std::vector<int64_t>& rp, const std::vector<int>& 
 bool schurUsable(Ctx& c, std::string* why);
 bool schurRefresh(Ctx& c, std::string* why);
 PrecondHook schurHook(Ctx& c);
-// the stage as a FusedStage (the one-sweep stage on a half-precision level; partials = 0 when
-// the stage cannot run fused and schurHook must be used alone)
+// the stage as a FusedStage (the one-sweep stage, on its half-precision or FP32 copy; partials
+// = 0 when the stage cannot run fused, the V-cycle or a dense one-level inverse, and schurHook
+// must be used alone)
 FusedStage schurFusedStage(Ctx& c);
 // constrained-DoF reactions of the last solve into vecR (solver.cpp), on `stream` (null: the handle's)
 void computeReactions(Ctx& c, hipStream_t stream = nullptr);

@@ -792,6 +792,21 @@ static void polySweepFused(SchurAmg& S, hipStream_t s, const double* R2, double*
   const int steps = polySteps(A.n);
   const ChebCoef k = chebCoef(A.lmax, steps, polyDiv(steps));
   const int* qrow = S.qrow.as<int>();
+  if (!A.half) {  // the FP32 copy: polySweep's two-term form
+    const AmgSell M = A.sell();
+    double* dinv = A.dinv.as<double>();
+    double* din = A.da.as<double>();
+    double* dout = A.db.as<double>();
+    launch_cheb_start_fused(s, A.n, qrow, R2, T, dinv, k.c0, A.r.as<double>(), A.res.as<double>(), din,
+                            A.x.as<double>(), st);
+    for (int i = 1; i < steps; i++) {
+      launch_cheb_step(s, M, din, dinv, k.c1[i], k.c2[i], A.res.as<double>(), dout, A.x.as<double>(),
+                       chebXMode(i, steps));
+      std::swap(din, dout);
+    }
+    launch_finish_fused(s, A.n, qrow, din, A.x.as<double>(), A.r.as<double>(), st, Z, partial);
+    return;
+  }
   const AmgHalf H = A.halfView();
   double* rhat = A.res.as<double>();
   double* cur = A.da.as<double>();
@@ -917,8 +932,8 @@ PrecondHook schurHook(Ctx& c) {
 FusedStage schurFusedStage(Ctx& c) {
   FusedStage f;
   std::shared_ptr<SchurAmg> S = c.schur;
-  if (!S || !S->available || !S->polynomial() || !S->L[0]->half) return f;  // partials = 0: not fused
-  f.partials = (S->L[0]->n + 511) / 512;  // the step kernel's workgroups
+  if (!S || !S->available || !S->polynomial()) return f;  // partials = 0: not fused
+  f.partials = (S->L[0]->n + 511) / 512;  // one per 512 rows (the last step's / the exit's workgroups)
   f.run = [S](hipStream_t s, const double* R2, double* T, double* Z, double* partial, const MinresState* st) {
     polySweepFused(*S, s, R2, T, Z, partial, st);
   };

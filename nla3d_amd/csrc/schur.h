@@ -175,6 +175,13 @@ void launch_cheb_start_gather(hipStream_t s, int n, const int* qrow, const doubl
                               double* r, double* res, double* d, double* x);
 void launch_finish_scatter(hipStream_t s, int n, const int* qrow, const double* d, const double* x,
                            const MinresState* st, double* z);
+// the same pair fused with MINRES's preconditioner pass: the start also does the pass's
+// T[qrow] -= c_r2 R2[qrow] (r = the result), the exit also writes partial[b] = sum of r . z over
+// rows 512 b .. 512 b + 511 (cdiv(n, 512) partials)
+void launch_cheb_start_fused(hipStream_t s, int n, const int* qrow, const double* R2, double* T, const double* dinv,
+                             double c0, double* r, double* res, double* d, double* x, const MinresState* st);
+void launch_finish_fused(hipStream_t s, int n, const int* qrow, const double* d, const double* x, const double* r,
+                         const MinresState* st, double* z, double* partial);
 // how Chebyshev step i of a sweep of `steps` terms (i = 1 .. steps-1) adds the increments to x:
 // odd steps with a step after them add their input and output increments (kChebPair), the even
 // steps after them add nothing (kChebSkip), and a last odd step adds its input (kChebPlain), so

@@ -536,6 +536,52 @@ __global__ __launch_bounds__(256) void cheb_start_gather_kernel(int n, const int
   }
 }
 
+// the same start fused with MINRES's preconditioner pass on the Schur rows (FusedStage): first
+// the pass's own update t = T[qrow] - c_r2 R2[qrow] (written back to T), then r = res = t,
+// d = c0 D^-1 t, x = 0; nothing once the solve has converged
+__global__ __launch_bounds__(256) void cheb_start_fused_kernel(int n, const int* __restrict__ qrow,
+                                                             const double* __restrict__ R2, double* __restrict__ T,
+                                                             const double* __restrict__ dinv, double c0,
+                                                             double* __restrict__ r, double* __restrict__ res,
+                                                             double* __restrict__ d, double* __restrict__ x,
+                                                             const MinresState* __restrict__ st) {
+  if (st->converged) return;
+  const double c2 = st->c_r2;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int q = qrow[i];
+    const double t = T[q] - c2 * R2[q];
+    T[q] = t;
+    r[i] = t;
+    res[i] = t;
+    d[i] = c0 * dinv[i] * t;
+    x[i] = 0.0;
+  }
+}
+
+// the exit of the fused stage: z[qrow] = d + x as finish_scatter, and workgroup b's sum of
+// r . z over its rows in partial[b] (one workgroup per 512 rows, two rows per thread)
+__global__ __launch_bounds__(256) void finish_fused_kernel(int n, const int* __restrict__ qrow,
+                                                         const double* __restrict__ d, const double* __restrict__ x,
+                                                         const double* __restrict__ r,
+                                                         const MinresState* __restrict__ st, double* __restrict__ z,
+                                                         double* __restrict__ partial) {
+  if (st->converged) return;
+  double contrib = 0.0;
+  for (int k = 0; k < 2; k++) {
+    const int i = blockIdx.x * 512 + k * 256 + (int)threadIdx.x;
+    if (i < n) {
+      const double zi = 1.0 * d[i] + 1.0 * x[i];
+      z[qrow[i]] = zi;
+      contrib += r[i] * zi;
+    }
+  }
+  __shared__ double sh[4];
+  for (int o = 32; o > 0; o >>= 1) contrib += __shfl_down(contrib, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = contrib;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
 // the MINRES stage's exit: z[qrow] = d + x (the last smoothing increment added on the way out),
 // nothing once the solve has converged
 __global__ __launch_bounds__(256) void finish_scatter_kernel(int n, const int* __restrict__ qrow,
@@ -902,6 +948,17 @@ void launch_cheb_start_gather(hipStream_t s, int n, const int* qrow, const doubl
 void launch_finish_scatter(hipStream_t s, int n, const int* qrow, const double* d, const double* x,
                            const MinresState* st, double* z) {
   hipLaunchKernelGGL(finish_scatter_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, qrow, d, x, st, z);
+}
+
+void launch_cheb_start_fused(hipStream_t s, int n, const int* qrow, const double* R2, double* T, const double* dinv,
+                             double c0, double* r, double* res, double* d, double* x, const MinresState* st) {
+  hipLaunchKernelGGL(cheb_start_fused_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, qrow, R2, T, dinv, c0, r, res, d,
+                     x, st);
+}
+
+void launch_finish_fused(hipStream_t s, int n, const int* qrow, const double* d, const double* x, const double* r,
+                         const MinresState* st, double* z, double* partial) {
+  hipLaunchKernelGGL(finish_fused_kernel, dim3(cdiv(n, 512)), dim3(256), 0, s, n, qrow, d, x, r, st, z, partial);
 }
 
 void launch_sell_fill(hipStream_t s, int nSlots, const int* pos, const double* csrVal, float* sellVal) {

@@ -141,9 +141,10 @@ def test_vcycle_matches_oracle_and_is_symmetric(gpu_lib, smoother, monkeypatch):
 
 @pytest.mark.parametrize("cycle", ["polynomial", "polynomial-half", "vcycle"])
 def test_minres_and_newton_with_schur_multigrid(gpu_lib, cycle, monkeypatch):
-    """polynomial-half: the sweep on the half-precision operator, which runs fused with MINRES's
-    preconditioner pass (the start takes the Schur rows' update from the block-Jacobi pass, the
-    last step writes their share of the r . z partials)"""
+    """the sweep (on the FP32 copy, or polynomial-half: on the half-precision operator) runs
+    fused with MINRES's preconditioner pass (the start takes the Schur rows' update from the
+    block-Jacobi pass, the exit writes their share of the r . z partials); the V-cycle runs
+    after the pass"""
     print(f"[gpu] minres / newton with the schur stage ({cycle})", flush=True)
     if cycle == "vcycle":
         monkeypatch.setenv("NLA3D_GPU_SCHUR_CYCLE", "vcycle")
