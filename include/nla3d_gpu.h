@@ -353,8 +353,9 @@ int nla3d_gpu_solve_block2_pc(nla3d_gpu_t h, const double* rhs, double* x, doubl
  * No counterpart in the reference.  The rows that block Jacobi treats by a scalar Schur
  * estimate (element pressures, multipliers) get z = B r with S = alpha B^T D^-1 B - C
  * (schur.h; alpha = 0.3 for the sweep, 1/2 for the V-cycle): by default one Jacobi-Chebyshev
- * sweep on S (8 terms up to 10^4 rows, 8 more
- * per decade, at most 24), or a V-cycle over an aggregation hierarchy of S
+ * sweep on S (8 terms up to 10^4 rows, 8 more per decade, at most 24), its result weighted
+ * against the block-Jacobi rows by 1 at 8 terms rising to 2.5 at 24, or a V-cycle over an
+ * aggregation hierarchy of S
  * (nla3d_gpu_schur_set_cycle; NLA3D_GPU_SCHUR_CYCLE=vcycle in the environment at set-up
  * makes the V-cycle the default).  A level-0 matrix of at most 256 rows is inverted
  * exactly either way.  Inspection entries: sizes of the hierarchy (sizes[l] = rows of level l,

@@ -64,6 +64,13 @@ static int polySteps(int n) {
   return std::max(kPolyMin, std::min(kPolyMax, (int)std::lround(s)));
 }
 static double polyDiv(int steps) { return std::min(400.0, 25.0 * steps); }
+// the stage's output weight against the block-Jacobi rows: z = beta p(D^-1 S) D^-1 r (any
+// weight keeps the preconditioner SPD), rising with the sweep's length from 1 at 8 terms to 2.5
+// at 24.  At 120x120x60 (24 terms, the driver's bench, alpha = 0.3): beta 0.6, 1, 1.6, 2, 2.5,
+// 3, 3.5, 5, 8 -> 394.3, 360.9, 340.8, 330.4, 332.3, 329.4, 338.0, 384.2, 453.6 MINRES iterations
+// per Newton iteration; 2.5 on every size slowed the batched cubes (10^4 - 3 10^4 Schur rows)
+// by 4 %.
+static double polyBeta(int steps) { return 1.0 + 1.5 * (steps - kPolyMin) / double(kPolyMax - kPolyMin); }
 constexpr int kCoefMax = kPolyMax > kSteps0 ? kPolyMax : kSteps0;  // longest sweep (coefficient arrays)
 static_assert(kStepsC <= kCoefMax, "coefficient arrays sized by the longest sweep");
 constexpr double kLmaxSafety = 1.05;  // the estimate approaches lmax from below
@@ -739,7 +746,8 @@ static ChebStartNext startNext(SchurAmg& S, size_t l) {
   return nx;
 }
 
-// the polynomial stage: one Chebyshev sweep of polySteps() terms on level 0, from zero; the MINRES
+// the polynomial stage: one Chebyshev sweep of polySteps() terms on level 0, from zero, its
+// result weighted by polyBeta; the MINRES
 // stage (rext != null) gathers the right-hand side from rext and scatters the result into zext
 // (st gates the write), otherwise A.r -> A.x
 static void polySweep(SchurAmg& S, hipStream_t s, const double* rext, double* zext, const MinresState* st) {
@@ -758,7 +766,7 @@ static void polySweep(SchurAmg& S, hipStream_t s, const double* rext, double* ze
       const bool last = i + 1 == steps;
       double* out = prev ? prev : A.db.as<double>();
       launch_half_step(s, H, cur, prev, rhat, k.c1[i], k.c2[i], out, last ? qrow : nullptr,
-                       last ? (zext ? zext : A.x.as<double>()) : nullptr, st);
+                       last ? (zext ? zext : A.x.as<double>()) : nullptr, st, nullptr, polyBeta(steps));
       prev = cur;
       cur = out;
     }
@@ -779,9 +787,9 @@ static void polySweep(SchurAmg& S, hipStream_t s, const double* rext, double* ze
     std::swap(din, dout);
   }
   if (zext)
-    launch_finish_scatter(s, A.n, qrow, din, A.x.as<double>(), st, zext);
+    launch_finish_scatter(s, A.n, qrow, din, A.x.as<double>(), st, zext, polyBeta(steps));
   else
-    launch_axpby(s, A.n, 1.0, din, 1.0, A.x.as<double>(), A.x.as<double>());
+    launch_axpby(s, A.n, polyBeta(steps), din, polyBeta(steps), A.x.as<double>(), A.x.as<double>());
 }
 
 // the same sweep fused with MINRES's preconditioner pass (FusedStage): the start also does the
@@ -804,7 +812,7 @@ static void polySweepFused(SchurAmg& S, hipStream_t s, const double* R2, double*
                        chebXMode(i, steps));
       std::swap(din, dout);
     }
-    launch_finish_fused(s, A.n, qrow, din, A.x.as<double>(), A.r.as<double>(), st, Z, partial);
+    launch_finish_fused(s, A.n, qrow, din, A.x.as<double>(), A.r.as<double>(), st, Z, partial, polyBeta(steps));
     return;
   }
   const AmgHalf H = A.halfView();
@@ -816,7 +824,7 @@ static void polySweepFused(SchurAmg& S, hipStream_t s, const double* R2, double*
     const bool last = i + 1 == steps;
     double* out = prev ? prev : A.db.as<double>();
     launch_half_step(s, H, cur, prev, rhat, k.c1[i], k.c2[i], out, last ? qrow : nullptr, last ? Z : nullptr, st,
-                     last ? partial : nullptr);
+                     last ? partial : nullptr, polyBeta(steps));
     prev = cur;
     cur = out;
   }

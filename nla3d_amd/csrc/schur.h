@@ -130,11 +130,11 @@ void launch_half_start_x(hipStream_t s, const AmgHalf& H, const double* x0, cons
                          const MinresState* st);
 // one step: xout = x^_(k+1) from xk = x^_k and xprev = x^_(k-1) (null: zero; xout may be xprev).
 // With z, the step is the sweep's last: z[qrow[i]] (z[i] without qrow) = s_i x^_(k+1) instead;
-// with partial too, workgroup b also writes partial[b] = its rows' sum of r^_i x^_(k+1),i (the
-// rows' share of r . z), one per workgroup of cdiv(n, 512).
+// with partial too, workgroup b also writes partial[b] = its rows' sum of r^_i zs x^_(k+1),i (the
+// rows' share of r . z), one per workgroup of cdiv(n, 512); zs scales z (the stage's weight).
 void launch_half_step(hipStream_t s, const AmgHalf& H, const double* xk, const double* xprev, const double* rhat,
                       double c1, double c2, double* xout, const int* qrow, double* z, const MinresState* st,
-                      double* partial = nullptr);
+                      double* partial = nullptr, double zs = 1.0);
 // The coarse level's cheb_start done by the restriction that feeds it (res = r, d = c0 D^-1 r,
 // x = 0 for every coarse row; res == nullptr: none, the coarse level starts itself)
 struct ChebStartNext {
@@ -174,14 +174,14 @@ void launch_cheb_start_x(hipStream_t s, const AmgSell& A, const double* r, const
 void launch_cheb_start_gather(hipStream_t s, int n, const int* qrow, const double* rext, const double* dinv, double c0,
                               double* r, double* res, double* d, double* x);
 void launch_finish_scatter(hipStream_t s, int n, const int* qrow, const double* d, const double* x,
-                           const MinresState* st, double* z);
+                           const MinresState* st, double* z, double zs = 1.0);
 // the same pair fused with MINRES's preconditioner pass: the start also does the pass's
 // T[qrow] -= c_r2 R2[qrow] (r = the result), the exit also writes partial[b] = sum of r . z over
 // rows 512 b .. 512 b + 511 (cdiv(n, 512) partials)
 void launch_cheb_start_fused(hipStream_t s, int n, const int* qrow, const double* R2, double* T, const double* dinv,
                              double c0, double* r, double* res, double* d, double* x, const MinresState* st);
 void launch_finish_fused(hipStream_t s, int n, const int* qrow, const double* d, const double* x, const double* r,
-                         const MinresState* st, double* z, double* partial);
+                         const MinresState* st, double* z, double* partial, double zs = 1.0);
 // how Chebyshev step i of a sweep of `steps` terms (i = 1 .. steps-1) adds the increments to x:
 // odd steps with a step after them add their input and output increments (kChebPair), the even
 // steps after them add nothing (kChebSkip), and a last odd step adds its input (kChebPlain), so

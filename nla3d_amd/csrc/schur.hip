@@ -347,7 +347,7 @@ __global__ __launch_bounds__(256) void half_step_kernel(AmgHalf H, const double*
                                                       const double* __restrict__ rhat, double c1, double c2,
                                                       double* xout, const int* __restrict__ qrow,
                                                       double* __restrict__ z, const MinresState* __restrict__ st,
-                                                      double* __restrict__ partial) {
+                                                      double* __restrict__ partial, double zs) {
   // the state is tested before the stores only, so that no load waits for it
   const bool done = st && st->converged;
   int j[2];
@@ -366,10 +366,10 @@ __global__ __launch_bounds__(256) void half_step_kernel(AmgHalf H, const double*
     xn.a = xc.a + c1 * (xc.a - xp.a) + c2 * (rh.a - a[0]);
     xn.b = xc.b + c1 * (xc.b - xp.b) + c2 * (rh.b - a[1]);
     if (z) {
-      z[qrow ? qrow[i] : i] = H.s[i] * xn.a;
-      z[qrow ? qrow[i + 1] : i + 1] = H.s[i + 1] * xn.b;
-      contrib += rh.a * xn.a;
-      contrib += rh.b * xn.b;
+      z[qrow ? qrow[i] : i] = H.s[i] * (zs * xn.a);
+      z[qrow ? qrow[i + 1] : i + 1] = H.s[i + 1] * (zs * xn.b);
+      contrib += rh.a * (zs * xn.a);
+      contrib += rh.b * (zs * xn.b);
     } else {
       __builtin_memcpy(xout + i, &xn, sizeof(Pair64));
     }
@@ -383,10 +383,10 @@ __global__ __launch_bounds__(256) void half_step_kernel(AmgHalf H, const double*
       const double rq = rhat[q];
       const double xn = xq + c1 * (xq - xp) + c2 * (rq - a[r]);
       if (z)
-        z[qrow ? qrow[q] : q] = H.s[q] * xn;
+        z[qrow ? qrow[q] : q] = H.s[q] * (zs * xn);
       else
         xout[q] = xn;
-      contrib += rq * xn;  // r . z on this row: (s t) x^ = t (s x^)
+      contrib += rq * (zs * xn);  // r . z on this row: (s t) x^ = t (s x^)
     }
   }
   if (partial) {  // the workgroup's sum of r . z (the fused stage's partials)
@@ -564,13 +564,13 @@ __global__ __launch_bounds__(256) void finish_fused_kernel(int n, const int* __r
                                                          const double* __restrict__ d, const double* __restrict__ x,
                                                          const double* __restrict__ r,
                                                          const MinresState* __restrict__ st, double* __restrict__ z,
-                                                         double* __restrict__ partial) {
+                                                         double* __restrict__ partial, double zs) {
   if (st->converged) return;
   double contrib = 0.0;
   for (int k = 0; k < 2; k++) {
     const int i = blockIdx.x * 512 + k * 256 + (int)threadIdx.x;
     if (i < n) {
-      const double zi = 1.0 * d[i] + 1.0 * x[i];
+      const double zi = zs * (1.0 * d[i] + 1.0 * x[i]);
       z[qrow[i]] = zi;
       contrib += r[i] * zi;
     }
@@ -588,10 +588,10 @@ __global__ __launch_bounds__(256) void finish_scatter_kernel(int n, const int* _
                                                            const double* __restrict__ d,
                                                            const double* __restrict__ x,
                                                            const MinresState* __restrict__ st,
-                                                           double* __restrict__ z) {
+                                                           double* __restrict__ z, double zs) {
   if (st && st->converged) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    z[qrow[i]] = 1.0 * d[i] + 1.0 * x[i];
+    z[qrow[i]] = zs * (1.0 * d[i] + 1.0 * x[i]);
 }
 
 // res = r - A x ; d = c0 D^-1 res   (x is read, not written)
@@ -892,9 +892,10 @@ struct PowerLaunch {
 template <bool kNarrow, int kND>
 struct StepLaunch {
   static void go(const AmgHalf& H, hipStream_t s, const double* xk, const double* xprev, const double* rhat, double c1,
-                 double c2, double* xout, const int* qrow, double* z, const MinresState* st, double* partial) {
+                 double c2, double* xout, const int* qrow, double* z, const MinresState* st, double* partial,
+                 double zs) {
     hipLaunchKernelGGL((half_step_kernel<kNarrow, kND>), dim3(cdiv(H.n, 512)), dim3(256), 0, s, H, xk, xprev, rhat,
-                       c1, c2, xout, qrow, z, st, partial);
+                       c1, c2, xout, qrow, z, st, partial, zs);
   }
 };
 template <bool kNarrow, int kND>
@@ -917,8 +918,8 @@ void launch_half_power(hipStream_t s, const AmgHalf& H, const double* x, const d
 
 void launch_half_step(hipStream_t s, const AmgHalf& H, const double* xk, const double* xprev, const double* rhat,
                       double c1, double c2, double* xout, const int* qrow, double* z, const MinresState* st,
-                      double* partial) {
-  half_dispatch<StepLaunch>(H, s, xk, xprev, rhat, c1, c2, xout, qrow, z, st, partial);
+                      double* partial, double zs) {
+  half_dispatch<StepLaunch>(H, s, xk, xprev, rhat, c1, c2, xout, qrow, z, st, partial, zs);
 }
 
 void launch_half_residual_restrict(hipStream_t s, const AmgHalf& H, const double* x, const double* rhat, double* t,
@@ -946,8 +947,8 @@ void launch_cheb_start_gather(hipStream_t s, int n, const int* qrow, const doubl
 }
 
 void launch_finish_scatter(hipStream_t s, int n, const int* qrow, const double* d, const double* x,
-                           const MinresState* st, double* z) {
-  hipLaunchKernelGGL(finish_scatter_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, qrow, d, x, st, z);
+                           const MinresState* st, double* z, double zs) {
+  hipLaunchKernelGGL(finish_scatter_kernel, dim3(elemGrid(n)), dim3(256), 0, s, n, qrow, d, x, st, z, zs);
 }
 
 void launch_cheb_start_fused(hipStream_t s, int n, const int* qrow, const double* R2, double* T, const double* dinv,
@@ -957,8 +958,9 @@ void launch_cheb_start_fused(hipStream_t s, int n, const int* qrow, const double
 }
 
 void launch_finish_fused(hipStream_t s, int n, const int* qrow, const double* d, const double* x, const double* r,
-                         const MinresState* st, double* z, double* partial) {
-  hipLaunchKernelGGL(finish_fused_kernel, dim3(cdiv(n, 512)), dim3(256), 0, s, n, qrow, d, x, r, st, z, partial);
+                         const MinresState* st, double* z, double* partial, double zs) {
+  hipLaunchKernelGGL(finish_fused_kernel, dim3(cdiv(n, 512)), dim3(256), 0, s, n, qrow, d, x, r, st, z, partial,
+                     zs);
 }
 
 void launch_sell_fill(hipStream_t s, int nSlots, const int* pos, const double* csrVal, float* sellVal) {

@@ -7,8 +7,8 @@ the device path implements:
   Schur estimate (element pressures, multipliers), B their couplings to the node
   rows, D^-1 the block-Jacobi inverse node blocks, C their own block;
 * polynomial: the default stage, one Jacobi-Chebyshev sweep on S from zero (no coarse
-  levels), 8 terms up to 10^4 rows and 8 more per decade to at most 24, on
-  [lmax/div, 1.05 lmax] with div = 25 terms (at most 400), on S's FP32-rounded entries
+  levels) weighted by poly_beta, 8 terms up to 10^4 rows and 8 more per decade to at most 24,
+  on [lmax/div, 1.05 lmax] with div = 25 terms (at most 400), on S's FP32-rounded entries
   or, for large matrices, on its half-precision operator (below);
 * vcycle: one V-cycle over given aggregation maps with Jacobi-Chebyshev smoothing
   on [lmax/div, 1.05 lmax], the same polynomial before and after (finest level:
@@ -31,8 +31,8 @@ scaled minimax Chebyshev polynomial in closed form; half_operator is exact on a 
 whose scaled entries FP16 holds, dominates the scaled matrix, and stays positive definite
 where plain FP16 rounding does not; the scaled three-term sweep on it equals the two-term
 Jacobi-Chebyshev sweep on the unscaled operator; the polynomial stage is a symmetric
-positive-definite operator that reduces the energy-norm error, and on a diagonal matrix it is
-the closed-form minimax polynomial (the sweep above).
+positive-definite operator whose unweighted sweep reduces the energy-norm error, and on a
+diagonal matrix it is the closed-form minimax polynomial (the sweep above) times the weight.
 """
 import math
 
@@ -79,11 +79,18 @@ def finest_div(steps):
 POLY_MIN, POLY_MAX = 8, 24  # the polynomial stage's shortest and longest sweep
 
 
+
 def poly_steps(n):
     """length of the polynomial stage's one sweep for n rows: 8 terms up to 10^4 rows, + 8 per
     decade, at most POLY_MAX (the library's polySteps)"""
     s = POLY_MIN + 8.0 * math.log10(max(n, 1) / 1.0e4)
     return max(POLY_MIN, min(POLY_MAX, int(math.floor(s + 0.5))))
+
+
+def poly_beta(steps):
+    """the stage's output weight against the block-Jacobi rows: 1 at 8 terms rising to 2.5 at 24
+    (the library's polyBeta)"""
+    return 1.0 + 1.5 * (steps - POLY_MIN) / (POLY_MAX - POLY_MIN)
 
 
 def poly_div(steps):
@@ -180,18 +187,20 @@ def smooth_scaled(H, rhat, x0, lmax, steps, div):
     return x
 
 
-def polynomial(S, lmax, r, half=None, use_half=False):
-    """the default Schur stage: one Jacobi-Chebyshev sweep from zero on S, z = p(D^-1 S) D^-1 r,
-    poly_steps(rows) terms on [lmax / poly_div, 1.05 lmax]; use_half: on the half-precision
-    operator in the scaled variables (half = (s, H) replaces half_operator(S)), else with S's
-    entries rounded to FP32 (products and sums in FP64, D the FP64 diagonal)"""
+def polynomial(S, lmax, r, half=None, use_half=False, beta=None):
+    """the default Schur stage: one Jacobi-Chebyshev sweep from zero on S weighted by beta
+    (default poly_beta(terms)), z = beta p(D^-1 S) D^-1 r, poly_steps(rows) terms on
+    [lmax / poly_div, 1.05 lmax]; use_half: on the half-precision operator in the scaled
+    variables (half = (s, H) replaces half_operator(S)), else with S's entries rounded to FP32
+    (products and sums in FP64, D the FP64 diagonal)"""
     S = sp.csr_matrix(S)
     steps = poly_steps(S.shape[0])
     div = poly_div(steps)
+    beta = poly_beta(steps) if beta is None else beta
     if use_half:
         s, H = half_operator(S) if half is None else half
-        return s * smooth_scaled(H, s * r, np.zeros_like(r), lmax, steps, div)
-    return smooth(rounded_f32(S), 1.0 / S.diagonal(), r, np.zeros_like(r), lmax, steps, div)
+        return s * (beta * smooth_scaled(H, s * r, np.zeros_like(r), lmax, steps, div))
+    return beta * smooth(rounded_f32(S), 1.0 / S.diagonal(), r, np.zeros_like(r), lmax, steps, div)
 
 
 def vcycle(mats, aggs, lmaxs, r, level=0, half=None, finest_half=True):

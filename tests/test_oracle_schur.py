@@ -150,19 +150,22 @@ def test_polynomial_length_follows_the_size():
     assert [ps.poly_steps(n) for n in (1, 3101, 10000, 31623, 100000, 907923, 10 ** 7)] == \
         [8, 8, 8, 12, 16, 24, 24]
     assert [ps.poly_div(k) for k in (8, 12, 16, 24)] == [200.0, 300.0, 400.0, 400.0]
+    assert [ps.poly_beta(k) for k in (8, 12, 16, 24)] == [1.0, 1.375, 1.75, 2.5]
 
 
 def test_polynomial_stage_is_spd_and_contracts():
-    """z = B r of the one-sweep stage, both forms: B symmetric positive definite, and I - B A
-    contracts in the energy norm (the minimax polynomial on [lmax/div, 1.05 lmax] with lmax
-    the largest eigenvalue of D^-1 A); on the half-precision form A is its operator's unscaled
-    matrix"""
+    """z = B r of the one-sweep stage, both forms: B symmetric positive definite, and without
+    its weight (beta = 1) I - B A contracts in the energy norm (the minimax polynomial on
+    [lmax/div, 1.05 lmax] with lmax the largest eigenvalue of D^-1 A); the weighted stage is
+    poly_beta(terms) times the unweighted one"""
     A = laplacian(8)
     n = A.shape[0]
     lmax = float(np.max(np.linalg.eigvals((sp.diags(1 / A.diagonal()) @ A).toarray()).real))
-    B = np.stack([ps.polynomial(A, lmax, e) for e in np.eye(n)], axis=1)
+    B = np.stack([ps.polynomial(A, lmax, e, beta=1.0) for e in np.eye(n)], axis=1)
     assert np.abs(B - B.T).max() <= 1e-12 * np.abs(B).max()
     assert np.linalg.eigvalsh(0.5 * (B + B.T)).min() > 0
+    Bw = np.stack([ps.polynomial(A, lmax, e) for e in np.eye(n)], axis=1)
+    assert np.abs(Bw - ps.poly_beta(ps.poly_steps(n)) * B).max() <= 1e-14 * np.abs(Bw).max()
     Ad = ps.rounded_f32(A).toarray()
     L = np.linalg.cholesky(Ad)
     E = L.T @ (np.eye(n) - B @ Ad) @ np.linalg.inv(L.T)
@@ -182,7 +185,8 @@ def test_polynomial_stage_is_the_sweep_from_zero():
     lmax = 1.7
     k = ps.poly_steps(30)
     ref = ps.smooth(ps.rounded_f32(A), 1 / A.diagonal(), r, np.zeros(30), lmax, k, ps.poly_div(k))
-    assert np.array_equal(ps.polynomial(A, lmax, r), ref)
+    assert np.array_equal(ps.polynomial(A, lmax, r, beta=1.0), ref)
+    assert np.array_equal(ps.polynomial(A, lmax, r), ps.poly_beta(k) * ref)
     s, H = ps.half_operator(A)
     refh = s * ps.smooth_scaled(H, s * r, np.zeros(30), lmax, k, ps.poly_div(k))
-    assert np.array_equal(ps.polynomial(A, lmax, r, use_half=True), refh)
+    assert np.array_equal(ps.polynomial(A, lmax, r, use_half=True, beta=1.0), refh)
