@@ -143,9 +143,10 @@ typedef struct {
   double krylov_tol;     /* MINRES: preconditioned residual <= krylov_tol * initial */
   int krylov_max_iter;
   int preconditioner;    /* 0: block Jacobi; 1: block Jacobi with the learned node-block corrections of the
-                            network attached by nla3d_gpu_attach_mlp; 2: block Jacobi on the node DoFs and one
-                            V-cycle of aggregation multigrid on the sparse Schur approximation
-                            S = B^T D^-1 B - C for the pressure and multiplier rows; 3: 1 and 2 together
+                            network attached by nla3d_gpu_attach_mlp; 2: block Jacobi on the node DoFs and the
+                            Schur-row stage (one Jacobi-Chebyshev sweep, or a V-cycle) on the sparse
+                            Schur approximation S = B^T D^-1 B - C for the pressure and multiplier
+                            rows; 3: 1 and 2 together
                             (S is formed from the plain block-Jacobi blocks).  A solve of 1, 2 or 3 that
                             breaks down or does not reach its tolerance is repeated with plain block Jacobi
                             (stats.precond_fallbacks) */
@@ -248,8 +249,9 @@ typedef struct {
   double tol;          /* MINRES: preconditioned residual <= tol * initial (default 1e-12) */
   int max_iter;        /* default 20000 */
   int preconditioner;  /* 0 (default): block Jacobi (3x3 node blocks, the diagonal Schur estimate on the
-                          other rows); 2: plus one V-cycle of the Schur-row multigrid on the rows that are
-                          not node DoFs (a solve it fails is repeated with block Jacobi) */
+                          other rows); 2: plus the Schur-row stage (one Jacobi-Chebyshev sweep, or a V-cycle)
+                          on the rows that are not node DoFs (a solve it fails is repeated with block
+                          Jacobi) */
 } nla3d_gpu_eqsolver_opts;
 typedef struct {
   int iterations;         /* MINRES iterations of the last substitute (both runs after a fallback) */
@@ -357,7 +359,9 @@ int nla3d_gpu_solve_block2_pc(nla3d_gpu_t h, const double* rhs, double* x, doubl
  * against the block-Jacobi rows by 1 at 8 terms rising to 2.5 at 24, or a V-cycle over an
  * aggregation hierarchy of S
  * (nla3d_gpu_schur_set_cycle; NLA3D_GPU_SCHUR_CYCLE=vcycle in the environment at set-up
- * makes the V-cycle the default).  A level-0 matrix of at most 256 rows is inverted
+ * makes the V-cycle the default).  The sweep runs inside MINRES's block-Jacobi pass
+ * (NLA3D_GPU_SCHUR_FUSED=0 in the environment, read at each solve: after it, the same
+ * arithmetic).  A level-0 matrix of at most 256 rows is inverted
  * exactly either way.  Inspection entries: sizes of the hierarchy (sizes[l] = rows of level l,
  * up to cap); S of level 0 for the current tangent (qrow[nq] = block(2) row of each Schur row;
  * CSR rowPtr[nq+1], col[nnz0], val[nnz0]); the aggregate of each row of a level and its

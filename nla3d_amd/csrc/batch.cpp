@@ -173,15 +173,21 @@ static void solveModels(Batch& b, const std::vector<int>& ids, double tol, int m
     double rr = 0;
     bool brk = false;
     PrecondHook hook;
+    FusedStage fused;
     bool mg = false;
     int fb = 0;
     if (pc >= 2) {
       std::string why;
       mg = schurUsable(c, &why) && schurRefresh(c, &why);
-      if (mg) hook = schurHook(c);
-      else fb++;
+      if (mg) {
+        hook = schurHook(c);
+        fused = schurFusedStage(c);
+      } else {
+        fb++;
+      }
     }
-    its[k] = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), tol, maxit, &rr, &brk, mg ? &hook : nullptr);
+    its[k] = c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), tol, maxit, &rr, &brk, mg ? &hook : nullptr,
+                           0.0, nullptr, 0.0, mg ? &fused : nullptr);
     if (mg && (brk || rr > tol)) {  // block Jacobi from a fresh start
       fb++;
       brk = false;
@@ -443,16 +449,21 @@ static void autotune(Batch& b, double t, int probeIters, int pc, std::vector<dou
     double r = 0, tRefresh = 0;
     bool brk = false;
     PrecondHook hook;
+    FusedStage fused;
     bool mg = false;
     if (pc >= 2) {
       std::string why;
       const double t0 = nowMsB();
       mg = schurUsable(c, &why) && schurRefresh(c, &why);
       tRefresh = nowMsB() - t0;
-      if (mg) hook = schurHook(c);
+      if (mg) {
+        hook = schurHook(c);
+        fused = schurFusedStage(c);
+      }
     }
     const double t0 = nowMsB();
-    c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), 0.0, probeIters, &r, &brk, mg ? &hook : nullptr);
+    c.minresSolve(c.d_rhs.as<double>(), c.d_dU.as<double>(), 0.0, probeIters, &r, &brk, mg ? &hook : nullptr, 0.0,
+                  nullptr, 0.0, mg ? &fused : nullptr);
     const double perIt = (nowMsB() - t0) / probeIters;
     devMs.push_back(tRefresh + perIt * (mg ? itersMG(order[i]) : itersBJ(order[i])));
   }

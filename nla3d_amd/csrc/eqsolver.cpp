@@ -154,11 +154,15 @@ static void substitute(EqSolver& S, const double* rhs, double* x) {
   st.breakdown = 0;
   if (S.opts.preconditioner == 2 && !S.schurOk) st.precond_fallbacks = 1;
   PrecondHook hook;
-  if (S.schurOk) hook = schurHook(c);
+  FusedStage fused;
+  if (S.schurOk) {
+    hook = schurHook(c);
+    fused = schurFusedStage(c);
+  }
   double relres = 0.0;
   bool brk = false;
   int it = c.minresSolve(c.d_rhs.as<double>(), S.d_x.as<double>(), S.opts.tol, S.opts.max_iter, &relres, &brk,
-                         S.schurOk ? &hook : nullptr);
+                         S.schurOk ? &hook : nullptr, 0.0, nullptr, 0.0, S.schurOk ? &fused : nullptr);
   int total = it;
   if (S.schurOk && (brk || !(relres <= S.opts.tol))) {  // the multigrid failed this solve: block Jacobi
     st.precond_fallbacks++;

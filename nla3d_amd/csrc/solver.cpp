@@ -542,6 +542,10 @@ int Ctx::minresSolve(const double* b, double* x, double tol, int maxit, double* 
                      const PrecondHook* hook, double trueAbs, bool* trueStop, double beta1Ref,
                      const FusedStage* fused) {
   if (fused && fused->partials <= 0) fused = nullptr;
+  if (fused) {  // NLA3D_GPU_SCHUR_FUSED=0: the same stage through the hook, after the block-Jacobi pass
+    const char* e = std::getenv("NLA3D_GPU_SCHUR_FUSED");
+    if (e && e[0] == '0') fused = nullptr;
+  }
   KrylovMatrix M = krylovMatrix();
   Precond P = precond();
   MinresWork w;

@@ -181,6 +181,56 @@ def test_minres_and_newton_with_schur_multigrid(gpu_lib, cycle, monkeypatch):
     assert np.abs(u2 - u0).max() <= 1e-8 * np.abs(u0).max()
 
 
+@pytest.mark.parametrize("smoother", ["auto", "half"])
+def test_fused_sweep_matches_the_sweep_after_the_pass(gpu_lib, smoother, monkeypatch):
+    """one system (12 x 12 x 6 rigid top, 864 Schur rows: two partials of the fused exit) solved
+    with the sweep fused into MINRES's block-Jacobi pass and with the same sweep run after the
+    pass (NLA3D_GPU_SCHUR_FUSED=0): the r . z partials are summed in another order, nothing else
+    differs, so the iteration counts agree within one and the solutions within 1e-9; the drop-in
+    equation solver given the model's DoF map builds the same stage (fused too) and the same
+    iteration count within one; all three within 1e-8 of a direct solve"""
+    import scipy.sparse.linalg as spla
+    print(f"[gpu] fused against unfused Schur sweep ({smoother})", flush=True)
+    monkeypatch.delenv("NLA3D_GPU_SCHUR_CYCLE", raising=False)
+    monkeypatch.delenv("NLA3D_GPU_SCHUR_FUSED", raising=False)
+    if smoother == "half":
+        monkeypatch.setenv("NLA3D_GPU_SCHUR_SMOOTHER", "half")
+    else:
+        monkeypatch.delenv("NLA3D_GPU_SCHUR_SMOOTHER", raising=False)
+    m = rigid_model(12, 12, 6)
+    assert not m.schur_vcycle()
+    assert m.schur_finest_precision() == (16 if smoother == "half" else 32)
+    K2, n2 = full_k2(m)
+    rhs = np.random.default_rng(6).standard_normal(n2)
+    xd = spla.spsolve(K2.tocsc(), rhs)
+    scale = np.abs(xd).max()
+    xf, itf, _ = m.solve_block2_pc(rhs, 1e-10, 50000, preconditioner=2)
+    monkeypatch.setenv("NLA3D_GPU_SCHUR_FUSED", "0")
+    xu, itu, _ = m.solve_block2_pc(rhs, 1e-10, 50000, preconditioner=2)
+    monkeypatch.delenv("NLA3D_GPU_SCHUR_FUSED")
+    sz = m.sizes
+    nC = sz["nConstrained"]
+    ne, ee = m.equation_numbers()
+    rn, rd = np.zeros(n2, dtype=np.uint32), np.full(n2, -1, dtype=np.int32)
+    nodes, types = np.nonzero(ne > nC)
+    q = ne[nodes, types].astype(np.int64) - nC - 1
+    rn[q], rd[q] = nodes + 1, types
+    els, types = np.nonzero(ee > nC)
+    rd[ee[els, types].astype(np.int64) - nC - 1] = types
+    s = nla3d_amd.EqSolver(0, tol=1e-10, max_iter=50000, preconditioner=2)
+    s.set_dof_map(rn, rd)
+    io, cols = m.pattern(2)
+    xe = s.solve(m.get_K(2), io, cols, rhs)
+    st = s.stats()
+    print(f"MINRES iterations: fused {itf}, after the pass {itu}, equation solver {st.iterations}; "
+          f"fused - unfused {np.abs(xf - xu).max() / scale:.2e}", flush=True)
+    assert st.precond_fallbacks == 0 and st.breakdown == 0
+    assert abs(itf - itu) <= 1 and abs(st.iterations - itf) <= 1
+    assert np.abs(xf - xu).max() <= 1e-9 * scale
+    for x in (xf, xu, xe):
+        assert np.abs(x - xd).max() <= 1e-8 * scale
+
+
 def test_smoother_timing_entry(gpu_lib):
     print("[gpu] smoother timing entry", flush=True)
     m = rigid_model(8, 8, 4)  # large enough for a smoothed level below the dense coarsest one
