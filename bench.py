@@ -234,6 +234,47 @@ def cpu_baseline(m, xyz, conn, U, n_sample):
                       f"the host's {fit['dgemm_gflops']:.0f} GFLOP/s matrix-product rate)"}
 
 
+def drop_in_leg(m, tol, preconditioner):
+    """the solver-level drop-in (nla3d_gpu_eqsolver_*: the caller's 1-based upper CSR and rhs in
+    host memory, EquationSolver.h:21-23) on the model's last Newton system, with the model's DoF
+    map: the first factorize (pattern analysis + values), a second one (values only: the upload
+    over PCIe, the mirror into the grouped copy, block Jacobi, the Schur stage), one substitute
+    (rhs up, MINRES, unknowns down), each by the solver's own host clock; beside it the
+    device-resident solve of the same system (nla3d_gpu_solve_block2_pc, host rhs and solution)"""
+    import nla3d_amd
+    io, cols = m.pattern(2)
+    vals = m.get_K(2)
+    rhs = m.get_rhs()
+    n2 = len(io) - 1
+    nC = m.sizes["nConstrained"]
+    ne, ee = m.equation_numbers()
+    rn, rd = np.zeros(n2, dtype=np.uint32), np.full(n2, -1, dtype=np.int32)
+    nodes, types = np.nonzero(ne > nC)
+    q = ne[nodes, types].astype(np.int64) - nC - 1
+    rn[q], rd[q] = nodes + 1, types
+    els, types = np.nonzero(ee > nC)
+    rd[ee[els, types].astype(np.int64) - nC - 1] = types
+    s = nla3d_amd.EqSolver(0, tol=tol, max_iter=100000, preconditioner=preconditioner)
+    try:
+        s.set_dof_map(rn, rd)
+        s.factorize(vals, io, cols)
+        ms_first = s.stats().ms_factorize
+        s.factorize(vals, io, cols)
+        ms_fact = s.stats().ms_factorize
+        x = s.substitute(rhs)
+        st = s.stats()
+    finally:
+        s.close()
+    t0 = time.perf_counter()
+    xm, itm, _ = m.solve_block2_pc(rhs, tol, 100000, preconditioner=preconditioner)
+    ms_model = (time.perf_counter() - t0) * 1e3
+    return {"factorize_first_ms": ms_first, "factorize_ms": ms_fact, "substitute_ms": st.ms_substitute,
+            "iterations": st.iterations, "precond_fallbacks": st.precond_fallbacks,
+            "values_bytes": int(vals.nbytes), "model_solve_ms": ms_model, "model_solve_iterations": itm,
+            "difference": float(np.abs(x - xm).max() / np.abs(xm).max()),
+            "system": "the last timed Newton iteration's K.block(2) and rhs"}
+
+
 def batch_mode(args):
     """Batched SOLID81: independent seeded cubes (n^3 hexahedra, n log-uniform in [4, 32],
     Mooney-Rivlin C10 ~ U[0.2, 1], C01 ~ U[0, 0.3], K/C10 ~ U[50, 2000], top UZ strain
@@ -522,6 +563,8 @@ def main():
                          "iterations alone)")
     ap.add_argument("--cpu-sample", type=int, default=250000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-drop-in", action="store_true",
+                    help="skip the solver-level drop-in's timing on the last Newton system")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU sharding is a design "
@@ -748,6 +791,13 @@ def main():
                   "assembly_elements_per_s": nE / ((ms_elem + ms_scatter) * 1e-3),
                   "spmv_ms": ms_spmv, "spmv_GBps": spmv_gbs,
                   "spmv_traffic": profiled_traffic("spmv", (nx, ny, nz))})
+
+    if args.mode == "newton" and not args.no_drop_in:
+        d = drop_in_leg(m, args.krylov_tol, 2 if opts.preconditioner >= 2 else 0)
+        extra["drop_in"] = d
+        print(f"[bench] drop-in: factorize {d['factorize_ms']:.1f} ms (first {d['factorize_first_ms']:.0f} ms), "
+              f"substitute {d['substitute_ms']:.1f} ms ({d['iterations']} MINRES iterations), device-resident "
+              f"solve {d['model_solve_ms']:.1f} ms ({d['model_solve_iterations']})", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
