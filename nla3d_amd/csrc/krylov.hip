@@ -369,9 +369,23 @@ __global__ __launch_bounds__(256) void dia_fill_kernel(KrylovMatrix M) {
 }
 
 // ---- fixed-order finish of a reduction + MINRES scalar steps --------------
+// Each thread issues its partials' loads 8 at a time before adding any: they were just written by
+// other compute units, so every load misses this one's caches, and a loop that adds as it loads
+// waits out that latency once per partial.  Fixed order: slot k of each thread's group of 8, then
+// the 8 slots pairwise, then the block.
 __device__ double finish_sum(const double* partial, int nPart, double* sh) {
-  double v = 0.0;
-  for (int i = threadIdx.x; i < nPart; i += kRedThreads) v += partial[i];
+  double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int base = threadIdx.x; base < nPart; base += 8 * kRedThreads) {
+    double a[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int i = base + k * kRedThreads;
+      a[k] = i < nPart ? partial[i] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc[k] += a[k];
+  }
+  const double v = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
   return block_sum<kRedThreads>(v, sh);
 }
 
