@@ -463,25 +463,53 @@ __global__ __launch_bounds__(256) void precond_kernel(int nGroups, const int* __
   for (int g0 = blockIdx.x * 256; g0 < nGroups; g0 += gridDim.x * 256) {
     const int ng = min(256, nGroups - g0);
     const int g = g0 + threadIdx.x;
-    const bool mine = g < nGroups && !(skipQ && (gInfo[g] >> 2) == 1);
+    const int info = g < nGroups ? gInfo[g] : 0;
+    const int r0 = g < nGroups ? gStart[g] : 0;
+    const bool mine = g < nGroups && !(skipQ && (info >> 2) == 1);
     // a run of 256 groups all left to the fused stage (the element pressures come first, the
     // multipliers last) is skipped whole, its inverse blocks unread
     if (!__syncthreads_or(mine)) continue;
-    for (int k = threadIdx.x; k < ng * 9; k += 256) sB[k] = binv[(size_t)g0 * 9 + k];
+    // every load of the pass is issued before any is used (two rounds: the inverse blocks with
+    // the group's rows of T and R2, whose addresses need gStart): a loop that used each load as it
+    // came would wait out the memory latency once per entry
+    const int size = info & 3;
+    double b[9], tv[3], rv[3];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const int e = threadIdx.x + k * 256;
+      b[k] = e < ng * 9 ? binv[(size_t)g0 * 9 + e] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {  // unconditional loads of rows in range (the group's first row, or
+                                   // row 0 past the last group), the unused ones zeroed after
+      const bool in = mine && i < size;
+      const int q = in ? r0 + i : r0;
+      const double tq = T[q], rq = R2[q];
+      tv[i] = in ? tq : 0.0;
+      rv[i] = in ? rq : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const int e = threadIdx.x + k * 256;
+      if (e < ng * 9) sB[e] = b[k];
+    }
     __syncthreads();
     if (mine) {
-      const int r0 = gStart[g], size = gInfo[g] & 3;
-      double t[3] = {0, 0, 0};
-      for (int i = 0; i < size; i++) {
-        t[i] = T[r0 + i] - c2 * R2[r0 + i];
-        T[r0 + i] = t[i];
-      }
+      double t[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) t[i] = tv[i] - c2 * rv[i];
       const double* B = sB + threadIdx.x * 9;
-      for (int i = 0; i < size; i++) {
-        double z = 0.0;
-        for (int j = 0; j < size; j++) z += B[i * 3 + j] * t[j];
-        Z[r0 + i] = z;
-        contrib += t[i] * z;
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        if (i < size) {
+          double z = 0.0;
+#pragma unroll
+          for (int j = 0; j < 3; j++)
+            if (j < size) z += B[i * 3 + j] * t[j];
+          T[r0 + i] = t[i];
+          Z[r0 + i] = z;
+          contrib += t[i] * z;
+        }
       }
     }
     __syncthreads();
